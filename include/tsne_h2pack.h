@@ -1,0 +1,206 @@
+/*
+ * tsne_h2pack.h -- C ABI of the MI355X-native t-SNE gradient library
+ * (libtsne_h2pack.so, hand-written gfx950 HIP kernels behind plain C entry points).
+ *
+ * The thirteen entry points marked [ref] are the ones the reference's ctypes layer
+ * binds (tsne_h2pack/_backend.py:345-448 of xinxing02/tsne-h2pack-python 3.3.1); their
+ * signatures, struct layouts, ownership and error conventions are those of the
+ * reference header include/tsne_h2pack.h, so the reference's Python package can load
+ * this library unchanged.  Entry points marked [add] have no counterpart in the
+ * reference; they are additive and the thirteen do not depend on them.
+ *
+ * Conventions (as the reference): every real is float64 at the ABI; X is pt_dim x N and
+ * Y is low_dim x N, dimension-major (element d*N+i); failures return NULL or a nonzero
+ * code and print a message to stderr.  Device state is private to a workspace: every
+ * workspace owns its own HIP stream, so workspaces on one device may run from several
+ * host threads at once.
+ */
+#ifndef TSNE_H2PACK_AMD_H
+#define TSNE_H2PACK_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* CSR matrix, 40 bytes; reference include/tsne_h2pack.h:17-24.  Created by
+ * sparse_matrix_create (malloc) and released by sparse_matrix_destroy (free). */
+typedef struct {
+    int n_rows;
+    int n_cols;
+    int nnz;
+    int *row_ptr;      /* n_rows + 1 */
+    int *col_idx;      /* nnz */
+    double *values;    /* nnz */
+} sparse_matrix_t;
+
+/* Run parameters, 160 bytes with the reference's field order and offsets
+ * (reference include/tsne_h2pack.h:27-85, mirrored by _backend.py:53-95). */
+typedef struct {
+    int n_iter;                    /*   0 */
+    double perplexity;             /*   8 */
+    double learning_rate;          /*  16  < 0: max(N/12, 200) */
+    double momentum;               /*  24  final momentum */
+    double initial_momentum;       /*  32 */
+    int momentum_switch_iter;      /*  40 */
+    double early_exag;             /*  48 */
+    int early_exag_iter;           /*  56 */
+    int use_vptree;                /*  60  legacy k-NN selector, unused on this path */
+    double h2_tol;                 /*  64  relative tolerance of the H2 approximation */
+    int h2_rebuild_freq;           /*  72 */
+    int h2pack_print_level;        /*  76 */
+    int init_method;               /*  80  0: random N(0, 1e-4); 1: copy init_Y */
+    int random_seed;               /*  84 */
+    int verbose;                   /*  88 */
+    int print_freq;                /*  92 */
+    int enable_early_stopping;     /*  96 */
+    double convergence_tol;        /* 104 */
+    int convergence_patience;      /* 112 */
+    int min_iter;                  /* 116 */
+    int sklearn_compatible;        /* 120 */
+    int enable_profiling;          /* 124 */
+    char *profiling_output_file;   /* 128 */
+    int num_threads;               /* 136  host threads for host-side work */
+    int h2_leaf_size;              /* 140  max points per H2 leaf */
+    int h2_rebuild_freq_early;     /* 144 */
+    const double *init_Y;          /* 152  low_dim x N, read only inside tsne_init* */
+} tsne_params_t;
+
+/* Opaque per-job state (device arrays, H2 structure, stream).  The reference
+ * exposes its struct (include/tsne_h2pack.h:102-157) but its Python layer treats it as
+ * opaque (_backend.py:123-127). */
+typedef struct tsne_workspace tsne_workspace_t;
+
+/* [ref] include/tsne_h2pack.h:167 / src/tsne_h2pack.c:56-106: defaults by value. */
+tsne_params_t tsne_create_default_params(void);
+
+/* [ref] include/tsne_h2pack.h:178-184 / src/tsne_h2pack.c:1152: legacy features path.
+ * The k-NN search of that path is outside this library's scope: it returns NULL with
+ * a message directing callers to compute_P_from_knn_distances + tsne_init_with_P. */
+tsne_workspace_t *tsne_init(const double *X, int n_point, int pt_dim, int low_dim,
+                            const tsne_params_t *params);
+
+/* [ref] include/tsne_h2pack.h:196-203 / src/tsne_h2pack.c:1356-1547.  Takes ownership of
+ * P (released with sparse_matrix_destroy by tsne_destroy) once validation passes;
+ * low_dim must be 2 or 3; P must be n_point x n_point.  Uploads P, initialises Y
+ * (copy of init_Y or seeded Box-Muller), and for N >= 2000 builds the first H2
+ * structure on the device.  NULL on failure. */
+tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int low_dim,
+                                   const tsne_params_t *params, sparse_matrix_t *P);
+
+/* [ref] include/tsne_h2pack.h:210 / src/tsne_h2pack.c:1652-2102: the optimisation loop.
+ * Blocking.  0 on success, -1 on failure (non-finite embedding, device error). */
+int tsne_run(tsne_workspace_t *workspace);
+
+/* [ref] include/tsne_h2pack.h:217 / src/tsne_h2pack.c:2108-2122: Y_out is low_dim x N in
+ * the caller's point order. */
+void tsne_get_embedding(tsne_workspace_t *workspace, double *Y_out);
+
+/* [ref] include/tsne_h2pack.h:223 / src/tsne_h2pack.c:2128-2184: frees all, including P. */
+void tsne_destroy(tsne_workspace_t *workspace);
+
+/* [ref] include/tsne_h2pack.h:235-242 / src/tsne_h2pack.c:2190-2212 (via tsne_init). */
+int tsne_fit_transform(const double *X, int n_point, int pt_dim, int low_dim,
+                       const tsne_params_t *params, double *Y_out);
+
+/* [ref] include/tsne_h2pack.h:254-262 / src/tsne_h2pack.c:2214-2237. */
+int tsne_fit_transform_with_P(const double *X, int n_point, int pt_dim, int low_dim,
+                              const tsne_params_t *params, sparse_matrix_t *P,
+                              double *Y_out);
+
+/* [ref] include/tsne_h2pack.h:275 / src/tsne_utils.c:18-39. */
+sparse_matrix_t *sparse_matrix_create(int n_rows, int n_cols, int nnz);
+
+/* [ref] include/tsne_h2pack.h:281 / src/tsne_utils.c:41-49. */
+void sparse_matrix_destroy(sparse_matrix_t *mat);
+
+/* [ref] include/tsne_h2pack.h:336-344 / src/tsne_utils.c:1022-1105: conditional P_j|i
+ * from CSR k-NN L2 distances (not squared) by per-row perplexity bisection, computed on
+ * the GPU in FP64 with one wavefront per row.  Returns a malloc'd CSR of the input's
+ * shape (free with sparse_matrix_destroy), NULL on failure.  num_threads is accepted
+ * for ABI compatibility. */
+sparse_matrix_t *compute_P_from_knn_distances(const double *knn_distances,
+                                              const int *knn_indices,
+                                              const int *knn_indptr, int n_samples,
+                                              int nnz, double perplexity, int num_threads);
+
+/* [ref] include/tsne_h2pack.h:430-434 / src/tsne_h2pack.c:1634-1646. */
+void tsne_set_init_profiling_times(tsne_workspace_t *workspace, double knn_time,
+                                   double p_construction_time);
+
+/* [ref] include/tsne_h2pack.h:405 / src/tsne_h2pack.c:2243-2247: last stored KL. */
+double compute_kl_divergence(tsne_workspace_t *workspace);
+
+/* ------------------------------------------------------------------------------------
+ * [add] Entry points with no counterpart in the reference.
+ * ---------------------------------------------------------------------------------- */
+
+/* Precision of the H2 product.  PLAIN: every stage in FP32 (far-field GEMMs on FP32
+ * VALU).  MIXED: near field FP32, far-field basis and coupling products in BF16 with
+ * FP32 accumulation on MFMA.  The skeleton selection (interpolative decomposition) is
+ * FP64 in both.  Default PLAIN, or the value of the environment variable
+ * TSNE_H2_PRECISION ("plain" / "mixed") read at tsne_init_with_P. */
+#define TSNE_H2_PLAIN 0
+#define TSNE_H2_MIXED 1
+int tsne_set_h2_precision(tsne_workspace_t *workspace, int mode);
+
+/* Number of points, embedding dimension and the effective learning rate. */
+int tsne_workspace_info(tsne_workspace_t *workspace, int *n_point, int *low_dim,
+                        double *learning_rate);
+
+/* Replace the current embedding (low_dim x N, caller's order); momentum and gains are
+ * left as they are.  0 on success. */
+int tsne_set_embedding(tsne_workspace_t *workspace, const double *Y);
+
+/* Gradient of the current embedding without updating it: the attractive force at
+ * exaggeration `exag` plus the repulsive force from a fresh H2 build and product (or
+ * the direct sum where the reference uses it).  grad_out is low_dim x N (caller's
+ * order), *Z_out the normalisation.  exact != 0 evaluates the repulsive part by the
+ * FP64 all-pairs sum on the device instead.  0 on success. */
+int tsne_compute_gradient(tsne_workspace_t *workspace, double exag, int exact,
+                          double *grad_out, double *Z_out);
+
+/* FP64 exact repulsive force (4/Z) sum_j K_ij (y_i - y_j) on selected rows of the
+ * current embedding, with Z from the FP64 all-pairs sum on the device.  out is
+ * n_rows x low_dim row-major; *Z_out receives Z.  0 on success. */
+int tsne_exact_repulsive_rows(tsne_workspace_t *workspace, const int *rows, int n_rows,
+                              double *out, double *Z_out);
+
+/* Statistics of the last H2 build: levels, nodes, leaves, admissible and near pairs,
+ * maximum and mean skeleton rank, proxy points of the finest level; see the
+ * tsne_h2_stats_t fields.  0 on success, -1 if no H2 structure exists. */
+typedef struct {
+    int levels, n_nodes, n_leaves, leaf_points_max;
+    long long n_adm_pairs, n_near_pairs, near_point_pairs;
+    int rank_max;
+    double rank_mean;
+    int proxy_max;
+    double far_flops, near_flops;
+} tsne_h2_stats_t;
+int tsne_h2_stats(tsne_workspace_t *workspace, tsne_h2_stats_t *stats);
+
+/* Phase timings of the last tsne_run, in milliseconds summed over iterations, measured
+ * with HIP events on the workspace stream: attractive+update, rebuild, product, other.
+ * Also the number of rebuilds.  0 on success. */
+int tsne_run_timings(tsne_workspace_t *workspace, double *attr_update_ms,
+                     double *rebuild_ms, double *product_ms, double *other_ms,
+                     int *n_rebuild);
+
+/* Run only iterations [it_begin, it_end) of the schedule of an n_iter-iteration run
+ * (the schedule follows the absolute iteration number).  tsne_run equals
+ * tsne_run_range(ws, 0, n_iter).  Lets a caller time a window of a run. */
+int tsne_run_range(tsne_workspace_t *workspace, int it_begin, int it_end);
+
+/* Block until every operation queued on the workspace stream is done. */
+int tsne_synchronize(tsne_workspace_t *workspace);
+
+/* Per-phase HIP-event timing of later runs (read with tsne_run_timings); it is also on
+ * when params.enable_profiling is set.  0 on success. */
+int tsne_enable_timing(tsne_workspace_t *workspace, int on);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TSNE_H2PACK_AMD_H */

@@ -1,0 +1,160 @@
+"""ctypes wrapper of the CPU oracle (oracle/tsne_oracle.c).
+
+TEST INFRASTRUCTURE ONLY.  This module is the checker: tests/, the smoke() of
+__graft_entry__ and the cpu_baseline leg of bench.py import it.  The product
+library (tsne_h2pack + libtsne_h2pack.so) never imports or links it.
+
+Arrays follow the reference's conventions: Y is (dim, n) dimension-major
+("d*n+i", include/tsne_h2pack.h:107-117 of the reference), FP64, and P is a CSR
+triple (row_ptr int32, col_idx int32, values float64).
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+_dp = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+_ip = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc, OpenMP)."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = ctypes.CDLL(_LIB_PATH)
+        L.oracle_perplexity_rows.argtypes = [_dp, _ip, ctypes.c_int, ctypes.c_double, _dp,
+                                             ctypes.c_void_p]
+        L.oracle_perplexity_rows.restype = None
+        L.oracle_attractive.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp,
+                                        ctypes.c_double, _dp]
+        L.oracle_attractive.restype = None
+        L.oracle_repulsive_exact.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_void_p,
+                                             ctypes.c_void_p, ctypes.c_void_p]
+        L.oracle_repulsive_exact.restype = ctypes.c_double
+        L.oracle_repulsive_rows.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_double,
+                                            _ip, ctypes.c_int, _dp]
+        L.oracle_repulsive_rows.restype = None
+        L.oracle_kl.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp,
+                                ctypes.c_double, ctypes.c_double]
+        L.oracle_kl.restype = ctypes.c_double
+        L.oracle_update.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
+                                    ctypes.c_double, ctypes.c_double]
+        L.oracle_update.restype = None
+        L.oracle_init_random.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_uint]
+        L.oracle_init_random.restype = None
+        L.oracle_is_rebuild_iter.argtypes = [ctypes.c_int] * 5
+        L.oracle_is_rebuild_iter.restype = ctypes.c_int
+        L.oracle_run.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
+                                 ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_int, ctypes.c_double,
+                                 ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.oracle_run.restype = ctypes.c_double
+        L.oracle_num_threads.argtypes = []
+        L.oracle_num_threads.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _csr(P):
+    return (np.ascontiguousarray(P.indptr, dtype=np.int32),
+            np.ascontiguousarray(P.indices, dtype=np.int32),
+            np.ascontiguousarray(P.data, dtype=np.float64))
+
+
+def perplexity_rows(dist, indptr, perplexity):
+    """Conditional P for CSR k-NN distances (L2, not squared); returns (p, steps)."""
+    dist = np.ascontiguousarray(dist, dtype=np.float64)
+    indptr = np.ascontiguousarray(indptr, dtype=np.int32)
+    n = indptr.size - 1
+    p = np.zeros_like(dist)
+    steps = np.zeros(n, dtype=np.int32)
+    lib().oracle_perplexity_rows(dist, indptr, n, float(perplexity), p,
+                                 steps.ctypes.data_as(ctypes.c_void_p))
+    return p, steps
+
+
+def attractive(P, Y, exag):
+    """4*exag*sum_j p_ij w_ij (y_i - y_j); Y is (dim, n). Returns (dim, n)."""
+    dim, n = Y.shape
+    rp, ci, va = _csr(P)
+    Yc = np.ascontiguousarray(Y, dtype=np.float64)
+    g = np.zeros((dim, n), dtype=np.float64)
+    lib().oracle_attractive(n, dim, rp, ci, va, Yc, float(exag), g)
+    return g
+
+
+def repulsive_exact(Y):
+    """(rep, Z, s, v): the reference's direct O(N^2) repulsive branch in FP64."""
+    dim, n = Y.shape
+    Yc = np.ascontiguousarray(Y, dtype=np.float64)
+    rep = np.zeros((dim, n))
+    s = np.zeros(n)
+    v = np.zeros((dim, n))
+    Z = lib().oracle_repulsive_exact(n, dim, Yc, rep.ctypes.data_as(ctypes.c_void_p),
+                                     s.ctypes.data_as(ctypes.c_void_p),
+                                     v.ctypes.data_as(ctypes.c_void_p))
+    return rep, Z, s, v
+
+
+def repulsive_rows(Y, Z, rows):
+    """Exact repulsive force (4/Z) sum_j K_ij (y_i-y_j) on selected rows: (len(rows), dim)."""
+    dim, n = Y.shape
+    Yc = np.ascontiguousarray(Y, dtype=np.float64)
+    rows = np.ascontiguousarray(rows, dtype=np.int32)
+    out = np.zeros((rows.size, dim))
+    lib().oracle_repulsive_rows(n, dim, Yc, float(Z), rows, rows.size, out)
+    return out
+
+
+def kl(P, Y, Z, exag=1.0):
+    dim, n = Y.shape
+    rp, ci, va = _csr(P)
+    return lib().oracle_kl(n, dim, rp, ci, va, np.ascontiguousarray(Y, dtype=np.float64),
+                           float(Z), float(exag))
+
+
+def update(Y, uY, gains, grad, momentum, lr):
+    """In-place gains/momentum/position update plus centring (all (dim, n) float64)."""
+    dim, n = Y.shape
+    lib().oracle_update(n, dim, Y, uY, gains, grad, float(momentum), float(lr))
+
+
+def init_random(n, dim, seed):
+    Y = np.zeros((dim, n))
+    lib().oracle_init_random(Y, n, dim, seed)
+    return Y
+
+
+def is_rebuild_iter(it, freq=3, freq_early=10, early_exag_iter=250, reuse=True):
+    return bool(lib().oracle_is_rebuild_iter(it, freq, freq_early, early_exag_iter, int(reuse)))
+
+
+def run(P, Y0, n_iter=1000, lr=None, mom0=0.5, mom1=0.8, mom_switch=250, exag=12.0,
+        exag_iter=250, freq=3, freq_early=10):
+    """The reference's tsne_run loop with exact repulsion; returns (Y (dim,n), KL)."""
+    dim, n = Y0.shape
+    if lr is None:
+        lr = max(n / 12.0, 200.0)
+    rp, ci, va = _csr(P)
+    Y = np.array(Y0, dtype=np.float64, order="C")
+    uY = np.zeros_like(Y)
+    gains = np.ones_like(Y)
+    klv = lib().oracle_run(n, dim, rp, ci, va, Y, uY, gains, int(n_iter), float(lr),
+                           float(mom0), float(mom1), int(mom_switch), float(exag),
+                           int(exag_iter), int(freq), int(max(freq_early, freq)), 0)
+    return Y, klv
+
+
+def num_threads():
+    return lib().oracle_num_threads()

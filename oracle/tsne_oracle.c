@@ -1,0 +1,333 @@
+/*
+ * tsne_oracle.c -- CPU restatement (FP64) of the reference's per-iteration t-SNE
+ * gradient path.  TEST INFRASTRUCTURE ONLY: this file is the checker that the
+ * HIP path is compared against.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py load it; the product library never links it.
+ *
+ * Every function restates one piece of the reference (xinxing02/tsne-h2pack-python
+ * 3.3.1) and cites the file:line it follows.  The H2 approximation of the
+ * reference is replaced here by the exact O(N^2) sum it approximates (the
+ * reference's own direct branch, src/tsne_h2pack.c:407-491), so that this file is
+ * the FP64 exact gradient the GPU path is held to within h2_tol.
+ *
+ * How each function is pinned (tests/test_oracle.py):
+ *   oracle_perplexity_rows  row sums 1 (1e-10), row perplexity (1e-3), scale invariance
+ *                           (tests/test_perplexity_calibration.py:48-59 of the reference),
+ *                           uniform rows, scipy brentq root of H(beta)=ln(perp), sklearn
+ *                           _binary_search_perplexity.
+ *   oracle_attractive,      sklearn.manifold._t_sne._kl_divergence (dense exact t-SNE
+ *   oracle_repulsive_exact, gradient and KL) on small N; translation invariance;
+ *   oracle_kl               sum of forces = 0.
+ *   oracle_update           sklearn.manifold._t_sne._gradient_descent, one step.
+ *   oracle_run              schedule constants (292 H2 products per 1000 iterations),
+ *                           the pins above composed.
+ *   oracle_init_random      glibc srand/rand Box-Muller, value checks on N(0,1e-4) moments.
+ */
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#include <stdint.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* Perplexity bisection for one row: src/tsne_utils.c:952-1005.
+ * beta starts at 1, grows x2 / shrinks x0.5 while a side of the bracket is open,
+ * bisects once both are known; stops at |H - ln(perp)| < 1e-5 nats or 200 steps.
+ * Distances are shifted by their minimum before exp (src/tsne_utils.c:972-974).
+ * Returns the number of entropy evaluations. */
+static int perplexity_row(const double *sqd, int k, double perplexity, double *p)
+{
+    const double tol = 1e-5;
+    const int max_steps = 200;
+    const double target = log(perplexity);
+    double dmin = sqd[0];
+    for (int j = 1; j < k; j++) if (sqd[j] < dmin) dmin = sqd[j];
+    double beta = 1.0, lo = 0.0, hi = 0.0;
+    int have_lo = 0, have_hi = 0, step;
+    for (step = 0; step < max_steps; step++) {
+        double sp = 0.0, sdp = 0.0;
+        for (int j = 0; j < k; j++) {
+            const double dd = sqd[j] - dmin;
+            p[j] = exp(-beta * dd);
+            sp += p[j];
+            sdp += dd * p[j];
+        }
+        const double H = log(sp) + beta * sdp / sp;
+        const double inv = 1.0 / sp;
+        for (int j = 0; j < k; j++) p[j] *= inv;
+        const double diff = H - target;
+        if (fabs(diff) < tol) break;
+        if (diff > 0.0) {
+            lo = beta; have_lo = 1;
+            beta = have_hi ? 0.5 * (beta + hi) : 2.0 * beta;
+        } else {
+            hi = beta; have_hi = 1;
+            beta = have_lo ? 0.5 * (beta + lo) : 0.5 * beta;
+        }
+    }
+    return step + 1;
+}
+
+/* compute_P_from_knn_distances, src/tsne_utils.c:1022-1105: squares the L2
+ * distances (:1052-1054) and calibrates each CSR row in storage order. */
+void oracle_perplexity_rows(const double *dist, const int *indptr, int n,
+                            double perplexity, double *p_out, int *steps_out)
+{
+    #pragma omp parallel for schedule(dynamic, 64)
+    for (int i = 0; i < n; i++) {
+        const int b = indptr[i], k = indptr[i + 1] - b;
+        if (k <= 0) { if (steps_out) steps_out[i] = 0; continue; }
+        double *sq = (double *)malloc(sizeof(double) * (size_t)k);
+        for (int j = 0; j < k; j++) sq[j] = dist[b + j] * dist[b + j];
+        const int s = perplexity_row(sq, k, perplexity, p_out + b);
+        if (steps_out) steps_out[i] = s;
+        free(sq);
+    }
+}
+
+/* Attractive force, generic FP64 form: src/tsne_h2pack.c:290-356.
+ * grad[d*n+i] += 4 * exag * sum_j p_ij (y_i - y_j) / (1 + |y_i - y_j|^2). */
+void oracle_attractive(int n, int dim, const int *row_ptr, const int *col_idx,
+                       const double *val, const double *Y, double exag, double *grad)
+{
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; i++) {
+        double acc[3] = {0.0, 0.0, 0.0};
+        for (int e = row_ptr[i]; e < row_ptr[i + 1]; e++) {
+            const int j = col_idx[e];
+            double r2 = 0.0, diff[3];
+            for (int d = 0; d < dim; d++) {
+                diff[d] = Y[(size_t)d * n + i] - Y[(size_t)d * n + j];
+                r2 += diff[d] * diff[d];
+            }
+            const double q = val[e] * exag / (1.0 + r2);
+            for (int d = 0; d < dim; d++) acc[d] += 4.0 * q * diff[d];
+        }
+        for (int d = 0; d < dim; d++) grad[(size_t)d * n + i] += acc[d];
+    }
+}
+
+/* Blocked sum whose rounding does not depend on threads: src/tsne_h2pack.c:358-383
+ * (blocks of 4096 summed in order). */
+static double sum_blocked(const double *x, int n)
+{
+    double total = 0.0;
+    for (int b = 0; b < n; b += 4096) {
+        const int e = (b + 4096 < n) ? b + 4096 : n;
+        double acc = 0.0;
+        for (int i = b; i < e; i++) acc += x[i];
+        total += acc;
+    }
+    return total;
+}
+
+/* Exact repulsive force, the reference's direct branch src/tsne_h2pack.c:407-491:
+ * s_i = sum_{j!=i} K_ij, v_i = sum_{j!=i} K_ij y_j with K = (1+r^2)^-2;
+ * Z = sum s + 2 sum |y|^2 s - 2 sum y.v; rep_i = (4/Z)(y_i s_i - v_i).
+ * rows == NULL: all rows; otherwise rep is filled only for rows[0..n_rows) (s and v
+ * still need every row for Z, so the pair loop runs over all i in that case too).
+ * Returns Z.  s (n), v (dim*n) may be NULL. */
+double oracle_repulsive_exact(int n, int dim, const double *Y, double *rep,
+                              double *s_out, double *v_out)
+{
+    double *s = (double *)calloc((size_t)n, sizeof(double));
+    double *v = (double *)calloc((size_t)n * dim, sizeof(double));
+    #pragma omp parallel for schedule(dynamic, 16)
+    for (int i = 0; i < n; i++) {
+        double yi[3], si = 0.0, vi[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < dim; d++) yi[d] = Y[(size_t)d * n + i];
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            double r2 = 0.0;
+            for (int d = 0; d < dim; d++) {
+                const double df = yi[d] - Y[(size_t)d * n + j];
+                r2 += df * df;
+            }
+            const double den = 1.0 + r2;
+            const double w = 1.0 / (den * den);
+            si += w;
+            for (int d = 0; d < dim; d++) vi[d] += w * Y[(size_t)d * n + j];
+        }
+        s[i] = si;
+        for (int d = 0; d < dim; d++) v[(size_t)d * n + i] = vi[d];
+    }
+    /* Z terms (src/tsne_h2pack.c:451-474), block-wise as in :560-590 */
+    double *t1 = (double *)malloc(sizeof(double) * (size_t)n);
+    double *t2 = (double *)malloc(sizeof(double) * (size_t)n);
+    double *t3 = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int i = 0; i < n; i++) {
+        double ny = 0.0, yv = 0.0;
+        for (int d = 0; d < dim; d++) {
+            const double y = Y[(size_t)d * n + i];
+            ny += y * y;
+            yv += y * v[(size_t)d * n + i];
+        }
+        t1[i] = s[i]; t2[i] = ny * s[i]; t3[i] = yv;
+    }
+    const double Z = sum_blocked(t1, n) + 2.0 * sum_blocked(t2, n) - 2.0 * sum_blocked(t3, n);
+    free(t1); free(t2); free(t3);
+    if (rep) {
+        const double c = 4.0 / Z;
+        for (int d = 0; d < dim; d++)
+            for (int i = 0; i < n; i++)
+                rep[(size_t)d * n + i] = c * (Y[(size_t)d * n + i] * s[i] - v[(size_t)d * n + i]);
+    }
+    if (s_out) memcpy(s_out, s, sizeof(double) * (size_t)n);
+    if (v_out) memcpy(v_out, v, sizeof(double) * (size_t)n * dim);
+    free(s); free(v);
+    return Z;
+}
+
+/* Exact repulsive force on selected rows only, given Z: for i in rows,
+ * rep_i = (4/Z) sum_{j!=i} K_ij (y_i - y_j).  Used to check large runs on a row
+ * sample (the GPU supplies Z from its own FP64 all-pairs pass, checked here on
+ * small N).  out is n_rows x dim, row-major. */
+void oracle_repulsive_rows(int n, int dim, const double *Y, double Z,
+                           const int *rows, int n_rows, double *out)
+{
+    #pragma omp parallel for schedule(dynamic, 4)
+    for (int r = 0; r < n_rows; r++) {
+        const int i = rows[r];
+        double yi[3], f[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < dim; d++) yi[d] = Y[(size_t)d * n + i];
+        for (int j = 0; j < n; j++) {
+            if (j == i) continue;
+            double r2 = 0.0, df[3];
+            for (int d = 0; d < dim; d++) {
+                df[d] = yi[d] - Y[(size_t)d * n + j];
+                r2 += df[d] * df[d];
+            }
+            const double den = 1.0 + r2;
+            const double w = 1.0 / (den * den);
+            for (int d = 0; d < dim; d++) f[d] += w * df[d];
+        }
+        for (int d = 0; d < dim; d++) out[(size_t)r * dim + d] = 4.0 / Z * f[d];
+    }
+}
+
+/* KL divergence at the exaggeration in force: src/tsne_h2pack.c:1870-1895.
+ * sum over nonzeros of p*exag*log(p*exag/q), q = 1/((1+d^2) Z), terms with
+ * p*exag > 1e-12 and q > 1e-12 only; per-row sums then the blocked total. */
+double oracle_kl(int n, int dim, const int *row_ptr, const int *col_idx,
+                 const double *val, const double *Y, double Z, double exag)
+{
+    double *row = (double *)malloc(sizeof(double) * (size_t)n);
+    #pragma omp parallel for schedule(dynamic, 256)
+    for (int i = 0; i < n; i++) {
+        double acc = 0.0;
+        for (int e = row_ptr[i]; e < row_ptr[i + 1]; e++) {
+            const int j = col_idx[e];
+            const double p = val[e] * exag;
+            double r2 = 0.0;
+            for (int d = 0; d < dim; d++) {
+                const double df = Y[(size_t)d * n + i] - Y[(size_t)d * n + j];
+                r2 += df * df;
+            }
+            const double q = 1.0 / ((1.0 + r2) * Z);
+            if (p > 1e-12 && q > 1e-12) acc += p * log(p / q);
+        }
+        row[i] = acc;
+    }
+    const double kl = sum_blocked(row, n);
+    free(row);
+    return kl;
+}
+
+/* Gains / momentum / position update, src/tsne_h2pack.c:1944-1968, then the
+ * centring src/tsne_h2pack.c:1978-1984 (mean by blocked sum). */
+void oracle_update(int n, int dim, double *Y, double *uY, double *gains,
+                   double *grad, double momentum, double lr)
+{
+    const size_t m = (size_t)n * dim;
+    for (size_t i = 0; i < m; i++) {
+        const int flip = (uY[i] * grad[i] < 0.0);
+        if (flip) gains[i] += 0.2; else gains[i] *= 0.8;
+        if (gains[i] < 0.01) gains[i] = 0.01;
+        grad[i] *= gains[i];
+        uY[i] = momentum * uY[i] - lr * grad[i];
+        Y[i] += uY[i];
+    }
+    for (int d = 0; d < dim; d++) {
+        double *Yd = Y + (size_t)d * n;
+        const double mean = sum_blocked(Yd, n) / n;
+        for (int i = 0; i < n; i++) Yd[i] -= mean;
+    }
+}
+
+/* Random initial embedding, src/tsne_utils.c:1129-1148: srand(seed), then
+ * Box-Muller pairs from rand()/RAND_MAX, u1 floored at 1e-10, scaled by 1e-4,
+ * filling the dim x n array in storage order. */
+void oracle_init_random(double *Y, int n, int dim, unsigned int seed)
+{
+    srand(seed);
+    const int m = n * dim;
+    for (int i = 0; i < m; i += 2) {
+        double u1 = (double)rand() / RAND_MAX;
+        double u2 = (double)rand() / RAND_MAX;
+        if (u1 < 1e-10) u1 = 1e-10;
+        const double rad = sqrt(-2.0 * log(u1));
+        Y[i] = 1e-4 * rad * cos(2.0 * M_PI * u2);
+        if (i + 1 < m) Y[i + 1] = 1e-4 * rad * sin(2.0 * M_PI * u2);
+    }
+}
+
+/* Rebuild cadence of tsne_run, src/tsne_h2pack.c:1765-1776: every iteration
+ * below 20, then every max(freq, freq_early) until early_exag_iter (only when the
+ * force is reused), then every freq.  Returns 1 when iteration `it` rebuilds. */
+int oracle_is_rebuild_iter(int it, int freq, int freq_early, int early_exag_iter, int reuse)
+{
+    if (freq <= 0) freq = 10;
+    int eff = (it < 20) ? 1 : freq;
+    if (it >= 20 && it < early_exag_iter && reuse && freq_early > eff) eff = freq_early;
+    return it > 0 && it % eff == 0;
+}
+
+/* The optimisation loop of tsne_run (src/tsne_h2pack.c:1652-2078) in sklearn mode
+ * with the exact repulsive force in place of the H2 product.  The reference's
+ * repulsive-force reuse (:1838-1850) is kept: for n >= 2000 the force (and its Z)
+ * is recomputed only on iteration 0 and on rebuild iterations and reused
+ * otherwise; below 2000 points it is recomputed every iteration.  KL is
+ * evaluated at the last iteration before its update, with that iteration's Z.
+ * Y, uY, gains are dim x n and updated in place.  Returns the final KL. */
+double oracle_run(int n, int dim, const int *row_ptr, const int *col_idx, const double *val,
+                  double *Y, double *uY, double *gains, int n_iter, double lr,
+                  double mom0, double mom1, int mom_switch, double exag0, int exag_iter,
+                  int freq, int freq_early, int it_begin)
+{
+    const size_t m = (size_t)n * dim;
+    double *grad = (double *)malloc(sizeof(double) * m);
+    double *rep = (double *)malloc(sizeof(double) * m);
+    const int reuse = (n >= 2000);
+    int rep_valid = 0;
+    double rep_Z = 1.0, kl = 0.0;
+    for (int it = it_begin; it < n_iter; it++) {
+        const double mom = (it < mom_switch) ? mom0 : mom1;
+        const double exag = (it < exag_iter) ? exag0 : 1.0;
+        const int rebuilt = (it == 0) || oracle_is_rebuild_iter(it, freq, freq_early, exag_iter, reuse);
+        memset(grad, 0, sizeof(double) * m);
+        oracle_attractive(n, dim, row_ptr, col_idx, val, Y, exag, grad);
+        double Z;
+        if (reuse && !rebuilt && rep_valid) {
+            Z = rep_Z;
+        } else {
+            Z = oracle_repulsive_exact(n, dim, Y, rep, NULL, NULL);
+            rep_Z = Z;
+            rep_valid = 1;
+        }
+        for (size_t i = 0; i < m; i++) grad[i] -= rep[i];
+        if (it == n_iter - 1) kl = oracle_kl(n, dim, row_ptr, col_idx, val, Y, Z, exag);
+        oracle_update(n, dim, Y, uY, gains, grad, mom, lr);
+    }
+    free(grad); free(rep);
+    return kl;
+}
+
+int oracle_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}

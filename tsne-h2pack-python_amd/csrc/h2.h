@@ -1,0 +1,32 @@
+// h2.h -- device H2 representation of the repulsive kernel K(x, y) = (1 + |x - y|^2)^-2:
+// rebuild (Morton-sorted cluster tree, admissible / near pair lists, proxy-point
+// interpolative decomposition per node) and the fused product that returns
+//   s_i = sum_{j != i} K_ij   and   F_i = sum_{j != i} K_ij (y_i - y_j).
+#pragma once
+#include <hip/hip_runtime.h>
+#include "../../include/tsne_h2pack.h"
+
+namespace tsne {
+
+struct H2;
+
+// n points of dimension dim (2 or 3); leaf_size is the maximum number of points per
+// leaf; tol the relative tolerance of the interpolative decompositions.
+H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st);
+void h2_destroy(H2 *h);
+
+// Rebuild from the float32 gather copy Yg (caller order).  Returns 0, or -1 on a
+// device error or a non-finite coordinate (reported in *nonfinite).
+int h2_build(H2 *h, const void *Yg, int *nonfinite);
+
+// Product: s (n) and F (n x dim, interleaved), caller order.  mixed selects the
+// BF16/MFMA far field.
+int h2_product(H2 *h, const void *Yg, float *s, float *F, int mixed);
+
+// Whether the last build produced a usable H2 structure (a tree too shallow for any
+// admissible pair falls back to the direct sum, as the reference does).
+int h2_usable(const H2 *h);
+
+int h2_stats(const H2 *h, tsne_h2_stats_t *st);
+
+}  // namespace tsne

@@ -1,0 +1,40 @@
+// kernels.h -- launchers of the per-iteration kernels (kernels_iter.hip) and the
+// device scalar slots they share.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace tsne {
+
+// Slots of the per-workspace device scalar array `scal` (double).
+enum ScalarSlot {
+    SLOT_Z = 0,        // normalisation Z of the current repulsive force
+    SLOT_MEAN = 1,     // 1..3: pending centring mean of Y64 (removed at next read)
+    SLOT_YSHIFT = 4,   // 4..6: the mean in force when the repulsive product was formed
+    SLOT_BAD = 7,      // non-finite embedding seen
+    SLOT_ZBAD = 8,     // non-positive / non-finite Z seen (replaced by 1)
+    SLOT_KL = 9,       // last KL divergence
+    SLOT_ZEXACT = 10,  // Z of the FP64 check path
+    SLOT_COUNT = 16
+};
+
+int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
+                       const void *Yg, const float *F, double *Y64, double *uY, double *gains,
+                       void *Yg_next, const double *scal, double exag, double mom, double lr,
+                       double *block_sums, int nblocks, hipStream_t st);
+int launch_repulsive_direct_f32(int D, int n, const void *Yg, float *s, float *F,
+                                hipStream_t st);
+int launch_z(int D, int n, const void *Yg, const float *s, const float *F, double *scal,
+             double *partial, int nblocks, hipStream_t st);
+int launch_shift_snapshot(double *scal, hipStream_t st);
+int launch_kl(int D, int n, const int *row_ptr, const int *col_idx, const double *pval64,
+              const double *Y64, const double *scal, double exag, double *row_kl,
+              double *partial, int nblocks, double *dst, hipStream_t st);
+int launch_repulsive_direct_f64(int D, int n, const double *Y64, const int *rows, int n_rows,
+                                double *s, double *F, hipStream_t st);
+int launch_z_f64(int D, int n, const double *Y64, const double *s, const double *F,
+                 const double *scal, double *partial, int nblocks, double *dst,
+                 hipStream_t st);
+int launch_perplexity(int n, const double *dist, const int *indptr, double perplexity,
+                      double *p_out, hipStream_t st);
+
+}  // namespace tsne

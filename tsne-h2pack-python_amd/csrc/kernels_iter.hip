@@ -1,0 +1,519 @@
+// kernels_iter.hip -- per-iteration kernels of the t-SNE gradient path on gfx950:
+//   * attractive CSR force with the gains/momentum update fused into its epilogue
+//     (one wavefront per row),
+//   * deterministic mean reduction for the re-centring,
+//   * direct O(N^2) repulsive sums (FP32 product path, FP64 check path),
+//   * Z from the repulsive sums, KL divergence,
+//   * perplexity bisection (one wavefront per row, FP64).
+//
+// State layout in HBM (per workspace):
+//   Y64, uY, gains   double [n][D]   interleaved, master copy of the embedding
+//   Yg               Vec<D>  [n]     float32 gather copy (float2 in 2D, float4 in 3D)
+//   F                float   [n][D]  unscaled repulsive sum  sum_j K_ij (y_i - y_j)
+//   s                float   [n]     sum_{j != i} K_ij
+//   scal             double  [..]    device scalars: Z, the pending centring mean, ...
+// The row order of all of these is the caller's point order.
+#include "common.h"
+#include "kernels.h"
+
+namespace tsne {
+
+// ---------------------------------------------------------------------------------
+// Attractive force + fused update.
+//
+// Reference: attractive force src/tsne_h2pack.c:199-356 (2D float32 gather,
+// src/tsne_attr_2d.h:22-67); update src/tsne_h2pack.c:1944-1968; centring :1978-1984.
+//
+// One wavefront per CSR row.  Lanes stride over the row's nonzeros, gather the
+// neighbour from the float32 copy Yg_cur and accumulate p_ij w_ij (y_i - y_j) in
+// float32; a wave reduction gives the row's force.  Lanes 0..D-1 then apply the
+// update to coordinate d of the row in FP64:
+//     g      = 4*exag*attr_d - (4/Z) F_d            (F: reused or fresh repulsive sum)
+//     gains  = (uY*g < 0) ? gains + 0.2 : gains * 0.8, floored at 0.01
+//     uY     = mom*uY - lr*gains*g
+//     y      = (Y64 - mean_prev) + uY
+// and write Y64, uY, gains and the next gather copy Yg_next.  mean_prev is the
+// mean removed lazily: the forces are translation invariant, so centring is applied
+// when Y64 is next read instead of in an extra pass.  Each block also writes the
+// sum of its new coordinates, in a fixed order, for the next mean.
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
+                   const float *__restrict__ pval, const typename Vec<D>::T *__restrict__ Yg,
+                   const float *__restrict__ F, double *__restrict__ Y64,
+                   double *__restrict__ uY, double *__restrict__ gains,
+                   typename Vec<D>::T *__restrict__ Yg_next, const double *__restrict__ scal,
+                   double exag4, double mom, double lr, int rows_per_block,
+                   double *__restrict__ block_sums)
+{
+    using V = typename Vec<D>::T;
+    __shared__ double wsum[4][3];
+    const int lane = threadIdx.x & 63;
+    const int wid = threadIdx.x >> 6;
+    const int row0 = blockIdx.x * rows_per_block;
+    const int row1 = min(n, row0 + rows_per_block);
+    const double rep_scale = 4.0 / scal[SLOT_Z];
+    double my_sum = 0.0;  // lane d: sum of new coordinate d over this wave's rows
+
+    for (int i = row0 + wid; i < row1; i += 4) {
+        const V yi = Yg[i];
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
+        for (int e = e0 + lane; e < e1; e += 64) {
+            const int j = col_idx[e];
+            const float p = pval[e];
+            const V yj = Yg[j];
+            const float dx = yi.x - yj.x, dy = yi.y - yj.y;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
+            float dz = 0.f;
+            if constexpr (D == 3) { dz = yi.z - yj.z; r2 = fmaf(dz, dz, r2); }
+            const float q = p * fast_rcp(r2);
+            ax = fmaf(q, dx, ax);
+            ay = fmaf(q, dy, ay);
+            if constexpr (D == 3) az = fmaf(q, dz, az);
+        }
+        ax = wave_sum(ax);
+        ay = wave_sum(ay);
+        if constexpr (D == 3) az = wave_sum(az);
+        double ynew = 0.0;
+        if (lane < D) {
+            const float a = (lane == 0) ? ax : (lane == 1) ? ay : az;
+            const size_t k = (size_t)i * D + lane;
+            const double g = exag4 * (double)a - rep_scale * (double)F[k];
+            double u = uY[k], gn = gains[k];
+            gn = (u * g < 0.0) ? gn + 0.2 : gn * 0.8;
+            if (gn < 0.01) gn = 0.01;
+            u = mom * u - lr * (gn * g);
+            ynew = (Y64[k] - scal[SLOT_MEAN + lane]) + u;
+            Y64[k] = ynew;
+            uY[k] = u;
+            gains[k] = gn;
+            my_sum += ynew;
+        }
+        // the new coordinates sit in lanes 0..D-1: move them to lane 0 for the record
+        const double n0 = __shfl(ynew, 0, 64);
+        const double n1 = __shfl(ynew, 1, 64);
+        const double n2 = (D == 3) ? __shfl(ynew, 2, 64) : 0.0;
+        if (lane == 0) {
+            V rec;
+            rec.x = (float)n0;
+            rec.y = (float)n1;
+            if constexpr (D == 3) { rec.z = (float)n2; rec.w = 0.f; }
+            Yg_next[i] = rec;
+        }
+    }
+    // per-wave sums (lane d holds coordinate d), then waves in order
+    if (lane < D) wsum[wid][lane] = my_sum;
+    __syncthreads();
+    if (threadIdx.x < D) {
+        double t = 0.0;
+        for (int w = 0; w < 4; w++) t += wsum[w][threadIdx.x];
+        block_sums[(size_t)blockIdx.x * 3 + threadIdx.x] = t;
+    }
+}
+
+// Sum of the per-block sums in block order -> new pending mean (SLOT_MEAN..+D-1).
+// Also folds a finiteness flag (SLOT_BAD) for the host.
+__global__ void mean_finalize_kernel(int nblocks, int D, int n,
+                                     const double *__restrict__ block_sums,
+                                     double *__restrict__ scal)
+{
+    __shared__ double part[256][3];
+    const int t = threadIdx.x;
+    double a[3] = {0.0, 0.0, 0.0};
+    // each thread sums a contiguous chunk of blocks in order, then chunks in order
+    const int per = (nblocks + 255) / 256;
+    const int b0 = t * per, b1 = min(nblocks, b0 + per);
+    for (int b = b0; b < b1; b++)
+        for (int d = 0; d < D; d++) a[d] += block_sums[(size_t)b * 3 + d];
+    for (int d = 0; d < 3; d++) part[t][d] = a[d];
+    __syncthreads();
+    if (t < D) {
+        double s = 0.0;
+        for (int k = 0; k < 256; k++) s += part[k][t];
+        const double m = s / (double)n;
+        scal[SLOT_MEAN + t] = m;
+        if (!isfinite(m)) scal[SLOT_BAD] = 1.0;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Direct O(N^2) repulsive sums in FP32 (the reference's direct branch,
+// src/tsne_h2pack.c:407-491, used below 2000 points and for method='exact').
+// One thread per target; sources staged through LDS in tiles of 256.  Produces
+//   s_i = sum_{j != i} K_ij,   F_i = sum_{j != i} K_ij (y_i - y_j),  K = (1+r^2)^-2.
+// F is formed from differences directly, which avoids the y*s - v cancellation of
+// the reference's assembly (src/tsne_h2pack.c:619) in float32.
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+repulsive_direct_f32(int n, const typename Vec<D>::T *__restrict__ Yg, float *__restrict__ s,
+                     float *__restrict__ F)
+{
+    using V = typename Vec<D>::T;
+    __shared__ V tile[256];
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    V yi;
+    if (i < n) yi = Yg[i]; else { yi.x = yi.y = 0.f; if constexpr (D == 3) { yi.z = yi.w = 0.f; } }
+    float si = 0.f, fx = 0.f, fy = 0.f, fz = 0.f;
+    for (int t0 = 0; t0 < n; t0 += 256) {
+        __syncthreads();
+        if (t0 + threadIdx.x < n) tile[threadIdx.x] = Yg[t0 + threadIdx.x];
+        __syncthreads();
+        const int tn = min(256, n - t0);
+        for (int u = 0; u < tn; u++) {
+            const V yj = tile[u];
+            const float dx = yi.x - yj.x, dy = yi.y - yj.y;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
+            float dz = 0.f;
+            if constexpr (D == 3) { dz = yi.z - yj.z; r2 = fmaf(dz, dz, r2); }
+            float w = fast_rcp(r2);
+            w = (t0 + u == i) ? 0.f : w * w;
+            si += w;
+            fx = fmaf(w, dx, fx);
+            fy = fmaf(w, dy, fy);
+            if constexpr (D == 3) fz = fmaf(w, dz, fz);
+        }
+    }
+    if (i < n) {
+        s[i] = si;
+        F[(size_t)i * D] = fx;
+        F[(size_t)i * D + 1] = fy;
+        if constexpr (D == 3) F[(size_t)i * D + 2] = fz;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Z = sum_i s_i + 2 sum_i (y_i - m) . F_i  (FP64 accumulation, fixed order).
+// This is the reference's identity Z = sum s + 2 sum |y|^2 s - 2 sum y.v
+// (src/tsne_h2pack.c:554-590) rewritten with F_i = y_i s_i - v_i, which keeps the two
+// large terms from cancelling.  Points are read through Yg (the coordinates the
+// product used) with the pending mean m removed.  Stage 1: per-block partials.
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+z_partial_kernel(int n, const typename Vec<D>::T *__restrict__ Yg, const float *__restrict__ s,
+                 const float *__restrict__ F, const double *__restrict__ scal,
+                 int per_block, double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    const int b0 = blockIdx.x * per_block, b1 = min(n, b0 + per_block);
+    double acc = 0.0;
+    const double m0 = scal[SLOT_YSHIFT], m1 = scal[SLOT_YSHIFT + 1];
+    const double m2 = (D == 3) ? scal[SLOT_YSHIFT + 2] : 0.0;
+    for (int i = b0 + threadIdx.x; i < b1; i += 256) {
+        const auto y = Yg[i];
+        double yf = ((double)y.x - m0) * (double)F[(size_t)i * D] +
+                    ((double)y.y - m1) * (double)F[(size_t)i * D + 1];
+        if constexpr (D == 3) yf += ((double)y.z - m2) * (double)F[(size_t)i * D + 2];
+        acc += (double)s[i] + 2.0 * yf;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// Stage 2: partials in order -> Z.  A non-positive or non-finite Z is replaced by 1
+// and flagged, as the reference returns 1.0 and continues (src/tsne_h2pack.c:604-611).
+__global__ void z_finalize_kernel(int nblocks, const double *__restrict__ partial,
+                                  double *__restrict__ scal)
+{
+    if (threadIdx.x != 0) return;
+    double z = 0.0;
+    for (int b = 0; b < nblocks; b++) z += partial[b];
+    if (!(z > 0.0) || !isfinite(z)) { scal[SLOT_ZBAD] = 1.0; z = 1.0; }
+    scal[SLOT_Z] = z;
+}
+
+// Copy the pending mean into the slot the Z pass subtracts (taken when the
+// repulsive product was formed).
+__global__ void shift_snapshot_kernel(double *__restrict__ scal)
+{
+    if (threadIdx.x < 3) scal[SLOT_YSHIFT + threadIdx.x] = scal[SLOT_MEAN + threadIdx.x];
+}
+
+// ---------------------------------------------------------------------------------
+// KL divergence, src/tsne_h2pack.c:1870-1895: per row sum of p*exag*log(p*exag/q) with
+// q = 1/((1+d^2) Z) over terms with p*exag > 1e-12 and q > 1e-12, in FP64; one
+// wavefront per row; rows summed in fixed blocks.
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+kl_rows_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
+               const double *__restrict__ pval64, const double *__restrict__ Y64,
+               const double *__restrict__ scal, double exag, double *__restrict__ row_kl)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const double Z = scal[SLOT_Z];
+    double yi[3];
+    for (int d = 0; d < D; d++) yi[d] = Y64[(size_t)i * D + d];
+    double acc = 0.0;
+    for (int e = row_ptr[i] + lane; e < row_ptr[i + 1]; e += 64) {
+        const int j = col_idx[e];
+        double r2 = 0.0;
+        for (int d = 0; d < D; d++) {
+            const double df = yi[d] - Y64[(size_t)j * D + d];
+            r2 += df * df;
+        }
+        const double p = pval64[e] * exag;
+        const double q = 1.0 / ((1.0 + r2) * Z);
+        if (p > 1e-12 && q > 1e-12) acc += p * log(p / q);
+    }
+    acc = wave_sum(acc);
+    if (lane == 0) row_kl[i] = acc;
+}
+
+__global__ void __launch_bounds__(256)
+sum_blocks_kernel(int n, const double *__restrict__ x, int per_block, double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    const int b0 = blockIdx.x * per_block, b1 = min(n, b0 + per_block);
+    double acc = 0.0;
+    for (int i = b0 + threadIdx.x; i < b1; i += 256) acc += x[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void sum_finalize_kernel(int nblocks, const double *__restrict__ partial,
+                                    double *__restrict__ dst)
+{
+    if (threadIdx.x != 0) return;
+    double z = 0.0;
+    for (int b = 0; b < nblocks; b++) z += partial[b];
+    *dst = z;
+}
+
+// ---------------------------------------------------------------------------------
+// FP64 all-pairs check path: s_i and F_i = sum_{j != i} K_ij (y_i - y_j) in FP64 from
+// the FP64 master coordinates (pending mean is a common shift and cancels).  Used to
+// measure the error of the product path, not inside the optimisation loop.
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+repulsive_direct_f64(int n, const double *__restrict__ Y64, const int *__restrict__ rows,
+                     int n_rows, double *__restrict__ s, double *__restrict__ F)
+{
+    __shared__ double tile[256][D];
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    const int i = (r < n_rows) ? (rows ? rows[r] : r) : -1;
+    double yi[3] = {0.0, 0.0, 0.0};
+    if (i >= 0) for (int d = 0; d < D; d++) yi[d] = Y64[(size_t)i * D + d];
+    double si = 0.0, f[3] = {0.0, 0.0, 0.0};
+    for (int t0 = 0; t0 < n; t0 += 256) {
+        __syncthreads();
+        if (t0 + threadIdx.x < n)
+            for (int d = 0; d < D; d++) tile[threadIdx.x][d] = Y64[(size_t)(t0 + threadIdx.x) * D + d];
+        __syncthreads();
+        const int tn = min(256, n - t0);
+        for (int u = 0; u < tn; u++) {
+            double df[3], r2 = 1.0;
+            for (int d = 0; d < D; d++) { df[d] = yi[d] - tile[u][d]; r2 += df[d] * df[d]; }
+            double w = 1.0 / r2;
+            w = (t0 + u == i) ? 0.0 : w * w;
+            si += w;
+            for (int d = 0; d < D; d++) f[d] += w * df[d];
+        }
+    }
+    if (i >= 0) {
+        s[r] = si;
+        for (int d = 0; d < D; d++) F[(size_t)r * D + d] = f[d];
+    }
+}
+
+// Z in FP64 from FP64 sums: Z = sum s + 2 sum (y - mean(y)) . F  (all rows).
+template <int D>
+__global__ void __launch_bounds__(256)
+z_partial_f64_kernel(int n, const double *__restrict__ Y64, const double *__restrict__ s,
+                     const double *__restrict__ F, const double *__restrict__ scal,
+                     int per_block, double *__restrict__ partial)
+{
+    __shared__ double red[256];
+    const int b0 = blockIdx.x * per_block, b1 = min(n, b0 + per_block);
+    double acc = 0.0;
+    for (int i = b0 + threadIdx.x; i < b1; i += 256) {
+        double yf = 0.0;
+        for (int d = 0; d < D; d++)
+            yf += (Y64[(size_t)i * D + d] - scal[SLOT_MEAN + d]) * F[(size_t)i * D + d];
+        acc += s[i] + 2.0 * yf;
+    }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) red[threadIdx.x] += red[threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+// ---------------------------------------------------------------------------------
+// Perplexity bisection, src/tsne_utils.c:952-1105, FP64, one wavefront per row.
+// Distances are squared and shifted by the row minimum; beta starts at 1 and is
+// doubled / halved until bracketed, then bisected, until |H - ln(perp)| < 1e-5 nats
+// or 200 steps; P = exp(-beta (d^2 - dmin)) / sum in the row's storage order.
+// ---------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256)
+perplexity_kernel(int n, const double *__restrict__ dist, const int *__restrict__ indptr,
+                  double perplexity, double *__restrict__ p_out)
+{
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (i >= n) return;
+    const int b = indptr[i], k = indptr[i + 1] - b;
+    if (k <= 0) return;
+    double dmin = INFINITY;
+    for (int j = lane; j < k; j += 64) { const double d = dist[b + j]; dmin = fmin(dmin, d * d); }
+    for (int off = 32; off > 0; off >>= 1) dmin = fmin(dmin, __shfl_xor(dmin, off, 64));
+    const double target = log(perplexity);
+    double beta = 1.0, lo = 0.0, hi = 0.0;
+    bool have_lo = false, have_hi = false;
+    double sp = 1.0;
+    for (int step = 0; step < 200; step++) {
+        double a = 0.0, c = 0.0;
+        for (int j = lane; j < k; j += 64) {
+            const double d0 = dist[b + j];
+            const double dd = d0 * d0 - dmin;
+            const double pj = exp(-beta * dd);
+            a += pj;
+            c += dd * pj;
+        }
+        sp = wave_sum(a);
+        const double sdp = wave_sum(c);
+        const double H = log(sp) + beta * sdp / sp;
+        const double diff = H - target;
+        if (fabs(diff) < 1e-5) break;
+        if (step == 199) break;
+        if (diff > 0.0) {
+            lo = beta; have_lo = true;
+            beta = have_hi ? 0.5 * (beta + hi) : 2.0 * beta;
+        } else {
+            hi = beta; have_hi = true;
+            beta = have_lo ? 0.5 * (beta + lo) : 0.5 * beta;
+        }
+    }
+    const double inv = 1.0 / sp;
+    for (int j = lane; j < k; j += 64) {
+        const double d0 = dist[b + j];
+        p_out[b + j] = exp(-beta * (d0 * d0 - dmin)) * inv;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Host-side launchers (declared in kernels.h).
+// ---------------------------------------------------------------------------------
+int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
+                       const void *Yg, const float *F, double *Y64, double *uY, double *gains,
+                       void *Yg_next, const double *scal, double exag, double mom, double lr,
+                       double *block_sums, int nblocks, hipStream_t st)
+{
+    const int rpb = (n + nblocks - 1) / nblocks;
+    if (D == 2)
+        hipLaunchKernelGGL(attr_update_kernel<2>, dim3(nblocks), dim3(256), 0, st, n, row_ptr,
+                           col_idx, pval, (const float2 *)Yg, F, Y64, uY, gains,
+                           (float2 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
+    else
+        hipLaunchKernelGGL(attr_update_kernel<3>, dim3(nblocks), dim3(256), 0, st, n, row_ptr,
+                           col_idx, pval, (const float4 *)Yg, F, Y64, uY, gains,
+                           (float4 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
+    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(256), 0, st, nblocks, D, n,
+                       block_sums, (double *)scal);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_repulsive_direct_f32(int D, int n, const void *Yg, float *s, float *F, hipStream_t st)
+{
+    const int g = (n + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(repulsive_direct_f32<2>, dim3(g), dim3(256), 0, st, n,
+                           (const float2 *)Yg, s, F);
+    else
+        hipLaunchKernelGGL(repulsive_direct_f32<3>, dim3(g), dim3(256), 0, st, n,
+                           (const float4 *)Yg, s, F);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_z(int D, int n, const void *Yg, const float *s, const float *F, double *scal,
+             double *partial, int nblocks, hipStream_t st)
+{
+    const int per = (n + nblocks - 1) / nblocks;
+    if (D == 2)
+        hipLaunchKernelGGL(z_partial_kernel<2>, dim3(nblocks), dim3(256), 0, st, n,
+                           (const float2 *)Yg, s, F, scal, per, partial);
+    else
+        hipLaunchKernelGGL(z_partial_kernel<3>, dim3(nblocks), dim3(256), 0, st, n,
+                           (const float4 *)Yg, s, F, scal, per, partial);
+    hipLaunchKernelGGL(z_finalize_kernel, dim3(1), dim3(64), 0, st, nblocks, partial, scal);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_shift_snapshot(double *scal, hipStream_t st)
+{
+    hipLaunchKernelGGL(shift_snapshot_kernel, dim3(1), dim3(64), 0, st, scal);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_kl(int D, int n, const int *row_ptr, const int *col_idx, const double *pval64,
+              const double *Y64, const double *scal, double exag, double *row_kl,
+              double *partial, int nblocks, double *dst, hipStream_t st)
+{
+    const int g = (n + 3) / 4;
+    if (D == 2)
+        hipLaunchKernelGGL(kl_rows_kernel<2>, dim3(g), dim3(256), 0, st, n, row_ptr, col_idx,
+                           pval64, Y64, scal, exag, row_kl);
+    else
+        hipLaunchKernelGGL(kl_rows_kernel<3>, dim3(g), dim3(256), 0, st, n, row_ptr, col_idx,
+                           pval64, Y64, scal, exag, row_kl);
+    const int per = (n + nblocks - 1) / nblocks;
+    hipLaunchKernelGGL(sum_blocks_kernel, dim3(nblocks), dim3(256), 0, st, n, row_kl, per, partial);
+    hipLaunchKernelGGL(sum_finalize_kernel, dim3(1), dim3(64), 0, st, nblocks, partial, dst);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_repulsive_direct_f64(int D, int n, const double *Y64, const int *rows, int n_rows,
+                                double *s, double *F, hipStream_t st)
+{
+    const int g = (n_rows + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(repulsive_direct_f64<2>, dim3(g), dim3(256), 0, st, n, Y64, rows,
+                           n_rows, s, F);
+    else
+        hipLaunchKernelGGL(repulsive_direct_f64<3>, dim3(g), dim3(256), 0, st, n, Y64, rows,
+                           n_rows, s, F);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_z_f64(int D, int n, const double *Y64, const double *s, const double *F,
+                 const double *scal, double *partial, int nblocks, double *dst, hipStream_t st)
+{
+    const int per = (n + nblocks - 1) / nblocks;
+    if (D == 2)
+        hipLaunchKernelGGL(z_partial_f64_kernel<2>, dim3(nblocks), dim3(256), 0, st, n, Y64, s,
+                           F, scal, per, partial);
+    else
+        hipLaunchKernelGGL(z_partial_f64_kernel<3>, dim3(nblocks), dim3(256), 0, st, n, Y64, s,
+                           F, scal, per, partial);
+    hipLaunchKernelGGL(sum_finalize_kernel, dim3(1), dim3(64), 0, st, nblocks, partial, dst);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_perplexity(int n, const double *dist, const int *indptr, double perplexity,
+                      double *p_out, hipStream_t st)
+{
+    const int g = (n + 3) / 4;
+    hipLaunchKernelGGL(perplexity_kernel, dim3(g), dim3(256), 0, st, n, dist, indptr, perplexity,
+                       p_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace tsne

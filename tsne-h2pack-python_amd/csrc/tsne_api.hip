@@ -1,0 +1,748 @@
+// tsne_api.hip -- the C ABI of libtsne_h2pack.so (include/tsne_h2pack.h) and the host
+// side of the optimisation loop.  Device work is queued on one HIP stream per
+// workspace; the loop synchronises only where the host needs a value (rebuild checks,
+// verbose prints, the end of a run).
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <ctime>
+#include <string>
+#include <vector>
+
+#include "common.h"
+#include "h2.h"
+#include "kernels.h"
+#include "../../include/tsne_h2pack.h"
+
+using namespace tsne;
+
+struct tsne_workspace {
+    int n = 0, pt_dim = 0, dim = 0;
+    tsne_params_t params{};
+    std::string profile_file;
+    double lr = 200.0;
+    sparse_matrix_t *P = nullptr;
+    hipStream_t st = nullptr;
+    int nnz = 0;
+    int *d_rowptr = nullptr, *d_col = nullptr;
+    float *d_val32 = nullptr;
+    double *d_val64 = nullptr;
+    double *d_Y64 = nullptr, *d_uY = nullptr, *d_gains = nullptr;
+    void *d_Yg[2] = {nullptr, nullptr};
+    int cur = 0;
+    float *d_s = nullptr, *d_F = nullptr;
+    double *d_scal = nullptr;
+    double *d_block_sums = nullptr;
+    int nblocks_upd = 0;
+    double *d_partial = nullptr;
+    int nblocks_red = 0;
+    double *d_rowkl = nullptr;
+    int rep_valid = 0;
+    double kl = 0.0;
+    int precision = TSNE_H2_PLAIN;
+    H2 *h2 = nullptr;
+    int h2_ok = 0;  // last rebuild produced a usable H2 structure
+    // timings of the last run (ms) and counters
+    double t_attr = 0, t_rebuild = 0, t_product = 0, t_other = 0;
+    int n_rebuild = 0;
+    int timing = 0;
+    double knn_time = 0, p_time = 0;
+    std::vector<double> prof;  // per-iteration phase times when profiling
+};
+
+namespace {
+
+const size_t kGatherBytes[4] = {0, 0, sizeof(float2), sizeof(float4)};
+
+template <typename T> int dalloc(T **p, size_t count)
+{
+    *p = nullptr;
+    if (count == 0) count = 1;
+    if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
+        fprintf(stderr, "tsne_h2pack: device allocation of %zu bytes failed\n", count * sizeof(T));
+        *p = nullptr;
+        return -1;
+    }
+    return 0;
+}
+
+template <typename T> void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+bool use_direct(const tsne_workspace *ws)
+{
+    // N < 2000 and method='exact' (rebuild frequency > 1e5) take the direct sum, as the
+    // reference does (src/tsne_h2pack.c:404-405, :1378).
+    return ws->n < 2000 || ws->params.h2_rebuild_freq > 100000;
+}
+
+int rebuild_iter(const tsne_workspace *ws, int it, int reuse)
+{
+    // src/tsne_h2pack.c:1765-1776
+    int freq = ws->params.h2_rebuild_freq > 0 ? ws->params.h2_rebuild_freq : 10;
+    int eff = (it < 20) ? 1 : freq;
+    if (it >= 20 && it < ws->params.early_exag_iter && reuse &&
+        ws->params.h2_rebuild_freq_early > eff)
+        eff = ws->params.h2_rebuild_freq_early;
+    return it > 0 && it % eff == 0;
+}
+
+// Seeded N(0, 1e-4) Box-Muller from glibc rand(), src/tsne_utils.c:1129-1148 (dim x n).
+void init_random(double *Y, int n, int dim, unsigned seed)
+{
+    srand(seed);
+    const int m = n * dim;
+    for (int i = 0; i < m; i += 2) {
+        double u1 = (double)rand() / RAND_MAX;
+        double u2 = (double)rand() / RAND_MAX;
+        if (u1 < 1e-10) u1 = 1e-10;
+        const double rad = std::sqrt(-2.0 * std::log(u1));
+        Y[i] = 1e-4 * rad * std::cos(2.0 * M_PI * u2);
+        if (i + 1 < m) Y[i + 1] = 1e-4 * rad * std::sin(2.0 * M_PI * u2);
+    }
+}
+
+void free_device(tsne_workspace *ws)
+{
+    if (ws->h2) { h2_destroy(ws->h2); ws->h2 = nullptr; }
+    dfree(ws->d_rowptr); dfree(ws->d_col); dfree(ws->d_val32); dfree(ws->d_val64);
+    dfree(ws->d_Y64); dfree(ws->d_uY); dfree(ws->d_gains);
+    dfree(ws->d_Yg[0]); dfree(ws->d_Yg[1]);
+    dfree(ws->d_s); dfree(ws->d_F); dfree(ws->d_scal);
+    dfree(ws->d_block_sums); dfree(ws->d_partial); dfree(ws->d_rowkl);
+    if (ws->st) { (void)hipStreamDestroy(ws->st); ws->st = nullptr; }
+}
+
+// Upload an embedding given dim x n in the caller's order: Y64 interleaved, gather copy.
+int upload_embedding(tsne_workspace *ws, const double *Ydn)
+{
+    const int n = ws->n, D = ws->dim;
+    std::vector<double> Yi((size_t)n * D);
+    std::vector<float> g((size_t)n * (D == 2 ? 2 : 4), 0.f);
+    const int gs = (D == 2) ? 2 : 4;
+    for (int i = 0; i < n; i++)
+        for (int d = 0; d < D; d++) {
+            Yi[(size_t)i * D + d] = Ydn[(size_t)d * n + i];
+            g[(size_t)i * gs + d] = (float)Ydn[(size_t)d * n + i];
+        }
+    TSNE_HIP_CHECK(hipMemcpyAsync(ws->d_Y64, Yi.data(), Yi.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, ws->st));
+    TSNE_HIP_CHECK(hipMemcpyAsync(ws->d_Yg[ws->cur], g.data(), g.size() * sizeof(float),
+                                  hipMemcpyHostToDevice, ws->st));
+    // the pending mean starts at zero: Y64 holds the embedding as given
+    std::vector<double> z(SLOT_COUNT, 0.0);
+    z[SLOT_Z] = 1.0;
+    TSNE_HIP_CHECK(hipMemcpyAsync(ws->d_scal, z.data(), z.size() * sizeof(double),
+                                  hipMemcpyHostToDevice, ws->st));
+    TSNE_HIP_CHECK(hipStreamSynchronize(ws->st));
+    ws->rep_valid = 0;
+    return 0;
+}
+
+// Repulsive sums s, F for the current gather copy; then the pending-mean snapshot and Z.
+int repulsive(tsne_workspace *ws, int allow_h2)
+{
+    const void *Yg = ws->d_Yg[ws->cur];
+    int rc;
+    if (allow_h2 && ws->h2 && ws->h2_ok)
+        rc = h2_product(ws->h2, Yg, ws->d_s, ws->d_F, ws->precision == TSNE_H2_MIXED);
+    else
+        rc = launch_repulsive_direct_f32(ws->dim, ws->n, Yg, ws->d_s, ws->d_F, ws->st);
+    if (rc) return rc;
+    if (launch_shift_snapshot(ws->d_scal, ws->st)) return -1;
+    return launch_z(ws->dim, ws->n, Yg, ws->d_s, ws->d_F, ws->d_scal, ws->d_partial,
+                    ws->nblocks_red, ws->st);
+}
+
+int rebuild(tsne_workspace *ws)
+{
+    if (!ws->h2) {
+        ws->h2 = h2_create(ws->n, ws->dim, ws->params.h2_leaf_size > 0 ? ws->params.h2_leaf_size : 100,
+                           ws->params.h2_tol > 0 ? ws->params.h2_tol : 1e-6, ws->st);
+        if (!ws->h2) return -1;
+    }
+    int bad = 0;
+    if (h2_build(ws->h2, ws->d_Yg[ws->cur], &bad) != 0) {
+        if (bad) fprintf(stderr, "tsne_h2pack: the embedding contains NaN or Inf\n");
+        ws->h2_ok = 0;
+        return -1;
+    }
+    ws->h2_ok = h2_usable(ws->h2);
+    return 0;
+}
+
+double read_scalar(tsne_workspace *ws, int slot)
+{
+    double v = 0.0;
+    if (hipMemcpyAsync(&v, ws->d_scal + slot, sizeof(double), hipMemcpyDeviceToHost, ws->st) !=
+            hipSuccess ||
+        hipStreamSynchronize(ws->st) != hipSuccess)
+        return NAN;
+    return v;
+}
+
+struct EvPair {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace
+
+extern "C" {
+
+tsne_params_t tsne_create_default_params(void)
+{
+    // src/tsne_h2pack.c:56-106
+    tsne_params_t p;
+    memset(&p, 0, sizeof(p));
+    p.n_iter = 1000;
+    p.perplexity = 30.0;
+    p.learning_rate = -1.0;
+    p.initial_momentum = 0.5;
+    p.momentum = 0.8;
+    p.momentum_switch_iter = 250;
+    p.early_exag = 12.0;
+    p.early_exag_iter = 250;
+    p.use_vptree = 2;
+    p.h2_tol = 1e-3;
+    p.h2_rebuild_freq = 10;
+    p.h2pack_print_level = 0;
+    p.init_method = 0;
+    p.init_Y = nullptr;
+    p.random_seed = (int)(unsigned)time(nullptr);
+    p.verbose = 1;
+    p.print_freq = 50;
+    p.enable_early_stopping = 0;
+    p.convergence_tol = 1e-5;
+    p.convergence_patience = 10;
+    p.min_iter = 250;
+    p.sklearn_compatible = 1;
+    p.enable_profiling = 0;
+    p.profiling_output_file = nullptr;
+    p.num_threads = 1;
+    p.h2_leaf_size = 100;
+    p.h2_rebuild_freq_early = 10;
+    return p;
+}
+
+sparse_matrix_t *sparse_matrix_create(int n_rows, int n_cols, int nnz)
+{
+    if (n_rows < 0 || nnz < 0) return nullptr;
+    sparse_matrix_t *m = (sparse_matrix_t *)malloc(sizeof(sparse_matrix_t));
+    if (!m) return nullptr;
+    m->n_rows = n_rows;
+    m->n_cols = n_cols;
+    m->nnz = nnz;
+    m->row_ptr = (int *)malloc(((size_t)n_rows + 1) * sizeof(int));
+    m->col_idx = (int *)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int));
+    m->values = (double *)malloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+    if (!m->row_ptr || !m->col_idx || !m->values) {
+        sparse_matrix_destroy(m);
+        return nullptr;
+    }
+    return m;
+}
+
+void sparse_matrix_destroy(sparse_matrix_t *m)
+{
+    if (!m) return;
+    free(m->row_ptr);
+    free(m->col_idx);
+    free(m->values);
+    free(m);
+}
+
+sparse_matrix_t *compute_P_from_knn_distances(const double *knn_distances, const int *knn_indices,
+                                              const int *knn_indptr, int n_samples, int nnz,
+                                              double perplexity, int num_threads)
+{
+    (void)num_threads;
+    bool args_ok = knn_distances && knn_indices && knn_indptr && n_samples > 0 && nnz >= 0 &&
+                   knn_indptr[0] == 0 && knn_indptr[n_samples] == nnz && perplexity > 0.0;
+    for (int i = 0; args_ok && i < n_samples; i++) args_ok = knn_indptr[i] <= knn_indptr[i + 1];
+    if (!args_ok) {
+        fprintf(stderr, "compute_P_from_knn_distances: invalid arguments\n");
+        return nullptr;
+    }
+    sparse_matrix_t *P = sparse_matrix_create(n_samples, n_samples, nnz);
+    if (!P) return nullptr;
+    memcpy(P->row_ptr, knn_indptr, ((size_t)n_samples + 1) * sizeof(int));
+    if (nnz > 0) memcpy(P->col_idx, knn_indices, (size_t)nnz * sizeof(int));
+    if (nnz == 0) return P;
+    hipStream_t st;
+    if (hipStreamCreate(&st) != hipSuccess) {
+        fprintf(stderr, "compute_P_from_knn_distances: no HIP device available\n");
+        sparse_matrix_destroy(P);
+        return nullptr;
+    }
+    double *d_dist = nullptr, *d_p = nullptr;
+    int *d_ptr = nullptr;
+    int ok = dalloc(&d_dist, nnz) == 0 && dalloc(&d_p, nnz) == 0 &&
+             dalloc(&d_ptr, (size_t)n_samples + 1) == 0;
+    ok = ok &&
+         hipMemcpyAsync(d_dist, knn_distances, (size_t)nnz * sizeof(double),
+                        hipMemcpyHostToDevice, st) == hipSuccess &&
+         hipMemcpyAsync(d_ptr, knn_indptr, ((size_t)n_samples + 1) * sizeof(int),
+                        hipMemcpyHostToDevice, st) == hipSuccess &&
+         launch_perplexity(n_samples, d_dist, d_ptr, perplexity, d_p, st) == 0 &&
+         hipMemcpyAsync(P->values, d_p, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost,
+                        st) == hipSuccess &&
+         hipStreamSynchronize(st) == hipSuccess;
+    dfree(d_dist); dfree(d_p); dfree(d_ptr);
+    (void)hipStreamDestroy(st);
+    if (!ok) {
+        fprintf(stderr, "compute_P_from_knn_distances: device failure\n");
+        sparse_matrix_destroy(P);
+        return nullptr;
+    }
+    return P;
+}
+
+tsne_workspace_t *tsne_init(const double *X, int n_point, int pt_dim, int low_dim,
+                            const tsne_params_t *params)
+{
+    (void)X; (void)n_point; (void)pt_dim; (void)low_dim; (void)params;
+    fprintf(stderr,
+            "tsne_init: the features path (built-in k-NN search) is not part of this library; "
+            "compute k-NN distances, call compute_P_from_knn_distances, symmetrise, and use "
+            "tsne_init_with_P\n");
+    return nullptr;
+}
+
+tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int low_dim,
+                                   const tsne_params_t *params, sparse_matrix_t *P)
+{
+    (void)X;
+    if (low_dim != 2 && low_dim != 3) {
+        fprintf(stderr, "Error: low_dim must be 2 or 3 (got %d)\n", low_dim);
+        return nullptr;
+    }
+    if (!params || !P || P->n_rows != n_point || P->n_cols != n_point || n_point < 2) {
+        fprintf(stderr, "Error: Invalid P matrix (expected %dx%d, got %dx%d)\n", n_point, n_point,
+                P ? P->n_rows : 0, P ? P->n_cols : 0);
+        return nullptr;
+    }
+    const int nnz = P->row_ptr[n_point];
+    bool csr_ok = nnz >= 0 && P->row_ptr[0] == 0;
+    for (int i = 0; csr_ok && i < n_point; i++) csr_ok = P->row_ptr[i] <= P->row_ptr[i + 1];
+    for (int e = 0; csr_ok && e < nnz; e++) csr_ok = P->col_idx[e] >= 0 && P->col_idx[e] < n_point;
+    if (!csr_ok) {
+        fprintf(stderr, "Error: Invalid P matrix (row pointers or column indices out of range)\n");
+        return nullptr;
+    }
+    tsne_workspace *ws = new tsne_workspace();
+    ws->n = n_point;
+    ws->pt_dim = pt_dim;
+    ws->dim = low_dim;
+    ws->params = *params;
+    if (params->profiling_output_file) ws->profile_file = params->profiling_output_file;
+    ws->params.profiling_output_file = nullptr;
+    ws->params.init_Y = nullptr;
+    ws->lr = params->learning_rate;
+    if (ws->lr < 0.0) ws->lr = std::fmax(200.0, n_point / 12.0);  // src/tsne_h2pack.c:1398-1405
+    ws->nnz = nnz;
+    const char *prec = getenv("TSNE_H2_PRECISION");
+    if (prec && strcmp(prec, "mixed") == 0) ws->precision = TSNE_H2_MIXED;
+
+    const int n = n_point, D = low_dim;
+    ws->nblocks_upd = std::max(1, std::min(2048, (n + 63) / 64));
+    ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
+    bool ok = hipStreamCreate(&ws->st) == hipSuccess;
+    if (!ok) {
+        fprintf(stderr, "tsne_init_with_P: no HIP device available\n");
+        delete ws;
+        return nullptr;
+    }
+    ok = dalloc(&ws->d_rowptr, (size_t)n + 1) == 0 && dalloc(&ws->d_col, nnz) == 0 &&
+         dalloc(&ws->d_val32, nnz) == 0 && dalloc(&ws->d_val64, nnz) == 0 &&
+         dalloc(&ws->d_Y64, (size_t)n * D) == 0 && dalloc(&ws->d_uY, (size_t)n * D) == 0 &&
+         dalloc(&ws->d_gains, (size_t)n * D) == 0 &&
+         dalloc((char **)&ws->d_Yg[0], (size_t)n * kGatherBytes[D]) == 0 &&
+         dalloc((char **)&ws->d_Yg[1], (size_t)n * kGatherBytes[D]) == 0 &&
+         dalloc(&ws->d_s, n) == 0 && dalloc(&ws->d_F, (size_t)n * D) == 0 &&
+         dalloc(&ws->d_scal, SLOT_COUNT) == 0 &&
+         dalloc(&ws->d_block_sums, (size_t)ws->nblocks_upd * 3) == 0 &&
+         dalloc(&ws->d_partial, ws->nblocks_red) == 0 && dalloc(&ws->d_rowkl, n) == 0;
+    if (ok) {
+        std::vector<float> v32(nnz > 0 ? nnz : 1);
+        for (int e = 0; e < nnz; e++) v32[e] = (float)P->values[e];
+        std::vector<double> ones((size_t)n * D, 1.0);
+        ok = hipMemcpyAsync(ws->d_rowptr, P->row_ptr, ((size_t)n + 1) * sizeof(int),
+                            hipMemcpyHostToDevice, ws->st) == hipSuccess &&
+             hipMemcpyAsync(ws->d_col, P->col_idx, (size_t)nnz * sizeof(int),
+                            hipMemcpyHostToDevice, ws->st) == hipSuccess &&
+             hipMemcpyAsync(ws->d_val64, P->values, (size_t)nnz * sizeof(double),
+                            hipMemcpyHostToDevice, ws->st) == hipSuccess &&
+             hipMemcpyAsync(ws->d_val32, v32.data(), (size_t)nnz * sizeof(float),
+                            hipMemcpyHostToDevice, ws->st) == hipSuccess &&
+             hipMemcpyAsync(ws->d_gains, ones.data(), ones.size() * sizeof(double),
+                            hipMemcpyHostToDevice, ws->st) == hipSuccess &&
+             hipMemsetAsync(ws->d_uY, 0, (size_t)n * D * sizeof(double), ws->st) == hipSuccess &&
+             hipMemsetAsync(ws->d_F, 0, (size_t)n * D * sizeof(float), ws->st) == hipSuccess &&
+             hipMemsetAsync(ws->d_Yg[1], 0, (size_t)n * kGatherBytes[D], ws->st) == hipSuccess &&
+             hipStreamSynchronize(ws->st) == hipSuccess;
+    }
+    if (ok) {
+        // initial embedding, src/tsne_utils.c:1111-1150
+        std::vector<double> Y0((size_t)n * D);
+        if (params->init_method == 1 && params->init_Y)
+            memcpy(Y0.data(), params->init_Y, Y0.size() * sizeof(double));
+        else
+            init_random(Y0.data(), n, D, (unsigned)params->random_seed);
+        ok = upload_embedding(ws, Y0.data()) == 0;
+    }
+    if (!ok) {
+        free_device(ws);
+        delete ws;
+        return nullptr;
+    }
+    ws->P = P;  // ownership taken once the workspace exists
+    if (!use_direct(ws)) {
+        if (rebuild(ws) != 0) {
+            fprintf(stderr, "Error: Failed to build initial H2 matrix\n");
+            tsne_destroy(ws);
+            return nullptr;
+        }
+    }
+    if (params->verbose)
+        printf("Initialized t-SNE on the GPU (n=%d, low_dim=%d, nnz=%d, %s repulsion)\n", n, D, nnz,
+               use_direct(ws) ? "direct" : "H2");
+    return ws;
+}
+
+int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
+{
+    if (!ws) return -1;
+    const tsne_params_t &pp = ws->params;
+    const int n_iter = pp.n_iter;
+    const int direct = use_direct(ws);
+    const int reuse = !direct;  // src/tsne_h2pack.c:1699 (rep_reuse when N >= 2000)
+    const int D = ws->dim;
+    if (it_begin < 0) it_begin = 0;
+    if (it_end > n_iter) it_end = n_iter;
+    const bool timing = ws->timing || pp.enable_profiling;
+    hipEvent_t ev[5];
+    if (timing)
+        for (auto &e : ev) (void)hipEventCreate(&e);
+    if (it_begin == 0) {
+        ws->t_attr = ws->t_rebuild = ws->t_product = ws->t_other = 0;
+        ws->n_rebuild = 0;
+        if (pp.enable_profiling) ws->prof.assign((size_t)n_iter * 4, 0.0);
+    }
+    if (pp.verbose && it_begin == 0)
+        printf("\nStarting t-SNE optimization (%d iterations, learning rate %.1f)\n", n_iter, ws->lr);
+    for (int it = it_begin; it < it_end; it++) {
+        const double mom = (it < pp.momentum_switch_iter) ? pp.initial_momentum : pp.momentum;
+        double exag;
+        if (pp.sklearn_compatible) {
+            exag = (it < pp.early_exag_iter) ? pp.early_exag : 1.0;
+        } else if (it < pp.early_exag_iter) {
+            exag = pp.early_exag;
+        } else if (it < pp.early_exag_iter + 100) {
+            exag = pp.early_exag - (pp.early_exag - 1.0) * (double)(it - pp.early_exag_iter) / 100.0;
+        } else {
+            exag = 1.0;
+        }
+        double lr = ws->lr;
+        if (!pp.sklearn_compatible && it >= pp.momentum_switch_iter)
+            lr = ws->lr * std::pow(0.995, it - pp.momentum_switch_iter);
+        if (timing) (void)hipEventRecord(ev[0], ws->st);
+        int rebuilt = (it == 0);
+        if (!direct && rebuild_iter(ws, it, reuse)) {
+            if (rebuild(ws) != 0) {
+                fprintf(stderr, "ERROR: Failed to rebuild H2 matrix at iteration %d\n", it);
+                return -1;
+            }
+            rebuilt = 1;
+            ws->n_rebuild++;
+        }
+        if (timing) (void)hipEventRecord(ev[1], ws->st);
+        if (!reuse || rebuilt || !ws->rep_valid) {
+            if (repulsive(ws, !direct) != 0) return -1;
+            ws->rep_valid = reuse;
+        }
+        if (timing) (void)hipEventRecord(ev[2], ws->st);
+        if ((pp.verbose && it % 50 == 0) || it == n_iter - 1) {
+            if (launch_kl(D, ws->n, ws->d_rowptr, ws->d_col, ws->d_val64, ws->d_Y64, ws->d_scal,
+                          exag, ws->d_rowkl, ws->d_partial, ws->nblocks_red, ws->d_scal + SLOT_KL,
+                          ws->st) != 0)
+                return -1;
+        }
+        if (timing) (void)hipEventRecord(ev[3], ws->st);
+        const int nxt = ws->cur ^ 1;
+        if (launch_attr_update(D, ws->n, ws->d_rowptr, ws->d_col, ws->d_val32, ws->d_Yg[ws->cur],
+                               ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains, ws->d_Yg[nxt], ws->d_scal,
+                               exag, mom, lr, ws->d_block_sums, ws->nblocks_upd, ws->st) != 0)
+            return -1;
+        ws->cur = nxt;
+        if (timing) {
+            (void)hipEventRecord(ev[4], ws->st);
+            (void)hipEventSynchronize(ev[4]);
+            float a = 0, b = 0, c = 0, d = 0;
+            (void)hipEventElapsedTime(&a, ev[0], ev[1]);
+            (void)hipEventElapsedTime(&b, ev[1], ev[2]);
+            (void)hipEventElapsedTime(&c, ev[2], ev[3]);
+            (void)hipEventElapsedTime(&d, ev[3], ev[4]);
+            ws->t_rebuild += a;
+            ws->t_product += b;
+            ws->t_other += c;
+            ws->t_attr += d;
+            if (pp.enable_profiling && (size_t)it * 4 + 3 < ws->prof.size()) {
+                ws->prof[(size_t)it * 4 + 0] = d * 1e-3;
+                ws->prof[(size_t)it * 4 + 1] = a * 1e-3;
+                ws->prof[(size_t)it * 4 + 2] = b * 1e-3;
+                ws->prof[(size_t)it * 4 + 3] = c * 1e-3;
+            }
+        }
+        if (pp.verbose && pp.print_freq > 0 && (it % pp.print_freq == 0 || it == n_iter - 1)) {
+            const double Z = read_scalar(ws, SLOT_Z);
+            printf("  Iteration %4d/%d: Z=%.3e\n", it + 1, n_iter, Z);
+            if (it % 50 == 0) printf("  [KL iter %4d] KL divergence: %.5f\n", it, read_scalar(ws, SLOT_KL));
+        }
+    }
+    if (timing)
+        for (auto &e : ev) (void)hipEventDestroy(e);
+    if (hipStreamSynchronize(ws->st) != hipSuccess) {
+        fprintf(stderr, "tsne_run: device failure\n");
+        return -1;
+    }
+    if (it_end == n_iter) ws->kl = read_scalar(ws, SLOT_KL);
+    if (read_scalar(ws, SLOT_BAD) != 0.0) {
+        fprintf(stderr, "ERROR: Y contains NaN or Inf\n");
+        return -1;
+    }
+    if (read_scalar(ws, SLOT_ZBAD) != 0.0)
+        fprintf(stderr, "ERROR: Z <= 0 or non-finite during the run (replaced by 1.0)\n");
+    if (pp.enable_profiling && it_end == n_iter && !ws->profile_file.empty()) {
+        FILE *f = fopen(ws->profile_file.c_str(), "w");
+        if (f) {
+            // the reference's CSV schema, src/tsne_h2pack.c:1553-1628
+            fprintf(f, "iteration,p_matvec,h2_rebuild,h2_matvec_s,h2_matvec_v,z_computation,"
+                       "gradient,other,total\n");
+            double tot[4] = {0, 0, 0, 0};
+            for (int it = 0; it < n_iter; it++) {
+                const double *r = &ws->prof[(size_t)it * 4];
+                fprintf(f, "%d,%.6e,%.6e,%.6e,0,0,0,%.6e,%.6e\n", it, r[0], r[1], r[2], r[3],
+                        r[0] + r[1] + r[2] + r[3]);
+                for (int k = 0; k < 4; k++) tot[k] += r[k];
+            }
+            fprintf(f, "# Total p_matvec time: %.6f\n", tot[0]);
+            fprintf(f, "# Total h2_rebuild time: %.6f\n", tot[1]);
+            fprintf(f, "# Total h2_matvec time: %.6f\n", tot[2]);
+            fprintf(f, "# Total other time: %.6f\n", tot[3]);
+            fprintf(f, "# Total k-NN time: %.6f\n", ws->knn_time);
+            fprintf(f, "# Total P construction time: %.6f\n", ws->p_time);
+            fclose(f);
+        }
+    }
+    return 0;
+}
+
+int tsne_run(tsne_workspace_t *ws)
+{
+    if (!ws) return -1;
+    const int rc = tsne_run_range(ws, 0, ws->params.n_iter);
+    if (rc == 0 && ws->params.verbose) printf("\nOptimization complete\n");
+    return rc;
+}
+
+void tsne_get_embedding(tsne_workspace_t *ws, double *Y_out)
+{
+    if (!ws || !Y_out) return;
+    const int n = ws->n, D = ws->dim;
+    std::vector<double> Yi((size_t)n * D);
+    double mean[SLOT_COUNT];
+    if (hipMemcpyAsync(Yi.data(), ws->d_Y64, Yi.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       ws->st) != hipSuccess ||
+        hipMemcpyAsync(mean, ws->d_scal, sizeof(mean), hipMemcpyDeviceToHost, ws->st) != hipSuccess ||
+        hipStreamSynchronize(ws->st) != hipSuccess) {
+        fprintf(stderr, "tsne_get_embedding: device failure\n");
+        return;
+    }
+    for (int d = 0; d < D; d++)
+        for (int i = 0; i < n; i++)
+            Y_out[(size_t)d * n + i] = Yi[(size_t)i * D + d] - mean[SLOT_MEAN + d];
+}
+
+void tsne_destroy(tsne_workspace_t *ws)
+{
+    if (!ws) return;
+    if (ws->st) (void)hipStreamSynchronize(ws->st);
+    free_device(ws);
+    sparse_matrix_destroy(ws->P);
+    delete ws;
+}
+
+int tsne_fit_transform(const double *X, int n_point, int pt_dim, int low_dim,
+                       const tsne_params_t *params, double *Y_out)
+{
+    tsne_workspace_t *ws = tsne_init(X, n_point, pt_dim, low_dim, params);
+    if (!ws) return -1;
+    int rc = tsne_run(ws);
+    if (rc == 0) tsne_get_embedding(ws, Y_out);
+    tsne_destroy(ws);
+    return rc;
+}
+
+int tsne_fit_transform_with_P(const double *X, int n_point, int pt_dim, int low_dim,
+                              const tsne_params_t *params, sparse_matrix_t *P, double *Y_out)
+{
+    tsne_workspace_t *ws = tsne_init_with_P(X, n_point, pt_dim, low_dim, params, P);
+    if (!ws) return -1;
+    int rc = tsne_run(ws);
+    if (rc == 0) tsne_get_embedding(ws, Y_out);
+    tsne_destroy(ws);
+    return rc;
+}
+
+void tsne_set_init_profiling_times(tsne_workspace_t *ws, double knn_time, double p_time)
+{
+    if (!ws || !ws->params.enable_profiling) return;
+    ws->knn_time = knn_time;
+    ws->p_time = p_time;
+}
+
+double compute_kl_divergence(tsne_workspace_t *ws) { return ws ? ws->kl : 0.0; }
+
+int tsne_set_h2_precision(tsne_workspace_t *ws, int mode)
+{
+    if (!ws || (mode != TSNE_H2_PLAIN && mode != TSNE_H2_MIXED)) return -1;
+    ws->precision = mode;
+    return 0;
+}
+
+int tsne_workspace_info(tsne_workspace_t *ws, int *n_point, int *low_dim, double *lr)
+{
+    if (!ws) return -1;
+    if (n_point) *n_point = ws->n;
+    if (low_dim) *low_dim = ws->dim;
+    if (lr) *lr = ws->lr;
+    return 0;
+}
+
+int tsne_set_embedding(tsne_workspace_t *ws, const double *Y)
+{
+    if (!ws || !Y) return -1;
+    return upload_embedding(ws, Y);
+}
+
+int tsne_compute_gradient(tsne_workspace_t *ws, double exag, int exact, double *grad_out,
+                          double *Z_out)
+{
+    if (!ws || !grad_out) return -1;
+    const int n = ws->n, D = ws->dim;
+    double Z = 0.0;
+    std::vector<double> rep((size_t)n * D);
+    if (exact) {
+        double *d_s = nullptr, *d_F = nullptr;
+        if (dalloc(&d_s, n) || dalloc(&d_F, (size_t)n * D)) { dfree(d_s); dfree(d_F); return -1; }
+        int rc = launch_repulsive_direct_f64(D, n, ws->d_Y64, nullptr, n, d_s, d_F, ws->st);
+        rc = rc || launch_z_f64(D, n, ws->d_Y64, d_s, d_F, ws->d_scal, ws->d_partial,
+                                ws->nblocks_red, ws->d_scal + SLOT_ZEXACT, ws->st);
+        rc = rc || hipMemcpyAsync(rep.data(), d_F, rep.size() * sizeof(double),
+                                  hipMemcpyDeviceToHost, ws->st) != hipSuccess;
+        rc = rc || hipStreamSynchronize(ws->st) != hipSuccess;
+        dfree(d_s); dfree(d_F);
+        if (rc) return -1;
+        Z = read_scalar(ws, SLOT_ZEXACT);
+        for (auto &v : rep) v *= 4.0 / Z;
+    } else {
+        if (!use_direct(ws) && rebuild(ws) != 0) return -1;
+        if (repulsive(ws, !use_direct(ws)) != 0) return -1;
+        std::vector<float> F((size_t)n * D);
+        if (hipMemcpyAsync(F.data(), ws->d_F, F.size() * sizeof(float), hipMemcpyDeviceToHost,
+                           ws->st) != hipSuccess)
+            return -1;
+        Z = read_scalar(ws, SLOT_Z);
+        for (size_t k = 0; k < F.size(); k++) rep[k] = 4.0 / Z * (double)F[k];
+        ws->rep_valid = 0;
+    }
+    // attractive part on the host in FP64 from the FP64 master coordinates
+    std::vector<double> Yi((size_t)n * D);
+    if (hipMemcpyAsync(Yi.data(), ws->d_Y64, Yi.size() * sizeof(double), hipMemcpyDeviceToHost,
+                       ws->st) != hipSuccess ||
+        hipStreamSynchronize(ws->st) != hipSuccess)
+        return -1;
+    const sparse_matrix_t *P = ws->P;
+    for (int i = 0; i < n; i++) {
+        double acc[3] = {0, 0, 0};
+        for (int e = P->row_ptr[i]; e < P->row_ptr[i + 1]; e++) {
+            const int j = P->col_idx[e];
+            double r2 = 1.0, df[3];
+            for (int d = 0; d < D; d++) {
+                df[d] = Yi[(size_t)i * D + d] - Yi[(size_t)j * D + d];
+                r2 += df[d] * df[d];
+            }
+            for (int d = 0; d < D; d++) acc[d] += P->values[e] / r2 * df[d];
+        }
+        for (int d = 0; d < D; d++)
+            grad_out[(size_t)d * n + i] = 4.0 * exag * acc[d] - rep[(size_t)i * D + d];
+    }
+    if (Z_out) *Z_out = Z;
+    return 0;
+}
+
+int tsne_exact_repulsive_rows(tsne_workspace_t *ws, const int *rows, int n_rows, double *out,
+                              double *Z_out)
+{
+    if (!ws || !rows || n_rows <= 0 || !out) return -1;
+    const int n = ws->n, D = ws->dim;
+    for (int r = 0; r < n_rows; r++)
+        if (rows[r] < 0 || rows[r] >= n) return -1;
+    double *d_s = nullptr, *d_F = nullptr, *d_s2 = nullptr, *d_F2 = nullptr;
+    int *d_rows = nullptr;
+    int rc = dalloc(&d_s, n) || dalloc(&d_F, (size_t)n * D) || dalloc(&d_s2, n_rows) ||
+             dalloc(&d_F2, (size_t)n_rows * D) || dalloc(&d_rows, n_rows);
+    rc = rc || hipMemcpyAsync(d_rows, rows, (size_t)n_rows * sizeof(int), hipMemcpyHostToDevice,
+                              ws->st) != hipSuccess;
+    rc = rc || launch_repulsive_direct_f64(D, n, ws->d_Y64, nullptr, n, d_s, d_F, ws->st);
+    rc = rc || launch_z_f64(D, n, ws->d_Y64, d_s, d_F, ws->d_scal, ws->d_partial, ws->nblocks_red,
+                            ws->d_scal + SLOT_ZEXACT, ws->st);
+    rc = rc || launch_repulsive_direct_f64(D, n, ws->d_Y64, d_rows, n_rows, d_s2, d_F2, ws->st);
+    std::vector<double> F((size_t)n_rows * D);
+    rc = rc || hipMemcpyAsync(F.data(), d_F2, F.size() * sizeof(double), hipMemcpyDeviceToHost,
+                              ws->st) != hipSuccess;
+    rc = rc || hipStreamSynchronize(ws->st) != hipSuccess;
+    dfree(d_s); dfree(d_F); dfree(d_s2); dfree(d_F2); dfree(d_rows);
+    if (rc) return -1;
+    const double Z = read_scalar(ws, SLOT_ZEXACT);
+    for (size_t k = 0; k < F.size(); k++) out[k] = 4.0 / Z * F[k];
+    if (Z_out) *Z_out = Z;
+    return 0;
+}
+
+int tsne_h2_stats(tsne_workspace_t *ws, tsne_h2_stats_t *stats)
+{
+    if (!ws || !stats || !ws->h2) return -1;
+    return h2_stats(ws->h2, stats);
+}
+
+int tsne_run_timings(tsne_workspace_t *ws, double *attr, double *reb, double *prod, double *other,
+                     int *n_rebuild)
+{
+    if (!ws) return -1;
+    if (attr) *attr = ws->t_attr;
+    if (reb) *reb = ws->t_rebuild;
+    if (prod) *prod = ws->t_product;
+    if (other) *other = ws->t_other;
+    if (n_rebuild) *n_rebuild = ws->n_rebuild;
+    return 0;
+}
+
+int tsne_synchronize(tsne_workspace_t *ws)
+{
+    if (!ws) return -1;
+    return hipStreamSynchronize(ws->st) == hipSuccess ? 0 : -1;
+}
+
+// Turns per-phase event timing on or off for later runs (also on with profiling).
+int tsne_enable_timing(tsne_workspace_t *ws, int on)
+{
+    if (!ws) return -1;
+    ws->timing = on;
+    return 0;
+}
+
+}  // extern "C"
