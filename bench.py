@@ -1,0 +1,263 @@
+"""Benchmark of the t-SNE gradient path on one MI355X.
+
+Workload (default): N=1,000,000 points, D=50 float32 from a seeded mixture of 100
+Gaussians (Dirichlet(1) weights, per-cluster std U[0.5, 2], centres N(0, 5^2)); exact
+k=90 graph (computed on the GPU with torch, harness only), perplexity 30, 2D,
+h2_tol 1e-2, rebuild every 3 (10 in early exaggeration), leaf 100.
+
+A step is one optimisation iteration of the library's loop (tsne_run_range) on
+device-resident state.  --warmup W iterations run untimed, then K iterations are
+timed (the schedule follows the absolute iteration index of a W+K iteration run).
+The metric is iterations/s of the mixed-precision H2 path; the plain FP32 path is
+timed the same way and reported beside it.  Prints ONE JSON line.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(ROOT, "tsne-h2pack-python_amd"), os.path.join(ROOT, "oracle"),
+           os.path.join(ROOT, "tests")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+HBM_PEAK_GBS = 8000.0
+FP32_VALU_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def make_inputs(n, dim_in, n_clusters, k, perplexity, seed=0):
+    """Seeded Gaussian mixture -> exact k-NN on the GPU (torch) -> symmetric P.
+    Returns (row_ptr, col_idx, values) int32/int32/float64 of the joint P."""
+    import torch
+    from scipy.sparse import csr_matrix
+
+    from tsne_h2pack._backend import get_backend
+    rng = np.random.default_rng(seed)
+    centers = rng.normal(scale=5.0, size=(n_clusters, dim_in))
+    w = rng.dirichlet(np.ones(n_clusters))
+    std = rng.uniform(0.5, 2.0, size=n_clusters)
+    labels = rng.choice(n_clusters, size=n, p=w)
+    X = (centers[labels] + rng.normal(size=(n, dim_in)) * std[labels, None]).astype(np.float32)
+    dev = torch.device("cuda:0")
+    Xt = torch.from_numpy(X).to(dev)
+    sq = (Xt * Xt).sum(1)
+    idx_all = torch.empty((n, k), dtype=torch.int32, device=dev)
+    dst_all = torch.empty((n, k), dtype=torch.float32, device=dev)
+    chunk = 4096
+    t0 = time.time()
+    for s in range(0, n, chunk):
+        e = min(n, s + chunk)
+        d2 = sq[s:e, None] - 2.0 * (Xt[s:e] @ Xt.T) + sq[None, :]
+        d2[torch.arange(e - s, device=dev), torch.arange(s, e, device=dev)] = float("inf")
+        vals, idx = torch.topk(d2, k, dim=1, largest=False, sorted=True)
+        idx_all[s:e] = idx.to(torch.int32)
+        dst_all[s:e] = vals.clamp_min(0).sqrt()
+        if (s // chunk) % 64 == 0:
+            log(f"knn rows {e}/{n} ({time.time() - t0:.1f}s)")
+    torch.cuda.synchronize()
+    idx = idx_all.cpu().numpy()
+    dist = dst_all.cpu().numpy()
+    del Xt, idx_all, dst_all
+    # CSR with sorted columns (reference _backend.py:651-653)
+    order = np.argsort(idx, axis=1, kind="stable")
+    idx = np.take_along_axis(idx, order, 1)
+    dist = np.take_along_axis(dist, order, 1).astype(np.float64)
+    indptr = np.arange(0, (n + 1) * k, k, dtype=np.int32)
+    be = get_backend()
+    t0 = time.time()
+    Pc = be.conditional_P(np.ascontiguousarray(dist.ravel()),
+                          np.ascontiguousarray(idx.ravel().astype(np.int32)), indptr, perplexity)
+    log(f"conditional P on GPU: {time.time() - t0:.2f}s")
+    # symmetrise on the GPU (the same sums as scipy's P + P.T, then / total)
+    t0 = time.time()
+    rows = torch.from_numpy(np.repeat(np.arange(n, dtype=np.int64), k)).to(dev)
+    cols = torch.from_numpy(Pc.indices.astype(np.int64)).to(dev)
+    vals = torch.from_numpy(Pc.data).to(dev)
+    S = torch.sparse_coo_tensor(torch.stack([torch.cat([rows, cols]), torch.cat([cols, rows])]),
+                                torch.cat([vals, vals]), (n, n)).coalesce()
+    v = S.values()
+    v = v / v.sum()
+    ij = S.indices()
+    counts = torch.bincount(ij[0], minlength=n)
+    row_ptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
+    row_ptr[1:] = torch.cumsum(counts, 0)
+    out = (row_ptr.cpu().numpy().astype(np.int32), ij[1].cpu().numpy().astype(np.int32),
+           v.cpu().numpy().astype(np.float64))
+    del S, rows, cols, vals, ij, v
+    torch.cuda.empty_cache()
+    log(f"symmetric P nnz={out[1].size} ({time.time() - t0:.2f}s)")
+    return out
+
+
+def make_workspace(be, csr, n, dim, precision, n_iter, leaf, tol, seed=0):
+    from scipy.sparse import csr_matrix
+    p = be.create_default_params()
+    p.verbose = 0
+    p.n_iter = n_iter
+    p.h2_tol = tol
+    p.h2_rebuild_freq = 3
+    p.h2_rebuild_freq_early = 10
+    p.h2_leaf_size = leaf
+    p.learning_rate = max(n / 12.0, 200.0)
+    p.random_seed = seed
+    P = csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n))
+    ws = be.init_workspace(P, dim, p, init_Y=None)
+    be.lib.tsne_set_h2_precision(ws, 1 if precision == "mixed" else 0)
+    return ws
+
+
+def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol):
+    lib = be.lib
+    ws = make_workspace(be, csr, n, dim, precision, warmup + steps, leaf, tol)
+    try:
+        if warmup:
+            assert lib.tsne_run_range(ws, 0, warmup) == 0
+        lib.tsne_synchronize(ws)
+        lib.tsne_enable_timing(ws, 1)
+        t0 = time.perf_counter()
+        rc = lib.tsne_run_range(ws, warmup, warmup + steps)
+        lib.tsne_synchronize(ws)
+        dt = time.perf_counter() - t0
+        assert rc == 0
+        # second pass with per-phase HIP events on the workspace stream would perturb the
+        # timed region; the phase timings below come from the same timed run
+        a, r, pr, o = (ctypes.c_double() for _ in range(4))
+        nr = ctypes.c_int()
+        lib.tsne_run_timings(ws, ctypes.byref(a), ctypes.byref(r), ctypes.byref(pr),
+                             ctypes.byref(o), ctypes.byref(nr))
+        st = None
+        from tsne_h2pack._backend import H2Stats
+        s = H2Stats()
+        if lib.tsne_h2_stats(ws, ctypes.byref(s)) == 0:
+            st = {f: getattr(s, f) for f, _ in H2Stats._fields_}
+        err = gradient_error(be, ws, n, dim)
+        kl = float(lib.compute_kl_divergence(ws))
+        return dict(seconds=dt, its=steps / dt, attr_ms=a.value, rebuild_ms=r.value,
+                    product_ms=pr.value, other_ms=o.value, n_rebuild=nr.value, h2=st,
+                    grad_err=err, kl=kl)
+    finally:
+        lib.tsne_destroy(ws)
+
+
+def gradient_error(be, ws, n, dim, n_rows=10000, seed=1):
+    """Relative error of the product-path gradient against the FP64 exact gradient at
+    exaggeration 1, ||g - g64|| / ||g64||, on all rows for n <= 100000 and on n_rows
+    sampled rows above.  Also the repulsive-only error and both Z."""
+    lib = be.lib
+    g1 = np.zeros((dim, n))
+    g0 = np.zeros((dim, n))
+    Z = ctypes.c_double()
+    if lib.tsne_compute_gradient(ws, 1.0, 0, g1, ctypes.byref(Z)) != 0:
+        return None
+    if lib.tsne_compute_gradient(ws, 0.0, 0, g0, ctypes.byref(Z)) != 0:
+        return None
+    rng = np.random.default_rng(seed)
+    rows = (np.arange(n, dtype=np.int32) if n <= 100000 else
+            np.sort(rng.choice(n, size=n_rows, replace=False)).astype(np.int32))
+    rep64 = np.zeros((rows.size, dim))
+    Ze = ctypes.c_double()
+    if lib.tsne_exact_repulsive_rows(ws, rows, rows.size, rep64, ctypes.byref(Ze)) != 0:
+        return None
+    rep = -g0[:, rows].T
+    attr = (g1 - g0)[:, rows].T
+    g = attr - rep
+    g64 = attr - rep64
+    return dict(rel_l2=float(np.linalg.norm(g - g64) / np.linalg.norm(g64)),
+                rep_rel_l2=float(np.linalg.norm(rep - rep64) / np.linalg.norm(rep64)),
+                max_row_rel=float((np.linalg.norm(g - g64, axis=1) /
+                                   np.maximum(np.linalg.norm(g64, axis=1), 1e-300)).max()),
+                Z=Z.value, Z64=Ze.value, rows=int(rows.size))
+
+
+def cpu_baseline(csr, n, dim, seconds_target=15.0):
+    """The oracle (FP64 restatement of the reference path with exact repulsion) timed on
+    the host cores on a bounded sample: the full attractive pass and the exact repulsive
+    force on a sample of rows, scaled to one iteration over all rows."""
+    import oracle
+    from scipy.sparse import csr_matrix
+    P = csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n))
+    rng = np.random.default_rng(0)
+    Y = rng.normal(scale=30.0, size=(dim, n))
+    t0 = time.perf_counter()
+    oracle.attractive(P, Y, 1.0)
+    t_attr = time.perf_counter() - t0
+    rows = np.sort(rng.choice(n, size=min(n, 64), replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    oracle.repulsive_rows(Y, 1.0, rows)
+    t_rep = (time.perf_counter() - t0) * n / rows.size
+    t_iter = t_attr + t_rep
+    return {"value": 1.0 / t_iter, "unit": "iters/s", "cores": oracle.num_threads(),
+            "kind": "port",
+            "sample": (f"oracle FP64 (exact O(N^2) repulsion; reference H2 not buildable): "
+                       f"full attractive pass {t_attr:.2f}s + exact repulsion on {rows.size} "
+                       f"sampled rows scaled to N={n}")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=990)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--dim", type=int, default=2)
+    ap.add_argument("--leaf", type=int, default=100)
+    ap.add_argument("--leaf-mixed", type=int, default=0)
+    ap.add_argument("--tol", type=float, default=1e-2)
+    ap.add_argument("--paths", default="mixed,plain")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the reference is single-process; see DESIGN.md on sharding)", file=sys.stderr)
+        sys.exit(2)
+    from tsne_h2pack._backend import get_backend
+    be = get_backend()
+    n, dim = a.n, a.dim
+    clusters = 100 if n >= 1_000_000 else 20
+    csr = make_inputs(n, 50, clusters, 90, 30.0)
+    res = {}
+    for path in a.paths.split(","):
+        leaf = a.leaf_mixed if (path == "mixed" and a.leaf_mixed) else a.leaf
+        log(f"timing {path} (leaf {leaf}) warmup={a.warmup} steps={a.steps}")
+        res[path] = time_path(be, csr, n, dim, path, a.warmup, a.steps, leaf, a.tol)
+        log(f"{path}: {res[path]['its']:.1f} it/s {json.dumps(res[path])}")
+    head = res.get("mixed") or res[next(iter(res))]
+    nnz = int(csr[1].size)
+    # dominant HBM kernel: attractive + fused update (every iteration)
+    attr_bytes = 8 * nnz + 4 * (n + 1) + n * (8 + 8 + 32 + 32 + 32 + 8) if dim == 2 else \
+        8 * nnz + 4 * (n + 1) + n * (16 + 12 + 48 + 48 + 48 + 16)
+    attr_s = head["attr_ms"] * 1e-3 / a.steps
+    roof = {"bound": "hbm", "achieved": attr_bytes / attr_s / 1e9, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": attr_bytes / attr_s / 1e9 / HBM_PEAK_GBS,
+            "traffic": None, "kernel": "attr_update_kernel (+mean_finalize)",
+            "algorithmic_bytes": attr_bytes}
+    cpu = None if a.no_cpu_baseline else cpu_baseline(csr, n, dim)
+    out = {
+        "metric": "iters/s at N=1M 2D, mixed vs plain FP32 GPU @<=1e-2 grad err; jobs/s + p99",
+        "value": head["its"], "unit": "iters/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": 1e3 / head["its"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded Gaussian mixture, exact k-NN on GPU)",
+        "config": {"workload": f"tsne-{dim}d N={n} D=50 k=90 perplexity=30 h2_tol={a.tol} "
+                               f"rebuild 3/10 leaf {a.leaf}", "n": n, "dim": dim, "nnz_P": nnz,
+                   "paths": a.paths},
+        "paths": res,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    if "plain" in res and "mixed" in res:
+        out["mixed_over_plain"] = res["mixed"]["its"] / res["plain"]["its"]
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -172,7 +172,8 @@ def test_random_init_is_seeded_box_muller(backend, small_problem):
         backend.lib.tsne_get_embedding(ws, Y)
     finally:
         backend.lib.tsne_destroy(ws)
-    np.testing.assert_array_equal(Y, oracle.init_random(P.shape[0], 2, 77))
+    # same rand() stream; the library's libm calls may differ from the oracle's by 1 ulp
+    np.testing.assert_allclose(Y, oracle.init_random(P.shape[0], 2, 77), rtol=4e-16, atol=0)
 
 
 def test_full_run_kl_within_one_percent(backend, small_problem):

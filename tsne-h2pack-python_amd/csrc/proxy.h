@@ -1,0 +1,26 @@
+// proxy.h -- proxy point sets of the H2 build (see proxy.cpp).
+#pragma once
+#include <vector>
+
+namespace tsne {
+
+// Ladder of node half-sides: h_k = kProxyBase * 2^(k / kProxyQ), k = 0..kProxyLevels-1.
+constexpr double kProxyBase = 1e-6;
+constexpr int kProxyQ = 2;
+constexpr int kProxyLevels = 2 * 34;  // up to ~1.7e4
+// Outer max-norm radius of the proxy candidates, in units of the half-side.
+constexpr double kProxyOuter = 48.0;
+
+struct ProxyTable {
+    int D = 0, pmax = 0, levels = 0;
+    std::vector<double> pts;  // [levels][pmax][D], relative to the node centre
+    std::vector<int> cnt;     // proxies per level
+};
+
+double proxy_half_side(int k);
+// Smallest ladder index whose half-side is >= h.
+int proxy_level_for(double h);
+// Cached table for (D, tolerance, eta, pmax); thread safe.
+const ProxyTable &proxy_table(int D, double tol, double eta, int pmax);
+
+}  // namespace tsne
