@@ -1,0 +1,140 @@
+"""H2 repulsive product on the GPU against the FP64 oracle (all rows), through the C ABI.
+
+Bar (north-star h2_tol): the gradient error ||g - g64|| / ||g64|| <= h2_tol (1e-2 in 2D)
+at exaggeration 1 and 12, on embeddings of several scales (cluster spread from 0.05 to
+200 units, which puts the node boxes on both sides of the kernel's unit length).  The
+repulsive force alone must be within 2 * h2_tol.  Determinism: two builds of the same
+embedding give bit-identical forces.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from datasets import gaussian_mixture, joint_P_oracle, knn_graph
+from tsne_h2pack._backend import H2Stats
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def problem():
+    X, labels = gaussian_mixture(6000, dim=20, n_clusters=12, seed=3)
+    P = joint_P_oracle(knn_graph(X, 45), 15.0)
+    return X, labels, P
+
+
+def _embedding(n, dim, scale, seed=0):
+    rng = np.random.default_rng(seed)
+    cent = rng.normal(scale=scale, size=(30, dim))
+    lab = rng.integers(0, 30, n)
+    Y = cent[lab] + rng.normal(size=(n, dim)) * rng.uniform(0.5, 2.0, 30)[lab, None] * scale / 20
+    return Y.T.copy()
+
+
+def _ws(backend, P, Y0, dim, tol, leaf=100):
+    p = backend.create_default_params()
+    p.verbose = 0
+    p.h2_tol = tol
+    p.h2_rebuild_freq = 3
+    p.h2_leaf_size = leaf
+    return backend.init_workspace(P, dim, p, init_Y=Y0.T.copy())
+
+
+def _grad(backend, ws, dim, n, exag, exact):
+    g = np.zeros((dim, n))
+    Z = ctypes.c_double()
+    assert backend.lib.tsne_compute_gradient(ws, exag, exact, g, ctypes.byref(Z)) == 0
+    return g, Z.value
+
+
+@pytest.mark.parametrize("scale", [0.05, 2.0, 8.0, 40.0, 200.0])
+def test_h2_gradient_within_tol_2d(backend, problem, scale):
+    _say(f"h2 2d scale={scale}")
+    _, _, P = problem
+    n = P.shape[0]
+    Y0 = _embedding(n, 2, scale)
+    ws = _ws(backend, P, Y0, 2, 1e-2)
+    try:
+        st = H2Stats()
+        for exag in (1.0, 12.0):
+            g, Z = _grad(backend, ws, 2, n, exag, 0)
+            rep64, Z64, _, _ = oracle.repulsive_exact(Y0)
+            g64 = oracle.attractive(P, Y0, exag) - rep64
+            err = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+            g0, _ = _grad(backend, ws, 2, n, 0.0, 0)
+            rep_err = np.linalg.norm(-g0 - rep64) / np.linalg.norm(rep64)
+            assert backend.lib.tsne_h2_stats(ws, ctypes.byref(st)) == 0
+            _say(f"  exag {exag}: grad err {err:.2e} rep err {rep_err:.2e} Z {Z:.6e}/{Z64:.6e} "
+                 f"levels {st.levels} nodes {st.n_nodes} rank max {st.rank_max} "
+                 f"mean {st.rank_mean:.1f} proxies {st.proxy_max}")
+            assert err <= 1e-2
+            assert rep_err <= 2e-2
+            assert abs(Z / Z64 - 1) <= 1e-2
+        assert st.levels >= 3 and st.n_adm_pairs > 0
+    finally:
+        backend.lib.tsne_destroy(ws)
+
+
+def test_h2_product_is_deterministic(backend, problem):
+    _, _, P = problem
+    n = P.shape[0]
+    Y0 = _embedding(n, 2, 20.0, seed=5)
+    ws = _ws(backend, P, Y0, 2, 1e-2)
+    try:
+        g1, Z1 = _grad(backend, ws, 2, n, 1.0, 0)
+        g2, Z2 = _grad(backend, ws, 2, n, 1.0, 0)
+    finally:
+        backend.lib.tsne_destroy(ws)
+    assert Z1 == Z2
+    assert np.array_equal(g1, g2)
+
+
+@pytest.mark.parametrize("leaf", [16, 64, 128])
+def test_h2_leaf_sizes(backend, problem, leaf):
+    """h2_leaf_size changes speed and memory only (reference manifold.py:150-152)."""
+    _, _, P = problem
+    n = P.shape[0]
+    Y0 = _embedding(n, 2, 30.0, seed=7)
+    ws = _ws(backend, P, Y0, 2, 1e-2, leaf=leaf)
+    try:
+        g, _ = _grad(backend, ws, 2, n, 1.0, 0)
+    finally:
+        backend.lib.tsne_destroy(ws)
+    rep64, _, _, _ = oracle.repulsive_exact(Y0)
+    g64 = oracle.attractive(P, Y0, 1.0) - rep64
+    assert np.linalg.norm(g - g64) / np.linalg.norm(g64) <= 1e-2
+
+
+def test_h2_run_kl_within_one_percent(backend, problem):
+    """Full 1000-iteration run (H2 + force reuse, rebuild 3/10) ends within 1% of the
+    oracle's KL (exact repulsion, same schedule) from the same init."""
+    _say("h2 full run N=6000")
+    _, labels, P = problem
+    n = P.shape[0]
+    Y0 = oracle.init_random(n, 2, 11)
+    p = backend.create_default_params()
+    p.verbose = 0
+    p.h2_tol = 1e-2
+    p.h2_rebuild_freq = 3
+    p.h2_rebuild_freq_early = 10
+    p.learning_rate = max(n / 12.0, 200.0)
+    ws = backend.init_workspace(P, 2, p, init_Y=Y0.T.copy())
+    try:
+        assert backend.lib.tsne_run(ws) == 0
+        kl = backend.lib.compute_kl_divergence(ws)
+        Y = np.zeros((2, n))
+        backend.lib.tsne_get_embedding(ws, Y)
+    finally:
+        backend.lib.tsne_destroy(ws)
+    _, kl_o = oracle.run(P, Y0, n_iter=1000)
+    _say(f"  KL gpu {kl:.5f} oracle {kl_o:.5f}")
+    assert abs(kl - kl_o) / kl_o < 0.01
+    assert np.isfinite(Y).all()

@@ -1,19 +1,354 @@
-// h2.hip -- placeholder H2 module: reports no usable structure, so the product falls
-// back to the direct sum.
-#include "h2.h"
+// h2.hip -- host side of the H2 representation: device allocations, the rebuild
+// sequence (root box, Morton sort, level-by-level tree and interaction lists, sorted
+// pair lists, per-level ID + upward pass) and the product (coupling, downward sweep,
+// leaves).  See h2_internal.h for the layout.
+#include <hipcub/hipcub.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
 #include <new>
+#include <vector>
+
+#include "h2.h"
+#include "h2_internal.h"
+#include "proxy.h"
 
 namespace tsne {
 
 struct H2 {
-    int n, dim;
+    int n = 0, D = 2, leaf = 100;
+    double tol = 1e-2, ptol = 3e-3;
+    hipStream_t st = nullptr;
+    H2Dev d{};
+    // buffers not in H2Dev
+    uint64_t *keys_alt = nullptr;
+    int *perm_alt = nullptr;
+    void *cub_tmp = nullptr;
+    size_t cub_bytes = 0;
+    double *part = nullptr;
+    int nblk = 0;
+    double *d_proxy = nullptr;
+    int *d_proxy_cnt = nullptr;
+    int *pin = nullptr;  // pinned host scratch
+    std::vector<int> lvl;  // level l nodes are [lvl[l], lvl[l+1])
+    int n_nodes = 0, n_leaves = 0, n_adm = 0, n_x = 0, n_w = 0, n_near = 0;
+    int usable = 0;
+    int hflags[FLAG_COUNT] = {0};
+    // growth factors (grown on overflow)
+    double node_f = 8.0, pair_f = 1.0, pool_f = 1.0;
 };
 
-H2 *h2_create(int n, int dim, int, double, hipStream_t) { return new (std::nothrow) H2{n, dim}; }
-void h2_destroy(H2 *h) { delete h; }
-int h2_build(H2 *, const void *, int *nonfinite) { *nonfinite = 0; return 0; }
-int h2_product(H2 *, const void *, float *, float *, int) { return -1; }
-int h2_usable(const H2 *) { return 0; }
-int h2_stats(const H2 *, tsne_h2_stats_t *) { return -1; }
+namespace {
+
+template <typename T> bool dmalloc(T **p, size_t n)
+{
+    if (n == 0) n = 1;
+    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+        *p = nullptr;
+        return false;
+    }
+    return true;
+}
+
+template <typename T> void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+int rmax_of(int D) { return D == 2 ? H2Cfg<2>::RMAX : H2Cfg<3>::RMAX; }
+int nch_of(int D) { return D == 2 ? 4 : 8; }
+int nearcap_of(int D) { return D == 2 ? H2Cfg<2>::NEAR_CAP : H2Cfg<3>::NEAR_CAP; }
+int pmax_of(int D) { return D == 2 ? H2Cfg<2>::PMAX : H2Cfg<3>::PMAX; }
+
+void free_sized(H2 *h)
+{
+    H2Dev &d = h->d;
+    dfree(d.nb); dfree(d.ne); dfree(d.level); dfree(d.parent); dfree(d.child0); dfree(d.nchild);
+    dfree(d.cent); dfree(d.icoord); dfree(d.bounds); dfree(d.childcnt); dfree(d.childoff);
+    dfree(d.near); dfree(d.near_cnt); dfree(d.leaves);
+    dfree(d.adm); dfree(d.xl); dfree(d.wl); dfree(d.nl); dfree(d.pend[0]); dfree(d.pend[1]);
+    dfree(d.sort_tmp);
+    dfree(d.adm_beg); dfree(d.adm_end); dfree(d.x_beg); dfree(d.x_end);
+    dfree(d.w_beg); dfree(d.w_end); dfree(d.nl_beg); dfree(d.nl_end);
+    dfree(d.rank); dfree(d.uoff); dfree(d.skel); dfree(d.skelf); dfree(d.upool);
+    dfree(d.wc); dfree(d.zc);
+    dfree(h->cub_tmp);
+    h->cub_bytes = 0;
+}
+
+// (Re)allocate everything whose size depends on the node / pair / pool capacities.
+bool alloc_sized(H2 *h)
+{
+    free_sized(h);
+    H2Dev &d = h->d;
+    const int D = h->D, RMAX = rmax_of(D), NCH = nch_of(D);
+    const long long ncap = std::max(1024LL, (long long)(h->node_f * h->n / h->leaf) + 64);
+    if (ncap > (1LL << 30)) return false;
+    d.node_cap = (int)ncap;
+    const size_t nc = (size_t)ncap;
+    const double pf = h->pair_f;
+    d.adm_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 32 : 200));
+    d.x_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 8 : 32));
+    d.w_cap = d.x_cap;
+    d.nl_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 16 : 64));
+    d.pend_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 8 : 32));
+    d.pool_cap = (long long)(h->pool_f * ((double)h->n * (D == 2 ? 20 : 48) + (double)nc * (D == 2 ? 64 : 256)));
+    const int maxpair = std::max({d.adm_cap, d.x_cap, d.w_cap, d.nl_cap});
+    bool ok = dmalloc(&d.nb, nc) && dmalloc(&d.ne, nc) && dmalloc(&d.level, nc) &&
+              dmalloc(&d.parent, nc) && dmalloc(&d.child0, nc) && dmalloc(&d.nchild, nc) &&
+              dmalloc(&d.cent, nc) && dmalloc(&d.icoord, nc * 3) &&
+              dmalloc(&d.bounds, nc * (NCH + 1)) && dmalloc(&d.childcnt, nc) &&
+              dmalloc(&d.childoff, nc) && dmalloc(&d.near, nc * nearcap_of(D)) &&
+              dmalloc(&d.near_cnt, nc) && dmalloc(&d.leaves, nc) &&
+              dmalloc(&d.adm, (size_t)d.adm_cap) && dmalloc(&d.xl, (size_t)d.x_cap) &&
+              dmalloc(&d.wl, (size_t)d.w_cap) && dmalloc(&d.nl, (size_t)d.nl_cap) &&
+              dmalloc(&d.pend[0], (size_t)d.pend_cap) && dmalloc(&d.pend[1], (size_t)d.pend_cap) &&
+              dmalloc(&d.sort_tmp, (size_t)maxpair) &&
+              dmalloc(&d.adm_beg, nc) && dmalloc(&d.adm_end, nc) && dmalloc(&d.x_beg, nc) &&
+              dmalloc(&d.x_end, nc) && dmalloc(&d.w_beg, nc) && dmalloc(&d.w_end, nc) &&
+              dmalloc(&d.nl_beg, nc) && dmalloc(&d.nl_end, nc) && dmalloc(&d.rank, nc) &&
+              dmalloc(&d.uoff, nc) && dmalloc(&d.skel, nc * RMAX * D) &&
+              dmalloc(&d.skelf, nc * RMAX) && dmalloc(&d.upool, (size_t)d.pool_cap) &&
+              dmalloc(&d.wc, nc * RMAX) && dmalloc(&d.zc, nc * RMAX);
+    if (!ok) return false;
+    // temporary storage for the library sorts / scans (largest of the three uses)
+    size_t b1 = 0, b2 = 0, b3 = 0;
+    hipcub::DoubleBuffer<uint64_t> kb(d.keys, h->keys_alt);
+    hipcub::DoubleBuffer<int> vb(d.perm, h->perm_alt);
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b1, kb, vb, h->n, 0, 64, h->st);
+    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b2, d.childcnt, d.childoff, (int)nc, h->st);
+    hipcub::DoubleBuffer<uint64_t> pb(d.adm, d.sort_tmp);
+    (void)hipcub::DeviceRadixSort::SortKeys(nullptr, b3, pb, maxpair, 0, 64, h->st);
+    h->cub_bytes = std::max({b1, b2, b3});
+    return hipMalloc(&h->cub_tmp, h->cub_bytes) == hipSuccess;
+}
+
+int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end)
+{
+    H2Dev &d = h->d;
+    if (hipMemsetAsync(beg, 0, sizeof(int) * d.node_cap, h->st) != hipSuccess ||
+        hipMemsetAsync(end, 0, sizeof(int) * d.node_cap, h->st) != hipSuccess)
+        return -1;
+    if (n <= 0) return 0;
+    int bits = 32;
+    while ((1LL << (bits - 32)) < d.node_cap) bits++;
+    hipcub::DoubleBuffer<uint64_t> db(buf, d.sort_tmp);
+    size_t bytes = h->cub_bytes;
+    if (hipcub::DeviceRadixSort::SortKeys(h->cub_tmp, bytes, db, n, 0, bits, h->st) != hipSuccess)
+        return -1;
+    if (db.Current() != buf) {  // keep the sorted list in the list's own buffer
+        std::swap(buf, d.sort_tmp);
+    }
+    return launch_pair_offsets(buf, n, beg, end, h->st);
+}
+
+// One attempt at a rebuild.  Returns 0 (built), 1 (capacity exceeded: grown, retry),
+// -1 (error), -2 (non-finite input).
+int build_once(H2 *h, const void *Yg)
+{
+    H2Dev &d = h->d;
+    hipStream_t st = h->st;
+    const int D = h->D;
+    if (hipMemsetAsync(d.cnt, 0, sizeof(int) * CNT_COUNT, st) != hipSuccess ||
+        hipMemsetAsync(d.flags, 0, sizeof(int) * FLAG_COUNT, st) != hipSuccess)
+        return -1;
+    if (launch_root_box(D, Yg, h->n, d, h->part, h->nblk, st)) return -1;
+    if (launch_morton(D, Yg, d, st)) return -1;
+    {
+        hipcub::DoubleBuffer<uint64_t> kb(d.keys, h->keys_alt);
+        hipcub::DoubleBuffer<int> vb(d.perm, h->perm_alt);
+        size_t bytes = h->cub_bytes;
+        const int endbit = D * (D == 2 ? H2Cfg<2>::B : H2Cfg<3>::B);
+        if (hipcub::DeviceRadixSort::SortPairs(h->cub_tmp, bytes, kb, vb, h->n, 0, endbit, st) !=
+            hipSuccess)
+            return -1;
+        if (kb.Current() != d.keys) std::swap(d.keys, h->keys_alt);
+        if (vb.Current() != d.perm) std::swap(d.perm, h->perm_alt);
+    }
+    if (launch_gather_sorted(D, Yg, d, st)) return -1;
+    if (launch_init_root(d, st)) return -1;
+    h->lvl.assign({0, 1});
+    int lb = 0, le = 1, l = 0, pend_in = 0, n_pend_in = 0;
+    bool first = true;
+    for (;;) {
+        if (launch_split_count(D, d, lb, le, l, st)) return -1;
+        size_t bytes = h->cub_bytes;
+        if (hipcub::DeviceScan::ExclusiveSum(h->cub_tmp, bytes, d.childcnt, d.childoff, le - lb, st) !=
+            hipSuccess)
+            return -1;
+        if (hipMemcpyAsync(&h->pin[0], d.childoff + (le - lb - 1), sizeof(int), hipMemcpyDeviceToHost, st) ||
+            hipMemcpyAsync(&h->pin[1], d.childcnt + (le - lb - 1), sizeof(int), hipMemcpyDeviceToHost, st) ||
+            hipMemcpyAsync(&h->pin[2], d.cnt + CNT_PEND0 + pend_in, sizeof(int), hipMemcpyDeviceToHost, st) ||
+            hipMemcpyAsync(&h->pin[3], d.flags + FLAG_NONFINITE, sizeof(int), hipMemcpyDeviceToHost, st) ||
+            hipStreamSynchronize(st) != hipSuccess)
+            return -1;
+        if (first && h->pin[3]) return -2;
+        first = false;
+        const int T = h->pin[0] + h->pin[1];
+        n_pend_in = std::min(h->pin[2], d.pend_cap);
+        if (T == 0) break;
+        if ((long long)le + T > d.node_cap) {
+            h->node_f *= 2.0;
+            return 1;
+        }
+        if (launch_split_write(D, d, lb, le, l, st)) return -1;
+        const int pend_out = pend_in ^ 1;
+        if (hipMemsetAsync(d.cnt + CNT_PEND0 + pend_out, 0, sizeof(int), st) != hipSuccess) return -1;
+        if (launch_lists(D, d, le, le + T, l + 1, pend_out, st)) return -1;
+        if (launch_pending(D, d, n_pend_in, pend_in, pend_out, st)) return -1;
+        lb = le;
+        le += T;
+        l++;
+        h->lvl.push_back(le);
+        pend_in = pend_out;
+    }
+    h->n_nodes = le;
+    if (hipMemcpyAsync(h->pin, d.cnt, sizeof(int) * CNT_COUNT, hipMemcpyDeviceToHost, st) ||
+        hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return -1;
+    if (h->hflags[FLAG_PAIR_OVF] || h->hflags[FLAG_NEAR_OVF]) {
+        if (h->hflags[FLAG_NEAR_OVF]) return -1;  // geometry the fixed near list cannot hold
+        h->pair_f *= 2.0;
+        return 1;
+    }
+    h->n_adm = h->pin[CNT_ADM];
+    h->n_x = h->pin[CNT_X];
+    h->n_w = h->pin[CNT_W];
+    h->n_near = h->pin[CNT_NEAR];
+    h->n_leaves = h->pin[CNT_LEAF];
+    if (sort_pairs_list(h, d.adm, h->n_adm, d.adm_beg, d.adm_end) ||
+        sort_pairs_list(h, d.xl, h->n_x, d.x_beg, d.x_end) ||
+        sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end) ||
+        sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end))
+        return -1;
+    // ID + upward, finest level first
+    const int nlev = (int)h->lvl.size() - 1;
+    for (int lv = nlev - 1; lv >= 1; lv--)
+        if (launch_id_up(D, d, h->lvl[lv], h->lvl[lv + 1], lv, h->tol, st)) return -1;
+    if (hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
+        hipStreamSynchronize(st) != hipSuccess)
+        return -1;
+    if (h->hflags[FLAG_POOL_OVF]) {
+        h->pool_f *= 2.0;
+        return 1;
+    }
+    return 0;
+}
+
+}  // namespace
+
+H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
+{
+    H2 *h = new (std::nothrow) H2();
+    if (!h) return nullptr;
+    h->n = n;
+    h->D = dim;
+    h->leaf = std::max(1, std::min(leaf_size, kLeafCap));
+    h->tol = tol;
+    // proxy selection a little tighter than the ID tolerance, clamped as the reference
+    // clamps its proxy tolerance to [1e-6, 1e-2] (src/tsne_h2pack.c:966-977)
+    h->ptol = std::min(1e-2, std::max(1e-6, 0.3 * tol));
+    h->st = st;
+    H2Dev &d = h->d;
+    d.n = n;
+    d.D = dim;
+    d.leaf = h->leaf;
+    h->nblk = std::max(1, std::min(512, (n + 4095) / 4096));
+    const size_t vb = (dim == 2) ? sizeof(float2) : sizeof(float4);
+    bool ok = dmalloc(&d.keys, n) && dmalloc(&h->keys_alt, n) && dmalloc(&d.perm, n) &&
+              dmalloc(&h->perm_alt, n) && dmalloc((char **)&d.Ys, (size_t)n * vb) &&
+              dmalloc(&d.root, 1) && dmalloc(&d.flags, FLAG_COUNT) && dmalloc(&d.cnt, CNT_COUNT) &&
+              dmalloc(&h->part, (size_t)h->nblk * 7) &&
+              hipHostMalloc((void **)&h->pin, sizeof(int) * 64, hipHostMallocDefault) == hipSuccess;
+    if (ok) {
+        const ProxyTable &pt = proxy_table(dim, h->ptol, 1.0, pmax_of(dim));
+        ok = dmalloc(&h->d_proxy, pt.pts.size()) && dmalloc(&h->d_proxy_cnt, pt.cnt.size()) &&
+             hipMemcpyAsync(h->d_proxy, pt.pts.data(), pt.pts.size() * sizeof(double),
+                            hipMemcpyHostToDevice, st) == hipSuccess &&
+             hipMemcpyAsync(h->d_proxy_cnt, pt.cnt.data(), pt.cnt.size() * sizeof(int),
+                            hipMemcpyHostToDevice, st) == hipSuccess &&
+             hipStreamSynchronize(st) == hipSuccess;
+        d.proxy_pts = h->d_proxy;
+        d.proxy_cnt = h->d_proxy_cnt;
+    }
+    ok = ok && alloc_sized(h);
+    if (!ok) {
+        fprintf(stderr, "tsne_h2pack: H2 allocation failed (n=%d)\n", n);
+        h2_destroy(h);
+        return nullptr;
+    }
+    return h;
+}
+
+void h2_destroy(H2 *h)
+{
+    if (!h) return;
+    if (h->st) (void)hipStreamSynchronize(h->st);
+    free_sized(h);
+    H2Dev &d = h->d;
+    dfree(d.keys); dfree(h->keys_alt); dfree(d.perm); dfree(h->perm_alt); dfree(d.Ys);
+    dfree(d.root); dfree(d.flags); dfree(d.cnt); dfree(h->part);
+    dfree(h->d_proxy); dfree(h->d_proxy_cnt);
+    if (h->pin) (void)hipHostFree(h->pin);
+    delete h;
+}
+
+int h2_build(H2 *h, const void *Yg, int *nonfinite)
+{
+    *nonfinite = 0;
+    h->usable = 0;
+    for (int attempt = 0; attempt < 6; attempt++) {
+        const int rc = build_once(h, Yg);
+        if (rc == 0) {
+            h->usable = !h->hflags[FLAG_BIG_LEAF];
+            return 0;
+        }
+        if (rc == -2) {
+            *nonfinite = 1;
+            return -1;
+        }
+        if (rc < 0) {
+            // the structure cannot be built (e.g. a near list beyond its capacity):
+            // the caller falls back to the direct sum
+            return hipGetLastError() == hipSuccess ? 0 : -1;
+        }
+        if (!alloc_sized(h)) return -1;
+    }
+    return 0;
+}
+
+int h2_product(H2 *h, const void *Yg, float *s, float *F, int mixed)
+{
+    (void)Yg;
+    if (!h->usable) return -1;
+    H2Dev &d = h->d;
+    if (launch_couple(h->D, d, h->n_nodes, mixed, h->st)) return -1;
+    const int nlev = (int)h->lvl.size() - 1;
+    for (int lv = 2; lv < nlev; lv++)
+        if (launch_down(h->D, d, h->lvl[lv], h->lvl[lv + 1], mixed, h->st)) return -1;
+    return launch_leaf_eval(h->D, d, h->n_leaves, s, F, h->st);
+}
+
+int h2_usable(const H2 *h) { return h->usable; }
+
+int h2_stats(const H2 *h, tsne_h2_stats_t *st)
+{
+    memset(st, 0, sizeof(*st));
+    st->levels = (int)h->lvl.size() - 1;
+    st->n_nodes = h->n_nodes;
+    st->n_leaves = h->n_leaves;
+    st->leaf_points_max = h->leaf;
+    st->n_adm_pairs = h->n_adm + h->n_x + h->n_w;
+    st->n_near_pairs = h->n_near;
+    st->near_point_pairs = -1;
+    st->rank_max = h->hflags[FLAG_RANK_MAX];
+    st->rank_mean = h->n_nodes > 1 ? (double)h->hflags[FLAG_RANK_SUM] / (h->n_nodes - 1) : 0.0;
+    st->proxy_max = h->hflags[FLAG_PROXY_MAX];
+    return 0;
+}
 
 }  // namespace tsne

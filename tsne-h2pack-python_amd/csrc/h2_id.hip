@@ -1,0 +1,279 @@
+// h2_id.hip -- H2 rebuild, part 2: proxy-point interpolative decomposition (ID) of every
+// node, fused with the upward pass of the product.
+//
+// Node i (level l >= 1) has m candidate points: its own points if it is a leaf, else
+// the skeleton points of its children.  With the proxy set P of its level translated to
+// its centre, A = K(candidates, P) (m x p) is column-normalised and row-ID'd by
+// column-pivoted modified Gram-Schmidt on A^T in FP64:  A ~= U A[J, :], |J| = r, where
+// the pivoting stops once the largest residual column norm is below tol (relative to
+// the first pivot).  U (m x r, FP32) and the skeleton J are stored; then the upward
+// coefficients w_i = U^T X_i are formed, X_i holding one row [1, y - c_i] per leaf point
+// or, for an internal node, the children's coefficients moved to c_i
+// ([w0, wY + (c_child - c_i) w0]).
+//
+// Reference counterpart: the U / J construction of H2P_build in H2Pack (QR_REL_NRM ID
+// against proxy points; src/tsne_h2pack.c:818, :908-919) and the upward sweep of the
+// fused product (src/tsne_h2_matmul.c:309-334).  One workgroup per node, one thread per
+// candidate column.
+#include "h2_internal.h"
+#include "proxy.h"
+
+namespace tsne {
+
+namespace {
+
+template <int D> using VT = typename Vec<D>::T;
+
+__device__ __forceinline__ double kern64(const double *a, const double *b, int D)
+{
+    double r2 = 1.0;
+    for (int d = 0; d < D; d++) {
+        const double df = a[d] - b[d];
+        r2 += df * df;
+    }
+    const double w = 1.0 / r2;
+    return w * w;
+}
+
+}  // namespace
+
+template <int D, int NT>
+__global__ void __launch_bounds__(NT)
+id_up_kernel(H2Dev h, int lb, int l, double tol)
+{
+    constexpr int PMAX = H2Cfg<D>::PMAX, RMAX = H2Cfg<D>::RMAX;
+    constexpr int NW = NT / 64;
+    constexpr size_t QBYTES = sizeof(double) * RMAX * PMAX;
+    constexpr size_t UBYTES = sizeof(float) * NT * RMAX;
+    __shared__ double s_cand[NT][D];
+    __shared__ float4 s_xin[NT];
+    __shared__ double s_prox[PMAX][D];
+    __shared__ double s_scale[PMAX];
+    __shared__ __align__(16) unsigned char s_qu[QBYTES > UBYTES ? QBYTES : UBYTES];
+    __shared__ double s_R[RMAX][RMAX];
+    __shared__ int s_piv[RMAX];
+    __shared__ double s_rv[NW];
+    __shared__ int s_ri[NW];
+    __shared__ double s_bv;
+    __shared__ int s_bi, s_base;
+    double(*s_Q)[PMAX] = reinterpret_cast<double(*)[PMAX]>(s_qu);
+    float(*s_U)[RMAX] = reinterpret_cast<float(*)[RMAX]>(s_qu);
+
+    const int node = lb + blockIdx.x;
+    const int t = threadIdx.x;
+    const int lane = t & 63, wid = t >> 6;
+    const float4 c = h.cent[node];
+    const double cd[3] = {(double)c.x, (double)c.y, (double)c.z};
+    const bool leafn = h.child0[node] < 0;
+
+    // ---- candidates and the rows of X
+    int m;
+    double my[3] = {0.0, 0.0, 0.0};
+    float4 xin = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (leafn) {
+        const int b = h.nb[node];
+        m = min(h.ne[node] - b, NT);
+        if (t < m) {
+            const VT<D> y = ((const VT<D> *)h.Ys)[b + t];
+            my[0] = y.x;
+            my[1] = y.y;
+            if constexpr (D == 3) my[2] = y.z;
+            xin = make_float4(1.f, y.x - c.x, y.y - c.y, 0.f);
+            if constexpr (D == 3) xin.w = y.z - c.z;
+        }
+    } else {
+        int acc = 0;
+        const int c0 = h.child0[node], nc = h.nchild[node];
+        for (int q = 0; q < nc; q++) {
+            const int cj = c0 + q;
+            const int rj = h.rank[cj];
+            if (t >= acc && t < acc + rj) {
+                const int k = t - acc;
+                for (int d = 0; d < D; d++) my[d] = h.skel[((size_t)cj * RMAX + k) * D + d];
+                const float4 w = h.wc[(size_t)cj * RMAX + k];
+                const float4 cc = h.cent[cj];
+                xin = make_float4(w.x, w.y + (cc.x - c.x) * w.x, w.z + (cc.y - c.y) * w.x,
+                                  w.w + (cc.z - c.z) * w.x);
+            }
+            acc += rj;
+        }
+        m = min(acc, NT);
+    }
+    for (int d = 0; d < D; d++) s_cand[t][d] = my[d];
+    s_xin[t] = xin;
+
+    // ---- proxies of this level, translated to the centre
+    const RootInfo root = *h.root;
+    const int kq = min(max(root.k0 - l, 0), kProxyLevels - 1);
+    const int p = min(h.proxy_cnt[kq], PMAX);
+    for (int q = t; q < p; q += NT)
+        for (int d = 0; d < D; d++) s_prox[q][d] = cd[d] + h.proxy_pts[((size_t)kq * PMAX + q) * D + d];
+    __syncthreads();
+    // column scales: 1 / ||K(candidates, proxy_q)||
+    for (int q = t; q < p; q += NT) {
+        double s2 = 0.0;
+        for (int u = 0; u < m; u++) {
+            const double v = kern64(s_cand[u], s_prox[q], D);
+            s2 += v * v;
+        }
+        s_scale[q] = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
+    }
+    __syncthreads();
+
+    // ---- this thread's normalised column of A^T
+    double a[PMAX];
+    double nrm = 0.0;
+#pragma unroll
+    for (int q = 0; q < PMAX; q++) {
+        double v = 0.0;
+        if (q < p && t < m) v = kern64(my, s_prox[q], D) * s_scale[q];
+        a[q] = v;
+        nrm += v * v;
+    }
+    if (t >= m) nrm = -1.0;
+
+    // ---- column-pivoted MGS
+    bool selected = false;
+    double nrm0 = 0.0;
+    int r = 0;
+    const int rlim = min(RMAX, min(m, p));
+    for (int k = 0; k < RMAX; k++) {
+        double v = (selected || t >= m) ? -1.0 : nrm;
+        int vi = t;
+        for (int off = 32; off > 0; off >>= 1) {
+            const double ov = __shfl_xor(v, off, 64);
+            const int oi = __shfl_xor(vi, off, 64);
+            if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
+        }
+        if (lane == 0) { s_rv[wid] = v; s_ri[wid] = vi; }
+        __syncthreads();
+        if (t == 0) {
+            double bv = s_rv[0];
+            int bi = s_ri[0];
+            for (int w = 1; w < NW; w++)
+                if (s_rv[w] > bv || (s_rv[w] == bv && s_ri[w] < bi)) { bv = s_rv[w]; bi = s_ri[w]; }
+            s_bv = bv;
+            s_bi = bi;
+        }
+        __syncthreads();
+        const double bv = s_bv;
+        const int bi = s_bi;
+        if (k == 0) nrm0 = bv;
+        if (k >= rlim || !(bv > tol * tol * nrm0) || !(bv > 0.0)) break;
+        if (t == bi) {
+            selected = true;
+            const double inv = 1.0 / sqrt(bv);
+#pragma unroll
+            for (int q = 0; q < PMAX; q++) s_Q[k][q] = a[q] * inv;
+            s_piv[k] = t;
+        }
+        __syncthreads();
+        if (!selected && t < m) {
+            double dot = 0.0;
+#pragma unroll
+            for (int q = 0; q < PMAX; q++) dot += s_Q[k][q] * a[q];
+            double s2 = 0.0;
+#pragma unroll
+            for (int q = 0; q < PMAX; q++) {
+                a[q] -= dot * s_Q[k][q];
+                s2 += a[q] * a[q];
+            }
+            nrm = s2;
+        }
+        r = k + 1;
+        if (k == RMAX - 1 && t == 0) atomicOr(&h.flags[FLAG_RANK_CAP], 1);
+    }
+
+    // ---- interpolation coefficients x_t = R11^{-1} Q^T a_t
+    double rt[RMAX];
+#pragma unroll
+    for (int k = 0; k < RMAX; k++) rt[k] = 0.0;
+    if (t < m) {
+        for (int q = 0; q < p; q++) {
+            const double v = kern64(my, s_prox[q], D) * s_scale[q];
+#pragma unroll
+            for (int k = 0; k < RMAX; k++)
+                if (k < r) rt[k] += s_Q[k][q] * v;
+        }
+    }
+    int kk = -1;
+    if (selected)
+        for (int k = 0; k < r; k++)
+            if (s_piv[k] == t) kk = k;
+    if (kk >= 0) {
+#pragma unroll
+        for (int j = 0; j < RMAX; j++)
+            if (j <= kk) s_R[j][kk] = rt[j];
+    }
+    __syncthreads();  // s_R complete; s_Q no longer read below (s_U aliases it)
+    if (kk >= 0) {
+#pragma unroll
+        for (int k = 0; k < RMAX; k++) rt[k] = (k == kk) ? 1.0 : 0.0;
+    } else if (t < m) {
+#pragma unroll
+        for (int k = RMAX - 1; k >= 0; k--) {
+            if (k < r) {
+                double v = rt[k];
+#pragma unroll
+                for (int j = 0; j < RMAX; j++)
+                    if (j > k && j < r) v -= s_R[k][j] * rt[j];
+                rt[k] = v / s_R[k][k];
+            }
+        }
+    }
+
+    // ---- store U (pool), skeleton, and the upward coefficients
+    if (t == 0) {
+        const int base = atomicAdd(&h.cnt[CNT_POOL], m * r);
+        s_base = base;
+        if ((long long)base + (long long)m * r > h.pool_cap) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
+        h.rank[node] = r;
+        h.uoff[node] = base;
+        atomicMax(&h.flags[FLAG_RANK_MAX], r);
+        atomicAdd(&h.flags[FLAG_RANK_SUM], r);
+        atomicMax(&h.flags[FLAG_PROXY_MAX], p);
+    }
+    __syncthreads();
+    const int base = s_base;
+    const bool pool_ok = (long long)base + (long long)m * r <= h.pool_cap;
+    if (t < m) {
+#pragma unroll
+        for (int k = 0; k < RMAX; k++)
+            if (k < r) {
+                const float u = (float)rt[k];
+                s_U[t][k] = u;
+                if (pool_ok) h.upool[(size_t)base + (size_t)t * r + k] = u;
+            }
+    }
+    if (kk >= 0) {
+        for (int d = 0; d < D; d++) h.skel[((size_t)node * RMAX + kk) * D + d] = my[d];
+        h.skelf[(size_t)node * RMAX + kk] =
+            make_float4((float)my[0], (float)my[1], (float)my[2], 0.f);
+    }
+    __syncthreads();
+    if (t < r) {
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int u = 0; u < m; u++) {
+            const float uk = s_U[u][t];
+            const float4 x = s_xin[u];
+            w.x = fmaf(uk, x.x, w.x);
+            w.y = fmaf(uk, x.y, w.y);
+            w.z = fmaf(uk, x.z, w.z);
+            w.w = fmaf(uk, x.w, w.w);
+        }
+        h.wc[(size_t)node * RMAX + t] = w;
+    }
+}
+
+int launch_id_up(int D, const H2Dev &h, int lb, int le, int l, double tol, hipStream_t st)
+{
+    const int g = le - lb;
+    if (g <= 0) return 0;
+    if (D == 2)
+        hipLaunchKernelGGL((id_up_kernel<2, 128>), dim3(g), dim3(128), 0, st, h, lb, l, tol);
+    else
+        hipLaunchKernelGGL((id_up_kernel<3, 384>), dim3(g), dim3(384), 0, st, h, lb, l, tol);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace tsne

@@ -1,0 +1,125 @@
+// h2_internal.h -- device layout of the H2 representation and the launchers of its
+// kernels (h2_tree.hip, h2_id.hip, h2_product.hip); orchestrated by h2.hip.
+//
+// Tree: adaptive 2^D-tree over the Morton-sorted points.  The root box is the
+// bounding square of the embedding with its half-side rounded up onto the proxy ladder
+// (H0 = 2^k0 * kProxyBase), so a node at level l is a dyadic box of half-side
+// H0 / 2^l whose proxy set is ladder entry k0 - l.  A node is split while it holds more
+// than leaf_size points (and its level is below the key depth).  Nodes are stored level
+// by level; the points of a node are a contiguous range [nb, ne) of the sorted order.
+//
+// Interaction lists (per target node, all computed level by level from the parent's
+// near list, reference call stack: H2P_build, src/tsne_h2pack.c:908-919):
+//   ADM  (i, j) same level, boxes separated by at least one box: skeleton <- skeleton
+//   X    (i, j) j a coarser leaf outside i's far region:          skeleton <- points
+//   W    (i, j) i a leaf outside j's far region, j finer:          points   <- skeleton
+//   NEAR (i, j) both leaves, neither far:                          points   <- points
+// Pair lists are sorted by (target, source) so the product is deterministic.
+#pragma once
+#include <cstdint>
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+
+namespace tsne {
+
+template <int D> struct H2Cfg;
+template <> struct H2Cfg<2> {
+    static constexpr int B = 30;         // key bits per dimension
+    static constexpr int NCH = 4;        // children per node
+    static constexpr int RMAX = 32;      // maximum skeleton rank
+    static constexpr int PMAX = 64;      // maximum proxy points
+    static constexpr int MCAP = 128;     // maximum ID candidates (NCH * RMAX, leaf cap)
+    static constexpr int NEAR_CAP = 48;  // near-list slots per node
+};
+template <> struct H2Cfg<3> {
+    static constexpr int B = 20;
+    static constexpr int NCH = 8;
+    static constexpr int RMAX = 48;
+    static constexpr int PMAX = 96;
+    static constexpr int MCAP = 384;
+    static constexpr int NEAR_CAP = 96;
+};
+
+constexpr int kLeafCap = 128;  // points per leaf handled by the leaf kernel
+
+// Error / status flags (device int array H2Dev::flags).
+enum H2Flag {
+    FLAG_NEAR_OVF = 0,   // a node's near list exceeded NEAR_CAP
+    FLAG_PAIR_OVF = 1,   // a pair buffer (ADM/X/W/NEAR/pending) overflowed
+    FLAG_POOL_OVF = 2,   // the U pool overflowed
+    FLAG_BIG_LEAF = 3,   // a leaf at the maximum depth holds more than kLeafCap points
+    FLAG_NONFINITE = 4,  // non-finite coordinate
+    FLAG_RANK_CAP = 5,   // an ID stopped at RMAX before reaching the tolerance
+    FLAG_RANK_MAX = 6,   // statistics: maximum rank
+    FLAG_RANK_SUM = 7,   // statistics: sum of ranks (low 31 bits)
+    FLAG_PROXY_MAX = 8,  // statistics: maximum proxies used
+    FLAG_COUNT = 16
+};
+
+// Counters of the pair buffers (device int array H2Dev::cnt).
+enum H2Count { CNT_ADM = 0, CNT_X, CNT_W, CNT_NEAR, CNT_PEND0, CNT_PEND1, CNT_LEAF, CNT_POOL,
+               CNT_COUNT = 16 };
+
+struct RootInfo {
+    double c[3];   // root box centre
+    double H0;     // root half-side (on the proxy ladder)
+    int k0;        // ladder index of H0
+    int pad;
+};
+
+// Pointers to device arrays shared by the kernels.
+struct H2Dev {
+    int n, D, leaf;
+    // sorted points
+    uint64_t *keys;
+    int *perm;
+    void *Ys;          // Vec<D> sorted coordinates
+    RootInfo *root;
+    // nodes
+    int node_cap;
+    int *nb, *ne, *level, *parent, *child0, *nchild;
+    float4 *cent;      // centre (xyz) of each node's box, w = half-side
+    int *icoord;       // [node][3] integer box coordinates at the node's level
+    int *bounds;       // split scratch [node][NCH+1]
+    int *childcnt, *childoff;
+    int *near, *near_cnt;  // [node][NEAR_CAP]
+    int *leaves;       // leaf node ids (CNT_LEAF of them)
+    // pair buffers (target << 32 | source)
+    uint64_t *adm, *xl, *wl, *nl, *pend[2];
+    int adm_cap, x_cap, w_cap, nl_cap, pend_cap;
+    uint64_t *sort_tmp;
+    int *adm_beg, *adm_end, *x_beg, *x_end, *w_beg, *w_end, *nl_beg, *nl_end;
+    // ID / product
+    int *rank, *uoff;
+    double *skel;      // [node][RMAX][D]
+    float4 *skelf;     // [node][RMAX]
+    float *upool;
+    long long pool_cap;
+    float4 *wc, *zc;   // [node][RMAX] upward / downward coefficients
+    const double *proxy_pts;  // [levels][PMAX][D]
+    const int *proxy_cnt;
+    int *flags, *cnt;
+};
+
+// h2_tree.hip
+int launch_root_box(int D, const void *Yg, int n, const H2Dev &h, double *partial, int nblk,
+                    hipStream_t st);
+int launch_morton(int D, const void *Yg, const H2Dev &h, hipStream_t st);
+int launch_gather_sorted(int D, const void *Yg, const H2Dev &h, hipStream_t st);
+int launch_init_root(const H2Dev &h, hipStream_t st);
+int launch_split_count(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st);
+int launch_split_write(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st);
+int launch_lists(int D, const H2Dev &h, int lb, int le, int l, int pend_out, hipStream_t st);
+int launch_pending(int D, const H2Dev &h, int n_pend, int pend_in, int pend_out, hipStream_t st);
+int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, hipStream_t st);
+
+// h2_id.hip
+int launch_id_up(int D, const H2Dev &h, int lb, int le, int l, double tol, hipStream_t st);
+
+// h2_product.hip
+int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, hipStream_t st);
+int launch_down(int D, const H2Dev &h, int lb, int le, int mixed, hipStream_t st);
+int launch_leaf_eval(int D, const H2Dev &h, int n_leaves, float *s, float *F, hipStream_t st);
+
+}  // namespace tsne

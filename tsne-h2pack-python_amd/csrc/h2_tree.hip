@@ -1,0 +1,451 @@
+// h2_tree.hip -- H2 rebuild, part 1: root box, Morton keys, adaptive 2^D-tree and
+// the interaction lists, built level by level on the device.
+//
+// Reference counterpart: H2P_partition_points / the admissible-pair search inside
+// H2P_build (H2Pack, called at src/tsne_h2pack.c:863 and :908-919) and the root box
+// tsne_calc_root_enbox (src/tsne_h2pack.c:651-698).
+#include <cmath>
+
+#include "h2_internal.h"
+#include "proxy.h"
+
+namespace tsne {
+
+namespace {
+
+template <int D> using VT = typename Vec<D>::T;
+
+__device__ __forceinline__ float vcomp(const float2 &v, int d) { return d == 0 ? v.x : v.y; }
+__device__ __forceinline__ float vcomp(const float4 &v, int d)
+{
+    return d == 0 ? v.x : (d == 1 ? v.y : v.z);
+}
+
+__device__ __forceinline__ void push_pair(uint64_t *buf, int *counter, int cap, int t, int s,
+                                          int *flags)
+{
+    const int k = atomicAdd(counter, 1);
+    if (k < cap)
+        buf[k] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)s;
+    else
+        atomicOr(&flags[FLAG_PAIR_OVF], 1);
+}
+
+// Separation of the boxes of nodes a (level la) and b (level lb <= la is not assumed),
+// measured in box widths of the finer of the two: max over d of the number of whole
+// fine boxes between them.  Exact integer arithmetic on the dyadic box coordinates.
+template <int D>
+__device__ __forceinline__ int box_gap(const int *ic, int a, int la, int b, int lb)
+{
+    int ga = 0;
+    if (la >= lb) {
+        const int s = 1 << (la - lb);
+        for (int d = 0; d < D; d++) {
+            const int x = ic[a * 3 + d];
+            const int lo = ic[b * 3 + d] * s, hi = lo + s - 1;
+            const int g = max(max(lo - x - 1, x - hi - 1), 0);
+            ga = max(ga, g);
+        }
+    } else {
+        const int s = 1 << (lb - la);
+        for (int d = 0; d < D; d++) {
+            const int x = ic[b * 3 + d];
+            const int lo = ic[a * 3 + d] * s, hi = lo + s - 1;
+            const int g = max(max(lo - x - 1, x - hi - 1), 0);
+            ga = max(ga, g);
+        }
+    }
+    return ga;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------------
+// Root box: bounding box of the embedding (and a finiteness test), then the root
+// square with its half-side rounded up onto the proxy ladder.
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+bbox_partial_kernel(const VT<D> *__restrict__ Y, int n, int per, double *__restrict__ part)
+{
+    __shared__ float red[2 * 3][256];
+    __shared__ int bad_s[256];
+    float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int bad = 0;
+    const int b0 = blockIdx.x * per, b1 = min(n, b0 + per);
+    for (int i = b0 + threadIdx.x; i < b1; i += 256) {
+        const VT<D> y = Y[i];
+        for (int d = 0; d < D; d++) {
+            const float v = vcomp(y, d);
+            if (!isfinite(v)) bad = 1;
+            lo[d] = fminf(lo[d], v);
+            hi[d] = fmaxf(hi[d], v);
+        }
+    }
+    for (int d = 0; d < D; d++) { red[d][threadIdx.x] = lo[d]; red[3 + d][threadIdx.x] = hi[d]; }
+    bad_s[threadIdx.x] = bad;
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off) {
+            for (int d = 0; d < D; d++) {
+                red[d][threadIdx.x] = fminf(red[d][threadIdx.x], red[d][threadIdx.x + off]);
+                red[3 + d][threadIdx.x] = fmaxf(red[3 + d][threadIdx.x], red[3 + d][threadIdx.x + off]);
+            }
+            bad_s[threadIdx.x] |= bad_s[threadIdx.x + off];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        for (int d = 0; d < D; d++) {
+            part[blockIdx.x * 7 + d] = red[d][0];
+            part[blockIdx.x * 7 + 3 + d] = red[3 + d][0];
+        }
+        part[blockIdx.x * 7 + 6] = bad_s[0];
+    }
+}
+
+template <int D>
+__global__ void bbox_final_kernel(int nblk, const double *__restrict__ part, H2Dev h)
+{
+    if (threadIdx.x != 0) return;
+    double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+    int bad = 0;
+    for (int b = 0; b < nblk; b++) {
+        for (int d = 0; d < D; d++) {
+            lo[d] = fmin(lo[d], part[b * 7 + d]);
+            hi[d] = fmax(hi[d], part[b * 7 + 3 + d]);
+        }
+        bad |= part[b * 7 + 6] != 0.0;
+    }
+    RootInfo r;
+    double ext = 0.0;
+    for (int d = 0; d < 3; d++) r.c[d] = 0.0;
+    for (int d = 0; d < D; d++) {
+        r.c[d] = 0.5 * (lo[d] + hi[d]);
+        ext = fmax(ext, hi[d] - lo[d]);
+    }
+    // half-side with a margin so the extreme points fall strictly inside
+    const double H = 0.5 * ext * (1.0 + 1e-6) + 1e-30;
+    int k = 0;
+    while (k < kProxyLevels - 1 && kProxyBase * exp2((double)k) < H) k++;
+    r.k0 = k;
+    r.H0 = kProxyBase * exp2((double)k);
+    r.pad = 0;
+    *h.root = r;
+    if (bad || !isfinite(ext)) h.flags[FLAG_NONFINITE] = 1;
+}
+
+int launch_root_box(int D, const void *Yg, int n, const H2Dev &h, double *partial, int nblk,
+                    hipStream_t st)
+{
+    const int per = (n + nblk - 1) / nblk;
+    if (D == 2) {
+        hipLaunchKernelGGL(bbox_partial_kernel<2>, dim3(nblk), dim3(256), 0, st,
+                           (const float2 *)Yg, n, per, partial);
+        hipLaunchKernelGGL(bbox_final_kernel<2>, dim3(1), dim3(64), 0, st, nblk, partial, h);
+    } else {
+        hipLaunchKernelGGL(bbox_partial_kernel<3>, dim3(nblk), dim3(256), 0, st,
+                           (const float4 *)Yg, n, per, partial);
+        hipLaunchKernelGGL(bbox_final_kernel<3>, dim3(1), dim3(64), 0, st, nblk, partial, h);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------------
+// Morton keys: coordinates quantised to 2^B cells per dimension of the root box, bits
+// interleaved most significant first, so the level-l digit of a key is its l-th group
+// of D bits from the top and points of one node are contiguous after sorting.
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+morton_kernel(const VT<D> *__restrict__ Y, H2Dev h)
+{
+    constexpr int B = H2Cfg<D>::B;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= h.n) return;
+    const RootInfo r = *h.root;
+    const VT<D> y = Y[i];
+    const double scale = (double)(1u << B) / (2.0 * r.H0);
+    uint64_t key = 0;
+    for (int d = 0; d < D; d++) {
+        double q = floor(((double)vcomp(y, d) - (r.c[d] - r.H0)) * scale);
+        q = fmin(fmax(q, 0.0), (double)((1u << B) - 1));
+        const uint64_t qi = (uint64_t)q;
+        for (int b = 0; b < B; b++) key |= ((qi >> b) & 1ull) << (D * b + d);
+    }
+    h.keys[i] = key;
+    h.perm[i] = i;
+}
+
+int launch_morton(int D, const void *Yg, const H2Dev &h, hipStream_t st)
+{
+    const int g = (h.n + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(morton_kernel<2>, dim3(g), dim3(256), 0, st, (const float2 *)Yg, h);
+    else
+        hipLaunchKernelGGL(morton_kernel<3>, dim3(g), dim3(256), 0, st, (const float4 *)Yg, h);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+gather_sorted_kernel(const VT<D> *__restrict__ Y, H2Dev h)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= h.n) return;
+    ((VT<D> *)h.Ys)[k] = Y[h.perm[k]];
+}
+
+int launch_gather_sorted(int D, const void *Yg, const H2Dev &h, hipStream_t st)
+{
+    const int g = (h.n + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(gather_sorted_kernel<2>, dim3(g), dim3(256), 0, st, (const float2 *)Yg, h);
+    else
+        hipLaunchKernelGGL(gather_sorted_kernel<3>, dim3(g), dim3(256), 0, st, (const float4 *)Yg, h);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// Root node: all points, its own near list {0}; a root that is already a leaf
+// interacts with itself directly.
+__global__ void init_root_kernel(H2Dev h, int B)
+{
+    if (threadIdx.x != 0) return;
+    const RootInfo r = *h.root;
+    h.nb[0] = 0;
+    h.ne[0] = h.n;
+    h.level[0] = 0;
+    h.parent[0] = -1;
+    h.child0[0] = -1;
+    h.nchild[0] = 0;
+    h.cent[0] = make_float4((float)r.c[0], (float)r.c[1], (float)r.c[2], (float)r.H0);
+    h.icoord[0] = h.icoord[1] = h.icoord[2] = 0;
+    h.near[0] = 0;
+    h.near_cnt[0] = 1;
+    if (h.n <= h.leaf || B == 0) {
+        h.nl[0] = 0;
+        h.cnt[CNT_NEAR] = 1;
+    }
+}
+
+int launch_init_root(const H2Dev &h, hipStream_t st)
+{
+    hipLaunchKernelGGL(init_root_kernel, dim3(1), dim3(64), 0, st, h,
+                       h.D == 2 ? H2Cfg<2>::B : H2Cfg<3>::B);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------------
+// Split of the nodes of level l: child ranges by binary search of the level-(l+1)
+// digit inside the node's sorted key range, then (after a scan of the child counts)
+// the children are written contiguously after the level.  Leaves are recorded.
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+split_count_kernel(H2Dev h, int lb, int le, int l)
+{
+    constexpr int B = H2Cfg<D>::B, NCH = H2Cfg<D>::NCH;
+    const int i = lb + blockIdx.x * 256 + threadIdx.x;
+    if (i >= le) return;
+    const int b = h.nb[i], e = h.ne[i];
+    int *bd = h.bounds + (size_t)(i - lb) * (NCH + 1);
+    if (e - b <= h.leaf || l >= B) {
+        h.childcnt[i - lb] = 0;
+        h.child0[i] = -1;
+        h.nchild[i] = 0;
+        const int k = atomicAdd(&h.cnt[CNT_LEAF], 1);
+        h.leaves[k] = i;
+        if (e - b > kLeafCap) atomicOr(&h.flags[FLAG_BIG_LEAF], 1);
+        return;
+    }
+    const int shift = D * (B - l - 1);
+    bd[0] = b;
+    bd[NCH] = e;
+    for (int v = 1; v < NCH; v++) {
+        int lo = b, hi = e;  // first index with digit >= v
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            const int dg = (int)((h.keys[mid] >> shift) & (uint64_t)(NCH - 1));
+            if (dg < v) lo = mid + 1; else hi = mid;
+        }
+        bd[v] = lo;
+    }
+    int c = 0;
+    for (int v = 0; v < NCH; v++) c += bd[v + 1] > bd[v];
+    h.childcnt[i - lb] = c;
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+split_write_kernel(H2Dev h, int lb, int le, int l)
+{
+    constexpr int NCH = H2Cfg<D>::NCH;
+    const int i = lb + blockIdx.x * 256 + threadIdx.x;
+    if (i >= le) return;
+    if (h.childcnt[i - lb] == 0) return;
+    const int *bd = h.bounds + (size_t)(i - lb) * (NCH + 1);
+    const int base = le + h.childoff[i - lb];
+    const float4 c = h.cent[i];
+    const float hc = 0.5f * c.w;
+    int k = 0;
+    for (int v = 0; v < NCH; v++) {
+        if (bd[v + 1] <= bd[v]) continue;
+        const int ch = base + k++;
+        h.nb[ch] = bd[v];
+        h.ne[ch] = bd[v + 1];
+        h.level[ch] = l + 1;
+        h.parent[ch] = i;
+        h.child0[ch] = -1;
+        h.nchild[ch] = 0;
+        float4 cc;
+        cc.x = c.x + ((v & 1) ? hc : -hc);
+        cc.y = c.y + ((v & 2) ? hc : -hc);
+        cc.z = (D == 3) ? c.z + ((v & 4) ? hc : -hc) : 0.f;
+        cc.w = hc;
+        h.cent[ch] = cc;
+        for (int d = 0; d < 3; d++)
+            h.icoord[ch * 3 + d] = (d < D) ? 2 * h.icoord[i * 3 + d] + ((v >> d) & 1) : 0;
+    }
+    h.child0[i] = base;
+    h.nchild[i] = k;
+}
+
+int launch_split_count(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st)
+{
+    const int g = (le - lb + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(split_count_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le, l);
+    else
+        hipLaunchKernelGGL(split_count_kernel<3>, dim3(g), dim3(256), 0, st, h, lb, le, l);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_split_write(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st)
+{
+    const int g = (le - lb + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(split_write_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le, l);
+    else
+        hipLaunchKernelGGL(split_write_kernel<3>, dim3(g), dim3(256), 0, st, h, lb, le, l);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------------
+// Interaction lists of the nodes of level l (>= 1).  Candidates are the children of
+// the parent's near-list members (a leaf member is its own candidate).  A same-level
+// candidate at least one box away is admissible (ADM); a coarser leaf at least one
+// box of i away is an X source; the rest stay in i's near list.  For a leaf i the near
+// list is final: leaf members become NEAR pairs, others are descended later (pending).
+// ---------------------------------------------------------------------------------
+template <int D>
+__global__ void __launch_bounds__(256)
+lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
+{
+    constexpr int B = H2Cfg<D>::B, NEAR_CAP = H2Cfg<D>::NEAR_CAP;
+    const int i = lb + blockIdx.x * 256 + threadIdx.x;
+    if (i >= le) return;
+    const int p = h.parent[i];
+    const int np = h.near_cnt[p];
+    int *my = h.near + (size_t)i * NEAR_CAP;
+    int cnt = 0;
+    for (int a = 0; a < np; a++) {
+        const int j = h.near[(size_t)p * NEAR_CAP + a];
+        const bool jl = h.child0[j] < 0;
+        const int c0 = jl ? j : h.child0[j];
+        const int nc = jl ? 1 : h.nchild[j];
+        for (int q = 0; q < nc; q++) {
+            const int c = c0 + q;
+            const int lc = h.level[c];
+            const int gap = box_gap<D>(h.icoord, i, l, c, lc);
+            if (gap >= 1 && c != i) {
+                if (lc == l)
+                    push_pair(h.adm, &h.cnt[CNT_ADM], h.adm_cap, i, c, h.flags);
+                else
+                    push_pair(h.xl, &h.cnt[CNT_X], h.x_cap, i, c, h.flags);
+            } else {
+                if (cnt < NEAR_CAP) my[cnt] = c;
+                else atomicOr(&h.flags[FLAG_NEAR_OVF], 1);
+                cnt++;
+            }
+        }
+    }
+    cnt = min(cnt, NEAR_CAP);
+    h.near_cnt[i] = cnt;
+    if ((h.ne[i] - h.nb[i]) <= h.leaf || l >= B) {
+        for (int a = 0; a < cnt; a++) {
+            const int c = my[a];
+            if ((h.ne[c] - h.nb[c]) <= h.leaf || h.level[c] >= B)
+                push_pair(h.nl, &h.cnt[CNT_NEAR], h.nl_cap, i, c, h.flags);
+            else
+                push_pair(h.pend[pend_out], &h.cnt[CNT_PEND0 + pend_out], h.pend_cap, i, c, h.flags);
+        }
+    }
+}
+
+// Pending pairs (leaf i, non-leaf c): each child d of c is a W source when i lies at
+// least one box of d away, a NEAR source when d is a leaf, otherwise pending again.
+template <int D>
+__global__ void __launch_bounds__(256)
+pending_kernel(H2Dev h, int n_pend, int pend_in, int pend_out)
+{
+    constexpr int B = H2Cfg<D>::B;
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n_pend) return;
+    const uint64_t pr = h.pend[pend_in][k];
+    const int i = (int)(pr >> 32), c = (int)(pr & 0xffffffffu);
+    const int li = h.level[i];
+    for (int q = 0; q < h.nchild[c]; q++) {
+        const int d = h.child0[c] + q;
+        const int ld = h.level[d];
+        if (box_gap<D>(h.icoord, i, li, d, ld) >= 1)
+            push_pair(h.wl, &h.cnt[CNT_W], h.w_cap, i, d, h.flags);
+        else if ((h.ne[d] - h.nb[d]) <= h.leaf || ld >= B)
+            push_pair(h.nl, &h.cnt[CNT_NEAR], h.nl_cap, i, d, h.flags);
+        else
+            push_pair(h.pend[pend_out], &h.cnt[CNT_PEND0 + pend_out], h.pend_cap, i, d, h.flags);
+    }
+}
+
+int launch_lists(int D, const H2Dev &h, int lb, int le, int l, int pend_out, hipStream_t st)
+{
+    const int g = (le - lb + 255) / 256;
+    if (g == 0) return 0;
+    if (D == 2)
+        hipLaunchKernelGGL(lists_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le, l, pend_out);
+    else
+        hipLaunchKernelGGL(lists_kernel<3>, dim3(g), dim3(256), 0, st, h, lb, le, l, pend_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_pending(int D, const H2Dev &h, int n_pend, int pend_in, int pend_out, hipStream_t st)
+{
+    if (n_pend <= 0) return 0;
+    const int g = (n_pend + 255) / 256;
+    if (D == 2)
+        hipLaunchKernelGGL(pending_kernel<2>, dim3(g), dim3(256), 0, st, h, n_pend, pend_in, pend_out);
+    else
+        hipLaunchKernelGGL(pending_kernel<3>, dim3(g), dim3(256), 0, st, h, n_pend, pend_in, pend_out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// CSR offsets of (target << 32 | source) pairs sorted by key: beg/end per target.
+__global__ void __launch_bounds__(256)
+pair_offsets_kernel(const uint64_t *__restrict__ pairs, int n, int *__restrict__ beg,
+                    int *__restrict__ end)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k >= n) return;
+    const int t = (int)(pairs[k] >> 32);
+    if (k == 0 || (int)(pairs[k - 1] >> 32) != t) beg[t] = k;
+    if (k == n - 1 || (int)(pairs[k + 1] >> 32) != t) end[t] = k + 1;
+}
+
+int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, hipStream_t st)
+{
+    if (n_pairs <= 0) return 0;
+    hipLaunchKernelGGL(pair_offsets_kernel, dim3((n_pairs + 255) / 256), dim3(256), 0, st, pairs,
+                       n_pairs, beg, end);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+}  // namespace tsne

@@ -6,8 +6,8 @@
 // decomposition of K(points, proxies); the proxies must span, up to the tolerance, the
 // functions x -> K(x, y) over the box for every y in that far region.  They are
 // selected from many candidates y (dense near the inner boundary, log-spaced outwards)
-// by column-pivoted Gram-Schmidt on K(box samples, candidates) in FP64, stopping when
-// the largest residual column norm falls below tol times the largest initial one.
+// by column-pivoted Gram-Schmidt on the column-normalised K(box samples, candidates) in
+// FP64, stopping when the largest residual column norm falls below tol.
 //
 // The sets depend only on (D, h, tolerance), not on the data, so they are computed once
 // per process for a fixed ladder of half-sides h_k = kProxyBase * 2^(k / kProxyQ) and
@@ -136,9 +136,19 @@ std::vector<double> make_set(int D, double h, double tol, double eta, int pmax, 
     const double r_out = kProxyOuter * h;
     const int ny = (D == 2) ? 1600 : 2400;
     const std::vector<double> Y = annulus_candidates(D, r_in, r_out, ny, seed);
+    // columns are normalised: every far direction is resolved to the same relative
+    // accuracy, not only the ones closest to the box (which dominate unscaled norms)
     std::vector<double> M((size_t)nx * ny);
-    for (int c = 0; c < ny; c++)
-        for (int r = 0; r < nx; r++) M[(size_t)c * nx + r] = kern(&X[(size_t)r * D], &Y[(size_t)c * D], D);
+    for (int c = 0; c < ny; c++) {
+        double s = 0.0;
+        for (int r = 0; r < nx; r++) {
+            const double v = kern(&X[(size_t)r * D], &Y[(size_t)c * D], D);
+            M[(size_t)c * nx + r] = v;
+            s += v * v;
+        }
+        const double inv = s > 0.0 ? 1.0 / std::sqrt(s) : 0.0;
+        for (int r = 0; r < nx; r++) M[(size_t)c * nx + r] *= inv;
+    }
     const std::vector<int> piv = select_columns(M, nx, ny, tol, pmax);
     std::vector<double> out;
     for (int c : piv)

@@ -5,9 +5,12 @@
 namespace tsne {
 
 // Ladder of node half-sides: h_k = kProxyBase * 2^(k / kProxyQ), k = 0..kProxyLevels-1.
-constexpr double kProxyBase = 1e-6;
-constexpr int kProxyQ = 2;
-constexpr int kProxyLevels = 2 * 34;  // up to ~1.7e4
+// The root box's half-side is rounded up onto the ladder, so every quadtree level
+// falls exactly on it (the reference rounds its root radius to minL * 2^k in the same
+// way, src/tsne_h2pack.c:679-690).
+constexpr double kProxyBase = 1.0 / (1 << 20);
+constexpr int kProxyQ = 1;
+constexpr int kProxyLevels = 48;  // half-sides 2^-20 .. 2^27
 // Outer max-norm radius of the proxy candidates, in units of the half-side.
 constexpr double kProxyOuter = 48.0;
 
