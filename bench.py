@@ -34,13 +34,10 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def make_inputs(n, dim_in, n_clusters, k, perplexity, seed=0):
-    """Seeded Gaussian mixture -> exact k-NN on the GPU (torch) -> symmetric P.
-    Returns (row_ptr, col_idx, values) int32/int32/float64 of the joint P."""
+def _torch_knn(n, dim_in, n_clusters, k, seed, out):
+    """Child process: seeded mixture -> exact k-NN on the GPU with torch.  torch ships its
+    own HIP runtime, so it runs apart from the library's process."""
     import torch
-    from scipy.sparse import csr_matrix
-
-    from tsne_h2pack._backend import get_backend
     rng = np.random.default_rng(seed)
     centers = rng.normal(scale=5.0, size=(n_clusters, dim_in))
     w = rng.dirichlet(np.ones(n_clusters))
@@ -66,36 +63,48 @@ def make_inputs(n, dim_in, n_clusters, k, perplexity, seed=0):
     torch.cuda.synchronize()
     idx = idx_all.cpu().numpy()
     dist = dst_all.cpu().numpy()
-    del Xt, idx_all, dst_all
-    # CSR with sorted columns (reference _backend.py:651-653)
+    # CSR rows with sorted columns (reference _backend.py:651-653)
     order = np.argsort(idx, axis=1, kind="stable")
-    idx = np.take_along_axis(idx, order, 1)
-    dist = np.take_along_axis(dist, order, 1).astype(np.float64)
+    np.save(out + "_idx.npy", np.take_along_axis(idx, order, 1))
+    np.save(out + "_dist.npy", np.take_along_axis(dist, order, 1))
+
+
+def _child(args):
+    import subprocess
+    r = subprocess.run([sys.executable, os.path.abspath(__file__)] + args)
+    if r.returncode != 0:
+        raise RuntimeError(f"input preparation step {args[0]} failed ({r.returncode})")
+
+
+def make_inputs(n, dim_in, n_clusters, k, perplexity, seed=0):
+    """Seeded Gaussian mixture -> exact k-NN (torch, child process) -> conditional P (the
+    library's perplexity kernel) -> symmetric P (torch, child process).
+    Returns (row_ptr, col_idx, values) int32/int32/float64 of the joint P."""
+    import tempfile
+    from tsne_h2pack._backend import get_backend
+    tmp = tempfile.mkdtemp(prefix="tsne_bench_")
+    out = os.path.join(tmp, "in")
+    t0 = time.time()
+    _child(["--_knn", out, str(n), str(dim_in), str(n_clusters), str(k), str(seed)])
+    log(f"k-NN graph: {time.time() - t0:.1f}s")
+    idx = np.load(out + "_idx.npy")
+    dist = np.load(out + "_dist.npy").astype(np.float64)
     indptr = np.arange(0, (n + 1) * k, k, dtype=np.int32)
-    be = get_backend()
     t0 = time.time()
-    Pc = be.conditional_P(np.ascontiguousarray(dist.ravel()),
-                          np.ascontiguousarray(idx.ravel().astype(np.int32)), indptr, perplexity)
-    log(f"conditional P on GPU: {time.time() - t0:.2f}s")
-    # symmetrise on the GPU (the same sums as scipy's P + P.T, then / total)
+    Pc = get_backend().conditional_P(np.ascontiguousarray(dist.ravel()),
+                                     np.ascontiguousarray(idx.ravel().astype(np.int32)), indptr,
+                                     perplexity)
+    log(f"conditional P on the GPU (library kernel): {time.time() - t0:.2f}s")
+    # symmetrise and normalise with scipy, as the reference does (_backend.py:709-718)
     t0 = time.time()
-    rows = torch.from_numpy(np.repeat(np.arange(n, dtype=np.int64), k)).to(dev)
-    cols = torch.from_numpy(Pc.indices.astype(np.int64)).to(dev)
-    vals = torch.from_numpy(Pc.data).to(dev)
-    S = torch.sparse_coo_tensor(torch.stack([torch.cat([rows, cols]), torch.cat([cols, rows])]),
-                                torch.cat([vals, vals]), (n, n)).coalesce()
-    v = S.values()
-    v = v / v.sum()
-    ij = S.indices()
-    counts = torch.bincount(ij[0], minlength=n)
-    row_ptr = torch.zeros(n + 1, dtype=torch.int64, device=dev)
-    row_ptr[1:] = torch.cumsum(counts, 0)
-    out = (row_ptr.cpu().numpy().astype(np.int32), ij[1].cpu().numpy().astype(np.int32),
-           v.cpu().numpy().astype(np.float64))
-    del S, rows, cols, vals, ij, v
-    torch.cuda.empty_cache()
-    log(f"symmetric P nnz={out[1].size} ({time.time() - t0:.2f}s)")
-    return out
+    P = Pc + Pc.T
+    P.data /= P.sum()
+    csr = (P.indptr.astype(np.int32), P.indices.astype(np.int32), P.data.astype(np.float64))
+    del Pc, P
+    log(f"symmetric P (scipy) nnz={csr[1].size} ({time.time() - t0:.1f}s)")
+    import shutil
+    shutil.rmtree(tmp, ignore_errors=True)
+    return csr
 
 
 def make_workspace(be, csr, n, dim, precision, n_iter, leaf, tol, seed=0):
@@ -214,6 +223,8 @@ def main():
     ap.add_argument("--tol", type=float, default=1e-2)
     ap.add_argument("--paths", default="mixed,plain")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--save-inputs", default="", help="write the prepared P here and exit")
+    ap.add_argument("--load-inputs", default="", help="read a P written by --save-inputs")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -223,7 +234,15 @@ def main():
     be = get_backend()
     n, dim = a.n, a.dim
     clusters = 100 if n >= 1_000_000 else 20
-    csr = make_inputs(n, 50, clusters, 90, 30.0)
+    if a.load_inputs:
+        z = np.load(a.load_inputs)
+        csr = (z["row_ptr"], z["col_idx"], z["values"])
+        assert csr[0].size == n + 1
+    else:
+        csr = make_inputs(n, 50, clusters, 90, 30.0)
+    if a.save_inputs:
+        np.savez(a.save_inputs, row_ptr=csr[0], col_idx=csr[1], values=csr[2])
+        return
     res = {}
     for path in a.paths.split(","):
         leaf = a.leaf_mixed if (path == "mixed" and a.leaf_mixed) else a.leaf
@@ -260,4 +279,8 @@ def main():
 
 
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "--_knn":
+        _torch_knn(int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]),
+                   int(sys.argv[7]), sys.argv[2])
+    else:
+        main()

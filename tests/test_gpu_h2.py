@@ -138,3 +138,61 @@ def test_h2_run_kl_within_one_percent(backend, problem):
     _say(f"  KL gpu {kl:.5f} oracle {kl_o:.5f}")
     assert abs(kl - kl_o) / kl_o < 0.01
     assert np.isfinite(Y).all()
+
+
+def _run(backend, P, Y0, n_iter, reorder):
+    import os
+    old = os.environ.get("TSNE_REORDER")
+    os.environ["TSNE_REORDER"] = "1" if reorder else "0"
+    try:
+        p = backend.create_default_params()
+        p.verbose = 0
+        p.n_iter = n_iter
+        p.h2_tol = 1e-2
+        p.h2_rebuild_freq = 3
+        p.h2_rebuild_freq_early = 10
+        p.learning_rate = max(P.shape[0] / 12.0, 200.0)
+        ws = backend.init_workspace(P, 2, p, init_Y=Y0.T.copy())
+    finally:
+        if old is None:
+            del os.environ["TSNE_REORDER"]
+        else:
+            os.environ["TSNE_REORDER"] = old
+    try:
+        assert backend.lib.tsne_run(ws) == 0
+        Y = np.zeros((2, P.shape[0]))
+        backend.lib.tsne_get_embedding(ws, Y)
+    finally:
+        backend.lib.tsne_destroy(ws)
+    return Y
+
+
+def test_renumbering_is_transparent(backend, problem):
+    """Reference tests/test_optimizations.py:77-95: the first renumbering happens at
+    iteration 30; the embedding comes back in the caller's order and matches a run
+    without renumbering up to float32 summation-order effects (a mis-tracked permutation
+    gives an O(1) difference)."""
+    _say("renumbering transparency")
+    _, _, P = problem
+    Y0 = oracle.init_random(P.shape[0], 2, 21)
+    ref = _run(backend, P, Y0, 34, False)
+    ren = _run(backend, P, Y0, 34, True)
+    rel = np.abs(ref - ren).max() / np.abs(ref).max()
+    _say(f"  rel diff {rel:.2e}")
+    assert rel < 1e-3
+
+
+def test_renumbering_keeps_structure(backend, problem):
+    """After all six renumberings the rows of the output still line up with the input:
+    10-NN overlap with the high-dimensional neighbours (reference
+    tests/test_optimizations.py:98-111)."""
+    from sklearn.neighbors import NearestNeighbors
+    X, _, P = problem
+    Y0 = oracle.init_random(P.shape[0], 2, 22)
+    Y = _run(backend, P, Y0, 1000, True).T
+    k = 10
+    hd = NearestNeighbors(n_neighbors=k).fit(X).kneighbors(return_distance=False)
+    ld = NearestNeighbors(n_neighbors=k).fit(Y).kneighbors(return_distance=False)
+    overlap = np.mean([len(set(a) & set(b)) for a, b in zip(hd, ld)]) / k
+    _say(f"  10-NN overlap {overlap:.3f}")
+    assert overlap > 0.15

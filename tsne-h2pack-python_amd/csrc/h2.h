@@ -29,4 +29,7 @@ int h2_usable(const H2 *h);
 
 int h2_stats(const H2 *h, tsne_h2_stats_t *st);
 
+// Device array (n): the point at Morton position k of the last build, i.e. the tree order.
+const int *h2_order(const H2 *h);
+
 }  // namespace tsne

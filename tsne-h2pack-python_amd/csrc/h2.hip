@@ -19,7 +19,7 @@ namespace tsne {
 
 struct H2 {
     int n = 0, D = 2, leaf = 100;
-    double tol = 1e-2, ptol = 3e-3;
+    double tol = 1e-2, ptol = 3e-3, id_tol = 5e-3;
     hipStream_t st = nullptr;
     H2Dev d{};
     // buffers not in H2Dev
@@ -225,10 +225,13 @@ int build_once(H2 *h, const void *Yg)
         sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end) ||
         sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end))
         return -1;
-    // ID + upward, finest level first
+    // ID + upward: every leaf at once (they need nothing from other nodes), then the
+    // internal nodes level by level, finest first
     const int nlev = (int)h->lvl.size() - 1;
+    if (h->n_nodes > 1 && launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, st)) return -1;
     for (int lv = nlev - 1; lv >= 1; lv--)
-        if (launch_id_up(D, d, h->lvl[lv], h->lvl[lv + 1], lv, h->tol, st)) return -1;
+        if (launch_id_up(D, d, nullptr, h->lvl[lv], h->lvl[lv + 1] - h->lvl[lv], h->id_tol, st))
+            return -1;
     if (hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
         hipStreamSynchronize(st) != hipSuccess)
         return -1;
@@ -251,7 +254,10 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
     h->tol = tol;
     // proxy selection a little tighter than the ID tolerance, clamped as the reference
     // clamps its proxy tolerance to [1e-6, 1e-2] (src/tsne_h2pack.c:966-977)
-    h->ptol = std::min(1e-2, std::max(1e-6, 0.3 * tol));
+    h->ptol = std::min(1e-2, std::max(1e-6, 0.15 * tol));
+    // the node IDs run at half the requested tolerance: the errors of the nested bases
+    // add up over the levels, and the gradient error is what h2_tol bounds
+    h->id_tol = 0.5 * tol;
     h->st = st;
     H2Dev &d = h->d;
     d.n = n;
@@ -334,6 +340,8 @@ int h2_product(H2 *h, const void *Yg, float *s, float *F, int mixed)
 }
 
 int h2_usable(const H2 *h) { return h->usable; }
+
+const int *h2_order(const H2 *h) { return h->d.perm; }
 
 int h2_stats(const H2 *h, tsne_h2_stats_t *st)
 {

@@ -31,7 +31,9 @@ __device__ __forceinline__ double kern64(const double *a, const double *b, int D
         const double df = a[d] - b[d];
         r2 += df * df;
     }
-    const double w = 1.0 / r2;
+    // hardware reciprocal refined by one Newton step (full FP64 accuracy, no IEEE divide)
+    double w = __builtin_amdgcn_rcp(r2);
+    w = fma(w, fma(-r2, w, 1.0), w);
     return w * w;
 }
 
@@ -39,7 +41,7 @@ __device__ __forceinline__ double kern64(const double *a, const double *b, int D
 
 template <int D, int NT>
 __global__ void __launch_bounds__(NT)
-id_up_kernel(H2Dev h, int lb, int l, double tol)
+id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, double tol)
 {
     constexpr int PMAX = H2Cfg<D>::PMAX, RMAX = H2Cfg<D>::RMAX;
     constexpr int NW = NT / 64;
@@ -59,7 +61,9 @@ id_up_kernel(H2Dev h, int lb, int l, double tol)
     double(*s_Q)[PMAX] = reinterpret_cast<double(*)[PMAX]>(s_qu);
     float(*s_U)[RMAX] = reinterpret_cast<float(*)[RMAX]>(s_qu);
 
-    const int node = lb + blockIdx.x;
+    const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
+    if (internal_only && h.child0[node] < 0) return;  // leaves were done in their own pass
+    const int l = h.level[node];
     const int t = threadIdx.x;
     const int lane = t & 63, wid = t >> 6;
     const float4 c = h.cent[node];
@@ -265,14 +269,19 @@ id_up_kernel(H2Dev h, int lb, int l, double tol)
     }
 }
 
-int launch_id_up(int D, const H2Dev &h, int lb, int le, int l, double tol, hipStream_t st)
+// list != nullptr: the nodes list[0..count) (all leaves, any level); otherwise the
+// internal nodes among [lb, lb + count).
+int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
+                 hipStream_t st)
 {
-    const int g = le - lb;
-    if (g <= 0) return 0;
+    if (count <= 0) return 0;
+    const int internal_only = list ? 0 : 1;
     if (D == 2)
-        hipLaunchKernelGGL((id_up_kernel<2, 128>), dim3(g), dim3(128), 0, st, h, lb, l, tol);
+        hipLaunchKernelGGL((id_up_kernel<2, 128>), dim3(count), dim3(128), 0, st, h, list, lb,
+                           internal_only, tol);
     else
-        hipLaunchKernelGGL((id_up_kernel<3, 384>), dim3(g), dim3(384), 0, st, h, lb, l, tol);
+        hipLaunchKernelGGL((id_up_kernel<3, 384>), dim3(count), dim3(384), 0, st, h, list, lb,
+                           internal_only, tol);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

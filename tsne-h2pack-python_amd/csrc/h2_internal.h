@@ -115,7 +115,8 @@ int launch_pending(int D, const H2Dev &h, int n_pend, int pend_in, int pend_out,
 int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, hipStream_t st);
 
 // h2_id.hip
-int launch_id_up(int D, const H2Dev &h, int lb, int le, int l, double tol, hipStream_t st);
+int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
+                 hipStream_t st);
 
 // h2_product.hip
 int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, hipStream_t st);

@@ -21,10 +21,25 @@ __device__ __forceinline__ float vcomp(const float4 &v, int d)
     return d == 0 ? v.x : (d == 1 ? v.y : v.z);
 }
 
-__device__ __forceinline__ void push_pair(uint64_t *buf, int *counter, int cap, int t, int s,
-                                          int *flags)
+// Slots for `count` entries of this lane in a buffer shared through `counter`: one
+// atomic per wavefront (all 64 lanes must call it), lanes get consecutive ranges.
+__device__ __forceinline__ int wave_alloc(int *counter, int count)
 {
-    const int k = atomicAdd(counter, 1);
+    const int lane = threadIdx.x & 63;
+    int x = count;
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    const int total = __shfl(x, 63, 64);
+    int base = 0;
+    if (lane == 63 && total > 0) base = atomicAdd(counter, total);
+    base = __shfl(base, 63, 64);
+    return base + x - count;
+}
+
+__device__ __forceinline__ void put_pair(uint64_t *buf, int k, int cap, int t, int s, int *flags)
+{
     if (k < cap)
         buf[k] = ((uint64_t)(uint32_t)t << 32) | (uint32_t)s;
     else
@@ -105,17 +120,31 @@ bbox_partial_kernel(const VT<D> *__restrict__ Y, int n, int per, double *__restr
 }
 
 template <int D>
-__global__ void bbox_final_kernel(int nblk, const double *__restrict__ part, H2Dev h)
+__global__ void __launch_bounds__(256)
+bbox_final_kernel(int nblk, const double *__restrict__ part, H2Dev h)
 {
-    if (threadIdx.x != 0) return;
+    __shared__ double slo[256][3], shi[256][3];
+    __shared__ int sbad[256];
+    const int t = threadIdx.x;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     int bad = 0;
-    for (int b = 0; b < nblk; b++) {
+    for (int b = t; b < nblk; b += 256) {
         for (int d = 0; d < D; d++) {
             lo[d] = fmin(lo[d], part[b * 7 + d]);
             hi[d] = fmax(hi[d], part[b * 7 + 3 + d]);
         }
         bad |= part[b * 7 + 6] != 0.0;
+    }
+    for (int d = 0; d < 3; d++) { slo[t][d] = lo[d]; shi[t][d] = hi[d]; }
+    sbad[t] = bad;
+    __syncthreads();
+    if (t != 0) return;
+    for (int k = 1; k < 256; k++) {
+        for (int d = 0; d < D; d++) {
+            lo[d] = fmin(lo[d], slo[k][d]);
+            hi[d] = fmax(hi[d], shi[k][d]);
+        }
+        bad |= sbad[k];
     }
     RootInfo r;
     double ext = 0.0;
@@ -142,11 +171,11 @@ int launch_root_box(int D, const void *Yg, int n, const H2Dev &h, double *partia
     if (D == 2) {
         hipLaunchKernelGGL(bbox_partial_kernel<2>, dim3(nblk), dim3(256), 0, st,
                            (const float2 *)Yg, n, per, partial);
-        hipLaunchKernelGGL(bbox_final_kernel<2>, dim3(1), dim3(64), 0, st, nblk, partial, h);
+        hipLaunchKernelGGL(bbox_final_kernel<2>, dim3(1), dim3(256), 0, st, nblk, partial, h);
     } else {
         hipLaunchKernelGGL(bbox_partial_kernel<3>, dim3(nblk), dim3(256), 0, st,
                            (const float4 *)Yg, n, per, partial);
-        hipLaunchKernelGGL(bbox_final_kernel<3>, dim3(1), dim3(64), 0, st, nblk, partial, h);
+        hipLaunchKernelGGL(bbox_final_kernel<3>, dim3(1), dim3(256), 0, st, nblk, partial, h);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
@@ -343,11 +372,15 @@ lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
 {
     constexpr int B = H2Cfg<D>::B, NEAR_CAP = H2Cfg<D>::NEAR_CAP;
     const int i = lb + blockIdx.x * 256 + threadIdx.x;
-    if (i >= le) return;
-    const int p = h.parent[i];
-    const int np = h.near_cnt[p];
-    int *my = h.near + (size_t)i * NEAR_CAP;
-    int cnt = 0;
+    const bool valid = i < le;
+    int n_adm = 0, n_x = 0, cnt = 0;
+    int *my = h.near + (size_t)(valid ? i : 0) * NEAR_CAP;
+    int p = 0, np = 0;
+    if (valid) {
+        p = h.parent[i];
+        np = h.near_cnt[p];
+    }
+    // pass 1: own near list, counts of ADM / X
     for (int a = 0; a < np; a++) {
         const int j = h.near[(size_t)p * NEAR_CAP + a];
         const bool jl = h.child0[j] < 0;
@@ -356,12 +389,8 @@ lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
         for (int q = 0; q < nc; q++) {
             const int c = c0 + q;
             const int lc = h.level[c];
-            const int gap = box_gap<D>(h.icoord, i, l, c, lc);
-            if (gap >= 1 && c != i) {
-                if (lc == l)
-                    push_pair(h.adm, &h.cnt[CNT_ADM], h.adm_cap, i, c, h.flags);
-                else
-                    push_pair(h.xl, &h.cnt[CNT_X], h.x_cap, i, c, h.flags);
+            if (box_gap<D>(h.icoord, i, l, c, lc) >= 1) {
+                if (lc == l) n_adm++; else n_x++;
             } else {
                 if (cnt < NEAR_CAP) my[cnt] = c;
                 else atomicOr(&h.flags[FLAG_NEAR_OVF], 1);
@@ -370,14 +399,45 @@ lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
         }
     }
     cnt = min(cnt, NEAR_CAP);
-    h.near_cnt[i] = cnt;
-    if ((h.ne[i] - h.nb[i]) <= h.leaf || l >= B) {
+    const bool leaf = valid && ((h.ne[i] - h.nb[i]) <= h.leaf || l >= B);
+    int n_near = 0, n_pend = 0;
+    if (valid) {
+        h.near_cnt[i] = cnt;
+        if (leaf)
+            for (int a = 0; a < cnt; a++) {
+                const int c = my[a];
+                if ((h.ne[c] - h.nb[c]) <= h.leaf || h.level[c] >= B) n_near++; else n_pend++;
+            }
+    }
+    int k_adm = wave_alloc(&h.cnt[CNT_ADM], n_adm);
+    int k_x = wave_alloc(&h.cnt[CNT_X], n_x);
+    int k_near = wave_alloc(&h.cnt[CNT_NEAR], n_near);
+    int k_pend = wave_alloc(&h.cnt[CNT_PEND0 + pend_out], n_pend);
+    if (!valid) return;
+    // pass 2: write the pairs
+    if (n_adm + n_x > 0) {
+        for (int a = 0; a < np; a++) {
+            const int j = h.near[(size_t)p * NEAR_CAP + a];
+            const bool jl = h.child0[j] < 0;
+            const int c0 = jl ? j : h.child0[j];
+            const int nc = jl ? 1 : h.nchild[j];
+            for (int q = 0; q < nc; q++) {
+                const int c = c0 + q;
+                const int lc = h.level[c];
+                if (box_gap<D>(h.icoord, i, l, c, lc) >= 1) {
+                    if (lc == l) put_pair(h.adm, k_adm++, h.adm_cap, i, c, h.flags);
+                    else put_pair(h.xl, k_x++, h.x_cap, i, c, h.flags);
+                }
+            }
+        }
+    }
+    if (leaf) {
         for (int a = 0; a < cnt; a++) {
             const int c = my[a];
             if ((h.ne[c] - h.nb[c]) <= h.leaf || h.level[c] >= B)
-                push_pair(h.nl, &h.cnt[CNT_NEAR], h.nl_cap, i, c, h.flags);
+                put_pair(h.nl, k_near++, h.nl_cap, i, c, h.flags);
             else
-                push_pair(h.pend[pend_out], &h.cnt[CNT_PEND0 + pend_out], h.pend_cap, i, c, h.flags);
+                put_pair(h.pend[pend_out], k_pend++, h.pend_cap, i, c, h.flags);
         }
     }
 }
@@ -390,19 +450,36 @@ pending_kernel(H2Dev h, int n_pend, int pend_in, int pend_out)
 {
     constexpr int B = H2Cfg<D>::B;
     const int k = blockIdx.x * 256 + threadIdx.x;
-    if (k >= n_pend) return;
-    const uint64_t pr = h.pend[pend_in][k];
-    const int i = (int)(pr >> 32), c = (int)(pr & 0xffffffffu);
-    const int li = h.level[i];
-    for (int q = 0; q < h.nchild[c]; q++) {
-        const int d = h.child0[c] + q;
+    const bool valid = k < n_pend;
+    int i = 0, c = 0, li = 0, nc = 0, c0 = 0;
+    if (valid) {
+        const uint64_t pr = h.pend[pend_in][k];
+        i = (int)(pr >> 32);
+        c = (int)(pr & 0xffffffffu);
+        li = h.level[i];
+        nc = h.nchild[c];
+        c0 = h.child0[c];
+    }
+    int nw = 0, nn = 0, np = 0;
+    for (int q = 0; q < nc; q++) {
+        const int d = c0 + q;
+        const int ld = h.level[d];
+        if (box_gap<D>(h.icoord, i, li, d, ld) >= 1) nw++;
+        else if ((h.ne[d] - h.nb[d]) <= h.leaf || ld >= B) nn++;
+        else np++;
+    }
+    int kw = wave_alloc(&h.cnt[CNT_W], nw);
+    int kn = wave_alloc(&h.cnt[CNT_NEAR], nn);
+    int kp = wave_alloc(&h.cnt[CNT_PEND0 + pend_out], np);
+    for (int q = 0; q < nc; q++) {
+        const int d = c0 + q;
         const int ld = h.level[d];
         if (box_gap<D>(h.icoord, i, li, d, ld) >= 1)
-            push_pair(h.wl, &h.cnt[CNT_W], h.w_cap, i, d, h.flags);
+            put_pair(h.wl, kw++, h.w_cap, i, d, h.flags);
         else if ((h.ne[d] - h.nb[d]) <= h.leaf || ld >= B)
-            push_pair(h.nl, &h.cnt[CNT_NEAR], h.nl_cap, i, d, h.flags);
+            put_pair(h.nl, kn++, h.nl_cap, i, d, h.flags);
         else
-            push_pair(h.pend[pend_out], &h.cnt[CNT_PEND0 + pend_out], h.pend_cap, i, d, h.flags);
+            put_pair(h.pend[pend_out], kp++, h.pend_cap, i, d, h.flags);
     }
 }
 

@@ -24,19 +24,22 @@ namespace tsne {
 // Reference: attractive force src/tsne_h2pack.c:199-356 (2D float32 gather,
 // src/tsne_attr_2d.h:22-67); update src/tsne_h2pack.c:1944-1968; centring :1978-1984.
 //
-// One wavefront per CSR row.  Lanes stride over the row's nonzeros, gather the
-// neighbour from the float32 copy Yg_cur and accumulate p_ij w_ij (y_i - y_j) in
-// float32; a wave reduction gives the row's force.  Lanes 0..D-1 then apply the
-// update to coordinate d of the row in FP64:
+// A group of 16 lanes per CSR row (4 rows per wavefront): lanes stride over the row's
+// nonzeros, gather the neighbour from the float32 copy Yg and accumulate
+// p_ij w_ij (y_i - y_j) in float32; a 16-lane reduction gives the row's force.  Lanes
+// 0..D-1 of the group then apply the update to coordinate d of the row in FP64:
 //     g      = 4*exag*attr_d - (4/Z) F_d            (F: reused or fresh repulsive sum)
 //     gains  = (uY*g < 0) ? gains + 0.2 : gains * 0.8, floored at 0.01
 //     uY     = mom*uY - lr*gains*g
 //     y      = (Y64 - mean_prev) + uY
-// and write Y64, uY, gains and the next gather copy Yg_next.  mean_prev is the
-// mean removed lazily: the forces are translation invariant, so centring is applied
-// when Y64 is next read instead of in an extra pass.  Each block also writes the
-// sum of its new coordinates, in a fixed order, for the next mean.
+// and write Y64, uY, gains and the next gather copy Yg_next.  mean_prev is the mean
+// removed lazily: the forces are translation invariant, so centring is applied when Y64
+// is next read instead of in an extra pass.  Each block covers a fixed contiguous row
+// range and writes the sum of its new coordinates in a fixed order, for the next mean.
 // ---------------------------------------------------------------------------------
+constexpr int kAttrLanes = 16;
+constexpr int kAttrGroups = 256 / kAttrLanes;  // rows in flight per block
+
 template <int D>
 __global__ void __launch_bounds__(256)
 attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
@@ -48,19 +51,20 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
                    double *__restrict__ block_sums)
 {
     using V = typename Vec<D>::T;
-    __shared__ double wsum[4][3];
-    const int lane = threadIdx.x & 63;
-    const int wid = threadIdx.x >> 6;
+    __shared__ double gsum[kAttrGroups][3];
+    const int sub = threadIdx.x & (kAttrLanes - 1);
+    const int grp = threadIdx.x / kAttrLanes;
     const int row0 = blockIdx.x * rows_per_block;
     const int row1 = min(n, row0 + rows_per_block);
     const double rep_scale = 4.0 / scal[SLOT_Z];
-    double my_sum = 0.0;  // lane d: sum of new coordinate d over this wave's rows
+    const double m_d = (sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
+    double my_sum = 0.0;  // lane d of the group: sum of new coordinate d over its rows
 
-    for (int i = row0 + wid; i < row1; i += 4) {
+    for (int i = row0 + grp; i < row1; i += kAttrGroups) {
         const V yi = Yg[i];
         float ax = 0.f, ay = 0.f, az = 0.f;
         const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
-        for (int e = e0 + lane; e < e1; e += 64) {
+        for (int e = e0 + sub; e < e1; e += kAttrLanes) {
             const int j = col_idx[e];
             const float p = pval[e];
             const V yj = Yg[j];
@@ -73,29 +77,32 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
             ay = fmaf(q, dy, ay);
             if constexpr (D == 3) az = fmaf(q, dz, az);
         }
-        ax = wave_sum(ax);
-        ay = wave_sum(ay);
-        if constexpr (D == 3) az = wave_sum(az);
+        for (int off = kAttrLanes / 2; off > 0; off >>= 1) {
+            ax += __shfl_xor(ax, off, 64);
+            ay += __shfl_xor(ay, off, 64);
+            if constexpr (D == 3) az += __shfl_xor(az, off, 64);
+        }
         double ynew = 0.0;
-        if (lane < D) {
-            const float a = (lane == 0) ? ax : (lane == 1) ? ay : az;
-            const size_t k = (size_t)i * D + lane;
+        if (sub < D) {
+            const float a = (sub == 0) ? ax : (sub == 1) ? ay : az;
+            const size_t k = (size_t)i * D + sub;
             const double g = exag4 * (double)a - rep_scale * (double)F[k];
             double u = uY[k], gn = gains[k];
             gn = (u * g < 0.0) ? gn + 0.2 : gn * 0.8;
             if (gn < 0.01) gn = 0.01;
             u = mom * u - lr * (gn * g);
-            ynew = (Y64[k] - scal[SLOT_MEAN + lane]) + u;
+            ynew = (Y64[k] - m_d) + u;
             Y64[k] = ynew;
             uY[k] = u;
             gains[k] = gn;
             my_sum += ynew;
         }
-        // the new coordinates sit in lanes 0..D-1: move them to lane 0 for the record
-        const double n0 = __shfl(ynew, 0, 64);
-        const double n1 = __shfl(ynew, 1, 64);
-        const double n2 = (D == 3) ? __shfl(ynew, 2, 64) : 0.0;
-        if (lane == 0) {
+        // the new coordinates sit in lanes 0..D-1 of the group: assemble the record
+        const int base = threadIdx.x & ~(kAttrLanes - 1);
+        const double n0 = __shfl(ynew, base, 64);
+        const double n1 = __shfl(ynew, base + 1, 64);
+        const double n2 = (D == 3) ? __shfl(ynew, base + 2, 64) : 0.0;
+        if (sub == 0) {
             V rec;
             rec.x = (float)n0;
             rec.y = (float)n1;
@@ -103,12 +110,12 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
             Yg_next[i] = rec;
         }
     }
-    // per-wave sums (lane d holds coordinate d), then waves in order
-    if (lane < D) wsum[wid][lane] = my_sum;
+    // per-group sums (lane d holds coordinate d), then groups in order
+    if (sub < D) gsum[grp][sub] = my_sum;
     __syncthreads();
     if (threadIdx.x < D) {
         double t = 0.0;
-        for (int w = 0; w < 4; w++) t += wsum[w][threadIdx.x];
+        for (int w = 0; w < kAttrGroups; w++) t += gsum[w][threadIdx.x];
         block_sums[(size_t)blockIdx.x * 3 + threadIdx.x] = t;
     }
 }
@@ -218,16 +225,25 @@ z_partial_kernel(int n, const typename Vec<D>::T *__restrict__ Yg, const float *
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-// Stage 2: partials in order -> Z.  A non-positive or non-finite Z is replaced by 1
-// and flagged, as the reference returns 1.0 and continues (src/tsne_h2pack.c:604-611).
-__global__ void z_finalize_kernel(int nblocks, const double *__restrict__ partial,
-                                  double *__restrict__ scal)
+// Stage 2: partials in order -> Z (256 threads take contiguous chunks, then the chunk
+// sums are added in order).  A non-positive or non-finite Z is replaced by 1 and
+// flagged, as the reference returns 1.0 and continues (src/tsne_h2pack.c:604-611).
+__global__ void __launch_bounds__(256)
+z_finalize_kernel(int nblocks, const double *__restrict__ partial, double *__restrict__ scal)
 {
-    if (threadIdx.x != 0) return;
-    double z = 0.0;
-    for (int b = 0; b < nblocks; b++) z += partial[b];
-    if (!(z > 0.0) || !isfinite(z)) { scal[SLOT_ZBAD] = 1.0; z = 1.0; }
-    scal[SLOT_Z] = z;
+    __shared__ double part[256];
+    const int t = threadIdx.x;
+    const int per = (nblocks + 255) / 256;
+    double a = 0.0;
+    for (int b = t * per; b < min(nblocks, (t + 1) * per); b++) a += partial[b];
+    part[t] = a;
+    __syncthreads();
+    if (t == 0) {
+        double z = 0.0;
+        for (int k = 0; k < 256; k++) z += part[k];
+        if (!(z > 0.0) || !isfinite(z)) { scal[SLOT_ZBAD] = 1.0; z = 1.0; }
+        scal[SLOT_Z] = z;
+    }
 }
 
 // Copy the pending mean into the slot the Z pass subtracts (taken when the
@@ -286,13 +302,21 @@ sum_blocks_kernel(int n, const double *__restrict__ x, int per_block, double *__
     if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
 }
 
-__global__ void sum_finalize_kernel(int nblocks, const double *__restrict__ partial,
-                                    double *__restrict__ dst)
+__global__ void __launch_bounds__(256)
+sum_finalize_kernel(int nblocks, const double *__restrict__ partial, double *__restrict__ dst)
 {
-    if (threadIdx.x != 0) return;
-    double z = 0.0;
-    for (int b = 0; b < nblocks; b++) z += partial[b];
-    *dst = z;
+    __shared__ double part[256];
+    const int t = threadIdx.x;
+    const int per = (nblocks + 255) / 256;
+    double a = 0.0;
+    for (int b = t * per; b < min(nblocks, (t + 1) * per); b++) a += partial[b];
+    part[t] = a;
+    __syncthreads();
+    if (t == 0) {
+        double z = 0.0;
+        for (int k = 0; k < 256; k++) z += part[k];
+        *dst = z;
+    }
 }
 
 // ---------------------------------------------------------------------------------
@@ -453,7 +477,7 @@ int launch_z(int D, int n, const void *Yg, const float *s, const float *F, doubl
     else
         hipLaunchKernelGGL(z_partial_kernel<3>, dim3(nblocks), dim3(256), 0, st, n,
                            (const float4 *)Yg, s, F, scal, per, partial);
-    hipLaunchKernelGGL(z_finalize_kernel, dim3(1), dim3(64), 0, st, nblocks, partial, scal);
+    hipLaunchKernelGGL(z_finalize_kernel, dim3(1), dim3(256), 0, st, nblocks, partial, scal);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -476,7 +500,7 @@ int launch_kl(int D, int n, const int *row_ptr, const int *col_idx, const double
                            pval64, Y64, scal, exag, row_kl);
     const int per = (n + nblocks - 1) / nblocks;
     hipLaunchKernelGGL(sum_blocks_kernel, dim3(nblocks), dim3(256), 0, st, n, row_kl, per, partial);
-    hipLaunchKernelGGL(sum_finalize_kernel, dim3(1), dim3(64), 0, st, nblocks, partial, dst);
+    hipLaunchKernelGGL(sum_finalize_kernel, dim3(1), dim3(256), 0, st, nblocks, partial, dst);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -503,7 +527,7 @@ int launch_z_f64(int D, int n, const double *Y64, const double *s, const double 
     else
         hipLaunchKernelGGL(z_partial_f64_kernel<3>, dim3(nblocks), dim3(256), 0, st, n, Y64, s,
                            F, scal, per, partial);
-    hipLaunchKernelGGL(sum_finalize_kernel, dim3(1), dim3(64), 0, st, nblocks, partial, dst);
+    hipLaunchKernelGGL(sum_finalize_kernel, dim3(1), dim3(256), 0, st, nblocks, partial, dst);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -12,6 +12,7 @@
 #include "common.h"
 #include "h2.h"
 #include "kernels.h"
+#include "renumber.h"
 #include "../../include/tsne_h2pack.h"
 
 using namespace tsne;
@@ -48,6 +49,15 @@ struct tsne_workspace {
     int timing = 0;
     double knn_time = 0, p_time = 0;
     std::vector<double> prof;  // per-iteration phase times when profiling
+    // renumbering along the tree order (reference src/tsne_h2pack.c:1706-1791)
+    std::vector<int> order;     // current point k is the caller's point order[k] (host)
+    int *d_order = nullptr;     // same on the device
+    std::vector<int> reorder_iters;
+    int next_reorder = 0;
+    int n_renumber = 0;
+    RenumberCsr P2;             // second CSR buffer set, alive while renumberings remain
+    RenumberScratch rs;
+    void *d_tmp = nullptr;      // n * 4 doubles: permutation staging
 };
 
 namespace {
@@ -105,8 +115,19 @@ void init_random(double *Y, int n, int dim, unsigned seed)
     }
 }
 
+void free_renumber_buffers(tsne_workspace *ws)
+{
+    dfree(ws->P2.rp); dfree(ws->P2.col); dfree(ws->P2.v32); dfree(ws->P2.v64);
+    dfree(ws->rs.inv); dfree(ws->rs.len); dfree(ws->rs.src); dfree(ws->rs.src_sorted);
+    dfree(ws->rs.col_tmp);
+    if (ws->rs.tmp) { (void)hipFree(ws->rs.tmp); ws->rs.tmp = nullptr; }
+    if (ws->d_tmp) { (void)hipFree(ws->d_tmp); ws->d_tmp = nullptr; }
+}
+
 void free_device(tsne_workspace *ws)
 {
+    free_renumber_buffers(ws);
+    dfree(ws->d_order);
     if (ws->h2) { h2_destroy(ws->h2); ws->h2 = nullptr; }
     dfree(ws->d_rowptr); dfree(ws->d_col); dfree(ws->d_val32); dfree(ws->d_val64);
     dfree(ws->d_Y64); dfree(ws->d_uY); dfree(ws->d_gains);
@@ -123,11 +144,13 @@ int upload_embedding(tsne_workspace *ws, const double *Ydn)
     std::vector<double> Yi((size_t)n * D);
     std::vector<float> g((size_t)n * (D == 2 ? 2 : 4), 0.f);
     const int gs = (D == 2) ? 2 : 4;
-    for (int i = 0; i < n; i++)
+    for (int i = 0; i < n; i++) {
+        const int o = ws->order.empty() ? i : ws->order[i];  // current point i = caller's o
         for (int d = 0; d < D; d++) {
-            Yi[(size_t)i * D + d] = Ydn[(size_t)d * n + i];
-            g[(size_t)i * gs + d] = (float)Ydn[(size_t)d * n + i];
+            Yi[(size_t)i * D + d] = Ydn[(size_t)d * n + o];
+            g[(size_t)i * gs + d] = (float)Ydn[(size_t)d * n + o];
         }
+    }
     TSNE_HIP_CHECK(hipMemcpyAsync(ws->d_Y64, Yi.data(), Yi.size() * sizeof(double),
                                   hipMemcpyHostToDevice, ws->st));
     TSNE_HIP_CHECK(hipMemcpyAsync(ws->d_Yg[ws->cur], g.data(), g.size() * sizeof(float),
@@ -172,6 +195,71 @@ int rebuild(tsne_workspace *ws)
     }
     ws->h2_ok = h2_usable(ws->h2);
     return 0;
+}
+
+// Renumber the points along the tree order of the last H2 build: new point k is the
+// current point ord[k].  P, every per-point array and the caller order are permuted
+// (reference tsne_renumber_points, src/tsne_reorder.c:112-167).  The caller rebuilds the
+// H2 structure right after.
+int renumber(tsne_workspace *ws)
+{
+    const int n = ws->n, D = ws->dim, nnz = ws->nnz;
+    const int *ord = h2_order(ws->h2);
+    hipStream_t st = ws->st;
+    if (!ws->P2.rp) {
+        ws->P2.n = n;
+        ws->P2.nnz = nnz;
+        ws->rs.tmp_bytes = renumber_tmp_bytes(n, nnz);
+        const bool ok = dalloc(&ws->P2.rp, (size_t)n + 1) == 0 && dalloc(&ws->P2.col, nnz) == 0 &&
+                        dalloc(&ws->P2.v32, nnz) == 0 && dalloc(&ws->P2.v64, nnz) == 0 &&
+                        dalloc(&ws->rs.inv, n) == 0 && dalloc(&ws->rs.len, n) == 0 &&
+                        dalloc(&ws->rs.src, nnz) == 0 && dalloc(&ws->rs.src_sorted, nnz) == 0 &&
+                        dalloc(&ws->rs.col_tmp, nnz) == 0 &&
+                        hipMalloc(&ws->rs.tmp, ws->rs.tmp_bytes) == hipSuccess &&
+                        hipMalloc(&ws->d_tmp, (size_t)n * 4 * sizeof(double)) == hipSuccess;
+        if (!ok) {
+            free_renumber_buffers(ws);
+            fprintf(stderr, "WARNING: point renumbering skipped (out of device memory)\n");
+            return 1;
+        }
+    }
+    RenumberCsr cur;
+    cur.n = n;
+    cur.nnz = nnz;
+    cur.rp = ws->d_rowptr;
+    cur.col = ws->d_col;
+    cur.v32 = ws->d_val32;
+    cur.v64 = ws->d_val64;
+    if (renumber_csr(cur, ord, ws->P2, ws->rs, st) != 0) return -1;
+    std::swap(ws->d_rowptr, ws->P2.rp);
+    std::swap(ws->d_col, ws->P2.col);
+    std::swap(ws->d_val32, ws->P2.v32);
+    std::swap(ws->d_val64, ws->P2.v64);
+    // per-point state: permute into the staging buffer, copy back
+    struct Arr { void *p; int width, bytes; };
+    const Arr arrs[] = {{ws->d_Y64, D, 8}, {ws->d_uY, D, 8}, {ws->d_gains, D, 8},
+                        {ws->d_Yg[ws->cur], D == 2 ? 2 : 4, 4}, {ws->d_F, D, 4},
+                        {ws->d_order, 1, 4}};
+    for (const Arr &a : arrs) {
+        if (permute_points(n, a.width, ord, a.p, ws->d_tmp, a.bytes, st) != 0) return -1;
+        TSNE_HIP_CHECK(hipMemcpyAsync(a.p, ws->d_tmp, (size_t)n * a.width * a.bytes,
+                                      hipMemcpyDeviceToDevice, st));
+    }
+    // host copy of the caller order
+    std::vector<int> ho(n);
+    TSNE_HIP_CHECK(hipMemcpyAsync(ho.data(), ws->d_order, sizeof(int) * n, hipMemcpyDeviceToHost, st));
+    TSNE_HIP_CHECK(hipStreamSynchronize(st));
+    ws->order.swap(ho);
+    ws->n_renumber++;
+    return 0;
+}
+
+// Current point index of each caller point.
+std::vector<int> inverse_order(const tsne_workspace *ws)
+{
+    std::vector<int> inv(ws->n);
+    for (int k = 0; k < ws->n; k++) inv[ws->order[k]] = k;
+    return inv;
 }
 
 double read_scalar(tsne_workspace *ws, int slot)
@@ -347,7 +435,7 @@ tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int
     if (prec && strcmp(prec, "mixed") == 0) ws->precision = TSNE_H2_MIXED;
 
     const int n = n_point, D = low_dim;
-    ws->nblocks_upd = std::max(1, std::min(2048, (n + 63) / 64));
+    ws->nblocks_upd = std::max(1, (n + 127) / 128);  // 128 rows per attractive block
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
     bool ok = hipStreamCreate(&ws->st) == hipSuccess;
     if (!ok) {
@@ -383,6 +471,27 @@ tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int
              hipMemsetAsync(ws->d_F, 0, (size_t)n * D * sizeof(float), ws->st) == hipSuccess &&
              hipMemsetAsync(ws->d_Yg[1], 0, (size_t)n * kGatherBytes[D], ws->st) == hipSuccess &&
              hipStreamSynchronize(ws->st) == hipSuccess;
+    }
+    if (ok) {
+        ws->order.resize(n);
+        for (int i = 0; i < n; i++) ws->order[i] = i;
+        ok = dalloc(&ws->d_order, n) == 0 &&
+             hipMemcpyAsync(ws->d_order, ws->order.data(), sizeof(int) * n, hipMemcpyHostToDevice,
+                            ws->st) == hipSuccess;
+        // renumbering schedule: src/tsne_h2pack.c:1706-1717 (TSNE_REORDER, TSNE_REORDER_ITERS)
+        const char *ro = getenv("TSNE_REORDER");
+        if (!(ro && ro[0] == '0')) {
+            ws->reorder_iters = {30, 100, 250, 400, 600, 800};
+            const char *ri = getenv("TSNE_REORDER_ITERS");
+            if (ri && *ri) {
+                ws->reorder_iters.clear();
+                for (const char *c = ri; *c && ws->reorder_iters.size() < 16;) {
+                    ws->reorder_iters.push_back(atoi(c));
+                    while (*c && *c != ',') c++;
+                    if (*c == ',') c++;
+                }
+            }
+        }
     }
     if (ok) {
         // initial embedding, src/tsne_utils.c:1111-1150
@@ -451,6 +560,14 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         if (timing) (void)hipEventRecord(ev[0], ws->st);
         int rebuilt = (it == 0);
         if (!direct && rebuild_iter(ws, it, reuse)) {
+            const int nro = (int)ws->reorder_iters.size();
+            if (ws->next_reorder < nro && it >= ws->reorder_iters[ws->next_reorder] && ws->h2 &&
+                ws->h2_ok) {
+                while (ws->next_reorder < nro && it >= ws->reorder_iters[ws->next_reorder])
+                    ws->next_reorder++;
+                if (renumber(ws) < 0) return -1;
+                if (ws->next_reorder >= nro) free_renumber_buffers(ws);  // last one
+            }
             if (rebuild(ws) != 0) {
                 fprintf(stderr, "ERROR: Failed to rebuild H2 matrix at iteration %d\n", it);
                 return -1;
@@ -563,7 +680,7 @@ void tsne_get_embedding(tsne_workspace_t *ws, double *Y_out)
     }
     for (int d = 0; d < D; d++)
         for (int i = 0; i < n; i++)
-            Y_out[(size_t)d * n + i] = Yi[(size_t)i * D + d] - mean[SLOT_MEAN + d];
+            Y_out[(size_t)d * n + ws->order[i]] = Yi[(size_t)i * D + d] - mean[SLOT_MEAN + d];
 }
 
 void tsne_destroy(tsne_workspace_t *ws)
@@ -659,12 +776,19 @@ int tsne_compute_gradient(tsne_workspace_t *ws, double exag, int exact, double *
         for (size_t k = 0; k < F.size(); k++) rep[k] = 4.0 / Z * (double)F[k];
         ws->rep_valid = 0;
     }
-    // attractive part on the host in FP64 from the FP64 master coordinates
-    std::vector<double> Yi((size_t)n * D);
-    if (hipMemcpyAsync(Yi.data(), ws->d_Y64, Yi.size() * sizeof(double), hipMemcpyDeviceToHost,
+    // attractive part on the host in FP64 from the FP64 master coordinates, all in the
+    // caller's order (P on the host is in that order)
+    std::vector<double> Yc((size_t)n * D), Yi((size_t)n * D), rc_((size_t)n * D);
+    if (hipMemcpyAsync(Yc.data(), ws->d_Y64, Yc.size() * sizeof(double), hipMemcpyDeviceToHost,
                        ws->st) != hipSuccess ||
         hipStreamSynchronize(ws->st) != hipSuccess)
         return -1;
+    for (int k = 0; k < n; k++)
+        for (int d = 0; d < D; d++) {
+            Yi[(size_t)ws->order[k] * D + d] = Yc[(size_t)k * D + d];
+            rc_[(size_t)ws->order[k] * D + d] = rep[(size_t)k * D + d];
+        }
+    rep.swap(rc_);
     const sparse_matrix_t *P = ws->P;
     for (int i = 0; i < n; i++) {
         double acc[3] = {0, 0, 0};
@@ -695,8 +819,11 @@ int tsne_exact_repulsive_rows(tsne_workspace_t *ws, const int *rows, int n_rows,
     int *d_rows = nullptr;
     int rc = dalloc(&d_s, n) || dalloc(&d_F, (size_t)n * D) || dalloc(&d_s2, n_rows) ||
              dalloc(&d_F2, (size_t)n_rows * D) || dalloc(&d_rows, n_rows);
-    rc = rc || hipMemcpyAsync(d_rows, rows, (size_t)n_rows * sizeof(int), hipMemcpyHostToDevice,
-                              ws->st) != hipSuccess;
+    const std::vector<int> inv = inverse_order(ws);
+    std::vector<int> cur_rows(n_rows);
+    for (int r = 0; r < n_rows; r++) cur_rows[r] = inv[rows[r]];
+    rc = rc || hipMemcpyAsync(d_rows, cur_rows.data(), (size_t)n_rows * sizeof(int),
+                              hipMemcpyHostToDevice, ws->st) != hipSuccess;
     rc = rc || launch_repulsive_direct_f64(D, n, ws->d_Y64, nullptr, n, d_s, d_F, ws->st);
     rc = rc || launch_z_f64(D, n, ws->d_Y64, d_s, d_F, ws->d_scal, ws->d_partial, ws->nblocks_red,
                             ws->d_scal + SLOT_ZEXACT, ws->st);
