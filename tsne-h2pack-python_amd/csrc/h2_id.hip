@@ -11,6 +11,10 @@
 // or, for an internal node, the children's coefficients moved to c_i
 // ([w0, wY + (c_child - c_i) w0]).
 //
+// A node with a single child keeps the child's skeleton (U = I): the child's far region
+// (outside 3 child half-sides of its centre) contains the parent's, so the child basis
+// already represents every interaction the parent's does.
+//
 // Reference counterpart: the U / J construction of H2P_build in H2Pack (QR_REL_NRM ID
 // against proxy points; src/tsne_h2pack.c:818, :908-919) and the upward sweep of the
 // fused product (src/tsne_h2_matmul.c:309-334).  One workgroup per node, one thread per
@@ -37,6 +41,12 @@ __device__ __forceinline__ double kern64(const double *a, const double *b, int D
     return w * w;
 }
 
+__device__ __forceinline__ double wsum64(double v)
+{
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
 }  // namespace
 
 template <int D, int NT>
@@ -44,12 +54,13 @@ __global__ void __launch_bounds__(NT)
 id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, double tol)
 {
     constexpr int PMAX = H2Cfg<D>::PMAX, RMAX = H2Cfg<D>::RMAX;
+    static_assert(PMAX % 4 == 0, "PMAX must be a multiple of 4");
     constexpr int NW = NT / 64;
     constexpr size_t QBYTES = sizeof(double) * RMAX * PMAX;
     constexpr size_t UBYTES = sizeof(float) * NT * RMAX;
-    __shared__ double s_cand[NT][D];
     __shared__ float4 s_xin[NT];
     __shared__ double s_prox[PMAX][D];
+    __shared__ double s_part[NW][PMAX];
     __shared__ double s_scale[PMAX];
     __shared__ __align__(16) unsigned char s_qu[QBYTES > UBYTES ? QBYTES : UBYTES];
     __shared__ double s_R[RMAX][RMAX];
@@ -102,8 +113,29 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, d
             acc += rj;
         }
         m = min(acc, NT);
+        if (nc == 1) {
+            // single child: same skeleton, U = I, coefficients moved to this centre
+            if (t == 0) {
+                const int base = atomicAdd(&h.cnt[CNT_POOL], m * m);
+                s_base = base;
+                if ((long long)base + (long long)m * m > h.pool_cap)
+                    atomicOr(&h.flags[FLAG_POOL_OVF], 1);
+                h.rank[node] = m;
+                h.uoff[node] = base;
+            }
+            __syncthreads();
+            const int base = s_base;
+            if (t < m) {
+                if ((long long)base + (long long)m * m <= h.pool_cap)
+                    for (int k = 0; k < m; k++) h.upool[(size_t)base + (size_t)t * m + k] = (k == t) ? 1.f : 0.f;
+                for (int d = 0; d < D; d++) h.skel[((size_t)node * RMAX + t) * D + d] = my[d];
+                h.skelf[(size_t)node * RMAX + t] =
+                    make_float4((float)my[0], (float)my[1], (float)my[2], 0.f);
+                h.wc[(size_t)node * RMAX + t] = xin;
+            }
+            return;
+        }
     }
-    for (int d = 0; d < D; d++) s_cand[t][d] = my[d];
     s_xin[t] = xin;
 
     // ---- proxies of this level, translated to the centre
@@ -113,28 +145,47 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, d
     for (int q = t; q < p; q += NT)
         for (int d = 0; d < D; d++) s_prox[q][d] = cd[d] + h.proxy_pts[((size_t)kq * PMAX + q) * D + d];
     __syncthreads();
-    // column scales: 1 / ||K(candidates, proxy_q)||
-    for (int q = t; q < p; q += NT) {
-        double s2 = 0.0;
-        for (int u = 0; u < m; u++) {
-            const double v = kern64(s_cand[u], s_prox[q], D);
-            s2 += v * v;
-        }
-        s_scale[q] = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
-    }
-    __syncthreads();
 
-    // ---- this thread's normalised column of A^T
+    // ---- this thread's column of A^T, then column norms over the candidates
     double a[PMAX];
-    double nrm = 0.0;
 #pragma unroll
     for (int q = 0; q < PMAX; q++) {
         double v = 0.0;
-        if (q < p && t < m) v = kern64(my, s_prox[q], D) * s_scale[q];
+        if (q < p && t < m) v = kern64(my, s_prox[q], D);
         a[q] = v;
-        nrm += v * v;
     }
-    if (t >= m) nrm = -1.0;
+#pragma unroll
+    for (int q = 0; q < PMAX; q++) {
+        if (q < p) {
+            const double v = wsum64(a[q] * a[q]);
+            if (lane == 0) s_part[wid][q] = v;
+        }
+    }
+    __syncthreads();
+    for (int q = t; q < p; q += NT) {
+        double s2 = 0.0;
+        for (int w = 0; w < NW; w++) s2 += s_part[w][q];
+        s_scale[q] = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
+    }
+    __syncthreads();
+    double nrm;
+    {
+        double n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
+#pragma unroll
+        for (int q = 0; q < PMAX; q += 4) {
+            if (q < p) {
+                a[q] *= s_scale[q];
+                a[q + 1] *= (q + 1 < p) ? s_scale[q + 1] : 0.0;
+                a[q + 2] *= (q + 2 < p) ? s_scale[q + 2] : 0.0;
+                a[q + 3] *= (q + 3 < p) ? s_scale[q + 3] : 0.0;
+                n0 += a[q] * a[q];
+                n1 += a[q + 1] * a[q + 1];
+                n2 += a[q + 2] * a[q + 2];
+                n3 += a[q + 3] * a[q + 3];
+            }
+        }
+        nrm = (t < m) ? (n0 + n1) + (n2 + n3) : -1.0;
+    }
 
     // ---- column-pivoted MGS
     bool selected = false;
@@ -149,40 +200,58 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, d
             const int oi = __shfl_xor(vi, off, 64);
             if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
         }
-        if (lane == 0) { s_rv[wid] = v; s_ri[wid] = vi; }
-        __syncthreads();
-        if (t == 0) {
-            double bv = s_rv[0];
-            int bi = s_ri[0];
-            for (int w = 1; w < NW; w++)
-                if (s_rv[w] > bv || (s_rv[w] == bv && s_ri[w] < bi)) { bv = s_rv[w]; bi = s_ri[w]; }
-            s_bv = bv;
-            s_bi = bi;
+        if (NW > 1) {
+            if (lane == 0) { s_rv[wid] = v; s_ri[wid] = vi; }
+            __syncthreads();
+            if (t == 0) {
+                double bv = s_rv[0];
+                int bi = s_ri[0];
+                for (int w = 1; w < NW; w++)
+                    if (s_rv[w] > bv || (s_rv[w] == bv && s_ri[w] < bi)) { bv = s_rv[w]; bi = s_ri[w]; }
+                s_bv = bv;
+                s_bi = bi;
+            }
+            __syncthreads();
+            v = s_bv;
+            vi = s_bi;
         }
-        __syncthreads();
-        const double bv = s_bv;
-        const int bi = s_bi;
+        const double bv = v;
+        const int bi = vi;
         if (k == 0) nrm0 = bv;
         if (k >= rlim || !(bv > tol * tol * nrm0) || !(bv > 0.0)) break;
         if (t == bi) {
             selected = true;
             const double inv = 1.0 / sqrt(bv);
 #pragma unroll
-            for (int q = 0; q < PMAX; q++) s_Q[k][q] = a[q] * inv;
+            for (int q = 0; q < PMAX; q++)
+                if (q < p) s_Q[k][q] = a[q] * inv;
             s_piv[k] = t;
         }
         __syncthreads();
         if (!selected && t < m) {
-            double dot = 0.0;
+            double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
 #pragma unroll
-            for (int q = 0; q < PMAX; q++) dot += s_Q[k][q] * a[q];
-            double s2 = 0.0;
-#pragma unroll
-            for (int q = 0; q < PMAX; q++) {
-                a[q] -= dot * s_Q[k][q];
-                s2 += a[q] * a[q];
+            for (int q = 0; q < PMAX; q += 4) {
+                if (q < p) {
+                    d0 += s_Q[k][q] * a[q];
+                    if (q + 1 < p) d1 += s_Q[k][q + 1] * a[q + 1];
+                    if (q + 2 < p) d2 += s_Q[k][q + 2] * a[q + 2];
+                    if (q + 3 < p) d3 += s_Q[k][q + 3] * a[q + 3];
+                }
             }
-            nrm = s2;
+            const double dot = (d0 + d1) + (d2 + d3);
+            double n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
+#pragma unroll
+            for (int q = 0; q < PMAX; q += 4) {
+                if (q < p) {
+                    a[q] -= dot * s_Q[k][q];
+                    n0 += a[q] * a[q];
+                    if (q + 1 < p) { a[q + 1] -= dot * s_Q[k][q + 1]; n1 += a[q + 1] * a[q + 1]; }
+                    if (q + 2 < p) { a[q + 2] -= dot * s_Q[k][q + 2]; n2 += a[q + 2] * a[q + 2]; }
+                    if (q + 3 < p) { a[q + 3] -= dot * s_Q[k][q + 3]; n3 += a[q + 3] * a[q + 3]; }
+                }
+            }
+            nrm = (n0 + n1) + (n2 + n3);
         }
         r = k + 1;
         if (k == RMAX - 1 && t == 0) atomicOr(&h.flags[FLAG_RANK_CAP], 1);

@@ -28,7 +28,7 @@ template <> struct H2Cfg<2> {
     static constexpr int B = 30;         // key bits per dimension
     static constexpr int NCH = 4;        // children per node
     static constexpr int RMAX = 32;      // maximum skeleton rank
-    static constexpr int PMAX = 64;      // maximum proxy points
+    static constexpr int PMAX = 48;      // maximum proxy points
     static constexpr int MCAP = 128;     // maximum ID candidates (NCH * RMAX, leaf cap)
     static constexpr int NEAR_CAP = 48;  // near-list slots per node
 };

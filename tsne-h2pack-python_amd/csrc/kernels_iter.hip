@@ -54,7 +54,12 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     __shared__ double gsum[kAttrGroups][3];
     const int sub = threadIdx.x & (kAttrLanes - 1);
     const int grp = threadIdx.x / kAttrLanes;
-    const int row0 = blockIdx.x * rows_per_block;
+    // XCD-aware placement: workgroups are dealt round-robin to the 8 XCDs, so logical
+    // block L = (b % 8) * (G / 8) + b / 8 gives each XCD one contiguous eighth of the
+    // rows; with the points renumbered along the tree, that XCD's gathers stay in its
+    // own L2.  (G, the launch size, is a multiple of 8.)
+    const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const int row0 = lblk * rows_per_block;
     const int row1 = min(n, row0 + rows_per_block);
     const double rep_scale = 4.0 / scal[SLOT_Z];
     const double m_d = (sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
@@ -64,18 +69,32 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
         const V yi = Yg[i];
         float ax = 0.f, ay = 0.f, az = 0.f;
         const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
-        for (int e = e0 + sub; e < e1; e += kAttrLanes) {
-            const int j = col_idx[e];
-            const float p = pval[e];
-            const V yj = Yg[j];
-            const float dx = yi.x - yj.x, dy = yi.y - yj.y;
-            float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
-            float dz = 0.f;
-            if constexpr (D == 3) { dz = yi.z - yj.z; r2 = fmaf(dz, dz, r2); }
-            const float q = p * fast_rcp(r2);
-            ax = fmaf(q, dx, ax);
-            ay = fmaf(q, dy, ay);
-            if constexpr (D == 3) az = fmaf(q, dz, az);
+        // two nonzeros per lane per step: both gathers are in flight together
+        for (int e = e0 + sub; e < e1; e += 2 * kAttrLanes) {
+            const int e2 = e + kAttrLanes;
+            const bool two = e2 < e1;
+            const int j1 = col_idx[e];
+            const float p1 = pval[e];
+            const int j2 = two ? col_idx[e2] : j1;
+            const float p2 = two ? pval[e2] : 0.f;
+            const V y1 = Yg[j1];
+            const V y2 = Yg[j2];
+            const float dx1 = yi.x - y1.x, dy1 = yi.y - y1.y;
+            const float dx2 = yi.x - y2.x, dy2 = yi.y - y2.y;
+            float r1 = fmaf(dx1, dx1, fmaf(dy1, dy1, 1.0f));
+            float r2 = fmaf(dx2, dx2, fmaf(dy2, dy2, 1.0f));
+            float dz1 = 0.f, dz2 = 0.f;
+            if constexpr (D == 3) {
+                dz1 = yi.z - y1.z;
+                dz2 = yi.z - y2.z;
+                r1 = fmaf(dz1, dz1, r1);
+                r2 = fmaf(dz2, dz2, r2);
+            }
+            const float q1 = p1 * fast_rcp(r1);
+            const float q2 = p2 * fast_rcp(r2);
+            ax = fmaf(q1, dx1, fmaf(q2, dx2, ax));
+            ay = fmaf(q1, dy1, fmaf(q2, dy2, ay));
+            if constexpr (D == 3) az = fmaf(q1, dz1, fmaf(q2, dz2, az));
         }
         for (int off = kAttrLanes / 2; off > 0; off >>= 1) {
             ax += __shfl_xor(ax, off, 64);
@@ -116,30 +135,33 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     if (threadIdx.x < D) {
         double t = 0.0;
         for (int w = 0; w < kAttrGroups; w++) t += gsum[w][threadIdx.x];
-        block_sums[(size_t)blockIdx.x * 3 + threadIdx.x] = t;
+        block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
     }
 }
 
 // Sum of the per-block sums in block order -> new pending mean (SLOT_MEAN..+D-1).
 // Also folds a finiteness flag (SLOT_BAD) for the host.
-__global__ void mean_finalize_kernel(int nblocks, int D, int n,
-                                     const double *__restrict__ block_sums,
-                                     double *__restrict__ scal)
+__global__ void __launch_bounds__(1024)
+mean_finalize_kernel(int nblocks, int D, int n, const double *__restrict__ block_sums,
+                     double *__restrict__ scal)
 {
-    __shared__ double part[256][3];
+    __shared__ double part[3][1024];
     const int t = threadIdx.x;
     double a[3] = {0.0, 0.0, 0.0};
-    // each thread sums a contiguous chunk of blocks in order, then chunks in order
-    const int per = (nblocks + 255) / 256;
+    // each thread sums a contiguous chunk of blocks in order, then a fixed tree
+    const int per = (nblocks + 1023) / 1024;
     const int b0 = t * per, b1 = min(nblocks, b0 + per);
     for (int b = b0; b < b1; b++)
         for (int d = 0; d < D; d++) a[d] += block_sums[(size_t)b * 3 + d];
-    for (int d = 0; d < 3; d++) part[t][d] = a[d];
+    for (int d = 0; d < 3; d++) part[d][t] = a[d];
     __syncthreads();
+    for (int off = 512; off > 0; off >>= 1) {
+        if (t < off)
+            for (int d = 0; d < D; d++) part[d][t] += part[d][t + off];
+        __syncthreads();
+    }
     if (t < D) {
-        double s = 0.0;
-        for (int k = 0; k < 256; k++) s += part[k][t];
-        const double m = s / (double)n;
+        const double m = part[t][0] / (double)n;
         scal[SLOT_MEAN + t] = m;
         if (!isfinite(m)) scal[SLOT_BAD] = 1.0;
     }
@@ -450,7 +472,7 @@ int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, con
         hipLaunchKernelGGL(attr_update_kernel<3>, dim3(nblocks), dim3(256), 0, st, n, row_ptr,
                            col_idx, pval, (const float4 *)Yg, F, Y64, uY, gains,
                            (float4 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
-    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(256), 0, st, nblocks, D, n,
+    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, nblocks, D, n,
                        block_sums, (double *)scal);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -435,7 +435,8 @@ tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int
     if (prec && strcmp(prec, "mixed") == 0) ws->precision = TSNE_H2_MIXED;
 
     const int n = n_point, D = low_dim;
-    ws->nblocks_upd = std::max(1, (n + 127) / 128);  // 128 rows per attractive block
+    // 128 rows per attractive block; a multiple of 8 blocks for the XCD-aware placement
+    ws->nblocks_upd = (((n + 127) / 128) + 7) & ~7;
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
     bool ok = hipStreamCreate(&ws->st) == hipSuccess;
     if (!ok) {
