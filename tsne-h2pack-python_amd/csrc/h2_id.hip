@@ -51,7 +51,8 @@ __device__ __forceinline__ double wsum64(double v)
 
 template <int D, int NT>
 __global__ void __launch_bounds__(NT)
-id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, double tol)
+id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, int m_lo,
+             double tol)
 {
     constexpr int PMAX = H2Cfg<D>::PMAX, RMAX = H2Cfg<D>::RMAX;
     static_assert(PMAX % 4 == 0, "PMAX must be a multiple of 4");
@@ -74,6 +75,17 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, d
 
     const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
     if (internal_only && h.child0[node] < 0) return;  // leaves were done in their own pass
+    {
+        // size class: this instantiation takes the nodes with m_lo < m <= NT candidates
+        int mt;
+        if (h.child0[node] < 0) {
+            mt = h.ne[node] - h.nb[node];
+        } else {
+            mt = 0;
+            for (int q = 0; q < h.nchild[node]; q++) mt += h.rank[h.child0[node] + q];
+        }
+        if (mt <= m_lo || (mt > NT && NT < H2Cfg<D>::MCAP)) return;
+    }
     const int l = h.level[node];
     const int t = threadIdx.x;
     const int lane = t & 63, wid = t >> 6;
@@ -339,18 +351,22 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, d
 }
 
 // list != nullptr: the nodes list[0..count) (all leaves, any level); otherwise the
-// internal nodes among [lb, lb + count).
+// internal nodes among [lb, lb + count).  Nodes with at most 64 candidates run in
+// single-wavefront workgroups (their barriers are free), larger ones in 128 threads.
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
                  hipStream_t st)
 {
     if (count <= 0) return 0;
     const int internal_only = list ? 0 : 1;
-    if (D == 2)
+    if (D == 2) {
+        hipLaunchKernelGGL((id_up_kernel<2, 64>), dim3(count), dim3(64), 0, st, h, list, lb,
+                           internal_only, 0, tol);
         hipLaunchKernelGGL((id_up_kernel<2, 128>), dim3(count), dim3(128), 0, st, h, list, lb,
-                           internal_only, tol);
-    else
+                           internal_only, 64, tol);
+    } else {
         hipLaunchKernelGGL((id_up_kernel<3, 384>), dim3(count), dim3(384), 0, st, h, list, lb,
-                           internal_only, tol);
+                           internal_only, 0, tol);
+    }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

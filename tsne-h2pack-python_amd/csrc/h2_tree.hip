@@ -366,80 +366,89 @@ int launch_split_write(int D, const H2Dev &h, int lb, int le, int l, hipStream_t
 // box of i away is an X source; the rest stay in i's near list.  For a leaf i the near
 // list is final: leaf members become NEAR pairs, others are descended later (pending).
 // ---------------------------------------------------------------------------------
+// One wavefront per node: lanes enumerate the candidates (children of the parent's
+// near-list members, in member order), classify them, and the near list is compacted
+// in candidate order with ballots; pairs get one atomic per category per wavefront.
 template <int D>
 __global__ void __launch_bounds__(256)
 lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
 {
     constexpr int B = H2Cfg<D>::B, NEAR_CAP = H2Cfg<D>::NEAR_CAP;
-    const int i = lb + blockIdx.x * 256 + threadIdx.x;
-    const bool valid = i < le;
-    int n_adm = 0, n_x = 0, cnt = 0;
-    int *my = h.near + (size_t)(valid ? i : 0) * NEAR_CAP;
-    int p = 0, np = 0;
-    if (valid) {
-        p = h.parent[i];
-        np = h.near_cnt[p];
-    }
-    // pass 1: own near list, counts of ADM / X
-    for (int a = 0; a < np; a++) {
-        const int j = h.near[(size_t)p * NEAR_CAP + a];
+    __shared__ int s_off[4][65], s_c0[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = lb + blockIdx.x * 4 + wv;
+    const bool valid = i < le;  // uniform over the wavefront
+    const unsigned long long lt = (1ull << lane) - 1ull;
+    const int p = valid ? h.parent[i] : 0;
+    const int np = valid ? h.near_cnt[p] : 0;
+    // candidates per member: a leaf member is its own candidate, else its children
+    int cc = 0, c0 = 0;
+    if (lane < np) {
+        const int j = h.near[(size_t)p * NEAR_CAP + lane];
         const bool jl = h.child0[j] < 0;
-        const int c0 = jl ? j : h.child0[j];
-        const int nc = jl ? 1 : h.nchild[j];
-        for (int q = 0; q < nc; q++) {
-            const int c = c0 + q;
-            const int lc = h.level[c];
-            if (box_gap<D>(h.icoord, i, l, c, lc) >= 1) {
-                if (lc == l) n_adm++; else n_x++;
-            } else {
-                if (cnt < NEAR_CAP) my[cnt] = c;
-                else atomicOr(&h.flags[FLAG_NEAR_OVF], 1);
-                cnt++;
-            }
-        }
+        c0 = jl ? j : h.child0[j];
+        cc = jl ? 1 : h.nchild[j];
     }
-    cnt = min(cnt, NEAR_CAP);
-    const bool leaf = valid && ((h.ne[i] - h.nb[i]) <= h.leaf || l >= B);
-    int n_near = 0, n_pend = 0;
-    if (valid) {
-        h.near_cnt[i] = cnt;
-        if (leaf)
-            for (int a = 0; a < cnt; a++) {
-                const int c = my[a];
-                if ((h.ne[c] - h.nb[c]) <= h.leaf || h.level[c] >= B) n_near++; else n_pend++;
-            }
+    int x = cc;  // inclusive scan over lanes
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
     }
-    int k_adm = wave_alloc(&h.cnt[CNT_ADM], n_adm);
-    int k_x = wave_alloc(&h.cnt[CNT_X], n_x);
-    int k_near = wave_alloc(&h.cnt[CNT_NEAR], n_near);
-    int k_pend = wave_alloc(&h.cnt[CNT_PEND0 + pend_out], n_pend);
+    const int T = __shfl(x, 63, 64);
+    s_off[wv][lane + 1] = x;
+    if (lane == 0) s_off[wv][0] = 0;
+    s_c0[wv][lane] = c0;
+    __syncthreads();  // reached by every wavefront of the block
     if (!valid) return;
-    // pass 2: write the pairs
-    if (n_adm + n_x > 0) {
-        for (int a = 0; a < np; a++) {
-            const int j = h.near[(size_t)p * NEAR_CAP + a];
-            const bool jl = h.child0[j] < 0;
-            const int c0 = jl ? j : h.child0[j];
-            const int nc = jl ? 1 : h.nchild[j];
-            for (int q = 0; q < nc; q++) {
-                const int c = c0 + q;
-                const int lc = h.level[c];
-                if (box_gap<D>(h.icoord, i, l, c, lc) >= 1) {
-                    if (lc == l) put_pair(h.adm, k_adm++, h.adm_cap, i, c, h.flags);
-                    else put_pair(h.xl, k_x++, h.x_cap, i, c, h.flags);
-                }
+    const bool leaf = (h.ne[i] - h.nb[i]) <= h.leaf || l >= B;
+    int *my = h.near + (size_t)i * NEAR_CAP;
+    int cnt = 0;
+    for (int k0 = 0; k0 < T; k0 += 64) {
+        const int k = k0 + lane;
+        int cand = -1, cat = 0;  // 1 ADM, 2 X, 3 near
+        if (k < T) {
+            int lo = 0, hi = np;  // member a with off[a] <= k < off[a+1]
+            while (hi - lo > 1) {
+                const int mid = (lo + hi) >> 1;
+                if (s_off[wv][mid] <= k) lo = mid; else hi = mid;
             }
+            cand = s_c0[wv][lo] + (k - s_off[wv][lo]);
+            const int lc = h.level[cand];
+            if (box_gap<D>(h.icoord, i, l, cand, lc) >= 1) cat = (lc == l) ? 1 : 2;
+            else cat = 3;
         }
-    }
-    if (leaf) {
-        for (int a = 0; a < cnt; a++) {
-            const int c = my[a];
-            if ((h.ne[c] - h.nb[c]) <= h.leaf || h.level[c] >= B)
-                put_pair(h.nl, k_near++, h.nl_cap, i, c, h.flags);
-            else
-                put_pair(h.pend[pend_out], k_pend++, h.pend_cap, i, c, h.flags);
+        // a leaf's near list is final: its members become NEAR pairs (leaf members) or
+        // pending pairs (members to descend at the next levels)
+        int sub = 0;  // 1 NEAR, 2 pending
+        if (cat == 3 && leaf)
+            sub = ((h.ne[cand] - h.nb[cand]) <= h.leaf || h.level[cand] >= B) ? 1 : 2;
+        const unsigned long long mA = __ballot(cat == 1), mX = __ballot(cat == 2),
+                                 mN = __ballot(cat == 3), mNL = __ballot(sub == 1),
+                                 mP = __ballot(sub == 2);
+        int bA = 0, bX = 0, bN = 0, bP = 0;
+        if (lane == 0) {
+            if (mA) bA = atomicAdd(&h.cnt[CNT_ADM], __popcll(mA));
+            if (mX) bX = atomicAdd(&h.cnt[CNT_X], __popcll(mX));
+            if (mNL) bN = atomicAdd(&h.cnt[CNT_NEAR], __popcll(mNL));
+            if (mP) bP = atomicAdd(&h.cnt[CNT_PEND0 + pend_out], __popcll(mP));
         }
+        bA = __shfl(bA, 0, 64);
+        bX = __shfl(bX, 0, 64);
+        bN = __shfl(bN, 0, 64);
+        bP = __shfl(bP, 0, 64);
+        if (cat == 1) put_pair(h.adm, bA + __popcll(mA & lt), h.adm_cap, i, cand, h.flags);
+        if (cat == 2) put_pair(h.xl, bX + __popcll(mX & lt), h.x_cap, i, cand, h.flags);
+        if (cat == 3) {
+            const int pos = cnt + __popcll(mN & lt);
+            if (pos < NEAR_CAP) my[pos] = cand;
+            else atomicOr(&h.flags[FLAG_NEAR_OVF], 1);
+        }
+        if (sub == 1) put_pair(h.nl, bN + __popcll(mNL & lt), h.nl_cap, i, cand, h.flags);
+        if (sub == 2)
+            put_pair(h.pend[pend_out], bP + __popcll(mP & lt), h.pend_cap, i, cand, h.flags);
+        cnt += __popcll(mN);
     }
+    if (lane == 0) h.near_cnt[i] = min(cnt, NEAR_CAP);
 }
 
 // Pending pairs (leaf i, non-leaf c): each child d of c is a W source when i lies at
@@ -485,7 +494,7 @@ pending_kernel(H2Dev h, int n_pend, int pend_in, int pend_out)
 
 int launch_lists(int D, const H2Dev &h, int lb, int le, int l, int pend_out, hipStream_t st)
 {
-    const int g = (le - lb + 255) / 256;
+    const int g = (le - lb + 3) / 4;
     if (g == 0) return 0;
     if (D == 2)
         hipLaunchKernelGGL(lists_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le, l, pend_out);

@@ -39,6 +39,7 @@ namespace tsne {
 // ---------------------------------------------------------------------------------
 constexpr int kAttrLanes = 16;
 constexpr int kAttrGroups = 256 / kAttrLanes;  // rows in flight per block
+constexpr int kAttrBatch = 10;                  // nonzeros per lane per batch
 
 template <int D>
 __global__ void __launch_bounds__(256)
@@ -66,35 +67,44 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     double my_sum = 0.0;  // lane d of the group: sum of new coordinate d over its rows
 
     for (int i = row0 + grp; i < row1; i += kAttrGroups) {
+        const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
+        // update operands of this row, fetched now and used after the reduction
+        const size_t kd = (size_t)i * D + (sub < D ? sub : 0);
+        double f_d = 0.0, u_d = 0.0, g_d = 0.0, y_d = 0.0;
+        if (sub < D) {
+            f_d = (double)F[kd];
+            u_d = uY[kd];
+            g_d = gains[kd];
+            y_d = Y64[kd];
+        }
         const V yi = Yg[i];
         float ax = 0.f, ay = 0.f, az = 0.f;
-        const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
-        // two nonzeros per lane per step: both gathers are in flight together
-        for (int e = e0 + sub; e < e1; e += 2 * kAttrLanes) {
-            const int e2 = e + kAttrLanes;
-            const bool two = e2 < e1;
-            const int j1 = col_idx[e];
-            const float p1 = pval[e];
-            const int j2 = two ? col_idx[e2] : j1;
-            const float p2 = two ? pval[e2] : 0.f;
-            const V y1 = Yg[j1];
-            const V y2 = Yg[j2];
-            const float dx1 = yi.x - y1.x, dy1 = yi.y - y1.y;
-            const float dx2 = yi.x - y2.x, dy2 = yi.y - y2.y;
-            float r1 = fmaf(dx1, dx1, fmaf(dy1, dy1, 1.0f));
-            float r2 = fmaf(dx2, dx2, fmaf(dy2, dy2, 1.0f));
-            float dz1 = 0.f, dz2 = 0.f;
-            if constexpr (D == 3) {
-                dz1 = yi.z - y1.z;
-                dz2 = yi.z - y2.z;
-                r1 = fmaf(dz1, dz1, r1);
-                r2 = fmaf(dz2, dz2, r2);
+        // a batch is kAttrBatch nonzeros per lane: all its column loads are issued, then
+        // all its gathers, so a row costs two memory round trips instead of one per step
+        for (int base = e0 + sub; base < e1; base += kAttrBatch * kAttrLanes) {
+            int jj[kAttrBatch];
+            float pp[kAttrBatch];
+#pragma unroll
+            for (int t = 0; t < kAttrBatch; t++) {
+                const int e = base + t * kAttrLanes;
+                const bool ok = e < e1;
+                jj[t] = ok ? col_idx[e] : i;
+                pp[t] = ok ? pval[e] : 0.f;
             }
-            const float q1 = p1 * fast_rcp(r1);
-            const float q2 = p2 * fast_rcp(r2);
-            ax = fmaf(q1, dx1, fmaf(q2, dx2, ax));
-            ay = fmaf(q1, dy1, fmaf(q2, dy2, ay));
-            if constexpr (D == 3) az = fmaf(q1, dz1, fmaf(q2, dz2, az));
+            V yy[kAttrBatch];
+#pragma unroll
+            for (int t = 0; t < kAttrBatch; t++) yy[t] = Yg[jj[t]];
+#pragma unroll
+            for (int t = 0; t < kAttrBatch; t++) {
+                const float dx = yi.x - yy[t].x, dy = yi.y - yy[t].y;
+                float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
+                float dz = 0.f;
+                if constexpr (D == 3) { dz = yi.z - yy[t].z; r2 = fmaf(dz, dz, r2); }
+                const float q = pp[t] * fast_rcp(r2);
+                ax = fmaf(q, dx, ax);
+                ay = fmaf(q, dy, ay);
+                if constexpr (D == 3) az = fmaf(q, dz, az);
+            }
         }
         for (int off = kAttrLanes / 2; off > 0; off >>= 1) {
             ax += __shfl_xor(ax, off, 64);
@@ -104,16 +114,15 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
         double ynew = 0.0;
         if (sub < D) {
             const float a = (sub == 0) ? ax : (sub == 1) ? ay : az;
-            const size_t k = (size_t)i * D + sub;
-            const double g = exag4 * (double)a - rep_scale * (double)F[k];
-            double u = uY[k], gn = gains[k];
+            const double g = exag4 * (double)a - rep_scale * f_d;
+            double u = u_d, gn = g_d;
             gn = (u * g < 0.0) ? gn + 0.2 : gn * 0.8;
             if (gn < 0.01) gn = 0.01;
             u = mom * u - lr * (gn * g);
-            ynew = (Y64[k] - m_d) + u;
-            Y64[k] = ynew;
-            uY[k] = u;
-            gains[k] = gn;
+            ynew = (y_d - m_d) + u;
+            Y64[kd] = ynew;
+            uY[kd] = u;
+            gains[kd] = gn;
             my_sum += ynew;
         }
         // the new coordinates sit in lanes 0..D-1 of the group: assemble the record
