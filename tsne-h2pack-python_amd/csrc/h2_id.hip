@@ -150,34 +150,32 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
     }
     s_xin[t] = xin;
 
-    // ---- proxies of this level, translated to the centre
+    // ---- proxies of this level, translated to the centre.  Everything below runs over
+    // all PMAX proxy slots and all RMAX rank slots without branches: unused slots hold
+    // zeros (scale 0, zero Q rows, identity R rows), which keeps the FP64 pipelines full.
     const RootInfo root = *h.root;
     const int kq = min(max(root.k0 - l, 0), kProxyLevels - 1);
     const int p = min(h.proxy_cnt[kq], PMAX);
-    for (int q = t; q < p; q += NT)
-        for (int d = 0; d < D; d++) s_prox[q][d] = cd[d] + h.proxy_pts[((size_t)kq * PMAX + q) * D + d];
+    for (int q = t; q < PMAX; q += NT)
+        for (int d = 0; d < D; d++)
+            s_prox[q][d] = cd[d] + (q < p ? h.proxy_pts[((size_t)kq * PMAX + q) * D + d] : 0.0);
+    for (int e = t; e < RMAX * RMAX; e += NT) s_R[e / RMAX][e % RMAX] = (e / RMAX == e % RMAX) ? 1.0 : 0.0;
     __syncthreads();
 
     // ---- this thread's column of A^T, then column norms over the candidates
     double a[PMAX];
 #pragma unroll
-    for (int q = 0; q < PMAX; q++) {
-        double v = 0.0;
-        if (q < p && t < m) v = kern64(my, s_prox[q], D);
-        a[q] = v;
-    }
+    for (int q = 0; q < PMAX; q++) a[q] = (t < m) ? kern64(my, s_prox[q], D) : 0.0;
 #pragma unroll
     for (int q = 0; q < PMAX; q++) {
-        if (q < p) {
-            const double v = wsum64(a[q] * a[q]);
-            if (lane == 0) s_part[wid][q] = v;
-        }
+        const double v = wsum64(a[q] * a[q]);
+        if (lane == 0) s_part[wid][q] = v;
     }
     __syncthreads();
-    for (int q = t; q < p; q += NT) {
+    for (int q = t; q < PMAX; q += NT) {
         double s2 = 0.0;
         for (int w = 0; w < NW; w++) s2 += s_part[w][q];
-        s_scale[q] = s2 > 0.0 ? 1.0 / sqrt(s2) : 0.0;
+        s_scale[q] = (q < p && s2 > 0.0) ? 1.0 / sqrt(s2) : 0.0;
     }
     __syncthreads();
     double nrm;
@@ -185,16 +183,14 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
         double n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
 #pragma unroll
         for (int q = 0; q < PMAX; q += 4) {
-            if (q < p) {
-                a[q] *= s_scale[q];
-                a[q + 1] *= (q + 1 < p) ? s_scale[q + 1] : 0.0;
-                a[q + 2] *= (q + 2 < p) ? s_scale[q + 2] : 0.0;
-                a[q + 3] *= (q + 3 < p) ? s_scale[q + 3] : 0.0;
-                n0 += a[q] * a[q];
-                n1 += a[q + 1] * a[q + 1];
-                n2 += a[q + 2] * a[q + 2];
-                n3 += a[q + 3] * a[q + 3];
-            }
+            a[q] *= s_scale[q];
+            a[q + 1] *= s_scale[q + 1];
+            a[q + 2] *= s_scale[q + 2];
+            a[q + 3] *= s_scale[q + 3];
+            n0 = fma(a[q], a[q], n0);
+            n1 = fma(a[q + 1], a[q + 1], n1);
+            n2 = fma(a[q + 2], a[q + 2], n2);
+            n3 = fma(a[q + 3], a[q + 3], n3);
         }
         nrm = (t < m) ? (n0 + n1) + (n2 + n3) : -1.0;
     }
@@ -235,50 +231,50 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
             selected = true;
             const double inv = 1.0 / sqrt(bv);
 #pragma unroll
-            for (int q = 0; q < PMAX; q++)
-                if (q < p) s_Q[k][q] = a[q] * inv;
+            for (int q = 0; q < PMAX; q++) s_Q[k][q] = a[q] * inv;
             s_piv[k] = t;
         }
         __syncthreads();
         if (!selected && t < m) {
             double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
 #pragma unroll
-            for (int q = 0; q < PMAX; q += 4) {
-                if (q < p) {
-                    d0 += s_Q[k][q] * a[q];
-                    if (q + 1 < p) d1 += s_Q[k][q + 1] * a[q + 1];
-                    if (q + 2 < p) d2 += s_Q[k][q + 2] * a[q + 2];
-                    if (q + 3 < p) d3 += s_Q[k][q + 3] * a[q + 3];
-                }
+            for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
+                d0 = fma(s_Q[k][q], a[q], d0);
+                d1 = fma(s_Q[k][q + 1], a[q + 1], d1);
+                d2 = fma(s_Q[k][q + 2], a[q + 2], d2);
+                d3 = fma(s_Q[k][q + 3], a[q + 3], d3);
             }
             const double dot = (d0 + d1) + (d2 + d3);
             double n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
 #pragma unroll
-            for (int q = 0; q < PMAX; q += 4) {
-                if (q < p) {
-                    a[q] -= dot * s_Q[k][q];
-                    n0 += a[q] * a[q];
-                    if (q + 1 < p) { a[q + 1] -= dot * s_Q[k][q + 1]; n1 += a[q + 1] * a[q + 1]; }
-                    if (q + 2 < p) { a[q + 2] -= dot * s_Q[k][q + 2]; n2 += a[q + 2] * a[q + 2]; }
-                    if (q + 3 < p) { a[q + 3] -= dot * s_Q[k][q + 3]; n3 += a[q + 3] * a[q + 3]; }
-                }
+            for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
+                a[q] = fma(-dot, s_Q[k][q], a[q]);
+                a[q + 1] = fma(-dot, s_Q[k][q + 1], a[q + 1]);
+                a[q + 2] = fma(-dot, s_Q[k][q + 2], a[q + 2]);
+                a[q + 3] = fma(-dot, s_Q[k][q + 3], a[q + 3]);
+                n0 = fma(a[q], a[q], n0);
+                n1 = fma(a[q + 1], a[q + 1], n1);
+                n2 = fma(a[q + 2], a[q + 2], n2);
+                n3 = fma(a[q + 3], a[q + 3], n3);
             }
             nrm = (n0 + n1) + (n2 + n3);
         }
         r = k + 1;
         if (k == RMAX - 1 && t == 0) atomicOr(&h.flags[FLAG_RANK_CAP], 1);
     }
+    // unused rank slots: zero Q rows
+    for (int e = t; e < (RMAX - r) * PMAX; e += NT) s_Q[r + e / PMAX][e % PMAX] = 0.0;
+    __syncthreads();
 
     // ---- interpolation coefficients x_t = R11^{-1} Q^T a_t
     double rt[RMAX];
 #pragma unroll
     for (int k = 0; k < RMAX; k++) rt[k] = 0.0;
     if (t < m) {
-        for (int q = 0; q < p; q++) {
+        for (int q = 0; q < PMAX; q++) {
             const double v = kern64(my, s_prox[q], D) * s_scale[q];
 #pragma unroll
-            for (int k = 0; k < RMAX; k++)
-                if (k < r) rt[k] += s_Q[k][q] * v;
+            for (int k = 0; k < RMAX; k++) rt[k] = fma(s_Q[k][q], v, rt[k]);
         }
     }
     int kk = -1;
@@ -290,20 +286,19 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
         for (int j = 0; j < RMAX; j++)
             if (j <= kk) s_R[j][kk] = rt[j];
     }
-    __syncthreads();  // s_R complete; s_Q no longer read below (s_U aliases it)
+    __syncthreads();  // s_R complete
+    if (t < RMAX) s_part[0][t] = 1.0 / s_R[t][t];  // s_part reused: inverse diagonal
+    __syncthreads();  // s_Q no longer read below (s_U aliases it)
     if (kk >= 0) {
 #pragma unroll
         for (int k = 0; k < RMAX; k++) rt[k] = (k == kk) ? 1.0 : 0.0;
     } else if (t < m) {
 #pragma unroll
         for (int k = RMAX - 1; k >= 0; k--) {
-            if (k < r) {
-                double v = rt[k];
+            double v = rt[k];
 #pragma unroll
-                for (int j = 0; j < RMAX; j++)
-                    if (j > k && j < r) v -= s_R[k][j] * rt[j];
-                rt[k] = v / s_R[k][k];
-            }
+            for (int j = k + 1; j < RMAX; j++) v = fma(-s_R[k][j], rt[j], v);
+            rt[k] = v * s_part[0][k];
         }
     }
 
@@ -318,24 +313,17 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
         atomicAdd(&h.flags[FLAG_RANK_SUM], r);
         atomicMax(&h.flags[FLAG_PROXY_MAX], p);
     }
-    __syncthreads();
-    const int base = s_base;
-    const bool pool_ok = (long long)base + (long long)m * r <= h.pool_cap;
-    if (t < m) {
 #pragma unroll
-        for (int k = 0; k < RMAX; k++)
-            if (k < r) {
-                const float u = (float)rt[k];
-                s_U[t][k] = u;
-                if (pool_ok) h.upool[(size_t)base + (size_t)t * r + k] = u;
-            }
-    }
+    for (int k = 0; k < RMAX; k++) s_U[t][k] = (float)rt[k];
     if (kk >= 0) {
         for (int d = 0; d < D; d++) h.skel[((size_t)node * RMAX + kk) * D + d] = my[d];
         h.skelf[(size_t)node * RMAX + kk] =
             make_float4((float)my[0], (float)my[1], (float)my[2], 0.f);
     }
     __syncthreads();
+    const int base = s_base;
+    if ((long long)base + (long long)m * r <= h.pool_cap)
+        for (int e = t; e < m * r; e += NT) h.upool[(size_t)base + e] = s_U[e / r][e % r];
     if (t < r) {
         float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
         for (int u = 0; u < m; u++) {

@@ -13,6 +13,8 @@
 //   s                float   [n]     sum_{j != i} K_ij
 //   scal             double  [..]    device scalars: Z, the pending centring mean, ...
 // The row order of all of these is the caller's point order.
+#include <cstdlib>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -37,11 +39,10 @@ namespace tsne {
 // is next read instead of in an extra pass.  Each block covers a fixed contiguous row
 // range and writes the sum of its new coordinates in a fixed order, for the next mean.
 // ---------------------------------------------------------------------------------
-constexpr int kAttrLanes = 16;
-constexpr int kAttrGroups = 256 / kAttrLanes;  // rows in flight per block
-constexpr int kAttrBatch = 10;                  // nonzeros per lane per batch
 
-template <int D>
+// LANES lanes per row (256 / LANES rows in flight per block), BATCH nonzeros per lane
+// per batch.
+template <int D, int LANES, int BATCH>
 __global__ void __launch_bounds__(256)
 attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
                    const float *__restrict__ pval, const typename Vec<D>::T *__restrict__ Yg,
@@ -52,9 +53,9 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
                    double *__restrict__ block_sums)
 {
     using V = typename Vec<D>::T;
-    __shared__ double gsum[kAttrGroups][3];
-    const int sub = threadIdx.x & (kAttrLanes - 1);
-    const int grp = threadIdx.x / kAttrLanes;
+    __shared__ double gsum[(256 / LANES)][3];
+    const int sub = threadIdx.x & (LANES - 1);
+    const int grp = threadIdx.x / LANES;
     // XCD-aware placement: workgroups are dealt round-robin to the 8 XCDs, so logical
     // block L = (b % 8) * (G / 8) + b / 8 gives each XCD one contiguous eighth of the
     // rows; with the points renumbered along the tree, that XCD's gathers stay in its
@@ -66,7 +67,7 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     const double m_d = (sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
     double my_sum = 0.0;  // lane d of the group: sum of new coordinate d over its rows
 
-    for (int i = row0 + grp; i < row1; i += kAttrGroups) {
+    for (int i = row0 + grp; i < row1; i += (256 / LANES)) {
         const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
         // update operands of this row, fetched now and used after the reduction
         const size_t kd = (size_t)i * D + (sub < D ? sub : 0);
@@ -79,23 +80,23 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
         }
         const V yi = Yg[i];
         float ax = 0.f, ay = 0.f, az = 0.f;
-        // a batch is kAttrBatch nonzeros per lane: all its column loads are issued, then
+        // a batch is BATCH nonzeros per lane: all its column loads are issued, then
         // all its gathers, so a row costs two memory round trips instead of one per step
-        for (int base = e0 + sub; base < e1; base += kAttrBatch * kAttrLanes) {
-            int jj[kAttrBatch];
-            float pp[kAttrBatch];
+        for (int base = e0 + sub; base < e1; base += BATCH * LANES) {
+            int jj[BATCH];
+            float pp[BATCH];
 #pragma unroll
-            for (int t = 0; t < kAttrBatch; t++) {
-                const int e = base + t * kAttrLanes;
+            for (int t = 0; t < BATCH; t++) {
+                const int e = base + t * LANES;
                 const bool ok = e < e1;
                 jj[t] = ok ? col_idx[e] : i;
                 pp[t] = ok ? pval[e] : 0.f;
             }
-            V yy[kAttrBatch];
+            V yy[BATCH];
 #pragma unroll
-            for (int t = 0; t < kAttrBatch; t++) yy[t] = Yg[jj[t]];
+            for (int t = 0; t < BATCH; t++) yy[t] = Yg[jj[t]];
 #pragma unroll
-            for (int t = 0; t < kAttrBatch; t++) {
+            for (int t = 0; t < BATCH; t++) {
                 const float dx = yi.x - yy[t].x, dy = yi.y - yy[t].y;
                 float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
                 float dz = 0.f;
@@ -106,7 +107,7 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
                 if constexpr (D == 3) az = fmaf(q, dz, az);
             }
         }
-        for (int off = kAttrLanes / 2; off > 0; off >>= 1) {
+        for (int off = LANES / 2; off > 0; off >>= 1) {
             ax += __shfl_xor(ax, off, 64);
             ay += __shfl_xor(ay, off, 64);
             if constexpr (D == 3) az += __shfl_xor(az, off, 64);
@@ -126,7 +127,7 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
             my_sum += ynew;
         }
         // the new coordinates sit in lanes 0..D-1 of the group: assemble the record
-        const int base = threadIdx.x & ~(kAttrLanes - 1);
+        const int base = threadIdx.x & ~(LANES - 1);
         const double n0 = __shfl(ynew, base, 64);
         const double n1 = __shfl(ynew, base + 1, 64);
         const double n2 = (D == 3) ? __shfl(ynew, base + 2, 64) : 0.0;
@@ -143,7 +144,7 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     __syncthreads();
     if (threadIdx.x < D) {
         double t = 0.0;
-        for (int w = 0; w < kAttrGroups; w++) t += gsum[w][threadIdx.x];
+        for (int w = 0; w < (256 / LANES); w++) t += gsum[w][threadIdx.x];
         block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
     }
 }
@@ -467,20 +468,51 @@ perplexity_kernel(int n, const double *__restrict__ dist, const int *__restrict_
 // ---------------------------------------------------------------------------------
 // Host-side launchers (declared in kernels.h).
 // ---------------------------------------------------------------------------------
+namespace {
+template <int D, int L, int BT>
+void attr_launch(int nblocks, hipStream_t st, int n, const int *row_ptr, const int *col_idx,
+                 const float *pval, const void *Yg, const float *F, double *Y64, double *uY,
+                 double *gains, void *Yg_next, const double *scal, double exag, double mom,
+                 double lr, int rpb, double *block_sums)
+{
+    using V = typename Vec<D>::T;
+    hipLaunchKernelGGL((attr_update_kernel<D, L, BT>), dim3(nblocks), dim3(256), 0, st, n,
+                       row_ptr, col_idx, pval, (const V *)Yg, F, Y64, uY, gains, (V *)Yg_next,
+                       scal, 4.0 * exag, mom, lr, rpb, block_sums);
+}
+}  // namespace
+
+// Row width of the attractive kernel: 8, 16 (default) or 32 lanes per row
+// (environment TSNE_ATTR_LANES, read once).
+static int attr_lanes()
+{
+    static int lanes = [] {
+        const char *e = getenv("TSNE_ATTR_LANES");
+        const int v = e ? atoi(e) : 16;
+        return (v == 8 || v == 32) ? v : 16;
+    }();
+    return lanes;
+}
+
 int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
                        const void *Yg, const float *F, double *Y64, double *uY, double *gains,
                        void *Yg_next, const double *scal, double exag, double mom, double lr,
                        double *block_sums, int nblocks, hipStream_t st)
 {
     const int rpb = (n + nblocks - 1) / nblocks;
-    if (D == 2)
-        hipLaunchKernelGGL(attr_update_kernel<2>, dim3(nblocks), dim3(256), 0, st, n, row_ptr,
-                           col_idx, pval, (const float2 *)Yg, F, Y64, uY, gains,
-                           (float2 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
-    else
-        hipLaunchKernelGGL(attr_update_kernel<3>, dim3(nblocks), dim3(256), 0, st, n, row_ptr,
-                           col_idx, pval, (const float4 *)Yg, F, Y64, uY, gains,
-                           (float4 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
+    const int L = attr_lanes();
+#define TSNE_ATTR_ARGS nblocks, st, n, row_ptr, col_idx, pval, Yg, F, Y64, uY, gains, Yg_next, \
+                       scal, exag, mom, lr, rpb, block_sums
+    if (D == 2) {
+        if (L == 8) attr_launch<2, 8, 12>(TSNE_ATTR_ARGS);
+        else if (L == 32) attr_launch<2, 32, 6>(TSNE_ATTR_ARGS);
+        else attr_launch<2, 16, 10>(TSNE_ATTR_ARGS);
+    } else {
+        if (L == 8) attr_launch<3, 8, 12>(TSNE_ATTR_ARGS);
+        else if (L == 32) attr_launch<3, 32, 6>(TSNE_ATTR_ARGS);
+        else attr_launch<3, 16, 10>(TSNE_ATTR_ARGS);
+    }
+#undef TSNE_ATTR_ARGS
     hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, nblocks, D, n,
                        block_sums, (double *)scal);
     return hipGetLastError() == hipSuccess ? 0 : -1;
