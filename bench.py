@@ -140,13 +140,18 @@ def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol):
             lib.tsne_destroy(ws)
     ws = make_workspace(be, csr, n, dim, precision, steps, leaf, tol)
     try:
-        lib.tsne_enable_timing(ws, 0)
+        # mode 2: HIP events on the job's stream around every attractive+update launch,
+        # read after the run (no per-iteration synchronisation)
+        lib.tsne_enable_timing(ws, 2)
         lib.tsne_synchronize(ws)
         t0 = time.perf_counter()
         rc = lib.tsne_run(ws)
         lib.tsne_synchronize(ws)
         dt = time.perf_counter() - t0
         assert rc == 0
+        ka, kn = ctypes.c_double(), ctypes.c_int()
+        lib.tsne_attr_kernel_time(ws, ctypes.byref(ka), ctypes.byref(kn))
+        attr_kernel_us = 1e3 * ka.value / max(kn.value, 1)
         kl = float(lib.compute_kl_divergence(ws))
         st = None
         from tsne_h2pack._backend import H2Stats
@@ -168,7 +173,8 @@ def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol):
                              ctypes.byref(o), ctypes.byref(nr))
     finally:
         lib.tsne_destroy(ws)
-    return dict(seconds=dt, its=steps / dt, kl=kl, attr_ms=a.value, rebuild_ms=r.value,
+    return dict(seconds=dt, its=steps / dt, kl=kl, attr_kernel_us=attr_kernel_us,
+                attr_kernel_launches=kn.value, attr_ms=a.value, rebuild_ms=r.value,
                 product_ms=pr.value, other_ms=o.value, n_rebuild=nr.value, h2=st,
                 grad_err_final=err, grad_err_after_warmup=err_warm)
 
@@ -215,7 +221,13 @@ def cpu_baseline(csr, n, dim, seconds_target=15.0):
     t0 = time.perf_counter()
     oracle.attractive(P, Y, 1.0)
     t_attr = time.perf_counter() - t0
+    # exact repulsion on a row sample sized to about seconds_target of CPU work
     rows = np.sort(rng.choice(n, size=min(n, 64), replace=False)).astype(np.int32)
+    t0 = time.perf_counter()
+    oracle.repulsive_rows(Y, 1.0, rows)
+    t64 = time.perf_counter() - t0
+    m = int(min(n, max(64, 64 * seconds_target / max(t64, 1e-6))))
+    rows = np.sort(rng.choice(n, size=m, replace=False)).astype(np.int32)
     t0 = time.perf_counter()
     oracle.repulsive_rows(Y, 1.0, rows)
     t_rep = (time.perf_counter() - t0) * n / rows.size
@@ -225,6 +237,75 @@ def cpu_baseline(csr, n, dim, seconds_target=15.0):
             "sample": (f"oracle FP64 (exact O(N^2) repulsion; reference H2 not buildable): "
                        f"full attractive pass {t_attr:.2f}s + exact repulsion on {rows.size} "
                        f"sampled rows scaled to N={n}")}
+
+
+def attr_algorithmic_bytes(n, dim, nnz):
+    """Bytes one attractive+update launch must move at least: P (int32 column + fp32
+    value per nonzero, int32 row pointers), per row its own fp32 gather record, its fp32
+    repulsive force (dim floats), FP64 Y/uY/gains read and written, and the next gather
+    record written.  The neighbour gathers are not counted (they hit the caches)."""
+    gath = 8 if dim == 2 else 16          # float2 / float4 gather record
+    return 8 * nnz + 4 * (n + 1) + n * (2 * gath + 4 * dim + 3 * 2 * 8 * dim)
+
+
+def attr_roofline(n, dim, nnz, kernel_us):
+    by = attr_algorithmic_bytes(n, dim, nnz)
+    ach = by / (kernel_us * 1e-6) / 1e9
+    roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "attr_update_kernel", "kernel_avg_us": kernel_us,
+            "algorithmic_bytes": by}
+    t = measured_traffic(n, dim, nnz)
+    if t is not None:
+        roof["traffic"] = t["bytes"]
+        roof["traffic_source"] = t["source"]
+    return roof
+
+
+def measured_traffic(n, dim, nnz):
+    """HBM bytes per attractive launch from the committed rocprofv3 counter passes
+    (FETCH_SIZE and WRITE_SIZE, KiB per dispatch), each scaled by the ratio of a plain
+    copy's known bytes to what the same counter read for it; None when no pass matches
+    this workload."""
+    path = os.path.join(ROOT, "profiles", "attr_update_kernel_traffic.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        m = json.load(f)
+    w = m.get("workload", {})
+    if (w.get("n"), w.get("dim"), w.get("nnz")) != (n, dim, nnz):
+        return None
+    cf = m["copy_bytes"] / (m["copy_fetch_kib"] * 1024.0)
+    cw = m["copy_bytes"] / (m["copy_write_kib"] * 1024.0)
+    b = m["attr_fetch_kib"] * 1024.0 * cf + m["attr_write_kib"] * 1024.0 * cw
+    return {"bytes": b, "source": os.path.relpath(path, ROOT)}
+
+
+def copy_probe(mib=1024, reps=5):
+    """A plain device copy of known size (torch; run under rocprofv3 --pmc to calibrate
+    FETCH_SIZE / WRITE_SIZE): `reps` copies of `mib` MiB."""
+    import torch
+    a = torch.ones(mib * (1 << 20) // 4, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes": mib * (1 << 20), "reps": reps}), flush=True)
+
+
+def pmc_summary(paths):
+    """Per-kernel mean of every counter over its dispatches, from rocprofv3
+    counter_collection.csv files."""
+    import csv
+    import collections
+    acc = collections.defaultdict(lambda: collections.defaultdict(list))
+    for p in paths:
+        with open(p) as f:
+            for row in csv.DictReader(f):
+                acc[row["Kernel_Name"]][row["Counter_Name"]].append(float(row["Counter_Value"]))
+    for k in sorted(acc):
+        for c, v in sorted(acc[k].items()):
+            print(f"{c:12s} dispatches {len(v):6d} mean {np.mean(v):14.6g} KiB-or-count  {k[:90]}")
 
 
 def main():
@@ -267,14 +348,7 @@ def main():
         log(f"{path}: {res[path]['its']:.1f} it/s {json.dumps(res[path])}")
     head = res.get("mixed") or res[next(iter(res))]
     nnz = int(csr[1].size)
-    # dominant HBM kernel: attractive + fused update (every iteration)
-    attr_bytes = 8 * nnz + 4 * (n + 1) + n * (8 + 8 + 32 + 32 + 32 + 8) if dim == 2 else \
-        8 * nnz + 4 * (n + 1) + n * (16 + 12 + 48 + 48 + 48 + 16)
-    attr_s = head["attr_ms"] * 1e-3 / a.steps  # instrumented run, HIP events on its stream
-    roof = {"bound": "hbm", "achieved": attr_bytes / attr_s / 1e9, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": attr_bytes / attr_s / 1e9 / HBM_PEAK_GBS,
-            "traffic": None, "kernel": "attr_update_kernel (+mean_finalize)",
-            "algorithmic_bytes": attr_bytes}
+    roof = attr_roofline(n, dim, nnz, head["attr_kernel_us"])
     cpu = None if a.no_cpu_baseline else cpu_baseline(csr, n, dim)
     out = {
         "metric": "iters/s at N=1M 2D, mixed vs plain FP32 GPU @<=1e-2 grad err; jobs/s + p99",
@@ -295,7 +369,11 @@ def main():
 
 
 if __name__ == "__main__":
-    if len(sys.argv) > 1 and sys.argv[1] == "--_knn":
+    if len(sys.argv) > 1 and sys.argv[1] == "--copy-probe":
+        copy_probe()
+    elif len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
+        pmc_summary(sys.argv[2:])
+    elif len(sys.argv) > 1 and sys.argv[1] == "--_knn":
         _torch_knn(int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]),
                    int(sys.argv[7]), sys.argv[2])
     else:

@@ -195,9 +195,16 @@ int tsne_run_range(tsne_workspace_t *workspace, int it_begin, int it_end);
 /* Block until every operation queued on the workspace stream is done. */
 int tsne_synchronize(tsne_workspace_t *workspace);
 
-/* Per-phase HIP-event timing of later runs (read with tsne_run_timings); it is also on
- * when params.enable_profiling is set.  0 on success. */
-int tsne_enable_timing(tsne_workspace_t *workspace, int on);
+/* Timing of later runs.  mode 1: per-phase HIP events, synchronising every iteration
+ * (read with tsne_run_timings; also on when params.enable_profiling is set).  mode 2:
+ * HIP events on the workspace stream around every attractive+update kernel launch,
+ * read once after the run (read with tsne_attr_kernel_time).  0 turns both off.
+ * 0 on success. */
+int tsne_enable_timing(tsne_workspace_t *workspace, int mode);
+
+/* Summed duration (ms) and number of the attractive+update kernel launches timed in
+ * the last run under timing mode 2.  0 on success. */
+int tsne_attr_kernel_time(tsne_workspace_t *workspace, double *total_ms, int *launches);
 
 #ifdef __cplusplus
 }

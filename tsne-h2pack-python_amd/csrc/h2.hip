@@ -356,6 +356,45 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     st->rank_max = h->hflags[FLAG_RANK_MAX];
     st->rank_mean = h->n_nodes > 1 ? (double)h->hflags[FLAG_RANK_SUM] / (h->n_nodes - 1) : 0.0;
     st->proxy_max = h->hflags[FLAG_PROXY_MAX];
+    if (!h->usable || h->n_nodes <= 0) return 0;
+    // flop counts of one product from the built structure (host copies; diagnostics only)
+    const int nn = h->n_nodes;
+    std::vector<int> nb(nn), ne(nn), rk(nn), c0(nn), par(nn);
+    std::vector<uint64_t> adm(h->n_adm), xl(h->n_x), wl(h->n_w), nl(h->n_near);
+    bool ok = hipMemcpyAsync(nb.data(), h->d.nb, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+              hipMemcpyAsync(ne.data(), h->d.ne, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+              hipMemcpyAsync(rk.data(), h->d.rank, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+              hipMemcpyAsync(c0.data(), h->d.child0, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+              hipMemcpyAsync(par.data(), h->d.parent, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess;
+    auto cp = [&](std::vector<uint64_t> &v, const uint64_t *src) {
+        return v.empty() || hipMemcpyAsync(v.data(), src, v.size() * sizeof(uint64_t),
+                                           hipMemcpyDeviceToHost, h->st) == hipSuccess;
+    };
+    ok = ok && cp(adm, h->d.adm) && cp(xl, h->d.xl) && cp(wl, h->d.wl) && cp(nl, h->d.nl) &&
+         hipStreamSynchronize(h->st) == hipSuccess;
+    if (!ok) return -1;
+    const double D = h->D;
+    auto tgt = [](uint64_t k) { return (int)(k >> 32); };
+    auto src = [](uint64_t k) { return (int)(uint32_t)k; };
+    auto npt = [&](int i) { return (double)(ne[i] - nb[i]); };
+    double near_pairs = 0.0;
+    for (uint64_t k : nl) near_pairs += npt(tgt(k)) * npt(src(k));
+    // kernel evaluation (3D+3) and accumulation into the D+1 coefficient columns 2(D+1)
+    const double per_entry = (3 * D + 3) + 2 * (D + 1);
+    double far_entries = 0.0;
+    for (uint64_t k : adm) far_entries += (double)rk[tgt(k)] * rk[src(k)];
+    for (uint64_t k : xl) far_entries += (double)rk[tgt(k)] * npt(src(k));
+    for (uint64_t k : wl) far_entries += npt(tgt(k)) * rk[src(k)];
+    // downward transfer (the rows of U_p that belong to child i) and the leaf
+    // expansion U z; the root (level 0) has no basis
+    double basis = 0.0;
+    for (int i = 1; i < nn; i++) {
+        if (c0[i] < 0) basis += npt(i) * rk[i];
+        if (par[i] > 0) basis += (double)rk[i] * rk[par[i]];
+    }
+    st->near_point_pairs = (long long)near_pairs;
+    st->near_flops = near_pairs * per_entry;
+    st->far_flops = far_entries * per_entry + basis * 2 * (D + 1);
     return 0;
 }
 

@@ -20,7 +20,8 @@ enum ScalarSlot {
 int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
                        const void *Yg, const float *F, double *Y64, double *uY, double *gains,
                        void *Yg_next, const double *scal, double exag, double mom, double lr,
-                       double *block_sums, int nblocks, hipStream_t st);
+                       double *block_sums, int nblocks, hipStream_t st,
+                       hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 int launch_repulsive_direct_f32(int D, int n, const void *Yg, float *s, float *F,
                                 hipStream_t st);
 int launch_z(int D, int n, const void *Yg, const float *s, const float *F, double *scal,

@@ -497,10 +497,12 @@ static int attr_lanes()
 int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
                        const void *Yg, const float *F, double *Y64, double *uY, double *gains,
                        void *Yg_next, const double *scal, double exag, double mom, double lr,
-                       double *block_sums, int nblocks, hipStream_t st)
+                       double *block_sums, int nblocks, hipStream_t st, hipEvent_t ev_begin,
+                       hipEvent_t ev_end)
 {
     const int rpb = (n + nblocks - 1) / nblocks;
     const int L = attr_lanes();
+    if (ev_begin) (void)hipEventRecord(ev_begin, st);
 #define TSNE_ATTR_ARGS nblocks, st, n, row_ptr, col_idx, pval, Yg, F, Y64, uY, gains, Yg_next, \
                        scal, exag, mom, lr, rpb, block_sums
     if (D == 2) {
@@ -513,6 +515,7 @@ int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, con
         else attr_launch<3, 16, 10>(TSNE_ATTR_ARGS);
     }
 #undef TSNE_ATTR_ARGS
+    if (ev_end) (void)hipEventRecord(ev_end, st);
     hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, nblocks, D, n,
                        block_sums, (double *)scal);
     return hipGetLastError() == hipSuccess ? 0 : -1;

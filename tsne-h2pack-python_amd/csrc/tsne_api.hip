@@ -46,7 +46,9 @@ struct tsne_workspace {
     // timings of the last run (ms) and counters
     double t_attr = 0, t_rebuild = 0, t_product = 0, t_other = 0;
     int n_rebuild = 0;
-    int timing = 0;
+    int timing = 0;             // 1: per-phase events (synchronising), 2: attractive kernel only
+    double t_attr_kernel = 0;   // timing 2: summed attractive-kernel time (ms) and launches
+    int n_attr_kernel = 0;
     double knn_time = 0, p_time = 0;
     std::vector<double> prof;  // per-iteration phase times when profiling
     // renumbering along the tree order (reference src/tsne_h2pack.c:1706-1791)
@@ -532,12 +534,22 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
     const int D = ws->dim;
     if (it_begin < 0) it_begin = 0;
     if (it_end > n_iter) it_end = n_iter;
-    const bool timing = ws->timing || pp.enable_profiling;
+    const bool timing = ws->timing == 1 || pp.enable_profiling;
     hipEvent_t ev[5];
     if (timing)
         for (auto &e : ev) (void)hipEventCreate(&e);
+    // timing 2: a pair of events around every attractive launch, read after the last
+    // synchronisation, so the run itself is not slowed by host round trips
+    std::vector<hipEvent_t> kev;
+    if (ws->timing == 2 && it_end > it_begin) {
+        kev.resize(2 * (size_t)(it_end - it_begin), nullptr);
+        for (auto &e : kev)
+            if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    }
     if (it_begin == 0) {
         ws->t_attr = ws->t_rebuild = ws->t_product = ws->t_other = 0;
+        ws->t_attr_kernel = 0;
+        ws->n_attr_kernel = 0;
         ws->n_rebuild = 0;
         if (pp.enable_profiling) ws->prof.assign((size_t)n_iter * 4, 0.0);
     }
@@ -592,7 +604,9 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         const int nxt = ws->cur ^ 1;
         if (launch_attr_update(D, ws->n, ws->d_rowptr, ws->d_col, ws->d_val32, ws->d_Yg[ws->cur],
                                ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains, ws->d_Yg[nxt], ws->d_scal,
-                               exag, mom, lr, ws->d_block_sums, ws->nblocks_upd, ws->st) != 0)
+                               exag, mom, lr, ws->d_block_sums, ws->nblocks_upd, ws->st,
+                               kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin)],
+                               kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin) + 1]) != 0)
             return -1;
         ws->cur = nxt;
         if (timing) {
@@ -626,6 +640,15 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         fprintf(stderr, "tsne_run: device failure\n");
         return -1;
     }
+    for (size_t k = 0; k + 1 < kev.size(); k += 2) {
+        float ms = 0.f;
+        if (kev[k] && kev[k + 1] && hipEventElapsedTime(&ms, kev[k], kev[k + 1]) == hipSuccess) {
+            ws->t_attr_kernel += ms;
+            ws->n_attr_kernel++;
+        }
+    }
+    for (auto &e : kev)
+        if (e) (void)hipEventDestroy(e);
     if (it_end == n_iter) ws->kl = read_scalar(ws, SLOT_KL);
     if (read_scalar(ws, SLOT_BAD) != 0.0) {
         fprintf(stderr, "ERROR: Y contains NaN or Inf\n");
@@ -859,13 +882,22 @@ int tsne_run_timings(tsne_workspace_t *ws, double *attr, double *reb, double *pr
     return 0;
 }
 
+int tsne_attr_kernel_time(tsne_workspace_t *ws, double *total_ms, int *launches)
+{
+    if (!ws) return -1;
+    if (total_ms) *total_ms = ws->t_attr_kernel;
+    if (launches) *launches = ws->n_attr_kernel;
+    return 0;
+}
+
 int tsne_synchronize(tsne_workspace_t *ws)
 {
     if (!ws) return -1;
     return hipStreamSynchronize(ws->st) == hipSuccess ? 0 : -1;
 }
 
-// Turns per-phase event timing on or off for later runs (also on with profiling).
+// Timing of later runs: 0 off, 1 per-phase events (also on with profiling), 2 events
+// around the attractive kernel only.
 int tsne_enable_timing(tsne_workspace_t *ws, int on)
 {
     if (!ws) return -1;

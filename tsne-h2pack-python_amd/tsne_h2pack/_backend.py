@@ -374,6 +374,8 @@ def configure_signatures(lib):
     lib.tsne_synchronize.restype = ctypes.c_int
     lib.tsne_enable_timing.argtypes = [c_ws, ctypes.c_int]
     lib.tsne_enable_timing.restype = ctypes.c_int
+    lib.tsne_attr_kernel_time.argtypes = [c_ws, dp, ip]
+    lib.tsne_attr_kernel_time.restype = ctypes.c_int
 
 
 def scipy_csr_to_sparse_matrix(P_scipy):
