@@ -308,6 +308,170 @@ def pmc_summary(paths):
             print(f"{c:12s} dispatches {len(v):6d} mean {np.mean(v):14.6g} KiB-or-count  {k[:90]}")
 
 
+# ----------------------------------------------------------------------------- serving
+
+def _serving_specs(n_jobs, seed=0):
+    """Job shapes of the serving workload: N log-uniform in [2e3, 2e5], 5-50 clusters,
+    every fourth job 3D (2D:3D = 3:1)."""
+    rng = np.random.default_rng(seed)
+    specs = []
+    for j in range(n_jobs):
+        n = int(round(np.exp(rng.uniform(np.log(2000), np.log(200000)))))
+        specs.append({"n": n, "clusters": int(rng.integers(5, 51)), "dim": 3 if j % 4 == 3 else 2,
+                      "seed": 1000 + j})
+    return specs
+
+
+def _torch_knn_many(spec_file, out_dir):
+    """Child process: k-NN graphs of every serving job (torch on the GPU)."""
+    with open(spec_file) as f:
+        specs = json.load(f)
+    for j, sp in enumerate(specs):
+        _torch_knn(sp["n"], 50, sp["clusters"], 90, sp["seed"], os.path.join(out_dir, f"j{j}"))
+
+
+def _symmetrise_job(args):
+    """Pool worker: joint P of one job from its conditional P files."""
+    path, n = args
+    from scipy.sparse import csr_matrix
+    z = np.load(path)
+    Pc = csr_matrix((z["p"], z["idx"], z["ptr"]), shape=(n, n))
+    P = Pc + Pc.T
+    P.data /= P.sum()
+    np.savez(path.replace("_pc.npz", "_P.npz"), ptr=P.indptr.astype(np.int32),
+             idx=P.indices.astype(np.int32), val=P.data)
+    return path
+
+
+def serving_inputs(specs, tmp):
+    import multiprocessing as mp
+    from tsne_h2pack._backend import get_backend
+    spec_file = os.path.join(tmp, "specs.json")
+    with open(spec_file, "w") as f:
+        json.dump(specs, f)
+    t0 = time.time()
+    _child(["--_knn_many", spec_file, tmp])
+    log(f"serving: k-NN graphs of {len(specs)} jobs in {time.time() - t0:.1f}s")
+    be = get_backend()
+    t0 = time.time()
+    for j, sp in enumerate(specs):
+        base = os.path.join(tmp, f"j{j}")
+        idx = np.load(base + "_idx.npy")
+        dist = np.load(base + "_dist.npy").astype(np.float64)
+        n, k = idx.shape
+        indptr = np.arange(0, (n + 1) * k, k, dtype=np.int32)
+        Pc = be.conditional_P(np.ascontiguousarray(dist.ravel()),
+                              np.ascontiguousarray(idx.ravel().astype(np.int32)), indptr, 30.0)
+        np.savez(base + "_pc.npz", p=Pc.data, idx=Pc.indices.astype(np.int32),
+                 ptr=Pc.indptr.astype(np.int32))
+    log(f"serving: conditional P (library kernel) in {time.time() - t0:.1f}s")
+    t0 = time.time()
+    ctx = mp.get_context("spawn")
+    with ctx.Pool(min(16, os.cpu_count() or 1)) as pool:
+        list(pool.imap_unordered(_symmetrise_job,
+                                 [(os.path.join(tmp, f"j{j}_pc.npz"), sp["n"])
+                                  for j, sp in enumerate(specs)]))
+    log(f"serving: symmetric P in {time.time() - t0:.1f}s")
+
+
+def _load_job_P(tmp, j, n):
+    from scipy.sparse import csr_matrix
+    z = np.load(os.path.join(tmp, f"j{j}_P.npz"))
+    return csr_matrix((z["val"], z["idx"], z["ptr"]), shape=(n, n))
+
+
+def _job_init(sp):
+    return np.random.default_rng(sp["seed"]).normal(scale=1e-2, size=(sp["n"], sp["dim"]))
+
+
+def serving_bench(n_jobs, slots, seq_sample, load=0.7):
+    """Serving workload: capacity (all jobs queued at once), then Poisson arrivals at
+    `load` x capacity with per-job latency (arrival -> result); baseline: one job at a
+    time through tsne_init_with_P + tsne_run (sequential plain runs) on a sample."""
+    import tempfile
+    import shutil
+    from tsne_h2pack._backend import get_backend
+    from tsne_h2pack.serving import TSNEServer
+    specs = _serving_specs(n_jobs)
+    tmp = tempfile.mkdtemp(prefix="tsne_serving_")
+    try:
+        serving_inputs(specs, tmp)
+        Ps = [_load_job_P(tmp, j, sp["n"]) for j, sp in enumerate(specs)]
+        inits = [_job_init(sp) for sp in specs]
+    finally:
+        shutil.rmtree(tmp, ignore_errors=True)
+    be = get_backend()
+    lib = be.lib
+    # sequential baseline on every (n_jobs // seq_sample)-th job
+    step = max(1, n_jobs // max(seq_sample, 1))
+    sample = list(range(0, n_jobs, step))[:seq_sample]
+    t_seq, kl_seq = 0.0, {}
+    for j in sample:
+        sp = specs[j]
+        p = be.create_default_params()
+        p.verbose = 0
+        p.learning_rate = max(sp["n"] / 12.0, 200.0)
+        t0 = time.perf_counter()
+        ws = be.init_workspace(Ps[j], sp["dim"], p, init_Y=inits[j])
+        assert lib.tsne_run(ws) == 0
+        Y = np.zeros((sp["dim"], sp["n"]))
+        lib.tsne_get_embedding(ws, Y)
+        kl_seq[j] = float(lib.compute_kl_divergence(ws))
+        lib.tsne_destroy(ws)
+        t_seq += time.perf_counter() - t0
+        log(f"serving: alone job {j} n={sp['n']} d={sp['dim']} {time.perf_counter() - t0:.2f}s")
+    seq_rate = len(sample) / t_seq
+    log(f"serving: sequential plain runs {seq_rate:.2f} jobs/s ({len(sample)} jobs, {t_seq:.1f}s)")
+
+    def run(arrivals):
+        with TSNEServer(slots=slots) as srv:
+            ids = []
+            t0 = time.perf_counter()
+            for j, sp in enumerate(specs):
+                if arrivals is not None:
+                    dt = arrivals[j] - (time.perf_counter() - t0)
+                    if dt > 0:
+                        time.sleep(dt)
+                ids.append(srv.submit(Ps[j], sp["dim"], srv.default_params(sp["n"]),
+                                      init_Y=inits[j]))
+            infos = []
+            for k, (i, sp) in enumerate(zip(ids, specs)):
+                infos.append(srv.wait(i, sp["n"], sp["dim"])[1])
+                if (k + 1) % 32 == 0:
+                    log(f"serving: {k + 1}/{n_jobs} jobs done ({time.perf_counter() - t0:.1f}s)")
+            wall = time.perf_counter() - t0
+        return infos, wall
+
+    infos_c, wall_c = run(None)
+    cap = n_jobs / wall_c
+    log(f"serving: capacity {cap:.2f} jobs/s ({n_jobs} jobs queued at once, {wall_c:.1f}s)")
+    rng = np.random.default_rng(1)
+    rate = load * cap
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_jobs))
+    infos_p, wall_p = run(arrivals)
+    lat = np.array([i["latency_s"] for i in infos_p])
+    done = np.array([i["end_s"] for i in infos_p])
+    rate_p = n_jobs / (done.max() - min(i["submit_s"] for i in infos_p))
+    kl_mix = {j: infos_c[j]["kl"] for j in sample}
+    kl_dev = max(abs(kl_mix[j] - kl_seq[j]) / abs(kl_seq[j]) for j in sample)
+    log(f"serving: Poisson arrivals at {rate:.2f} jobs/s: p50 {np.percentile(lat, 50):.2f}s "
+        f"p99 {np.percentile(lat, 99):.2f}s")
+    return {
+        "metric": "jobs/s + p99 (serving: 256 t-SNE jobs, N log-uniform 2e3..2e5, 2D:3D 3:1)",
+        "value": cap, "unit": "jobs/s", "n_gpus": 1, "higher_is_better": True,
+        "dtype": "f32", "data": "synthetic (seeded Gaussian mixtures, exact k-NN on GPU)",
+        "config": {"workload": f"serving {n_jobs} jobs, 1000 iterations each, slots {slots}",
+                   "jobs": n_jobs, "slots": slots, "n_total": int(sum(s["n"] for s in specs))},
+        "capacity_jobs_per_s": cap,
+        "sequential_jobs_per_s": seq_rate, "sequential_sample": len(sample),
+        "speedup_vs_sequential": cap / seq_rate,
+        "poisson": {"offered_jobs_per_s": rate, "achieved_jobs_per_s": rate_p,
+                    "p50_s": float(np.percentile(lat, 50)),
+                    "p99_s": float(np.percentile(lat, 99)), "max_s": float(lat.max())},
+        "kl_max_rel_diff_mix_vs_alone": kl_dev,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -322,11 +486,19 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--save-inputs", default="", help="write the prepared P here and exit")
     ap.add_argument("--load-inputs", default="", help="read a P written by --save-inputs")
+    ap.add_argument("--serving", action="store_true",
+                    help="serving workload (jobs/s, p50/p99) instead of the 1M iteration rate")
+    ap.add_argument("--jobs", type=int, default=256)
+    ap.add_argument("--slots", type=int, default=8)
+    ap.add_argument("--seq-sample", type=int, default=32)
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the reference is single-process; see DESIGN.md on sharding)", file=sys.stderr)
         sys.exit(2)
+    if a.serving:
+        print(json.dumps(serving_bench(a.jobs, a.slots, a.seq_sample)), flush=True)
+        return
     from tsne_h2pack._backend import get_backend
     be = get_backend()
     n, dim = a.n, a.dim
@@ -373,6 +545,8 @@ if __name__ == "__main__":
         copy_probe()
     elif len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
         pmc_summary(sys.argv[2:])
+    elif len(sys.argv) > 1 and sys.argv[1] == "--_knn_many":
+        _torch_knn_many(sys.argv[2], sys.argv[3])
     elif len(sys.argv) > 1 and sys.argv[1] == "--_knn":
         _torch_knn(int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]),
                    int(sys.argv[7]), sys.argv[2])

@@ -206,6 +206,59 @@ int tsne_enable_timing(tsne_workspace_t *workspace, int mode);
  * the last run under timing mode 2.  0 on success. */
 int tsne_attr_kernel_time(tsne_workspace_t *workspace, double *total_ms, int *launches);
 
+/* ------------------------------------------------------------------------------------
+ * [add] Job server: continuous batching of independent jobs on one GPU.  No counterpart
+ * in the reference (one job per process there).  Jobs are admitted in arrival order
+ * into `slots` concurrent slots while their estimated device footprint fits the HBM
+ * budget; each runs on its own HIP stream with its state resident on the device, and a
+ * finished job's slot and memory go to the next queued job at once.  A job's result is
+ * the same as that of tsne_init_with_P + tsne_run + tsne_get_embedding on the same
+ * inputs.
+ * ---------------------------------------------------------------------------------- */
+#define TSNE_JOB_QUEUED 0
+#define TSNE_JOB_RUNNING 1
+#define TSNE_JOB_DONE 2
+#define TSNE_JOB_FAILED (-1)
+
+typedef struct tsne_server tsne_server_t;
+
+typedef struct {
+    int job_id;
+    int status;                /* TSNE_JOB_* */
+    int n_point, low_dim;
+    double kl;                 /* final KL divergence (done jobs) */
+    double submit_s, start_s, end_s;  /* seconds since the server was created */
+    long long footprint_bytes; /* device memory reserved for the job at admission */
+} tsne_job_info_t;
+
+/* Estimated peak device bytes of one job (the admission charge). */
+long long tsne_job_footprint(int n_point, int low_dim, long long nnz);
+
+/* slots: concurrent jobs (1..64); hbm_budget_bytes <= 0: 90% of the free device memory.
+ * NULL without a device. */
+tsne_server_t *tsne_server_create(int slots, long long hbm_budget_bytes);
+
+/* Queue a job: P is the joint P (n_point x n_point, from sparse_matrix_create), taken by
+ * the server on success; init_Y (low_dim x n_point, may be NULL for the seeded random
+ * init) is copied.  Returns the job id (>= 0), or -1 (P not taken). */
+int tsne_server_submit(tsne_server_t *server, int n_point, int low_dim,
+                       const tsne_params_t *params, sparse_matrix_t *P, const double *init_Y);
+
+/* Status of a job (TSNE_JOB_*), -2 for an unknown id; fills *info if not NULL. */
+int tsne_server_poll(tsne_server_t *server, int job_id, tsne_job_info_t *info);
+
+/* Block until the job ends; copy its embedding (low_dim x n_point, caller order) to Y_out
+ * if not NULL, fill *info, and release the job.  0 if it succeeded, -1 otherwise. */
+int tsne_server_wait(tsne_server_t *server, int job_id, double *Y_out, tsne_job_info_t *info);
+
+/* Queued and running job counts and the device bytes charged to running jobs. */
+int tsne_server_status(tsne_server_t *server, int *queued, int *running,
+                       long long *hbm_in_use);
+
+/* Drops queued jobs, waits for running ones, frees everything.  Call it after every
+ * tsne_server_wait of other threads has returned. */
+void tsne_server_destroy(tsne_server_t *server);
+
 #ifdef __cplusplus
 }
 #endif

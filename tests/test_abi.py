@@ -98,3 +98,15 @@ def test_init_rejects_bad_dims_without_device():
     assert not be.lib.tsne_init_with_P(None, 4, 0, 4, ctypes.byref(p), ptr)
     assert not be.lib.tsne_init_with_P(None, 5, 0, 2, ctypes.byref(p), ptr)
     be.lib.sparse_matrix_destroy(ptr)
+
+
+def test_job_footprint_and_server_without_device():
+    """The admission charge grows with N and nnz; without a device the server cannot be
+    created (no host fallback)."""
+    be = B.get_backend()
+    f = be.lib.tsne_job_footprint
+    assert f(1000, 2, 90000) < f(2000, 2, 90000) < f(2000, 2, 180000) < f(2000, 3, 180000)
+    assert f(1_000_000, 2, 150_000_000) > 7_000_000_000
+    import os
+    if not os.path.exists("/dev/kfd"):
+        assert not be.lib.tsne_server_create(4, 0)

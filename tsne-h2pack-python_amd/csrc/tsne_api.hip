@@ -9,6 +9,7 @@
 #include <string>
 #include <vector>
 
+#include "api_internal.h"
 #include "common.h"
 #include "h2.h"
 #include "kernels.h"
@@ -405,6 +406,21 @@ tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int
                                    const tsne_params_t *params, sparse_matrix_t *P)
 {
     (void)X;
+    bool taken = false;
+    return tsne::init_with_P(n_point, pt_dim, low_dim, params, P, &taken);
+}
+
+}  // extern "C"
+
+namespace tsne {
+
+// Body of tsne_init_with_P.  *taken tells whether P now belongs to the library (it does
+// once the workspace exists, also when the initial build then fails and the workspace
+// is destroyed with it).
+tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_params_t *params,
+                              sparse_matrix_t *P, bool *taken)
+{
+    *taken = false;
     if (low_dim != 2 && low_dim != 3) {
         fprintf(stderr, "Error: low_dim must be 2 or 3 (got %d)\n", low_dim);
         return nullptr;
@@ -511,6 +527,7 @@ tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int
         return nullptr;
     }
     ws->P = P;  // ownership taken once the workspace exists
+    *taken = true;
     if (!use_direct(ws)) {
         if (rebuild(ws) != 0) {
             fprintf(stderr, "Error: Failed to build initial H2 matrix\n");
@@ -523,6 +540,10 @@ tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int
                use_direct(ws) ? "direct" : "H2");
     return ws;
 }
+
+}  // namespace tsne
+
+extern "C" {
 
 int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
 {

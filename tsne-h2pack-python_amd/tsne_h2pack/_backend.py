@@ -87,6 +87,16 @@ class H2Stats(ctypes.Structure):
     ]
 
 
+class JobInfo(ctypes.Structure):
+    """tsne_job_info_t."""
+    _fields_ = [
+        ('job_id', ctypes.c_int), ('status', ctypes.c_int), ('n_point', ctypes.c_int),
+        ('low_dim', ctypes.c_int), ('kl', ctypes.c_double), ('submit_s', ctypes.c_double),
+        ('start_s', ctypes.c_double), ('end_s', ctypes.c_double),
+        ('footprint_bytes', ctypes.c_longlong),
+    ]
+
+
 def _optional_ndpointer(**kwargs):
     """ndpointer that also accepts None (NULL)."""
     base = np.ctypeslib.ndpointer(**kwargs)
@@ -376,6 +386,24 @@ def configure_signatures(lib):
     lib.tsne_enable_timing.restype = ctypes.c_int
     lib.tsne_attr_kernel_time.argtypes = [c_ws, dp, ip]
     lib.tsne_attr_kernel_time.restype = ctypes.c_int
+    # job server
+    c_srv = ctypes.c_void_p
+    lp = ctypes.POINTER(ctypes.c_longlong)
+    f64n = _optional_ndpointer(dtype=NP_DTYPE, flags='C_CONTIGUOUS')
+    lib.tsne_job_footprint.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_longlong]
+    lib.tsne_job_footprint.restype = ctypes.c_longlong
+    lib.tsne_server_create.argtypes = [ctypes.c_int, ctypes.c_longlong]
+    lib.tsne_server_create.restype = c_srv
+    lib.tsne_server_submit.argtypes = [c_srv, ctypes.c_int, ctypes.c_int, c_pp, c_sm, f64n]
+    lib.tsne_server_submit.restype = ctypes.c_int
+    lib.tsne_server_poll.argtypes = [c_srv, ctypes.c_int, ctypes.POINTER(JobInfo)]
+    lib.tsne_server_poll.restype = ctypes.c_int
+    lib.tsne_server_wait.argtypes = [c_srv, ctypes.c_int, f64n, ctypes.POINTER(JobInfo)]
+    lib.tsne_server_wait.restype = ctypes.c_int
+    lib.tsne_server_status.argtypes = [c_srv, ip, ip, lp]
+    lib.tsne_server_status.restype = ctypes.c_int
+    lib.tsne_server_destroy.argtypes = [c_srv]
+    lib.tsne_server_destroy.restype = None
 
 
 def scipy_csr_to_sparse_matrix(P_scipy):
