@@ -196,3 +196,35 @@ def test_renumbering_keeps_structure(backend, problem):
     overlap = np.mean([len(set(a) & set(b)) for a, b in zip(hd, ld)]) / k
     _say(f"  10-NN overlap {overlap:.3f}")
     assert overlap > 0.15
+
+
+def test_default_tol_100k_early_iterations(backend):
+    """Reference defaults (h2_tol 1e-3, rebuild every 10) on N=100,000 from a compressed
+    start (init scale 1e-2): the U pool grows past its first size, ranks reach the cap,
+    and the interaction lists fill several rebuilds of list and sort buffers.  Regression
+    test for pair lists written past their buffers after the sort scratch was exchanged
+    with a smaller list."""
+    from scipy.sparse import csr_matrix
+    _say("default tol N=100000")
+    n, k = 100_000, 30
+    X, _ = gaussian_mixture(n, dim=10, n_clusters=30, seed=8, dirichlet=True,
+                            std_range=(0.5, 2.0), dtype=np.float32)
+    G = knn_graph(X, k)
+    P = joint_P_oracle(G, 10.0)
+    p = backend.create_default_params()
+    p.verbose = 0
+    p.n_iter = 40
+    p.learning_rate = n / 12.0
+    Y0 = np.random.default_rng(8).normal(scale=1e-2, size=(n, 2))
+    ws = backend.init_workspace(P, 2, p, init_Y=Y0)
+    try:
+        assert backend.lib.tsne_run(ws) == 0
+        st = H2Stats()
+        assert backend.lib.tsne_h2_stats(ws, ctypes.byref(st)) == 0
+        Y = np.zeros((2, n))
+        backend.lib.tsne_get_embedding(ws, Y)
+    finally:
+        backend.lib.tsne_destroy(ws)
+    _say(f"  nodes {st.n_nodes} far pairs {st.n_adm_pairs} near pairs {st.n_near_pairs} "
+         f"rank max {st.rank_max}")
+    assert np.isfinite(Y).all()

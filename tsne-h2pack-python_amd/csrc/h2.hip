@@ -103,8 +103,10 @@ bool alloc_sized(H2 *h)
               dmalloc(&d.bounds, nc * (NCH + 1)) && dmalloc(&d.childcnt, nc) &&
               dmalloc(&d.childoff, nc) && dmalloc(&d.near, nc * nearcap_of(D)) &&
               dmalloc(&d.near_cnt, nc) && dmalloc(&d.leaves, nc) &&
-              dmalloc(&d.adm, (size_t)d.adm_cap) && dmalloc(&d.xl, (size_t)d.x_cap) &&
-              dmalloc(&d.wl, (size_t)d.w_cap) && dmalloc(&d.nl, (size_t)d.nl_cap) &&
+              // the four lists and the sort scratch trade buffers after every sort (the
+              // sorted keys stay where the sort left them), so all five have the same size
+              dmalloc(&d.adm, (size_t)maxpair) && dmalloc(&d.xl, (size_t)maxpair) &&
+              dmalloc(&d.wl, (size_t)maxpair) && dmalloc(&d.nl, (size_t)maxpair) &&
               dmalloc(&d.pend[0], (size_t)d.pend_cap) && dmalloc(&d.pend[1], (size_t)d.pend_cap) &&
               dmalloc(&d.sort_tmp, (size_t)maxpair) &&
               dmalloc(&d.adm_beg, nc) && dmalloc(&d.adm_end, nc) && dmalloc(&d.x_beg, nc) &&
@@ -139,8 +141,8 @@ int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end)
     size_t bytes = h->cub_bytes;
     if (hipcub::DeviceRadixSort::SortKeys(h->cub_tmp, bytes, db, n, 0, bits, h->st) != hipSuccess)
         return -1;
-    if (db.Current() != buf) {  // keep the sorted list in the list's own buffer
-        std::swap(buf, d.sort_tmp);
+    if (db.Current() != buf) {  // the sorted list now lives in the scratch buffer:
+        std::swap(buf, d.sort_tmp);  // exchange them (both hold maxpair entries)
     }
     return launch_pair_offsets(buf, n, beg, end, h->st);
 }
