@@ -42,19 +42,20 @@ struct H2 {
 
 namespace {
 
-template <typename T> bool dmalloc(T **p, size_t n)
+// Stream-ordered device memory on the structure's stream (see the workspace allocator).
+template <typename T> bool dmalloc(T **p, size_t n, hipStream_t st)
 {
     if (n == 0) n = 1;
-    if (hipMalloc((void **)p, n * sizeof(T)) != hipSuccess) {
+    if (hipMallocAsync((void **)p, n * sizeof(T), st) != hipSuccess) {
         *p = nullptr;
         return false;
     }
     return true;
 }
 
-template <typename T> void dfree(T *&p)
+template <typename T> void dfree(T *&p, hipStream_t st)
 {
-    if (p) (void)hipFree((void *)p);
+    if (p) (void)hipFreeAsync((void *)p, st);
     p = nullptr;
 }
 
@@ -66,16 +67,16 @@ int pmax_of(int D) { return D == 2 ? H2Cfg<2>::PMAX : H2Cfg<3>::PMAX; }
 void free_sized(H2 *h)
 {
     H2Dev &d = h->d;
-    dfree(d.nb); dfree(d.ne); dfree(d.level); dfree(d.parent); dfree(d.child0); dfree(d.nchild);
-    dfree(d.cent); dfree(d.icoord); dfree(d.bounds); dfree(d.childcnt); dfree(d.childoff);
-    dfree(d.near); dfree(d.near_cnt); dfree(d.leaves);
-    dfree(d.adm); dfree(d.xl); dfree(d.wl); dfree(d.nl); dfree(d.pend[0]); dfree(d.pend[1]);
-    dfree(d.sort_tmp);
-    dfree(d.adm_beg); dfree(d.adm_end); dfree(d.x_beg); dfree(d.x_end);
-    dfree(d.w_beg); dfree(d.w_end); dfree(d.nl_beg); dfree(d.nl_end);
-    dfree(d.rank); dfree(d.uoff); dfree(d.skel); dfree(d.skelf); dfree(d.upool);
-    dfree(d.wc); dfree(d.zc);
-    dfree(h->cub_tmp);
+    dfree(d.nb, h->st); dfree(d.ne, h->st); dfree(d.level, h->st); dfree(d.parent, h->st); dfree(d.child0, h->st); dfree(d.nchild, h->st);
+    dfree(d.cent, h->st); dfree(d.icoord, h->st); dfree(d.bounds, h->st); dfree(d.childcnt, h->st); dfree(d.childoff, h->st);
+    dfree(d.near, h->st); dfree(d.near_cnt, h->st); dfree(d.leaves, h->st);
+    dfree(d.adm, h->st); dfree(d.xl, h->st); dfree(d.wl, h->st); dfree(d.nl, h->st); dfree(d.pend[0], h->st); dfree(d.pend[1], h->st);
+    dfree(d.sort_tmp, h->st);
+    dfree(d.adm_beg, h->st); dfree(d.adm_end, h->st); dfree(d.x_beg, h->st); dfree(d.x_end, h->st);
+    dfree(d.w_beg, h->st); dfree(d.w_end, h->st); dfree(d.nl_beg, h->st); dfree(d.nl_end, h->st);
+    dfree(d.rank, h->st); dfree(d.uoff, h->st); dfree(d.skel, h->st); dfree(d.skelf, h->st); dfree(d.upool, h->st);
+    dfree(d.wc, h->st); dfree(d.zc, h->st);
+    dfree(h->cub_tmp, h->st);
     h->cub_bytes = 0;
 }
 
@@ -97,24 +98,24 @@ bool alloc_sized(H2 *h)
     d.pend_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 8 : 32));
     d.pool_cap = (long long)(h->pool_f * ((double)h->n * (D == 2 ? 20 : 48) + (double)nc * (D == 2 ? 64 : 256)));
     const int maxpair = std::max({d.adm_cap, d.x_cap, d.w_cap, d.nl_cap});
-    bool ok = dmalloc(&d.nb, nc) && dmalloc(&d.ne, nc) && dmalloc(&d.level, nc) &&
-              dmalloc(&d.parent, nc) && dmalloc(&d.child0, nc) && dmalloc(&d.nchild, nc) &&
-              dmalloc(&d.cent, nc) && dmalloc(&d.icoord, nc * 3) &&
-              dmalloc(&d.bounds, nc * (NCH + 1)) && dmalloc(&d.childcnt, nc) &&
-              dmalloc(&d.childoff, nc) && dmalloc(&d.near, nc * nearcap_of(D)) &&
-              dmalloc(&d.near_cnt, nc) && dmalloc(&d.leaves, nc) &&
+    bool ok = dmalloc(&d.nb, nc, h->st) && dmalloc(&d.ne, nc, h->st) && dmalloc(&d.level, nc, h->st) &&
+              dmalloc(&d.parent, nc, h->st) && dmalloc(&d.child0, nc, h->st) && dmalloc(&d.nchild, nc, h->st) &&
+              dmalloc(&d.cent, nc, h->st) && dmalloc(&d.icoord, nc * 3, h->st) &&
+              dmalloc(&d.bounds, nc * (NCH + 1), h->st) && dmalloc(&d.childcnt, nc, h->st) &&
+              dmalloc(&d.childoff, nc, h->st) && dmalloc(&d.near, nc * nearcap_of(D), h->st) &&
+              dmalloc(&d.near_cnt, nc, h->st) && dmalloc(&d.leaves, nc, h->st) &&
               // the four lists and the sort scratch trade buffers after every sort (the
               // sorted keys stay where the sort left them), so all five have the same size
-              dmalloc(&d.adm, (size_t)maxpair) && dmalloc(&d.xl, (size_t)maxpair) &&
-              dmalloc(&d.wl, (size_t)maxpair) && dmalloc(&d.nl, (size_t)maxpair) &&
-              dmalloc(&d.pend[0], (size_t)d.pend_cap) && dmalloc(&d.pend[1], (size_t)d.pend_cap) &&
-              dmalloc(&d.sort_tmp, (size_t)maxpair) &&
-              dmalloc(&d.adm_beg, nc) && dmalloc(&d.adm_end, nc) && dmalloc(&d.x_beg, nc) &&
-              dmalloc(&d.x_end, nc) && dmalloc(&d.w_beg, nc) && dmalloc(&d.w_end, nc) &&
-              dmalloc(&d.nl_beg, nc) && dmalloc(&d.nl_end, nc) && dmalloc(&d.rank, nc) &&
-              dmalloc(&d.uoff, nc) && dmalloc(&d.skel, nc * RMAX * D) &&
-              dmalloc(&d.skelf, nc * RMAX) && dmalloc(&d.upool, (size_t)d.pool_cap) &&
-              dmalloc(&d.wc, nc * RMAX) && dmalloc(&d.zc, nc * RMAX);
+              dmalloc(&d.adm, (size_t)maxpair, h->st) && dmalloc(&d.xl, (size_t)maxpair, h->st) &&
+              dmalloc(&d.wl, (size_t)maxpair, h->st) && dmalloc(&d.nl, (size_t)maxpair, h->st) &&
+              dmalloc(&d.pend[0], (size_t)d.pend_cap, h->st) && dmalloc(&d.pend[1], (size_t)d.pend_cap, h->st) &&
+              dmalloc(&d.sort_tmp, (size_t)maxpair, h->st) &&
+              dmalloc(&d.adm_beg, nc, h->st) && dmalloc(&d.adm_end, nc, h->st) && dmalloc(&d.x_beg, nc, h->st) &&
+              dmalloc(&d.x_end, nc, h->st) && dmalloc(&d.w_beg, nc, h->st) && dmalloc(&d.w_end, nc, h->st) &&
+              dmalloc(&d.nl_beg, nc, h->st) && dmalloc(&d.nl_end, nc, h->st) && dmalloc(&d.rank, nc, h->st) &&
+              dmalloc(&d.uoff, nc, h->st) && dmalloc(&d.skel, nc * RMAX * D, h->st) &&
+              dmalloc(&d.skelf, nc * RMAX, h->st) && dmalloc(&d.upool, (size_t)d.pool_cap, h->st) &&
+              dmalloc(&d.wc, nc * RMAX, h->st) && dmalloc(&d.zc, nc * RMAX, h->st);
     if (!ok) return false;
     // temporary storage for the library sorts / scans (largest of the three uses)
     size_t b1 = 0, b2 = 0, b3 = 0;
@@ -125,7 +126,7 @@ bool alloc_sized(H2 *h)
     hipcub::DoubleBuffer<uint64_t> pb(d.adm, d.sort_tmp);
     (void)hipcub::DeviceRadixSort::SortKeys(nullptr, b3, pb, maxpair, 0, 64, h->st);
     h->cub_bytes = std::max({b1, b2, b3});
-    return hipMalloc(&h->cub_tmp, h->cub_bytes) == hipSuccess;
+    return hipMallocAsync(&h->cub_tmp, h->cub_bytes, h->st) == hipSuccess;
 }
 
 int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end)
@@ -267,14 +268,14 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
     d.leaf = h->leaf;
     h->nblk = std::max(1, std::min(512, (n + 4095) / 4096));
     const size_t vb = (dim == 2) ? sizeof(float2) : sizeof(float4);
-    bool ok = dmalloc(&d.keys, n) && dmalloc(&h->keys_alt, n) && dmalloc(&d.perm, n) &&
-              dmalloc(&h->perm_alt, n) && dmalloc((char **)&d.Ys, (size_t)n * vb) &&
-              dmalloc(&d.root, 1) && dmalloc(&d.flags, FLAG_COUNT) && dmalloc(&d.cnt, CNT_COUNT) &&
-              dmalloc(&h->part, (size_t)h->nblk * 7) &&
+    bool ok = dmalloc(&d.keys, n, h->st) && dmalloc(&h->keys_alt, n, h->st) && dmalloc(&d.perm, n, h->st) &&
+              dmalloc(&h->perm_alt, n, h->st) && dmalloc((char **)&d.Ys, (size_t)n * vb, h->st) &&
+              dmalloc(&d.root, 1, h->st) && dmalloc(&d.flags, FLAG_COUNT, h->st) && dmalloc(&d.cnt, CNT_COUNT, h->st) &&
+              dmalloc(&h->part, (size_t)h->nblk * 7, h->st) &&
               hipHostMalloc((void **)&h->pin, sizeof(int) * 64, hipHostMallocDefault) == hipSuccess;
     if (ok) {
         const ProxyTable &pt = proxy_table(dim, h->ptol, 1.0, pmax_of(dim));
-        ok = dmalloc(&h->d_proxy, pt.pts.size()) && dmalloc(&h->d_proxy_cnt, pt.cnt.size()) &&
+        ok = dmalloc(&h->d_proxy, pt.pts.size(), h->st) && dmalloc(&h->d_proxy_cnt, pt.cnt.size(), h->st) &&
              hipMemcpyAsync(h->d_proxy, pt.pts.data(), pt.pts.size() * sizeof(double),
                             hipMemcpyHostToDevice, st) == hipSuccess &&
              hipMemcpyAsync(h->d_proxy_cnt, pt.cnt.data(), pt.cnt.size() * sizeof(int),
@@ -298,9 +299,9 @@ void h2_destroy(H2 *h)
     if (h->st) (void)hipStreamSynchronize(h->st);
     free_sized(h);
     H2Dev &d = h->d;
-    dfree(d.keys); dfree(h->keys_alt); dfree(d.perm); dfree(h->perm_alt); dfree(d.Ys);
-    dfree(d.root); dfree(d.flags); dfree(d.cnt); dfree(h->part);
-    dfree(h->d_proxy); dfree(h->d_proxy_cnt);
+    dfree(d.keys, h->st); dfree(h->keys_alt, h->st); dfree(d.perm, h->st); dfree(h->perm_alt, h->st); dfree(d.Ys, h->st);
+    dfree(d.root, h->st); dfree(d.flags, h->st); dfree(d.cnt, h->st); dfree(h->part, h->st);
+    dfree(h->d_proxy, h->st); dfree(h->d_proxy_cnt, h->st);
     if (h->pin) (void)hipHostFree(h->pin);
     delete h;
 }

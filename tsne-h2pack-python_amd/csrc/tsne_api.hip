@@ -67,11 +67,13 @@ namespace {
 
 const size_t kGatherBytes[4] = {0, 0, sizeof(float2), sizeof(float4)};
 
-template <typename T> int dalloc(T **p, size_t count)
+// Device memory is stream-ordered (hipMallocAsync / hipFreeAsync on the job's stream):
+// allocating or releasing one job's buffers never waits for the other jobs on the device.
+template <typename T> int dalloc(T **p, size_t count, hipStream_t st)
 {
     *p = nullptr;
     if (count == 0) count = 1;
-    if (hipMalloc((void **)p, count * sizeof(T)) != hipSuccess) {
+    if (hipMallocAsync((void **)p, count * sizeof(T), st) != hipSuccess) {
         fprintf(stderr, "tsne_h2pack: device allocation of %zu bytes failed\n", count * sizeof(T));
         *p = nullptr;
         return -1;
@@ -79,9 +81,9 @@ template <typename T> int dalloc(T **p, size_t count)
     return 0;
 }
 
-template <typename T> void dfree(T *&p)
+template <typename T> void dfree(T *&p, hipStream_t st)
 {
-    if (p) (void)hipFree((void *)p);
+    if (p) (void)hipFreeAsync((void *)p, st);
     p = nullptr;
 }
 
@@ -120,23 +122,23 @@ void init_random(double *Y, int n, int dim, unsigned seed)
 
 void free_renumber_buffers(tsne_workspace *ws)
 {
-    dfree(ws->P2.rp); dfree(ws->P2.col); dfree(ws->P2.v32); dfree(ws->P2.v64);
-    dfree(ws->rs.inv); dfree(ws->rs.len); dfree(ws->rs.src); dfree(ws->rs.src_sorted);
-    dfree(ws->rs.col_tmp);
-    if (ws->rs.tmp) { (void)hipFree(ws->rs.tmp); ws->rs.tmp = nullptr; }
-    if (ws->d_tmp) { (void)hipFree(ws->d_tmp); ws->d_tmp = nullptr; }
+    dfree(ws->P2.rp, ws->st); dfree(ws->P2.col, ws->st); dfree(ws->P2.v32, ws->st); dfree(ws->P2.v64, ws->st);
+    dfree(ws->rs.inv, ws->st); dfree(ws->rs.len, ws->st); dfree(ws->rs.src, ws->st); dfree(ws->rs.src_sorted, ws->st);
+    dfree(ws->rs.col_tmp, ws->st);
+    if (ws->rs.tmp) { (void)hipFreeAsync(ws->rs.tmp, ws->st); ws->rs.tmp = nullptr; }
+    if (ws->d_tmp) { (void)hipFreeAsync(ws->d_tmp, ws->st); ws->d_tmp = nullptr; }
 }
 
 void free_device(tsne_workspace *ws)
 {
     free_renumber_buffers(ws);
-    dfree(ws->d_order);
+    dfree(ws->d_order, ws->st);
     if (ws->h2) { h2_destroy(ws->h2); ws->h2 = nullptr; }
-    dfree(ws->d_rowptr); dfree(ws->d_col); dfree(ws->d_val32); dfree(ws->d_val64);
-    dfree(ws->d_Y64); dfree(ws->d_uY); dfree(ws->d_gains);
-    dfree(ws->d_Yg[0]); dfree(ws->d_Yg[1]);
-    dfree(ws->d_s); dfree(ws->d_F); dfree(ws->d_scal);
-    dfree(ws->d_block_sums); dfree(ws->d_partial); dfree(ws->d_rowkl);
+    dfree(ws->d_rowptr, ws->st); dfree(ws->d_col, ws->st); dfree(ws->d_val32, ws->st); dfree(ws->d_val64, ws->st);
+    dfree(ws->d_Y64, ws->st); dfree(ws->d_uY, ws->st); dfree(ws->d_gains, ws->st);
+    dfree(ws->d_Yg[0], ws->st); dfree(ws->d_Yg[1], ws->st);
+    dfree(ws->d_s, ws->st); dfree(ws->d_F, ws->st); dfree(ws->d_scal, ws->st);
+    dfree(ws->d_block_sums, ws->st); dfree(ws->d_partial, ws->st); dfree(ws->d_rowkl, ws->st);
     if (ws->st) { (void)hipStreamDestroy(ws->st); ws->st = nullptr; }
 }
 
@@ -213,13 +215,13 @@ int renumber(tsne_workspace *ws)
         ws->P2.n = n;
         ws->P2.nnz = nnz;
         ws->rs.tmp_bytes = renumber_tmp_bytes(n, nnz);
-        const bool ok = dalloc(&ws->P2.rp, (size_t)n + 1) == 0 && dalloc(&ws->P2.col, nnz) == 0 &&
-                        dalloc(&ws->P2.v32, nnz) == 0 && dalloc(&ws->P2.v64, nnz) == 0 &&
-                        dalloc(&ws->rs.inv, n) == 0 && dalloc(&ws->rs.len, n) == 0 &&
-                        dalloc(&ws->rs.src, nnz) == 0 && dalloc(&ws->rs.src_sorted, nnz) == 0 &&
-                        dalloc(&ws->rs.col_tmp, nnz) == 0 &&
-                        hipMalloc(&ws->rs.tmp, ws->rs.tmp_bytes) == hipSuccess &&
-                        hipMalloc(&ws->d_tmp, (size_t)n * 4 * sizeof(double)) == hipSuccess;
+        const bool ok = dalloc(&ws->P2.rp, (size_t)n + 1, ws->st) == 0 && dalloc(&ws->P2.col, nnz, ws->st) == 0 &&
+                        dalloc(&ws->P2.v32, nnz, ws->st) == 0 && dalloc(&ws->P2.v64, nnz, ws->st) == 0 &&
+                        dalloc(&ws->rs.inv, n, ws->st) == 0 && dalloc(&ws->rs.len, n, ws->st) == 0 &&
+                        dalloc(&ws->rs.src, nnz, ws->st) == 0 && dalloc(&ws->rs.src_sorted, nnz, ws->st) == 0 &&
+                        dalloc(&ws->rs.col_tmp, nnz, ws->st) == 0 &&
+                        hipMallocAsync(&ws->rs.tmp, ws->rs.tmp_bytes, ws->st) == hipSuccess &&
+                        hipMallocAsync(&ws->d_tmp, (size_t)n * 4 * sizeof(double), ws->st) == hipSuccess;
         if (!ok) {
             free_renumber_buffers(ws);
             fprintf(stderr, "WARNING: point renumbering skipped (out of device memory)\n");
@@ -363,15 +365,15 @@ sparse_matrix_t *compute_P_from_knn_distances(const double *knn_distances, const
     if (nnz > 0) memcpy(P->col_idx, knn_indices, (size_t)nnz * sizeof(int));
     if (nnz == 0) return P;
     hipStream_t st;
-    if (hipStreamCreate(&st) != hipSuccess) {
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
         fprintf(stderr, "compute_P_from_knn_distances: no HIP device available\n");
         sparse_matrix_destroy(P);
         return nullptr;
     }
     double *d_dist = nullptr, *d_p = nullptr;
     int *d_ptr = nullptr;
-    int ok = dalloc(&d_dist, nnz) == 0 && dalloc(&d_p, nnz) == 0 &&
-             dalloc(&d_ptr, (size_t)n_samples + 1) == 0;
+    int ok = dalloc(&d_dist, nnz, st) == 0 && dalloc(&d_p, nnz, st) == 0 &&
+             dalloc(&d_ptr, (size_t)n_samples + 1, st) == 0;
     ok = ok &&
          hipMemcpyAsync(d_dist, knn_distances, (size_t)nnz * sizeof(double),
                         hipMemcpyHostToDevice, st) == hipSuccess &&
@@ -381,7 +383,7 @@ sparse_matrix_t *compute_P_from_knn_distances(const double *knn_distances, const
          hipMemcpyAsync(P->values, d_p, (size_t)nnz * sizeof(double), hipMemcpyDeviceToHost,
                         st) == hipSuccess &&
          hipStreamSynchronize(st) == hipSuccess;
-    dfree(d_dist); dfree(d_p); dfree(d_ptr);
+    dfree(d_dist, st); dfree(d_p, st); dfree(d_ptr, st);
     (void)hipStreamDestroy(st);
     if (!ok) {
         fprintf(stderr, "compute_P_from_knn_distances: device failure\n");
@@ -456,22 +458,22 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
     // 128 rows per attractive block; a multiple of 8 blocks for the XCD-aware placement
     ws->nblocks_upd = (((n + 127) / 128) + 7) & ~7;
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
-    bool ok = hipStreamCreate(&ws->st) == hipSuccess;
+    bool ok = hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
         fprintf(stderr, "tsne_init_with_P: no HIP device available\n");
         delete ws;
         return nullptr;
     }
-    ok = dalloc(&ws->d_rowptr, (size_t)n + 1) == 0 && dalloc(&ws->d_col, nnz) == 0 &&
-         dalloc(&ws->d_val32, nnz) == 0 && dalloc(&ws->d_val64, nnz) == 0 &&
-         dalloc(&ws->d_Y64, (size_t)n * D) == 0 && dalloc(&ws->d_uY, (size_t)n * D) == 0 &&
-         dalloc(&ws->d_gains, (size_t)n * D) == 0 &&
-         dalloc((char **)&ws->d_Yg[0], (size_t)n * kGatherBytes[D]) == 0 &&
-         dalloc((char **)&ws->d_Yg[1], (size_t)n * kGatherBytes[D]) == 0 &&
-         dalloc(&ws->d_s, n) == 0 && dalloc(&ws->d_F, (size_t)n * D) == 0 &&
-         dalloc(&ws->d_scal, SLOT_COUNT) == 0 &&
-         dalloc(&ws->d_block_sums, (size_t)ws->nblocks_upd * 3) == 0 &&
-         dalloc(&ws->d_partial, ws->nblocks_red) == 0 && dalloc(&ws->d_rowkl, n) == 0;
+    ok = dalloc(&ws->d_rowptr, (size_t)n + 1, ws->st) == 0 && dalloc(&ws->d_col, nnz, ws->st) == 0 &&
+         dalloc(&ws->d_val32, nnz, ws->st) == 0 && dalloc(&ws->d_val64, nnz, ws->st) == 0 &&
+         dalloc(&ws->d_Y64, (size_t)n * D, ws->st) == 0 && dalloc(&ws->d_uY, (size_t)n * D, ws->st) == 0 &&
+         dalloc(&ws->d_gains, (size_t)n * D, ws->st) == 0 &&
+         dalloc((char **)&ws->d_Yg[0], (size_t)n * kGatherBytes[D], ws->st) == 0 &&
+         dalloc((char **)&ws->d_Yg[1], (size_t)n * kGatherBytes[D], ws->st) == 0 &&
+         dalloc(&ws->d_s, n, ws->st) == 0 && dalloc(&ws->d_F, (size_t)n * D, ws->st) == 0 &&
+         dalloc(&ws->d_scal, SLOT_COUNT, ws->st) == 0 &&
+         dalloc(&ws->d_block_sums, (size_t)ws->nblocks_upd * 3, ws->st) == 0 &&
+         dalloc(&ws->d_partial, ws->nblocks_red, ws->st) == 0 && dalloc(&ws->d_rowkl, n, ws->st) == 0;
     if (ok) {
         std::vector<float> v32(nnz > 0 ? nnz : 1);
         for (int e = 0; e < nnz; e++) v32[e] = (float)P->values[e];
@@ -494,7 +496,7 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
     if (ok) {
         ws->order.resize(n);
         for (int i = 0; i < n; i++) ws->order[i] = i;
-        ok = dalloc(&ws->d_order, n) == 0 &&
+        ok = dalloc(&ws->d_order, n, ws->st) == 0 &&
              hipMemcpyAsync(ws->d_order, ws->order.data(), sizeof(int) * n, hipMemcpyHostToDevice,
                             ws->st) == hipSuccess;
         // renumbering schedule: src/tsne_h2pack.c:1706-1717 (TSNE_REORDER, TSNE_REORDER_ITERS)
@@ -799,14 +801,14 @@ int tsne_compute_gradient(tsne_workspace_t *ws, double exag, int exact, double *
     std::vector<double> rep((size_t)n * D);
     if (exact) {
         double *d_s = nullptr, *d_F = nullptr;
-        if (dalloc(&d_s, n) || dalloc(&d_F, (size_t)n * D)) { dfree(d_s); dfree(d_F); return -1; }
+        if (dalloc(&d_s, n, ws->st) || dalloc(&d_F, (size_t)n * D, ws->st)) { dfree(d_s, ws->st); dfree(d_F, ws->st); return -1; }
         int rc = launch_repulsive_direct_f64(D, n, ws->d_Y64, nullptr, n, d_s, d_F, ws->st);
         rc = rc || launch_z_f64(D, n, ws->d_Y64, d_s, d_F, ws->d_scal, ws->d_partial,
                                 ws->nblocks_red, ws->d_scal + SLOT_ZEXACT, ws->st);
         rc = rc || hipMemcpyAsync(rep.data(), d_F, rep.size() * sizeof(double),
                                   hipMemcpyDeviceToHost, ws->st) != hipSuccess;
         rc = rc || hipStreamSynchronize(ws->st) != hipSuccess;
-        dfree(d_s); dfree(d_F);
+        dfree(d_s, ws->st); dfree(d_F, ws->st);
         if (rc) return -1;
         Z = read_scalar(ws, SLOT_ZEXACT);
         for (auto &v : rep) v *= 4.0 / Z;
@@ -862,8 +864,8 @@ int tsne_exact_repulsive_rows(tsne_workspace_t *ws, const int *rows, int n_rows,
         if (rows[r] < 0 || rows[r] >= n) return -1;
     double *d_s = nullptr, *d_F = nullptr, *d_s2 = nullptr, *d_F2 = nullptr;
     int *d_rows = nullptr;
-    int rc = dalloc(&d_s, n) || dalloc(&d_F, (size_t)n * D) || dalloc(&d_s2, n_rows) ||
-             dalloc(&d_F2, (size_t)n_rows * D) || dalloc(&d_rows, n_rows);
+    int rc = dalloc(&d_s, n, ws->st) || dalloc(&d_F, (size_t)n * D, ws->st) || dalloc(&d_s2, n_rows, ws->st) ||
+             dalloc(&d_F2, (size_t)n_rows * D, ws->st) || dalloc(&d_rows, n_rows, ws->st);
     const std::vector<int> inv = inverse_order(ws);
     std::vector<int> cur_rows(n_rows);
     for (int r = 0; r < n_rows; r++) cur_rows[r] = inv[rows[r]];
@@ -877,7 +879,7 @@ int tsne_exact_repulsive_rows(tsne_workspace_t *ws, const int *rows, int n_rows,
     rc = rc || hipMemcpyAsync(F.data(), d_F2, F.size() * sizeof(double), hipMemcpyDeviceToHost,
                               ws->st) != hipSuccess;
     rc = rc || hipStreamSynchronize(ws->st) != hipSuccess;
-    dfree(d_s); dfree(d_F); dfree(d_s2); dfree(d_F2); dfree(d_rows);
+    dfree(d_s, ws->st); dfree(d_F, ws->st); dfree(d_s2, ws->st); dfree(d_F2, ws->st); dfree(d_rows, ws->st);
     if (rc) return -1;
     const double Z = read_scalar(ws, SLOT_ZEXACT);
     for (size_t k = 0; k < F.size(); k++) out[k] = 4.0 / Z * F[k];
