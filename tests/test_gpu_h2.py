@@ -228,3 +228,72 @@ def test_default_tol_100k_early_iterations(backend):
     _say(f"  nodes {st.n_nodes} far pairs {st.n_adm_pairs} near pairs {st.n_near_pairs} "
          f"rank max {st.rank_max}")
     assert np.isfinite(Y).all()
+
+
+@pytest.mark.parametrize("scale", [0.05, 3.0, 30.0])
+def test_h2_gradient_within_tol_3d(backend, problem, scale):
+    """3D embeddings at h2_tol 3e-2 (the 3D bound): gradient error <= 3e-2 against the
+    FP64 exact gradient on all rows, exaggeration 1 and 12."""
+    _say(f"h2 3d scale={scale}")
+    _, _, P = problem
+    n = P.shape[0]
+    Y0 = _embedding(n, 3, scale, seed=4)
+    ws = _ws(backend, P, Y0, 3, 3e-2)
+    try:
+        st = H2Stats()
+        for exag in (1.0, 12.0):
+            g, Z = _grad(backend, ws, 3, n, exag, 0)
+            rep64, Z64, _, _ = oracle.repulsive_exact(Y0)
+            g64 = oracle.attractive(P, Y0, exag) - rep64
+            err = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+            assert backend.lib.tsne_h2_stats(ws, ctypes.byref(st)) == 0
+            _say(f"  exag {exag}: grad err {err:.2e} Z {Z:.6e}/{Z64:.6e} levels {st.levels} "
+                 f"rank max {st.rank_max} mean {st.rank_mean:.1f} proxies {st.proxy_max}")
+            assert err <= 3e-2
+            assert abs(Z / Z64 - 1) <= 3e-2
+        assert st.levels >= 3 and st.n_adm_pairs > 0
+    finally:
+        backend.lib.tsne_destroy(ws)
+
+
+def test_plain_gradient_all_rows_100k(backend):
+    """N=100,000 (20 Gaussians, Dirichlet weights, per-cluster std U[0.5,2], centres
+    N(0,5^2), D=50 float32; exact k=90 graph; perplexity 30; 2D; h2_tol 1e-2): after the
+    250 early-exaggeration iterations the plain FP32 gradient is within 1e-2 of the FP64
+    exact gradient on all rows (the exact repulsion comes from the FP64 all-pairs sum on
+    the device)."""
+    _say("plain gradient all rows N=100000")
+    n = 100_000
+    X, _ = gaussian_mixture(n, dim=50, n_clusters=20, seed=0, spread=5.0, dirichlet=True,
+                            std_range=(0.5, 2.0), dtype=np.float32)
+    P = joint_P_oracle(knn_graph(X, 90), 30.0)
+    p = backend.create_default_params()
+    p.verbose = 0
+    p.n_iter = 1000
+    p.h2_tol = 1e-2
+    p.h2_rebuild_freq = 3
+    p.h2_rebuild_freq_early = 10
+    p.learning_rate = n / 12.0
+    Y0 = np.random.default_rng(0).normal(scale=1e-4, size=(n, 2))
+    ws = backend.init_workspace(P, 2, p, init_Y=Y0)
+    try:
+        assert backend.lib.tsne_run_range(ws, 0, 250) == 0
+        g1 = np.zeros((2, n))
+        g0 = np.zeros((2, n))
+        Z = ctypes.c_double()
+        assert backend.lib.tsne_compute_gradient(ws, 1.0, 0, g1, ctypes.byref(Z)) == 0
+        assert backend.lib.tsne_compute_gradient(ws, 0.0, 0, g0, ctypes.byref(Z)) == 0
+        rows = np.arange(n, dtype=np.int32)
+        rep64 = np.zeros((n, 2))
+        Ze = ctypes.c_double()
+        assert backend.lib.tsne_exact_repulsive_rows(ws, rows, n, rep64, ctypes.byref(Ze)) == 0
+    finally:
+        backend.lib.tsne_destroy(ws)
+    attr = (g1 - g0).T
+    g = attr + g0.T
+    g64 = attr - rep64
+    err = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+    rep_err = np.linalg.norm(-g0.T - rep64) / np.linalg.norm(rep64)
+    _say(f"  grad err {err:.2e} rep err {rep_err:.2e} Z {Z.value:.6e}/{Ze.value:.6e}")
+    assert err <= 1e-2
+    assert rep_err <= 1e-2

@@ -19,6 +19,8 @@
 // against proxy points; src/tsne_h2pack.c:818, :908-919) and the upward sweep of the
 // fused product (src/tsne_h2_matmul.c:309-334).  One workgroup per node, one thread per
 // candidate column.
+#include <type_traits>
+
 #include "h2_internal.h"
 #include "proxy.h"
 
@@ -49,10 +51,11 @@ __device__ __forceinline__ double wsum64(double v)
 
 }  // namespace
 
+// ID + upward pass of one node, by one workgroup of NT threads; nodes with at most
+// m_lo candidates, or (below the maximum size class) more than NT, are left to another
+// size class.
 template <int D, int NT>
-__global__ void __launch_bounds__(NT)
-id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, int m_lo,
-             double tol)
+__device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m_lo, double tol)
 {
     constexpr int PMAX = H2Cfg<D>::PMAX, RMAX = H2Cfg<D>::RMAX;
     static_assert(PMAX % 4 == 0, "PMAX must be a multiple of 4");
@@ -73,8 +76,6 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
     double(*s_Q)[PMAX] = reinterpret_cast<double(*)[PMAX]>(s_qu);
     float(*s_U)[RMAX] = reinterpret_cast<float(*)[RMAX]>(s_qu);
 
-    const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
-    if (internal_only && h.child0[node] < 0) return;  // leaves were done in their own pass
     {
         // size class: this instantiation takes the nodes with m_lo < m <= NT candidates
         int mt;
@@ -262,45 +263,58 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
         r = k + 1;
         if (k == RMAX - 1 && t == 0) atomicOr(&h.flags[FLAG_RANK_CAP], 1);
     }
-    // unused rank slots: zero Q rows
-    for (int e = t; e < (RMAX - r) * PMAX; e += NT) s_Q[r + e / PMAX][e % PMAX] = 0.0;
-    __syncthreads();
-
-    // ---- interpolation coefficients x_t = R11^{-1} Q^T a_t
-    double rt[RMAX];
-#pragma unroll
-    for (int k = 0; k < RMAX; k++) rt[k] = 0.0;
-    if (t < m) {
-        for (int q = 0; q < PMAX; q++) {
-            const double v = kern64(my, s_prox[q], D) * s_scale[q];
-#pragma unroll
-            for (int k = 0; k < RMAX; k++) rt[k] = fma(s_Q[k][q], v, rt[k]);
-        }
-    }
+    // The rest runs over RB rank slots, RB the smallest bucket (RMAX/4, RMAX/2, RMAX)
+    // that holds r: a uniform branch per workgroup, each bucket fully unrolled.
     int kk = -1;
     if (selected)
         for (int k = 0; k < r; k++)
             if (s_piv[k] == t) kk = k;
-    if (kk >= 0) {
+    auto tail = [&](auto rb) {
+        constexpr int RB = decltype(rb)::value;
+        // unused rank slots below RB: zero Q rows (R rows there are identity)
+        for (int e = t; e < (RB - r) * PMAX; e += NT) s_Q[r + e / PMAX][e % PMAX] = 0.0;
+        __syncthreads();
+
+        // ---- interpolation coefficients x_t = R11^{-1} Q^T a_t
+        double rt[RB];
 #pragma unroll
-        for (int j = 0; j < RMAX; j++)
-            if (j <= kk) s_R[j][kk] = rt[j];
-    }
-    __syncthreads();  // s_R complete
-    if (t < RMAX) s_part[0][t] = 1.0 / s_R[t][t];  // s_part reused: inverse diagonal
-    __syncthreads();  // s_Q no longer read below (s_U aliases it)
-    if (kk >= 0) {
+        for (int k = 0; k < RB; k++) rt[k] = 0.0;
+        if (t < m) {
+            for (int q = 0; q < PMAX; q++) {
+                const double v = kern64(my, s_prox[q], D) * s_scale[q];
 #pragma unroll
-        for (int k = 0; k < RMAX; k++) rt[k] = (k == kk) ? 1.0 : 0.0;
-    } else if (t < m) {
-#pragma unroll
-        for (int k = RMAX - 1; k >= 0; k--) {
-            double v = rt[k];
-#pragma unroll
-            for (int j = k + 1; j < RMAX; j++) v = fma(-s_R[k][j], rt[j], v);
-            rt[k] = v * s_part[0][k];
+                for (int k = 0; k < RB; k++) rt[k] = fma(s_Q[k][q], v, rt[k]);
+            }
         }
-    }
+        if (kk >= 0) {
+#pragma unroll
+            for (int j = 0; j < RB; j++)
+                if (j <= kk) s_R[j][kk] = rt[j];
+        }
+        __syncthreads();  // s_R complete
+        if (t < RB) s_part[0][t] = 1.0 / s_R[t][t];  // s_part reused: inverse diagonal
+        __syncthreads();  // s_Q no longer read below (s_U aliases it)
+        if (kk >= 0) {
+#pragma unroll
+            for (int k = 0; k < RB; k++) rt[k] = (k == kk) ? 1.0 : 0.0;
+        } else if (t < m) {
+#pragma unroll
+            for (int k = RB - 1; k >= 0; k--) {
+                double v = rt[k];
+#pragma unroll
+                for (int j = k + 1; j < RB; j++) v = fma(-s_R[k][j], rt[j], v);
+                rt[k] = v * s_part[0][k];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < RB; k++) s_U[t][k] = (float)rt[k];
+    };
+    if (r <= RMAX / 4)
+        tail(std::integral_constant<int, RMAX / 4>{});
+    else if (r <= RMAX / 2)
+        tail(std::integral_constant<int, RMAX / 2>{});
+    else
+        tail(std::integral_constant<int, RMAX>{});
 
     // ---- store U (pool), skeleton, and the upward coefficients
     if (t == 0) {
@@ -313,8 +327,6 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
         atomicAdd(&h.flags[FLAG_RANK_SUM], r);
         atomicMax(&h.flags[FLAG_PROXY_MAX], p);
     }
-#pragma unroll
-    for (int k = 0; k < RMAX; k++) s_U[t][k] = (float)rt[k];
     if (kk >= 0) {
         for (int d = 0; d < D; d++) h.skel[((size_t)node * RMAX + kk) * D + d] = my[d];
         h.skelf[(size_t)node * RMAX + kk] =
@@ -338,22 +350,30 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int internal_only, i
     }
 }
 
-// list != nullptr: the nodes list[0..count) (all leaves, any level); otherwise the
-// internal nodes among [lb, lb + count).  Nodes with at most 64 candidates run in
-// single-wavefront workgroups (their barriers are free), larger ones in 128 threads.
+// list != nullptr: the nodes list[0..count) (the leaves); otherwise the internal nodes
+// among [lb, lb + count).  One workgroup per node, in one size class.
+template <int D, int NT>
+__global__ void __launch_bounds__(NT)
+id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int m_lo, double tol)
+{
+    const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
+    if (!list && h.child0[node] < 0) return;  // leaves were done in their own launch
+    id_up_node<D, NT>(h, node, m_lo, tol);
+}
+
+// Nodes with at most 64 candidates run in single-wavefront workgroups (their barriers
+// are free), larger ones in 128 threads (2D); 3D uses 384 threads.
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
                  hipStream_t st)
 {
     if (count <= 0) return 0;
-    const int internal_only = list ? 0 : 1;
     if (D == 2) {
-        hipLaunchKernelGGL((id_up_kernel<2, 64>), dim3(count), dim3(64), 0, st, h, list, lb,
-                           internal_only, 0, tol);
-        hipLaunchKernelGGL((id_up_kernel<2, 128>), dim3(count), dim3(128), 0, st, h, list, lb,
-                           internal_only, 64, tol);
+        hipLaunchKernelGGL((id_up_kernel<2, 64>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
+        hipLaunchKernelGGL((id_up_kernel<2, 128>), dim3(count), dim3(128), 0, st, h, list, lb, 64,
+                           tol);
     } else {
-        hipLaunchKernelGGL((id_up_kernel<3, 384>), dim3(count), dim3(384), 0, st, h, list, lb,
-                           internal_only, 0, tol);
+        hipLaunchKernelGGL((id_up_kernel<3, 384>), dim3(count), dim3(384), 0, st, h, list, lb, 0,
+                           tol);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
