@@ -96,7 +96,9 @@ bool alloc_sized(H2 *h)
     d.w_cap = d.x_cap;
     d.nl_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 16 : 64));
     d.pend_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 8 : 32));
-    d.pool_cap = (long long)(h->pool_f * ((double)h->n * (D == 2 ? 20 : 48) + (double)nc * (D == 2 ? 64 : 256)));
+    // U pool: n x RMAX floats for the leaves (placed by point), then the internal nodes'
+    // m x RMAX reservations (grown on overflow)
+    d.pool_cap = (long long)h->n * RMAX + (long long)(h->pool_f * (double)nc * (D == 2 ? 24 : 96) * RMAX);
     const int maxpair = std::max({d.adm_cap, d.x_cap, d.w_cap, d.nl_cap});
     bool ok = dmalloc(&d.nb, nc, h->st) && dmalloc(&d.ne, nc, h->st) && dmalloc(&d.level, nc, h->st) &&
               dmalloc(&d.parent, nc, h->st) && dmalloc(&d.child0, nc, h->st) && dmalloc(&d.nchild, nc, h->st) &&

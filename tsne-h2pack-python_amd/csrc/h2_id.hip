@@ -76,33 +76,48 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     double(*s_Q)[PMAX] = reinterpret_cast<double(*)[PMAX]>(s_qu);
     float(*s_U)[RMAX] = reinterpret_cast<float(*)[RMAX]>(s_qu);
 
-    {
-        // size class: this instantiation takes the nodes with m_lo < m <= NT candidates
-        int mt;
-        if (h.child0[node] < 0) {
-            mt = h.ne[node] - h.nb[node];
-        } else {
-            mt = 0;
-            for (int q = 0; q < h.nchild[node]; q++) mt += h.rank[h.child0[node] + q];
-        }
-        if (mt <= m_lo || (mt > NT && NT < H2Cfg<D>::MCAP)) return;
-    }
-    const int l = h.level[node];
+    constexpr int NCH = H2Cfg<D>::NCH;
     const int t = threadIdx.x;
     const int lane = t & 63, wid = t >> 6;
+    // node record and, independent of it, the proxy set of the node's level: all these
+    // loads are issued together
+    const int ch0 = h.child0[node];
+    const bool leafn = ch0 < 0;
+    const int l = h.level[node];
     const float4 c = h.cent[node];
+    const RootInfo root = *h.root;
+    const int kq = min(max(root.k0 - l, 0), kProxyLevels - 1);
+    const int p = min(h.proxy_cnt[kq], PMAX);
+    double px[3] = {0.0, 0.0, 0.0};  // proxy t (PMAX <= NT), relative to the centre
+    if (t < p)
+        for (int d = 0; d < D; d++) px[d] = h.proxy_pts[((size_t)kq * PMAX + t) * D + d];
+    int nb = 0, mt = 0, nc = 0;
+    int rk[NCH];
+    if (leafn) {
+        nb = h.nb[node];
+        mt = h.ne[node] - nb;
+    } else {
+        nc = h.nchild[node];
+#pragma unroll
+        for (int q = 0; q < NCH; q++) rk[q] = (q < nc) ? h.rank[ch0 + q] : 0;
+#pragma unroll
+        for (int q = 0; q < NCH; q++) mt += rk[q];
+    }
+    // size class: this instantiation takes the nodes with m_lo < m <= NT candidates
+    if (mt <= m_lo || (mt > NT && NT < H2Cfg<D>::MCAP)) return;
+    const int m = min(mt, NT);
+    // U of the node: a leaf's rows are its points, so its block sits at its first sorted
+    // point (m x RMAX floats); an internal node reserves m x RMAX floats after the n x RMAX
+    // of the leaves (the reservation is issued now, its latency hidden by the ID)
+    if (t == 0) s_base = leafn ? nb * RMAX : h.n * RMAX + atomicAdd(&h.cnt[CNT_POOL], m * RMAX);
     const double cd[3] = {(double)c.x, (double)c.y, (double)c.z};
-    const bool leafn = h.child0[node] < 0;
 
     // ---- candidates and the rows of X
-    int m;
     double my[3] = {0.0, 0.0, 0.0};
     float4 xin = make_float4(0.f, 0.f, 0.f, 0.f);
     if (leafn) {
-        const int b = h.nb[node];
-        m = min(h.ne[node] - b, NT);
         if (t < m) {
-            const VT<D> y = ((const VT<D> *)h.Ys)[b + t];
+            const VT<D> y = ((const VT<D> *)h.Ys)[nb + t];
             my[0] = y.x;
             my[1] = y.y;
             if constexpr (D == 3) my[2] = y.z;
@@ -110,37 +125,36 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             if constexpr (D == 3) xin.w = y.z - c.z;
         }
     } else {
-        int acc = 0;
-        const int c0 = h.child0[node], nc = h.nchild[node];
-        for (int q = 0; q < nc; q++) {
-            const int cj = c0 + q;
-            const int rj = h.rank[cj];
-            if (t >= acc && t < acc + rj) {
-                const int k = t - acc;
-                for (int d = 0; d < D; d++) my[d] = h.skel[((size_t)cj * RMAX + k) * D + d];
-                const float4 w = h.wc[(size_t)cj * RMAX + k];
-                const float4 cc = h.cent[cj];
-                xin = make_float4(w.x, w.y + (cc.x - c.x) * w.x, w.z + (cc.y - c.y) * w.x,
-                                  w.w + (cc.z - c.z) * w.x);
+        // this thread's child and skeleton slot, from the children's ranks
+        int cj = -1, k = 0, acc = 0;
+#pragma unroll
+        for (int q = 0; q < NCH; q++) {
+            if (t >= acc && t < acc + rk[q]) {
+                cj = ch0 + q;
+                k = t - acc;
             }
-            acc += rj;
+            acc += rk[q];
         }
-        m = min(acc, NT);
+        if (cj >= 0) {
+            for (int d = 0; d < D; d++) my[d] = h.skel[((size_t)cj * RMAX + k) * D + d];
+            const float4 w = h.wc[(size_t)cj * RMAX + k];
+            const float4 cc = h.cent[cj];
+            xin = make_float4(w.x, w.y + (cc.x - c.x) * w.x, w.z + (cc.y - c.y) * w.x,
+                              w.w + (cc.z - c.z) * w.x);
+        }
         if (nc == 1) {
             // single child: same skeleton, U = I, coefficients moved to this centre
+            __syncthreads();
+            const int base = s_base;
+            const bool fits = (long long)base + (long long)m * RMAX <= h.pool_cap;
             if (t == 0) {
-                const int base = atomicAdd(&h.cnt[CNT_POOL], m * m);
-                s_base = base;
-                if ((long long)base + (long long)m * m > h.pool_cap)
-                    atomicOr(&h.flags[FLAG_POOL_OVF], 1);
+                if (!fits) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
                 h.rank[node] = m;
                 h.uoff[node] = base;
             }
-            __syncthreads();
-            const int base = s_base;
             if (t < m) {
-                if ((long long)base + (long long)m * m <= h.pool_cap)
-                    for (int k = 0; k < m; k++) h.upool[(size_t)base + (size_t)t * m + k] = (k == t) ? 1.f : 0.f;
+                if (fits)
+                    for (int q = 0; q < m; q++) h.upool[(size_t)base + (size_t)t * m + q] = (q == t) ? 1.f : 0.f;
                 for (int d = 0; d < D; d++) h.skel[((size_t)node * RMAX + t) * D + d] = my[d];
                 h.skelf[(size_t)node * RMAX + t] =
                     make_float4((float)my[0], (float)my[1], (float)my[2], 0.f);
@@ -154,12 +168,8 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     // ---- proxies of this level, translated to the centre.  Everything below runs over
     // all PMAX proxy slots and all RMAX rank slots without branches: unused slots hold
     // zeros (scale 0, zero Q rows, identity R rows), which keeps the FP64 pipelines full.
-    const RootInfo root = *h.root;
-    const int kq = min(max(root.k0 - l, 0), kProxyLevels - 1);
-    const int p = min(h.proxy_cnt[kq], PMAX);
-    for (int q = t; q < PMAX; q += NT)
-        for (int d = 0; d < D; d++)
-            s_prox[q][d] = cd[d] + (q < p ? h.proxy_pts[((size_t)kq * PMAX + q) * D + d] : 0.0);
+    if (t < PMAX)
+        for (int d = 0; d < D; d++) s_prox[t][d] = cd[d] + px[d];
     for (int e = t; e < RMAX * RMAX; e += NT) s_R[e / RMAX][e % RMAX] = (e / RMAX == e % RMAX) ? 1.0 : 0.0;
     __syncthreads();
 
@@ -167,10 +177,41 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     double a[PMAX];
 #pragma unroll
     for (int q = 0; q < PMAX; q++) a[q] = (t < m) ? kern64(my, s_prox[q], D) : 0.0;
+    if constexpr (D == 2) {
+        // reduce-scatter over the wavefront: at each step a lane keeps half of its
+        // partial column sums and adds the partner's copy of that half, so after six
+        // steps lane j holds the wavefront's sum for column j (63 independent shuffles
+        // instead of a 6-step chain per column)
+        static_assert(PMAX <= 64 && PMAX > 32, "2D column count");
+        double w1[32], w2[16], w3[8], w4[4], w5[2];
+        const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4, b1 = lane & 2,
+                   b0 = lane & 1;
 #pragma unroll
-    for (int q = 0; q < PMAX; q++) {
-        const double v = wsum64(a[q] * a[q]);
-        if (lane == 0) s_part[wid][q] = v;
+        for (int j = 0; j < 32; j++) {
+            const double lo = a[j] * a[j];
+            const double hi = (j + 32 < PMAX) ? a[j + 32] * a[j + 32] : 0.0;
+            w1[j] = (b5 ? hi : lo) + __shfl_xor(b5 ? lo : hi, 32, 64);
+        }
+#pragma unroll
+        for (int j = 0; j < 16; j++)
+            w2[j] = (b4 ? w1[j + 16] : w1[j]) + __shfl_xor(b4 ? w1[j] : w1[j + 16], 16, 64);
+#pragma unroll
+        for (int j = 0; j < 8; j++)
+            w3[j] = (b3 ? w2[j + 8] : w2[j]) + __shfl_xor(b3 ? w2[j] : w2[j + 8], 8, 64);
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            w4[j] = (b2 ? w3[j + 4] : w3[j]) + __shfl_xor(b2 ? w3[j] : w3[j + 4], 4, 64);
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            w5[j] = (b1 ? w4[j + 2] : w4[j]) + __shfl_xor(b1 ? w4[j] : w4[j + 2], 2, 64);
+        const double w6 = (b0 ? w5[1] : w5[0]) + __shfl_xor(b0 ? w5[0] : w5[1], 1, 64);
+        if (lane < PMAX) s_part[wid][lane] = w6;
+    } else {
+#pragma unroll
+        for (int q = 0; q < PMAX; q++) {
+            const double v = wsum64(a[q] * a[q]);
+            if (lane == 0) s_part[wid][q] = v;
+        }
     }
     __syncthreads();
     for (int q = t; q < PMAX; q += NT) {
@@ -318,9 +359,8 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
 
     // ---- store U (pool), skeleton, and the upward coefficients
     if (t == 0) {
-        const int base = atomicAdd(&h.cnt[CNT_POOL], m * r);
-        s_base = base;
-        if ((long long)base + (long long)m * r > h.pool_cap) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
+        const int base = s_base;
+        if ((long long)base + (long long)m * RMAX > h.pool_cap) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
         h.rank[node] = r;
         h.uoff[node] = base;
         atomicMax(&h.flags[FLAG_RANK_MAX], r);
@@ -334,7 +374,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     }
     __syncthreads();
     const int base = s_base;
-    if ((long long)base + (long long)m * r <= h.pool_cap)
+    if ((long long)base + (long long)m * RMAX <= h.pool_cap)
         for (int e = t; e < m * r; e += NT) h.upool[(size_t)base + e] = s_U[e / r][e % r];
     if (t < r) {
         float4 w = make_float4(0.f, 0.f, 0.f, 0.f);

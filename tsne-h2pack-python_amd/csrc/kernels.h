@@ -6,6 +6,10 @@
 namespace tsne {
 
 // Slots of the per-workspace device scalar array `scal` (double).
+// The attractive pass reads P's columns and fp32 values in aligned 4-element chunks,
+// so those two arrays carry this many elements of padding past nnz.
+constexpr int kCsrPad = 4;
+
 enum ScalarSlot {
     SLOT_Z = 0,        // normalisation Z of the current repulsive force
     SLOT_MEAN = 1,     // 1..3: pending centring mean of Y64 (removed at next read)

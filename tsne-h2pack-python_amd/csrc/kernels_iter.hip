@@ -40,9 +40,15 @@ namespace tsne {
 // range and writes the sum of its new coordinates in a fixed order, for the next mean.
 // ---------------------------------------------------------------------------------
 
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
 // LANES lanes per row (256 / LANES rows in flight per block), BATCH nonzeros per lane
-// per batch.
-template <int D, int LANES, int BATCH>
+// per batch.  VEC: P's columns and values are read as aligned 4-element chunks (BATCH
+// chunks per lane per batch) with non-temporal loads, so the 8 bytes per nonzero of the
+// stream take a quarter of the load instructions and do not displace the gather records
+// from the caches; elements of a chunk outside the row are masked.
+template <int D, int LANES, int BATCH, bool VEC>
 __global__ void __launch_bounds__(256)
 attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
                    const float *__restrict__ pval, const typename Vec<D>::T *__restrict__ Yg,
@@ -80,9 +86,51 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
         }
         const V yi = Yg[i];
         float ax = 0.f, ay = 0.f, az = 0.f;
+        if constexpr (VEC) {
+            const int c1 = (e1 + 3) >> 2;  // chunks [e0/4, c1) cover the row
+            for (int cb = (e0 >> 2) + sub; cb < c1; cb += BATCH * LANES) {
+                int jj[BATCH][4];
+                float pp[BATCH][4];
+#pragma unroll
+                for (int t = 0; t < BATCH; t++) {
+                    const int c = cb + t * LANES;
+                    v4i cc = {i, i, i, i};
+                    v4f pv = {0.f, 0.f, 0.f, 0.f};
+                    if (c < c1) {
+                        cc = __builtin_nontemporal_load((const v4i *)col_idx + c);
+                        pv = __builtin_nontemporal_load((const v4f *)pval + c);
+                    }
+                    const int e = 4 * c;
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const bool ok = e + u >= e0 && e + u < e1;
+                        jj[t][u] = ok ? cc[u] : i;
+                        pp[t][u] = ok ? pv[u] : 0.f;
+                    }
+                }
+                V yy[BATCH][4];
+#pragma unroll
+                for (int t = 0; t < BATCH; t++)
+#pragma unroll
+                    for (int u = 0; u < 4; u++) yy[t][u] = Yg[jj[t][u]];
+#pragma unroll
+                for (int t = 0; t < BATCH; t++)
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const float dx = yi.x - yy[t][u].x, dy = yi.y - yy[t][u].y;
+                        float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
+                        float dz = 0.f;
+                        if constexpr (D == 3) { dz = yi.z - yy[t][u].z; r2 = fmaf(dz, dz, r2); }
+                        const float q = pp[t][u] * fast_rcp(r2);
+                        ax = fmaf(q, dx, ax);
+                        ay = fmaf(q, dy, ay);
+                        if constexpr (D == 3) az = fmaf(q, dz, az);
+                    }
+            }
+        }
         // a batch is BATCH nonzeros per lane: all its column loads are issued, then
         // all its gathers, so a row costs two memory round trips instead of one per step
-        for (int base = e0 + sub; base < e1; base += BATCH * LANES) {
+        for (int base = e0 + sub; !VEC && base < e1; base += BATCH * LANES) {
             int jj[BATCH];
             float pp[BATCH];
 #pragma unroll
@@ -469,14 +517,14 @@ perplexity_kernel(int n, const double *__restrict__ dist, const int *__restrict_
 // Host-side launchers (declared in kernels.h).
 // ---------------------------------------------------------------------------------
 namespace {
-template <int D, int L, int BT>
+template <int D, int L, int BT, bool VEC>
 void attr_launch(int nblocks, hipStream_t st, int n, const int *row_ptr, const int *col_idx,
                  const float *pval, const void *Yg, const float *F, double *Y64, double *uY,
                  double *gains, void *Yg_next, const double *scal, double exag, double mom,
                  double lr, int rpb, double *block_sums)
 {
     using V = typename Vec<D>::T;
-    hipLaunchKernelGGL((attr_update_kernel<D, L, BT>), dim3(nblocks), dim3(256), 0, st, n,
+    hipLaunchKernelGGL((attr_update_kernel<D, L, BT, VEC>), dim3(nblocks), dim3(256), 0, st, n,
                        row_ptr, col_idx, pval, (const V *)Yg, F, Y64, uY, gains, (V *)Yg_next,
                        scal, 4.0 * exag, mom, lr, rpb, block_sums);
 }
@@ -494,6 +542,17 @@ static int attr_lanes()
     return lanes;
 }
 
+// Chunked non-temporal reads of P (default) or per-element reads (TSNE_ATTR_VEC=0, which
+// also honours TSNE_ATTR_LANES); read once.
+static bool attr_vec()
+{
+    static bool v = [] {
+        const char *e = getenv("TSNE_ATTR_VEC");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+
 int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
                        const void *Yg, const float *F, double *Y64, double *uY, double *gains,
                        void *Yg_next, const double *scal, double exag, double mom, double lr,
@@ -505,14 +564,17 @@ int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, con
     if (ev_begin) (void)hipEventRecord(ev_begin, st);
 #define TSNE_ATTR_ARGS nblocks, st, n, row_ptr, col_idx, pval, Yg, F, Y64, uY, gains, Yg_next, \
                        scal, exag, mom, lr, rpb, block_sums
-    if (D == 2) {
-        if (L == 8) attr_launch<2, 8, 12>(TSNE_ATTR_ARGS);
-        else if (L == 32) attr_launch<2, 32, 6>(TSNE_ATTR_ARGS);
-        else attr_launch<2, 16, 10>(TSNE_ATTR_ARGS);
+    if (attr_vec()) {
+        if (D == 2) attr_launch<2, 16, 3, true>(TSNE_ATTR_ARGS);
+        else attr_launch<3, 16, 3, true>(TSNE_ATTR_ARGS);
+    } else if (D == 2) {
+        if (L == 8) attr_launch<2, 8, 12, false>(TSNE_ATTR_ARGS);
+        else if (L == 32) attr_launch<2, 32, 6, false>(TSNE_ATTR_ARGS);
+        else attr_launch<2, 16, 10, false>(TSNE_ATTR_ARGS);
     } else {
-        if (L == 8) attr_launch<3, 8, 12>(TSNE_ATTR_ARGS);
-        else if (L == 32) attr_launch<3, 32, 6>(TSNE_ATTR_ARGS);
-        else attr_launch<3, 16, 10>(TSNE_ATTR_ARGS);
+        if (L == 8) attr_launch<3, 8, 12, false>(TSNE_ATTR_ARGS);
+        else if (L == 32) attr_launch<3, 32, 6, false>(TSNE_ATTR_ARGS);
+        else attr_launch<3, 16, 10, false>(TSNE_ATTR_ARGS);
     }
 #undef TSNE_ATTR_ARGS
     if (ev_end) (void)hipEventRecord(ev_end, st);

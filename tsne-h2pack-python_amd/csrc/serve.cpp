@@ -77,7 +77,9 @@ void fill_info(const Job *j, tsne_job_info_t *info)
     info->footprint_bytes = j->footprint;
 }
 
-void run_job(Job *j)
+// Runs the job outside the server lock; returns its final status, which the caller
+// publishes under the lock (with the end time) before any waiter may see it.
+int run_job(Job *j)
 {
     tsne_params_t p = j->params;
     p.profiling_output_file = nullptr;
@@ -100,7 +102,7 @@ void run_job(Job *j)
     }
     if (ws) tsne_destroy(ws);
     std::vector<double>().swap(j->init_Y);
-    j->status = ok ? TSNE_JOB_DONE : TSNE_JOB_FAILED;
+    return ok ? TSNE_JOB_DONE : TSNE_JOB_FAILED;
 }
 
 void worker(tsne_server *s)
@@ -125,9 +127,10 @@ void worker(tsne_server *s)
             j->status = TSNE_JOB_RUNNING;
             j->t_start = s->now();
         }
-        run_job(j);
+        const int status = run_job(j);
         {
             std::lock_guard<std::mutex> lk(s->mu);
+            j->status = status;
             j->t_end = s->now();
             s->in_use -= j->footprint;
             s->running--;
@@ -146,7 +149,7 @@ long long tsne_job_footprint(int n_point, int low_dim, long long nnz)
     // P on the device (int32 columns, fp32 + fp64 values) and, until the last
     // renumbering, a second copy with its sort scratch; per-point state, gather copies,
     // forces and the H2 structure; fixed buffers (proxy tables, sort scratch)
-    return 48LL * nnz + (long long)n_point * (32LL * low_dim + 300) + (48LL << 20);
+    return 48LL * nnz + (long long)n_point * (32LL * low_dim + 600) + (48LL << 20);
 }
 
 tsne_server_t *tsne_server_create(int slots, long long hbm_budget_bytes)

@@ -215,8 +215,8 @@ int renumber(tsne_workspace *ws)
         ws->P2.n = n;
         ws->P2.nnz = nnz;
         ws->rs.tmp_bytes = renumber_tmp_bytes(n, nnz);
-        const bool ok = dalloc(&ws->P2.rp, (size_t)n + 1, ws->st) == 0 && dalloc(&ws->P2.col, nnz, ws->st) == 0 &&
-                        dalloc(&ws->P2.v32, nnz, ws->st) == 0 && dalloc(&ws->P2.v64, nnz, ws->st) == 0 &&
+        const bool ok = dalloc(&ws->P2.rp, (size_t)n + 1, ws->st) == 0 && dalloc(&ws->P2.col, (size_t)nnz + kCsrPad, ws->st) == 0 &&
+                        dalloc(&ws->P2.v32, (size_t)nnz + kCsrPad, ws->st) == 0 && dalloc(&ws->P2.v64, nnz, ws->st) == 0 &&
                         dalloc(&ws->rs.inv, n, ws->st) == 0 && dalloc(&ws->rs.len, n, ws->st) == 0 &&
                         dalloc(&ws->rs.src, nnz, ws->st) == 0 && dalloc(&ws->rs.src_sorted, nnz, ws->st) == 0 &&
                         dalloc(&ws->rs.col_tmp, nnz, ws->st) == 0 &&
@@ -464,8 +464,8 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         delete ws;
         return nullptr;
     }
-    ok = dalloc(&ws->d_rowptr, (size_t)n + 1, ws->st) == 0 && dalloc(&ws->d_col, nnz, ws->st) == 0 &&
-         dalloc(&ws->d_val32, nnz, ws->st) == 0 && dalloc(&ws->d_val64, nnz, ws->st) == 0 &&
+    ok = dalloc(&ws->d_rowptr, (size_t)n + 1, ws->st) == 0 && dalloc(&ws->d_col, (size_t)nnz + kCsrPad, ws->st) == 0 &&
+         dalloc(&ws->d_val32, (size_t)nnz + kCsrPad, ws->st) == 0 && dalloc(&ws->d_val64, nnz, ws->st) == 0 &&
          dalloc(&ws->d_Y64, (size_t)n * D, ws->st) == 0 && dalloc(&ws->d_uY, (size_t)n * D, ws->st) == 0 &&
          dalloc(&ws->d_gains, (size_t)n * D, ws->st) == 0 &&
          dalloc((char **)&ws->d_Yg[0], (size_t)n * kGatherBytes[D], ws->st) == 0 &&
