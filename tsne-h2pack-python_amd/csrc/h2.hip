@@ -31,10 +31,12 @@ struct H2 {
     int nblk = 0;
     double *d_proxy = nullptr;
     int *d_proxy_cnt = nullptr;
-    int *pin = nullptr;  // pinned host scratch
+    int *pin = nullptr;      // pinned host scratch
+    int *pin_dev = nullptr;  // the same memory as the device sees it
     std::vector<int> lvl;  // level l nodes are [lvl[l], lvl[l+1])
     int n_nodes = 0, n_leaves = 0, n_adm = 0, n_x = 0, n_w = 0, n_near = 0;
     int usable = 0;
+    int pq = 0;  // proxy slots the ID processes: the table's largest count, rounded up
     int hflags[FLAG_COUNT] = {0};
     // growth factors (grown on overflow)
     double node_f = 8.0, pair_f = 1.0, pool_f = 1.0;
@@ -184,10 +186,9 @@ int build_once(H2 *h, const void *Yg)
         if (hipcub::DeviceScan::ExclusiveSum(h->cub_tmp, bytes, d.childcnt, d.childoff, le - lb, st) !=
             hipSuccess)
             return -1;
-        if (hipMemcpyAsync(&h->pin[0], d.childoff + (le - lb - 1), sizeof(int), hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(&h->pin[1], d.childcnt + (le - lb - 1), sizeof(int), hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(&h->pin[2], d.cnt + CNT_PEND0 + pend_in, sizeof(int), hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(&h->pin[3], d.flags + FLAG_NONFINITE, sizeof(int), hipMemcpyDeviceToHost, st) ||
+        // the level's child count, pending pairs and finiteness flag, written by one
+        // thread straight into the pinned host words (one launch instead of four copies)
+        if (launch_publish_level(d, le - lb - 1, pend_in, h->pin_dev, st) ||
             hipStreamSynchronize(st) != hipSuccess)
             return -1;
         if (first && h->pin[3]) return -2;
@@ -233,9 +234,11 @@ int build_once(H2 *h, const void *Yg)
     // ID + upward: every leaf at once (they need nothing from other nodes), then the
     // internal nodes level by level, finest first
     const int nlev = (int)h->lvl.size() - 1;
-    if (h->n_nodes > 1 && launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, st)) return -1;
+    if (h->n_nodes > 1 && launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, h->pq, st))
+        return -1;
     for (int lv = nlev - 1; lv >= 1; lv--)
-        if (launch_id_up(D, d, nullptr, h->lvl[lv], h->lvl[lv + 1] - h->lvl[lv], h->id_tol, st))
+        if (launch_id_up(D, d, nullptr, h->lvl[lv], h->lvl[lv + 1] - h->lvl[lv], h->id_tol, h->pq,
+                         st))
             return -1;
     if (hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -274,9 +277,14 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
               dmalloc(&h->perm_alt, n, h->st) && dmalloc((char **)&d.Ys, (size_t)n * vb, h->st) &&
               dmalloc(&d.root, 1, h->st) && dmalloc(&d.flags, FLAG_COUNT, h->st) && dmalloc(&d.cnt, CNT_COUNT, h->st) &&
               dmalloc(&h->part, (size_t)h->nblk * 7, h->st) &&
-              hipHostMalloc((void **)&h->pin, sizeof(int) * 64, hipHostMallocDefault) == hipSuccess;
+              hipHostMalloc((void **)&h->pin, sizeof(int) * 64,
+                            hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
+              hipHostGetDevicePointer((void **)&h->pin_dev, h->pin, 0) == hipSuccess;
     if (ok) {
         const ProxyTable &pt = proxy_table(dim, h->ptol, 1.0, pmax_of(dim));
+        int cmax = 0;
+        for (int c : pt.cnt) cmax = std::max(cmax, c);
+        h->pq = cmax;
         ok = dmalloc(&h->d_proxy, pt.pts.size(), h->st) && dmalloc(&h->d_proxy_cnt, pt.cnt.size(), h->st) &&
              hipMemcpyAsync(h->d_proxy, pt.pts.data(), pt.pts.size() * sizeof(double),
                             hipMemcpyHostToDevice, st) == hipSuccess &&

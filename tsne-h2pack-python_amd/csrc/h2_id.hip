@@ -54,11 +54,14 @@ __device__ __forceinline__ double wsum64(double v)
 // ID + upward pass of one node, by one workgroup of NT threads; nodes with at most
 // m_lo candidates, or (below the maximum size class) more than NT, are left to another
 // size class.
-template <int D, int NT>
+// PQ proxy slots are processed (PQ <= the table's PMAX; at least the largest proxy count
+// of any level).
+template <int D, int NT, int PQ>
 __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m_lo, double tol)
 {
-    constexpr int PMAX = H2Cfg<D>::PMAX, RMAX = H2Cfg<D>::RMAX;
-    static_assert(PMAX % 4 == 0, "PMAX must be a multiple of 4");
+    constexpr int PMAX = PQ, RMAX = H2Cfg<D>::RMAX;
+    constexpr int PSTRIDE = H2Cfg<D>::PMAX;  // row stride of the proxy table
+    static_assert(PMAX % 4 == 0 && PMAX <= PSTRIDE, "proxy slots");
     constexpr int NW = NT / 64;
     constexpr size_t QBYTES = sizeof(double) * RMAX * PMAX;
     constexpr size_t UBYTES = sizeof(float) * NT * RMAX;
@@ -90,7 +93,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     const int p = min(h.proxy_cnt[kq], PMAX);
     double px[3] = {0.0, 0.0, 0.0};  // proxy t (PMAX <= NT), relative to the centre
     if (t < p)
-        for (int d = 0; d < D; d++) px[d] = h.proxy_pts[((size_t)kq * PMAX + t) * D + d];
+        for (int d = 0; d < D; d++) px[d] = h.proxy_pts[((size_t)kq * PSTRIDE + t) * D + d];
     int nb = 0, mt = 0, nc = 0;
     int rk[NCH];
     if (leafn) {
@@ -180,17 +183,22 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     if constexpr (D == 2) {
         // reduce-scatter over the wavefront: at each step a lane keeps half of its
         // partial column sums and adds the partner's copy of that half, so after six
-        // steps lane j holds the wavefront's sum for column j (63 independent shuffles
-        // instead of a 6-step chain per column)
-        static_assert(PMAX <= 64 && PMAX > 32, "2D column count");
+        // steps lane j holds the wavefront's sum for column j (mod 32 when PMAX <= 32):
+        // 63 independent shuffles instead of a 6-step chain per column
+        static_assert(PMAX <= 64, "2D column count");
         double w1[32], w2[16], w3[8], w4[4], w5[2];
         const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4, b1 = lane & 2,
                    b0 = lane & 1;
 #pragma unroll
         for (int j = 0; j < 32; j++) {
-            const double lo = a[j] * a[j];
-            const double hi = (j + 32 < PMAX) ? a[j + 32] * a[j + 32] : 0.0;
-            w1[j] = (b5 ? hi : lo) + __shfl_xor(b5 ? lo : hi, 32, 64);
+            if constexpr (PMAX <= 32) {
+                const double v = (j < PMAX) ? a[j] * a[j] : 0.0;
+                w1[j] = v + __shfl_xor(v, 32, 64);
+            } else {
+                const double lo = a[j] * a[j];
+                const double hi = (j + 32 < PMAX) ? a[j + 32] * a[j + 32] : 0.0;
+                w1[j] = (b5 ? hi : lo) + __shfl_xor(b5 ? lo : hi, 32, 64);
+            }
         }
 #pragma unroll
         for (int j = 0; j < 16; j++)
@@ -206,6 +214,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             w5[j] = (b1 ? w4[j + 2] : w4[j]) + __shfl_xor(b1 ? w4[j] : w4[j + 2], 2, 64);
         const double w6 = (b0 ? w5[1] : w5[0]) + __shfl_xor(b0 ? w5[0] : w5[1], 1, 64);
         if (lane < PMAX) s_part[wid][lane] = w6;
+        (void)b5;
     } else {
 #pragma unroll
         for (int q = 0; q < PMAX; q++) {
@@ -392,27 +401,33 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
 
 // list != nullptr: the nodes list[0..count) (the leaves); otherwise the internal nodes
 // among [lb, lb + count).  One workgroup per node, in one size class.
-template <int D, int NT>
+template <int D, int NT, int PQ>
 __global__ void __launch_bounds__(NT)
 id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int m_lo, double tol)
 {
     const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
     if (!list && h.child0[node] < 0) return;  // leaves were done in their own launch
-    id_up_node<D, NT>(h, node, m_lo, tol);
+    id_up_node<D, NT, PQ>(h, node, m_lo, tol);
 }
 
 // Nodes with at most 64 candidates run in single-wavefront workgroups (their barriers
 // are free), larger ones in 128 threads (2D); 3D uses 384 threads.
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
-                 hipStream_t st)
+                 int pq, hipStream_t st)
 {
     if (count <= 0) return 0;
     if (D == 2) {
-        hipLaunchKernelGGL((id_up_kernel<2, 64>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
-        hipLaunchKernelGGL((id_up_kernel<2, 128>), dim3(count), dim3(128), 0, st, h, list, lb, 64,
-                           tol);
+        if (pq <= 32) {
+            hipLaunchKernelGGL((id_up_kernel<2, 64, 32>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
+            hipLaunchKernelGGL((id_up_kernel<2, 128, 32>), dim3(count), dim3(128), 0, st, h, list, lb,
+                               64, tol);
+        } else {
+            hipLaunchKernelGGL((id_up_kernel<2, 64, 48>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
+            hipLaunchKernelGGL((id_up_kernel<2, 128, 48>), dim3(count), dim3(128), 0, st, h, list, lb,
+                               64, tol);
+        }
     } else {
-        hipLaunchKernelGGL((id_up_kernel<3, 384>), dim3(count), dim3(384), 0, st, h, list, lb, 0,
+        hipLaunchKernelGGL((id_up_kernel<3, 384, 96>), dim3(count), dim3(384), 0, st, h, list, lb, 0,
                            tol);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -112,11 +112,13 @@ int launch_split_count(int D, const H2Dev &h, int lb, int le, int l, hipStream_t
 int launch_split_write(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st);
 int launch_lists(int D, const H2Dev &h, int lb, int le, int l, int pend_out, hipStream_t st);
 int launch_pending(int D, const H2Dev &h, int n_pend, int pend_in, int pend_out, hipStream_t st);
+int launch_publish_level(const H2Dev &h, int last, int pend_in, int *out, hipStream_t st);
 int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, hipStream_t st);
 
 // h2_id.hip
+// pq: proxy slots to process (>= the largest proxy count of the table)
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
-                 hipStream_t st);
+                 int pq, hipStream_t st);
 
 // h2_product.hip
 int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, hipStream_t st);

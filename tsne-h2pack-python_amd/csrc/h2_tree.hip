@@ -514,6 +514,24 @@ int launch_pending(int D, const H2Dev &h, int n_pend, int pend_in, int pend_out,
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// After the scan of a level's child counts: the total child count (exclusive offset +
+// count of the last node), the pending-pair count of the input buffer and the
+// non-finite flag, into the pinned host words out[0..3].
+__global__ void publish_level_kernel(H2Dev h, int last, int pend_in, int *out)
+{
+    if (threadIdx.x != 0) return;
+    out[0] = h.childoff[last];
+    out[1] = h.childcnt[last];
+    out[2] = h.cnt[CNT_PEND0 + pend_in];
+    out[3] = h.flags[FLAG_NONFINITE];
+}
+
+int launch_publish_level(const H2Dev &h, int last, int pend_in, int *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(publish_level_kernel, dim3(1), dim3(64), 0, st, h, last, pend_in, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 // CSR offsets of (target << 32 | source) pairs sorted by key: beg/end per target.
 __global__ void __launch_bounds__(256)
 pair_offsets_kernel(const uint64_t *__restrict__ pairs, int n, int *__restrict__ beg,

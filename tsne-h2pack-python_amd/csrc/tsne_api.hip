@@ -456,7 +456,14 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
 
     const int n = n_point, D = low_dim;
     // 128 rows per attractive block; a multiple of 8 blocks for the XCD-aware placement
-    ws->nblocks_upd = (((n + 127) / 128) + 7) & ~7;
+    // rows per attractive block (environment TSNE_ATTR_RPB, default 32); the block count
+    // is a multiple of 8 for the XCD-aware placement
+    {
+        const char *e = getenv("TSNE_ATTR_RPB");
+        int rpb = e ? atoi(e) : 32;
+        rpb = std::max(16, std::min(rpb, 4096));
+        ws->nblocks_upd = (((n + rpb - 1) / rpb) + 7) & ~7;
+    }
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
     bool ok = hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
