@@ -152,6 +152,14 @@ def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol):
         ka, kn = ctypes.c_double(), ctypes.c_int()
         lib.tsne_attr_kernel_time(ws, ctypes.byref(ka), ctypes.byref(kn))
         attr_kernel_us = 1e3 * ka.value / max(kn.value, 1)
+        ta, tn = ctypes.c_double(), ctypes.c_int()
+        lib.tsne_attr_kernel_time_tiled(ws, ctypes.byref(ta), ctypes.byref(tn))
+        tiled_us = 1e3 * ta.value / tn.value if tn.value else None
+        row_us = (1e3 * (ka.value - ta.value) / (kn.value - tn.value)
+                  if kn.value > tn.value else None)
+        tv = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()]
+        lib.tsne_attr_tile_stats(ws, *[ctypes.byref(v) for v in tv])
+        tiles = dict(zip(["rows", "ntiles", "umax", "ucount", "n_global"], [v.value for v in tv]))
         kl = float(lib.compute_kl_divergence(ws))
         st = None
         from tsne_h2pack._backend import H2Stats
@@ -174,7 +182,8 @@ def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol):
     finally:
         lib.tsne_destroy(ws)
     return dict(seconds=dt, its=steps / dt, kl=kl, attr_kernel_us=attr_kernel_us,
-                attr_kernel_launches=kn.value, attr_ms=a.value, rebuild_ms=r.value,
+                attr_kernel_launches=kn.value, attr_tiled_us=tiled_us,
+                attr_tiled_launches=tn.value, attr_row_us=row_us, tiles=tiles, attr_ms=a.value, rebuild_ms=r.value,
                 product_ms=pr.value, other_ms=o.value, n_rebuild=nr.value, h2=st,
                 grad_err_final=err, grad_err_after_warmup=err_warm)
 
@@ -248,26 +257,47 @@ def attr_algorithmic_bytes(n, dim, nnz):
     return 8 * nnz + 4 * (n + 1) + n * (2 * gath + 4 * dim + 3 * 2 * 8 * dim)
 
 
-def attr_roofline(n, dim, nnz, kernel_us):
-    by = attr_algorithmic_bytes(n, dim, nnz)
+def attr_tiled_algorithmic_bytes(n, dim, nnz, tiles):
+    """Bytes one launch of the tiled attractive+update kernel must move at least: per
+    nonzero its 16-bit tile-local index and fp32 value; per row the row pointer and the
+    same per-row state as the per-row kernel; per tile its offset, and per distinct
+    neighbour of a tile its column (int32) and one read of its gather record (the
+    staging into LDS)."""
+    gath = 8 if dim == 2 else 16
+    return (6 * nnz + 4 * (n + 1) + n * (2 * gath + 4 * dim + 3 * 2 * 8 * dim)
+            + 4 * (tiles["ntiles"] + 1) + tiles["ucount"] * (4 + gath))
+
+
+def attr_roofline(n, dim, nnz, head):
+    """Roofline of the dominant kernel: the tiled attractive kernel when the run used it
+    (every launch after the first renumbering), else the per-row kernel."""
+    tiled = head.get("attr_tiled_us") is not None
+    if tiled:
+        kernel_us = head["attr_tiled_us"]
+        by = attr_tiled_algorithmic_bytes(n, dim, nnz, head["tiles"])
+        name = "attr_tile_kernel"
+    else:
+        kernel_us = head["attr_kernel_us"]
+        by = attr_algorithmic_bytes(n, dim, nnz)
+        name = "attr_update_kernel"
     ach = by / (kernel_us * 1e-6) / 1e9
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": None,
-            "kernel": "attr_update_kernel", "kernel_avg_us": kernel_us,
-            "algorithmic_bytes": by}
-    t = measured_traffic(n, dim, nnz)
+            "kernel": name, "kernel_avg_us": kernel_us, "algorithmic_bytes": by,
+            "launches": head["attr_tiled_launches"] if tiled else head["attr_kernel_launches"]}
+    t = measured_traffic(n, dim, nnz, name)
     if t is not None:
         roof["traffic"] = t["bytes"]
         roof["traffic_source"] = t["source"]
     return roof
 
 
-def measured_traffic(n, dim, nnz):
+def measured_traffic(n, dim, nnz, kernel="attr_update_kernel"):
     """HBM bytes per attractive launch from the committed rocprofv3 counter passes
     (FETCH_SIZE and WRITE_SIZE, KiB per dispatch), each scaled by the ratio of a plain
     copy's known bytes to what the same counter read for it; None when no pass matches
     this workload."""
-    path = os.path.join(ROOT, "profiles", "attr_update_kernel_traffic.json")
+    path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -504,7 +534,11 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for this process (serving; 0 leaves the default)")
     ap.add_argument("--seq-sample", type=int, default=32)
+    ap.add_argument("--tile-rows", type=int, default=-1,
+                    help="rows per attractive tile (0: per-row kernel only; default: library's)")
     a = ap.parse_args()
+    if a.tile_rows >= 0:
+        os.environ["TSNE_ATTR_TILE_ROWS"] = str(a.tile_rows)
     if a.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the reference is single-process; see DESIGN.md on sharding)", file=sys.stderr)
@@ -536,7 +570,7 @@ def main():
         log(f"{path}: {res[path]['its']:.1f} it/s {json.dumps(res[path])}")
     head = res.get("mixed") or res[next(iter(res))]
     nnz = int(csr[1].size)
-    roof = attr_roofline(n, dim, nnz, head["attr_kernel_us"])
+    roof = attr_roofline(n, dim, nnz, head)
     cpu = None if a.no_cpu_baseline else cpu_baseline(csr, n, dim)
     out = {
         "metric": "iters/s at N=1M 2D, mixed vs plain FP32 GPU @<=1e-2 grad err; jobs/s + p99",

@@ -206,6 +206,17 @@ int tsne_enable_timing(tsne_workspace_t *workspace, int mode);
  * the last run under timing mode 2.  0 on success. */
 int tsne_attr_kernel_time(tsne_workspace_t *workspace, double *total_ms, int *launches);
 
+/* The part of tsne_attr_kernel_time spent in launches of the tiled attractive kernel
+ * (after the first renumbering) and their number.  0 on success. */
+int tsne_attr_kernel_time_tiled(tsne_workspace_t *workspace, double *total_ms, int *launches);
+
+/* Row tiles of the attractive pass after the last renumbering: rows per tile, tile count,
+ * largest distinct-neighbour count among the tiles staged in LDS, total distinct
+ * neighbours over all tiles, and tiles whose neighbours are read from HBM instead
+ * (ntiles 0: no renumbering yet, the per-row kernel runs).  0 on success. */
+int tsne_attr_tile_stats(tsne_workspace_t *workspace, int *tile_rows, int *ntiles, int *umax,
+                         long long *ucount, int *n_global);
+
 /* ------------------------------------------------------------------------------------
  * [add] Job server: continuous batching of independent jobs on one GPU.  No counterpart
  * in the reference (one job per process there).  Jobs are admitted in arrival order

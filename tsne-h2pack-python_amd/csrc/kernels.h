@@ -6,9 +6,9 @@
 namespace tsne {
 
 // Slots of the per-workspace device scalar array `scal` (double).
-// The attractive pass reads P's columns and fp32 values in aligned 4-element chunks,
-// so those two arrays carry this many elements of padding past nnz.
-constexpr int kCsrPad = 4;
+// The attractive pass reads P's columns and fp32 values in aligned 4- and 8-element
+// chunks, so those two arrays carry this many elements of padding past nnz.
+constexpr int kCsrPad = 8;
 
 enum ScalarSlot {
     SLOT_Z = 0,        // normalisation Z of the current repulsive force
@@ -26,6 +26,14 @@ int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, con
                        void *Yg_next, const double *scal, double exag, double mom, double lr,
                        double *block_sums, int nblocks, hipStream_t st,
                        hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
+struct AttrTiles;
+// The same pass over row tiles with LDS-staged neighbours (tiles.h), nt (256 or 1024)
+// threads per tile; nblocks is the capacity of block_sums in records.
+int launch_attr_tile(int D, int n, const AttrTiles &tiles, int nt, const int *row_ptr,
+                     const int *col_idx, const float *pval, const void *Yg, const float *F,
+                     double *Y64, double *uY, double *gains, void *Yg_next, const double *scal,
+                     double exag, double mom, double lr, double *block_sums, int nblocks,
+                     hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 int launch_repulsive_direct_f32(int D, int n, const void *Yg, float *s, float *F,
                                 hipStream_t st);
 int launch_z(int D, int n, const void *Yg, const float *s, const float *F, double *scal,

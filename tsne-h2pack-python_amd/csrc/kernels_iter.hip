@@ -13,10 +13,12 @@
 //   s                float   [n]     sum_{j != i} K_ij
 //   scal             double  [..]    device scalars: Z, the pending centring mean, ...
 // The row order of all of these is the caller's point order.
+#include <algorithm>
 #include <cstdlib>
 
 #include "common.h"
 #include "kernels.h"
+#include "tiles.h"
 
 namespace tsne {
 
@@ -42,6 +44,52 @@ namespace tsne {
 
 typedef int v4i __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
+
+// End of one row, shared by the attractive kernels: the group's LANES partial forces are
+// reduced, lanes 0..D-1 apply the update to coordinate `sub` in FP64 and store Y64, uY,
+// gains; lane 0 writes the row's next gather record.  Returns the new coordinate (lanes
+// 0..D-1) for the centring sums, 0 elsewhere.
+template <int D, int LANES>
+__device__ __forceinline__ double row_epilogue(float ax, float ay, float az, int sub, size_t kd,
+                                               double f_d, double u_d, double g_d, double y_d,
+                                               double m_d, double exag4, double rep_scale,
+                                               double mom, double lr, double *__restrict__ Y64,
+                                               double *__restrict__ uY,
+                                               double *__restrict__ gains,
+                                               typename Vec<D>::T *__restrict__ rec_out)
+{
+    for (int off = LANES / 2; off > 0; off >>= 1) {
+        ax += __shfl_xor(ax, off, 64);
+        ay += __shfl_xor(ay, off, 64);
+        if constexpr (D == 3) az += __shfl_xor(az, off, 64);
+    }
+    double ynew = 0.0;
+    if (sub < D) {
+        const float a = (sub == 0) ? ax : (sub == 1) ? ay : az;
+        const double g = exag4 * (double)a - rep_scale * f_d;
+        double u = u_d, gn = g_d;
+        gn = (u * g < 0.0) ? gn + 0.2 : gn * 0.8;
+        if (gn < 0.01) gn = 0.01;
+        u = mom * u - lr * (gn * g);
+        ynew = (y_d - m_d) + u;
+        Y64[kd] = ynew;
+        uY[kd] = u;
+        gains[kd] = gn;
+    }
+    // the new coordinates sit in lanes 0..D-1 of the group: assemble the record
+    const int base = threadIdx.x & ~(LANES - 1);
+    const double n0 = __shfl(ynew, base, 64);
+    const double n1 = __shfl(ynew, base + 1, 64);
+    const double n2 = (D == 3) ? __shfl(ynew, base + 2, 64) : 0.0;
+    if (sub == 0) {
+        typename Vec<D>::T rec;
+        rec.x = (float)n0;
+        rec.y = (float)n1;
+        if constexpr (D == 3) { rec.z = (float)n2; rec.w = 0.f; }
+        *rec_out = rec;
+    }
+    return ynew;
+}
 
 // LANES lanes per row (256 / LANES rows in flight per block), BATCH nonzeros per lane
 // per batch.  VEC: P's columns and values are read as aligned 4-element chunks (BATCH
@@ -155,37 +203,8 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
                 if constexpr (D == 3) az = fmaf(q, dz, az);
             }
         }
-        for (int off = LANES / 2; off > 0; off >>= 1) {
-            ax += __shfl_xor(ax, off, 64);
-            ay += __shfl_xor(ay, off, 64);
-            if constexpr (D == 3) az += __shfl_xor(az, off, 64);
-        }
-        double ynew = 0.0;
-        if (sub < D) {
-            const float a = (sub == 0) ? ax : (sub == 1) ? ay : az;
-            const double g = exag4 * (double)a - rep_scale * f_d;
-            double u = u_d, gn = g_d;
-            gn = (u * g < 0.0) ? gn + 0.2 : gn * 0.8;
-            if (gn < 0.01) gn = 0.01;
-            u = mom * u - lr * (gn * g);
-            ynew = (y_d - m_d) + u;
-            Y64[kd] = ynew;
-            uY[kd] = u;
-            gains[kd] = gn;
-            my_sum += ynew;
-        }
-        // the new coordinates sit in lanes 0..D-1 of the group: assemble the record
-        const int base = threadIdx.x & ~(LANES - 1);
-        const double n0 = __shfl(ynew, base, 64);
-        const double n1 = __shfl(ynew, base + 1, 64);
-        const double n2 = (D == 3) ? __shfl(ynew, base + 2, 64) : 0.0;
-        if (sub == 0) {
-            V rec;
-            rec.x = (float)n0;
-            rec.y = (float)n1;
-            if constexpr (D == 3) { rec.z = (float)n2; rec.w = 0.f; }
-            Yg_next[i] = rec;
-        }
+        my_sum += row_epilogue<D, LANES>(ax, ay, az, sub, kd, f_d, u_d, g_d, y_d, m_d, exag4,
+                                         rep_scale, mom, lr, Y64, uY, gains, Yg_next + i);
     }
     // per-group sums (lane d holds coordinate d), then groups in order
     if (sub < D) gsum[grp][sub] = my_sum;
@@ -193,6 +212,140 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     if (threadIdx.x < D) {
         double t = 0.0;
         for (int w = 0; w < (256 / LANES); w++) t += gsum[w][threadIdx.x];
+        block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
+    }
+}
+
+// ---------------------------------------------------------------------------------
+// Attractive force + fused update over row tiles (points numbered along the tree).
+//
+// One workgroup per tile of `tile_rows` consecutive rows.  The gather records of the
+// tile's distinct neighbours (tcols[toff[b] .. toff[b+1]), sorted, so the staging reads
+// are nearly contiguous) are copied into LDS once; each nonzero then carries a 16-bit
+// position in that list (lidx) and its neighbour is an LDS read.  P is streamed as
+// aligned 8-element chunks: 16 bytes of lidx and 32 bytes of fp32 values, non-temporal.
+// A tile with more distinct neighbours than fit (ucap) reads its neighbours from HBM
+// through the int32 columns instead.  Same arithmetic and update as attr_update_kernel;
+// block_sums gets one record per tile.
+// ---------------------------------------------------------------------------------
+template <int D, int LANES, int BATCH, int NT>
+__global__ void __launch_bounds__(NT)
+attr_tile_kernel(int n, int ntiles, int tile_rows, const int *__restrict__ row_ptr,
+                 const uint16_t *__restrict__ lidx, const int *__restrict__ col_idx,
+                 const float *__restrict__ pval, const int *__restrict__ toff,
+                 const int *__restrict__ tcols, int ucap,
+                 const typename Vec<D>::T *__restrict__ Yg, const float *__restrict__ F,
+                 double *__restrict__ Y64, double *__restrict__ uY, double *__restrict__ gains,
+                 typename Vec<D>::T *__restrict__ Yg_next, const double *__restrict__ scal,
+                 double exag4, double mom, double lr, double *__restrict__ block_sums)
+{
+    using V = typename Vec<D>::T;
+    extern __shared__ __align__(16) unsigned char s_dyn[];
+    V *s_rec = reinterpret_cast<V *>(s_dyn);
+    __shared__ double gsum[(NT / LANES)][3];
+    const int sub = threadIdx.x & (LANES - 1);
+    const int grp = threadIdx.x / LANES;
+    // XCD-aware placement as in attr_update_kernel (gridDim.x is a multiple of 8;
+    // tiles past ntiles are empty)
+    const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const int row0 = min(n, lblk * tile_rows);
+    const int row1 = min(n, row0 + tile_rows);
+    const int u0 = (lblk < ntiles) ? toff[lblk] : 0;
+    const int U = (lblk < ntiles) ? toff[lblk + 1] - u0 : 0;
+    const bool local = U <= ucap;  // uniform over the workgroup
+    if (local)
+        for (int u = threadIdx.x; u < U; u += NT) s_rec[u] = Yg[tcols[u0 + u]];
+    const double rep_scale = 4.0 / scal[SLOT_Z];
+    const double m_d = (sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
+    double my_sum = 0.0;
+    __syncthreads();
+
+    for (int i = row0 + grp; i < row1; i += (NT / LANES)) {
+        const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
+        const size_t kd = (size_t)i * D + (sub < D ? sub : 0);
+        double f_d = 0.0, u_d = 0.0, g_d = 0.0, y_d = 0.0;
+        if (sub < D) {
+            f_d = (double)F[kd];
+            u_d = uY[kd];
+            g_d = gains[kd];
+            y_d = Y64[kd];
+        }
+        const V yi = Yg[i];
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        auto accum = [&](const V &y, float p) {
+            const float dx = yi.x - y.x, dy = yi.y - y.y;
+            float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
+            float dz = 0.f;
+            if constexpr (D == 3) { dz = yi.z - y.z; r2 = fmaf(dz, dz, r2); }
+            const float q = p * fast_rcp(r2);
+            ax = fmaf(q, dx, ax);
+            ay = fmaf(q, dy, ay);
+            if constexpr (D == 3) az = fmaf(q, dz, az);
+        };
+        if (local) {
+            const int c1 = (e1 + 7) >> 3;  // 8-element chunks [e0/8, c1) cover the row
+            for (int cb = (e0 >> 3) + sub; cb < c1; cb += BATCH * LANES) {
+                v4i li[BATCH];
+                v4f pa[BATCH], pb[BATCH];
+#pragma unroll
+                for (int t = 0; t < BATCH; t++) {
+                    const int c = cb + t * LANES;
+                    li[t] = v4i{0, 0, 0, 0};
+                    pa[t] = pb[t] = v4f{0.f, 0.f, 0.f, 0.f};
+                    if (c < c1) {
+                        li[t] = __builtin_nontemporal_load((const v4i *)lidx + c);
+                        pa[t] = __builtin_nontemporal_load((const v4f *)pval + 2 * c);
+                        pb[t] = __builtin_nontemporal_load((const v4f *)pval + 2 * c + 1);
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < BATCH; t++) {
+                    const int e = 8 * (cb + t * LANES);
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const bool ok = e + u >= e0 && e + u < e1;
+                        const int l = (li[t][u >> 1] >> (16 * (u & 1))) & 0xffff;
+                        const V y = s_rec[ok ? l : 0];
+                        const float p = (u < 4) ? pa[t][u] : pb[t][u - 4];
+                        accum(ok ? y : yi, ok ? p : 0.f);
+                    }
+                }
+            }
+        } else {
+            const int c1 = (e1 + 3) >> 2;  // 4-element chunks of the int32 columns
+            for (int cb = (e0 >> 2) + sub; cb < c1; cb += BATCH * LANES) {
+                v4i cc[BATCH];
+                v4f pv[BATCH];
+#pragma unroll
+                for (int t = 0; t < BATCH; t++) {
+                    const int c = cb + t * LANES;
+                    cc[t] = v4i{i, i, i, i};
+                    pv[t] = v4f{0.f, 0.f, 0.f, 0.f};
+                    if (c < c1) {
+                        cc[t] = __builtin_nontemporal_load((const v4i *)col_idx + c);
+                        pv[t] = __builtin_nontemporal_load((const v4f *)pval + c);
+                    }
+                }
+#pragma unroll
+                for (int t = 0; t < BATCH; t++) {
+                    const int e = 4 * (cb + t * LANES);
+#pragma unroll
+                    for (int u = 0; u < 4; u++) {
+                        const bool ok = e + u >= e0 && e + u < e1;
+                        const V y = Yg[ok ? cc[t][u] : i];
+                        accum(y, ok ? pv[t][u] : 0.f);
+                    }
+                }
+            }
+        }
+        my_sum += row_epilogue<D, LANES>(ax, ay, az, sub, kd, f_d, u_d, g_d, y_d, m_d, exag4,
+                                         rep_scale, mom, lr, Y64, uY, gains, Yg_next + i);
+    }
+    if (sub < D) gsum[grp][sub] = my_sum;
+    __syncthreads();
+    if (threadIdx.x < D) {
+        double t = 0.0;
+        for (int w = 0; w < (NT / LANES); w++) t += gsum[w][threadIdx.x];
         block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
     }
 }
@@ -206,10 +359,9 @@ mean_finalize_kernel(int nblocks, int D, int n, const double *__restrict__ block
     __shared__ double part[3][1024];
     const int t = threadIdx.x;
     double a[3] = {0.0, 0.0, 0.0};
-    // each thread sums a contiguous chunk of blocks in order, then a fixed tree
-    const int per = (nblocks + 1023) / 1024;
-    const int b0 = t * per, b1 = min(nblocks, b0 + per);
-    for (int b = b0; b < b1; b++)
+    // thread t sums blocks t, t + 1024, ... in order (neighbouring threads read
+    // neighbouring records, so each pass is one coalesced sweep), then a fixed tree
+    for (int b = t; b < nblocks; b += 1024)
         for (int d = 0; d < D; d++) a[d] += block_sums[(size_t)b * 3 + d];
     for (int d = 0; d < 3; d++) part[d][t] = a[d];
     __syncthreads();
@@ -551,6 +703,79 @@ static bool attr_vec()
         return !(e && e[0] == '0');
     }();
     return v;
+}
+
+int launch_attr_tile(int D, int n, const AttrTiles &tl, int nt, const int *row_ptr, const int *col_idx,
+                     const float *pval, const void *Yg, const float *F, double *Y64, double *uY,
+                     double *gains, void *Yg_next, const double *scal, double exag, double mom,
+                     double lr, double *block_sums, int nblocks, hipStream_t st,
+                     hipEvent_t ev_begin, hipEvent_t ev_end)
+{
+    const int grid = (tl.ntiles + 7) & ~7;
+    if (grid > nblocks) return -1;  // block_sums holds nblocks records
+    const int ucap = tl.umax;       // tiles with more distinct neighbours gather from HBM
+    const size_t lds = (size_t)std::max(ucap, 1) * (D == 2 ? sizeof(float2) : sizeof(float4));
+    if (lds > (size_t)kTileLdsMaxKB * 1024) return -1;  // beyond the CU's LDS with the static part
+    {
+        // the device's per-workgroup LDS limit (read once): a launch beyond it would be
+        // rejected by the runtime, so refuse it here
+        static int lim = [] {
+            int dev = 0, v = 0;
+            if (hipGetDevice(&dev) != hipSuccess ||
+                hipDeviceGetAttribute(&v, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+                v = 65536;
+            return v;
+        }();
+        if (lds + 4096 > (size_t)lim) {
+            fprintf(stderr, "tsne_h2pack: tile LDS %zu B exceeds the %d B per workgroup\n", lds, lim);
+            return -1;
+        }
+    }
+    if (ev_begin) (void)hipEventRecord(ev_begin, st);
+#define TSNE_TILE_ARGS dim3(grid), dim3(NT_), lds, st, n, tl.ntiles, tl.rows, row_ptr, tl.lidx, \
+        col_idx, pval, tl.off, tl.cols, ucap
+    if (D == 2) {
+        if (nt == 1024) {
+            constexpr int NT_ = 1024;
+            if (lds > 65536)
+                (void)hipFuncSetAttribute((const void *)attr_tile_kernel<2, 16, 2, 1024>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL((attr_tile_kernel<2, 16, 2, 1024>), TSNE_TILE_ARGS, (const float2 *)Yg,
+                               F, Y64, uY, gains, (float2 *)Yg_next, scal, 4.0 * exag, mom, lr,
+                               block_sums);
+        } else {
+            constexpr int NT_ = 256;
+            if (lds > 65536)
+                (void)hipFuncSetAttribute((const void *)attr_tile_kernel<2, 16, 2, 256>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL((attr_tile_kernel<2, 16, 2, 256>), TSNE_TILE_ARGS, (const float2 *)Yg,
+                               F, Y64, uY, gains, (float2 *)Yg_next, scal, 4.0 * exag, mom, lr,
+                               block_sums);
+        }
+    } else {
+        if (nt == 1024) {
+            constexpr int NT_ = 1024;
+            if (lds > 65536)
+                (void)hipFuncSetAttribute((const void *)attr_tile_kernel<3, 16, 2, 1024>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL((attr_tile_kernel<3, 16, 2, 1024>), TSNE_TILE_ARGS, (const float4 *)Yg,
+                               F, Y64, uY, gains, (float4 *)Yg_next, scal, 4.0 * exag, mom, lr,
+                               block_sums);
+        } else {
+            constexpr int NT_ = 256;
+            if (lds > 65536)
+                (void)hipFuncSetAttribute((const void *)attr_tile_kernel<3, 16, 2, 256>,
+                                          hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            hipLaunchKernelGGL((attr_tile_kernel<3, 16, 2, 256>), TSNE_TILE_ARGS, (const float4 *)Yg,
+                               F, Y64, uY, gains, (float4 *)Yg_next, scal, 4.0 * exag, mom, lr,
+                               block_sums);
+        }
+    }
+#undef TSNE_TILE_ARGS
+    if (ev_end) (void)hipEventRecord(ev_end, st);
+    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, grid, D, n, block_sums,
+                       (double *)scal);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,

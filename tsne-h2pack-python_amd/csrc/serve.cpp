@@ -146,10 +146,11 @@ extern "C" {
 
 long long tsne_job_footprint(int n_point, int low_dim, long long nnz)
 {
-    // P on the device (int32 columns, fp32 + fp64 values) and, until the last
-    // renumbering, a second copy with its sort scratch; per-point state, gather copies,
-    // forces and the H2 structure; fixed buffers (proxy tables, sort scratch)
-    return 48LL * nnz + (long long)n_point * (32LL * low_dim + 600) + (48LL << 20);
+    // P on the device (int32 columns, fp32 + fp64 values, 16-bit tile indices) and, until
+    // the last renumbering, a second copy with its sort scratch and the tile build's;
+    // per-point state, gather copies, forces and the H2 structure; fixed buffers (proxy
+    // tables, sort scratch)
+    return 64LL * nnz + (long long)n_point * (32LL * low_dim + 600) + (48LL << 20);
 }
 
 tsne_server_t *tsne_server_create(int slots, long long hbm_budget_bytes)

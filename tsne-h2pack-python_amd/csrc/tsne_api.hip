@@ -14,6 +14,7 @@
 #include "h2.h"
 #include "kernels.h"
 #include "renumber.h"
+#include "tiles.h"
 #include "../../include/tsne_h2pack.h"
 
 using namespace tsne;
@@ -50,6 +51,8 @@ struct tsne_workspace {
     int timing = 0;             // 1: per-phase events (synchronising), 2: attractive kernel only
     double t_attr_kernel = 0;   // timing 2: summed attractive-kernel time (ms) and launches
     int n_attr_kernel = 0;
+    double t_attr_tiled = 0;    // the part of it spent in the tiled kernel, and its launches
+    int n_attr_tiled = 0;
     double knn_time = 0, p_time = 0;
     std::vector<double> prof;  // per-iteration phase times when profiling
     // renumbering along the tree order (reference src/tsne_h2pack.c:1706-1791)
@@ -61,6 +64,12 @@ struct tsne_workspace {
     RenumberCsr P2;             // second CSR buffer set, alive while renumberings remain
     RenumberScratch rs;
     void *d_tmp = nullptr;      // n * 4 doubles: permutation staging
+    // row tiles of P for the attractive pass, built after each renumbering (ntiles == 0:
+    // the points are still in the caller's order and the per-row kernel runs)
+    AttrTiles tiles;
+    int tile_rows = 0;          // 0: tiles off
+    int tile_lds = 0;           // LDS bytes per tile workgroup for the staged records
+    int tile_nt = 256;          // threads per tile workgroup
 };
 
 namespace {
@@ -132,6 +141,7 @@ void free_renumber_buffers(tsne_workspace *ws)
 void free_device(tsne_workspace *ws)
 {
     free_renumber_buffers(ws);
+    tiles_free(ws->tiles, ws->st);
     dfree(ws->d_order, ws->st);
     if (ws->h2) { h2_destroy(ws->h2); ws->h2 = nullptr; }
     dfree(ws->d_rowptr, ws->st); dfree(ws->d_col, ws->st); dfree(ws->d_val32, ws->st); dfree(ws->d_val64, ws->st);
@@ -240,6 +250,14 @@ int renumber(tsne_workspace *ws)
     std::swap(ws->d_col, ws->P2.col);
     std::swap(ws->d_val32, ws->P2.v32);
     std::swap(ws->d_val64, ws->P2.v64);
+    // the attractive pass's row tiles follow the new numbering (a failed build leaves
+    // the per-row kernel in use)
+    if (ws->tile_rows > 0 &&
+        tiles_build(ws->tiles, n, nnz, ws->d_rowptr, ws->d_col, ws->tile_rows, D,
+                    tile_ucap(D, ws->tile_lds), st) != 0) {
+        (void)hipGetLastError();
+        fprintf(stderr, "WARNING: attractive row tiles not built (out of device memory?)\n");
+    }
     // per-point state: permute into the staging buffer, copy back
     struct Arr { void *p; int width, bytes; };
     const Arr arrs[] = {{ws->d_Y64, D, 8}, {ws->d_uY, D, 8}, {ws->d_gains, D, 8},
@@ -464,6 +482,21 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         rpb = std::max(16, std::min(rpb, 4096));
         ws->nblocks_upd = (((n + rpb - 1) / rpb) + 7) & ~7;
     }
+    // rows per attractive tile (environment TSNE_ATTR_TILE_ROWS, 0 = per-row kernel only);
+    // tiles are at least as large as the per-row blocks, so block_sums holds either
+    {
+        const char *e = getenv("TSNE_ATTR_TILE_ROWS");
+        int tr = e ? atoi(e) : 0;
+        const int rpb = (n + ws->nblocks_upd - 1) / ws->nblocks_upd;
+        ws->tile_rows = (tr <= 0) ? 0 : std::max(std::max(tr, rpb), 16);
+        // LDS for the staged records (TSNE_ATTR_TILE_LDS_KB, 16..kTileLdsMaxKB KiB; the
+        // kernel's static LDS takes the rest of the CU's 160 KiB) and the workgroup size
+        // (TSNE_ATTR_TILE_NT, 256 or 1024)
+        const char *l = getenv("TSNE_ATTR_TILE_LDS_KB");
+        ws->tile_lds = 1024 * std::max(16, std::min(l ? atoi(l) : 64, kTileLdsMaxKB));
+        const char *t = getenv("TSNE_ATTR_TILE_NT");
+        ws->tile_nt = (t && atoi(t) == 1024) ? 1024 : 256;
+    }
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
     bool ok = hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
     if (!ok) {
@@ -571,8 +604,10 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
     // timing 2: a pair of events around every attractive launch, read after the last
     // synchronisation, so the run itself is not slowed by host round trips
     std::vector<hipEvent_t> kev;
+    std::vector<char> ktiled;  // which timed launches ran the tiled kernel
     if (ws->timing == 2 && it_end > it_begin) {
         kev.resize(2 * (size_t)(it_end - it_begin), nullptr);
+        ktiled.assign((size_t)(it_end - it_begin), 0);
         for (auto &e : kev)
             if (hipEventCreate(&e) != hipSuccess) e = nullptr;
     }
@@ -580,6 +615,8 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         ws->t_attr = ws->t_rebuild = ws->t_product = ws->t_other = 0;
         ws->t_attr_kernel = 0;
         ws->n_attr_kernel = 0;
+        ws->t_attr_tiled = 0;
+        ws->n_attr_tiled = 0;
         ws->n_rebuild = 0;
         if (pp.enable_profiling) ws->prof.assign((size_t)n_iter * 4, 0.0);
     }
@@ -632,12 +669,20 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         }
         if (timing) (void)hipEventRecord(ev[3], ws->st);
         const int nxt = ws->cur ^ 1;
-        if (launch_attr_update(D, ws->n, ws->d_rowptr, ws->d_col, ws->d_val32, ws->d_Yg[ws->cur],
-                               ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains, ws->d_Yg[nxt], ws->d_scal,
-                               exag, mom, lr, ws->d_block_sums, ws->nblocks_upd, ws->st,
-                               kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin)],
-                               kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin) + 1]) != 0)
-            return -1;
+        hipEvent_t kb = kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin)];
+        hipEvent_t ke = kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin) + 1];
+        const int arc =
+            ws->tiles.ntiles > 0
+                ? launch_attr_tile(D, ws->n, ws->tiles, ws->tile_nt, ws->d_rowptr, ws->d_col, ws->d_val32,
+                                   ws->d_Yg[ws->cur], ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains,
+                                   ws->d_Yg[nxt], ws->d_scal, exag, mom, lr, ws->d_block_sums,
+                                   ws->nblocks_upd, ws->st, kb, ke)
+                : launch_attr_update(D, ws->n, ws->d_rowptr, ws->d_col, ws->d_val32,
+                                     ws->d_Yg[ws->cur], ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains,
+                                     ws->d_Yg[nxt], ws->d_scal, exag, mom, lr, ws->d_block_sums,
+                                     ws->nblocks_upd, ws->st, kb, ke);
+        if (arc != 0) return -1;
+        if (!ktiled.empty()) ktiled[(size_t)(it - it_begin)] = ws->tiles.ntiles > 0;
         ws->cur = nxt;
         if (timing) {
             (void)hipEventRecord(ev[4], ws->st);
@@ -675,6 +720,10 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         if (kev[k] && kev[k + 1] && hipEventElapsedTime(&ms, kev[k], kev[k + 1]) == hipSuccess) {
             ws->t_attr_kernel += ms;
             ws->n_attr_kernel++;
+            if (ktiled[k / 2]) {
+                ws->t_attr_tiled += ms;
+                ws->n_attr_tiled++;
+            }
         }
     }
     for (auto &e : kev)
@@ -909,6 +958,26 @@ int tsne_run_timings(tsne_workspace_t *ws, double *attr, double *reb, double *pr
     if (prod) *prod = ws->t_product;
     if (other) *other = ws->t_other;
     if (n_rebuild) *n_rebuild = ws->n_rebuild;
+    return 0;
+}
+
+int tsne_attr_kernel_time_tiled(tsne_workspace_t *ws, double *total_ms, int *launches)
+{
+    if (!ws) return -1;
+    if (total_ms) *total_ms = ws->t_attr_tiled;
+    if (launches) *launches = ws->n_attr_tiled;
+    return 0;
+}
+
+int tsne_attr_tile_stats(tsne_workspace_t *ws, int *tile_rows, int *ntiles, int *umax,
+                         long long *ucount, int *n_global)
+{
+    if (!ws) return -1;
+    *tile_rows = ws->tiles.rows;
+    *ntiles = ws->tiles.ntiles;
+    *umax = ws->tiles.umax;
+    *ucount = ws->tiles.ucount;
+    *n_global = ws->tiles.n_global;
     return 0;
 }
 

@@ -386,6 +386,10 @@ def configure_signatures(lib):
     lib.tsne_enable_timing.restype = ctypes.c_int
     lib.tsne_attr_kernel_time.argtypes = [c_ws, dp, ip]
     lib.tsne_attr_kernel_time.restype = ctypes.c_int
+    lib.tsne_attr_kernel_time_tiled.argtypes = [c_ws, dp, ip]
+    lib.tsne_attr_kernel_time_tiled.restype = ctypes.c_int
+    lib.tsne_attr_tile_stats.argtypes = [c_ws, ip, ip, ip, ctypes.POINTER(ctypes.c_longlong), ip]
+    lib.tsne_attr_tile_stats.restype = ctypes.c_int
     # job server
     c_srv = ctypes.c_void_p
     lp = ctypes.POINTER(ctypes.c_longlong)
