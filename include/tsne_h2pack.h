@@ -168,8 +168,9 @@ int tsne_exact_repulsive_rows(tsne_workspace_t *workspace, const int *rows, int 
                               double *out, double *Z_out);
 
 /* Statistics of the last H2 build: levels, nodes, leaves, admissible and near pairs,
- * maximum and mean skeleton rank, proxy points of the finest level; see the
- * tsne_h2_stats_t fields.  0 on success, -1 if no H2 structure exists. */
+ * maximum and mean skeleton rank, proxy points of the finest level, flop counts of one
+ * product; and of the last mixed-precision product, the nodes whose BF16 coupling was
+ * redone in FP32.  0 on success, -1 if no H2 structure exists. */
 typedef struct {
     int levels, n_nodes, n_leaves, leaf_points_max;
     long long n_adm_pairs, n_near_pairs, near_point_pairs;
@@ -177,6 +178,7 @@ typedef struct {
     double rank_mean;
     int proxy_max;
     double far_flops, near_flops;
+    int mixed_fallback_nodes;
 } tsne_h2_stats_t;
 int tsne_h2_stats(tsne_workspace_t *workspace, tsne_h2_stats_t *stats);
 

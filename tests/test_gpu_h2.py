@@ -297,3 +297,40 @@ def test_plain_gradient_all_rows_100k(backend):
     _say(f"  grad err {err:.2e} rep err {rep_err:.2e} Z {Z.value:.6e}/{Ze.value:.6e}")
     assert err <= 1e-2
     assert rep_err <= 1e-2
+
+
+def test_overlapped_attractive_pass_matches_fused(backend, problem):
+    """On rebuild iterations the attractive sums run on a second stream next to the
+    rebuild and the product, and a separate kernel applies the update; the result
+    equals the fused single-stream pass up to the order of the centring sums."""
+    import os
+    _say("overlap vs fused")
+    _, _, P = problem
+    Y0 = oracle.init_random(P.shape[0], 2, 23)
+    out = {}
+    for ov in ("0", "1"):
+        old = os.environ.get("TSNE_OVERLAP")
+        os.environ["TSNE_OVERLAP"] = ov
+        try:
+            p = backend.create_default_params()
+            p.verbose = 0
+            p.n_iter = 12  # rebuilds at iterations 1..11 (every iteration below 20)
+            p.h2_tol = 1e-2
+            p.h2_rebuild_freq = 3
+            p.learning_rate = max(P.shape[0] / 12.0, 200.0)
+            ws = backend.init_workspace(P, 2, p, init_Y=Y0.T.copy())
+        finally:
+            if old is None:
+                del os.environ["TSNE_OVERLAP"]
+            else:
+                os.environ["TSNE_OVERLAP"] = old
+        try:
+            assert backend.lib.tsne_run(ws) == 0
+            Y = np.zeros((2, P.shape[0]))
+            backend.lib.tsne_get_embedding(ws, Y)
+            out[ov] = (Y, backend.lib.compute_kl_divergence(ws))
+        finally:
+            backend.lib.tsne_destroy(ws)
+    rel = np.abs(out["0"][0] - out["1"][0]).max() / np.abs(out["0"][0]).max()
+    _say(f"  rel diff {rel:.2e}, KL {out['0'][1]:.8f} / {out['1'][1]:.8f}")
+    assert rel < 1e-6

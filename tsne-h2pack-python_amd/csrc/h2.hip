@@ -77,7 +77,7 @@ void free_sized(H2 *h)
     dfree(d.adm_beg, h->st); dfree(d.adm_end, h->st); dfree(d.x_beg, h->st); dfree(d.x_end, h->st);
     dfree(d.w_beg, h->st); dfree(d.w_end, h->st); dfree(d.nl_beg, h->st); dfree(d.nl_end, h->st);
     dfree(d.rank, h->st); dfree(d.uoff, h->st); dfree(d.skel, h->st); dfree(d.skelf, h->st); dfree(d.upool, h->st);
-    dfree(d.wc, h->st); dfree(d.zc, h->st);
+    dfree(d.wc, h->st); dfree(d.zc, h->st); dfree(d.nflag, h->st);
     dfree(h->cub_tmp, h->st);
     h->cub_bytes = 0;
 }
@@ -119,7 +119,8 @@ bool alloc_sized(H2 *h)
               dmalloc(&d.nl_beg, nc, h->st) && dmalloc(&d.nl_end, nc, h->st) && dmalloc(&d.rank, nc, h->st) &&
               dmalloc(&d.uoff, nc, h->st) && dmalloc(&d.skel, nc * RMAX * D, h->st) &&
               dmalloc(&d.skelf, nc * RMAX, h->st) && dmalloc(&d.upool, (size_t)d.pool_cap, h->st) &&
-              dmalloc(&d.wc, nc * RMAX, h->st) && dmalloc(&d.zc, nc * RMAX, h->st);
+              dmalloc(&d.wc, nc * RMAX, h->st) && dmalloc(&d.zc, nc * RMAX, h->st) &&
+              dmalloc(&d.nflag, nc, h->st);
     if (!ok) return false;
     // temporary storage for the library sorts / scans (largest of the three uses)
     size_t b1 = 0, b2 = 0, b3 = 0;
@@ -345,7 +346,7 @@ int h2_product(H2 *h, const void *Yg, float *s, float *F, int mixed)
     (void)Yg;
     if (!h->usable) return -1;
     H2Dev &d = h->d;
-    if (launch_couple(h->D, d, h->n_nodes, mixed, h->st)) return -1;
+    if (launch_couple(h->D, d, h->n_nodes, mixed, (float)h->tol, h->st)) return -1;
     const int nlev = (int)h->lvl.size() - 1;
     for (int lv = 2; lv < nlev; lv++)
         if (launch_down(h->D, d, h->lvl[lv], h->lvl[lv + 1], mixed, h->st)) return -1;
@@ -369,7 +370,11 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     st->rank_max = h->hflags[FLAG_RANK_MAX];
     st->rank_mean = h->n_nodes > 1 ? (double)h->hflags[FLAG_RANK_SUM] / (h->n_nodes - 1) : 0.0;
     st->proxy_max = h->hflags[FLAG_PROXY_MAX];
+    st->mixed_fallback_nodes = 0;
     if (!h->usable || h->n_nodes <= 0) return 0;
+    if (hipMemcpyAsync(&st->mixed_fallback_nodes, h->d.cnt + CNT_FALLBACK, sizeof(int),
+                       hipMemcpyDeviceToHost, h->st) != hipSuccess)
+        return -1;
     // flop counts of one product from the built structure (host copies; diagnostics only)
     const int nn = h->n_nodes;
     std::vector<int> nb(nn), ne(nn), rk(nn), c0(nn), par(nn);

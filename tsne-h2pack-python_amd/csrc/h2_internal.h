@@ -59,6 +59,7 @@ enum H2Flag {
 
 // Counters of the pair buffers (device int array H2Dev::cnt).
 enum H2Count { CNT_ADM = 0, CNT_X, CNT_W, CNT_NEAR, CNT_PEND0, CNT_PEND1, CNT_LEAF, CNT_POOL,
+               CNT_FALLBACK,  // nodes whose BF16 coupling was redone in FP32 (last product)
                CNT_COUNT = 16 };
 
 struct RootInfo {
@@ -97,6 +98,7 @@ struct H2Dev {
     float *upool;
     long long pool_cap;
     float4 *wc, *zc;   // [node][RMAX] upward / downward coefficients
+    int *nflag;        // [node] the mixed coupling of this node needs the FP32 pass
     const double *proxy_pts;  // [levels][PMAX][D]
     const int *proxy_cnt;
     int *flags, *cnt;
@@ -121,7 +123,9 @@ int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, doub
                  int pq, hipStream_t st);
 
 // h2_product.hip
-int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, hipStream_t st);
+// mixed: coupling on MFMA with BF16 operands and FP32 accumulation, nodes whose
+// rounding-error estimate exceeds tau x |z| are flagged and recomputed by the FP32 pass.
+int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, float tau, hipStream_t st);
 int launch_down(int D, const H2Dev &h, int lb, int le, int mixed, hipStream_t st);
 int launch_leaf_eval(int D, const H2Dev &h, int n_leaves, float *s, float *F, hipStream_t st);
 

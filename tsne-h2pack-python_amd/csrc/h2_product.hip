@@ -45,14 +45,16 @@ __device__ __forceinline__ float4 group_sum(float4 v, int RS)
 
 // One wavefront per target node (all non-root nodes); lanes are (group g, row k): row k
 // of the node's skeleton, groups split the source list.
+// only_flagged: recompute just the nodes the mixed kernel flagged.
 template <int D>
 __global__ void __launch_bounds__(256)
-couple_kernel(H2Dev h, int n_nodes)
+couple_kernel(H2Dev h, int n_nodes, int only_flagged)
 {
     constexpr int RMAX = H2Cfg<D>::RMAX;
     const int lane = threadIdx.x & 63;
     const int i = 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n_nodes) return;
+    if (only_flagged && h.nflag[i] == 0) return;
     const int r = h.rank[i];
     const int RS = rank_stride(r), G = 64 / RS;
     const int k = lane % RS, g = lane / RS;
@@ -95,6 +97,178 @@ couple_kernel(H2Dev h, int n_nodes)
     }
     acc = group_sum(acc, RS);
     if (g == 0 && k < r) h.zc[(size_t)i * RMAX + k] = acc;
+}
+
+// ---------------------------------------------------------------------------------
+// Mixed-precision coupling: the same z_i as couple_kernel, as a batched small GEMM on
+// MFMA.  For target node i (one wavefront) the sources are the skeleton points of its
+// ADM nodes and the points of its X leaves, concatenated; with W'_s the source's row
+// [w0, (c_i - c_j) w0 - wY] (ADM) or [1, c_i - y] (X),
+//     z_i = K(J_i, sources) W',
+// an (r_i x S) by (S x 4) product.  v_mfma_f32_4x4x4bf16_1k computes 16 independent
+// 4x4x4 blocks: block b = lane / 4 takes target rows 4g..4g+3 (g = b mod G, G = the
+// rank in groups of 4, rounded up to a power of two) against 4 sources of phase
+// b / G, so one instruction consumes 64 / G sources; the phases are summed at the end.
+// Lane u of a block supplies row 4g+u of K (four kernel values, computed in FP32 and
+// rounded to BF16) and column u of W' for the block's four sources, split into two
+// BF16 terms (hi + lo) so that W' keeps ~16 bits; accumulation is FP32.
+//
+// Error control: rounding K to BF16 perturbs each term by at most 2^-9 relative with
+// signs that vary over the sources, so the error of an entry of z is estimated as
+// 2^-9 sqrt(sum (K W')^2) (a third MFMA accumulates K^2 W'^2).  A node where the estimate
+// exceeds tau |z| for any entry (strong cancellation between its sources) is flagged
+// and left to the FP32 pass (couple_kernel with only_flagged).
+// ---------------------------------------------------------------------------------
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+typedef __bf16 v4bf __attribute__((ext_vector_type(4)));
+
+// four floats to BF16 (round to nearest even; two v_cvt_pk_bf16_f32)
+__device__ __forceinline__ v4s bf16x4(v4f x)
+{
+    return __builtin_bit_cast(v4s, __builtin_convertvector(x, v4bf));
+}
+
+__device__ __forceinline__ v4f f32x4(v4s x)
+{
+    return __builtin_convertvector(__builtin_bit_cast(v4bf, x), v4f);
+}
+
+template <int D>
+__global__ void __launch_bounds__(256)
+couple_mfma_kernel(H2Dev h, int n_nodes, float tau)
+{
+    constexpr int RMAX = H2Cfg<D>::RMAX;
+    constexpr int CH = 64;  // sources staged per round
+    __shared__ float4 s_x[4][CH];
+    // W' of the staged sources as BF16, column-major so that a lane's four sources of
+    // one column are one 8-byte read: hi and lo terms, and W'^2 for the error estimate
+    __shared__ __align__(8) short s_wh[4][4][CH], s_wl[4][4][CH], s_w2[4][4][CH];
+    __shared__ int s_off[4][65], s_j[4][64];
+    __shared__ float s_z[4][16][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = 1 + blockIdx.x * 4 + wv;
+    if (i >= n_nodes) return;  // per wavefront: only wavefront-level LDS traffic below
+    const int r = h.rank[i];
+    if (r <= 0) return;
+    const int G = r <= 4 ? 1 : r <= 8 ? 2 : r <= 16 ? 4 : r <= 32 ? 8 : 16;
+    const int P = 16 / G;
+    const int b = lane >> 2, u = lane & 3, g = b % G, ph = b / G;
+    const float4 ci = h.cent[i];
+    const int tr = 4 * g + u;
+    const float4 xk = (tr < r) ? h.skelf[(size_t)i * RMAX + tr] : ci;
+    const VT<D> *ys = (const VT<D> *)h.Ys;
+    v4f acc = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
+    const int a0 = h.adm_beg[i], a1 = h.adm_end[i], x0 = h.x_beg[i], x1 = h.x_end[i];
+    const int na = a1 - a0, nm = na + (x1 - x0);
+    for (int m0 = 0; m0 < nm; m0 += 64) {
+        // this round's members (one per lane) and their source counts, scanned
+        const int m = m0 + lane;
+        int sz = 0, j = -1;
+        if (m < nm) {
+            j = (m < na) ? (int)(uint32_t)h.adm[a0 + m] : (int)(uint32_t)h.xl[x0 + m - na];
+            sz = (m < na) ? h.rank[j] : (h.ne[j] - h.nb[j]);
+        }
+        int x = sz;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        const int T = __shfl(x, 63, 64);
+        const int nmem = min(64, nm - m0);
+        s_off[wv][lane + 1] = x;
+        if (lane == 0) s_off[wv][0] = 0;
+        s_j[wv][lane] = j;
+        __builtin_amdgcn_wave_barrier();
+        for (int c0 = 0; c0 < T; c0 += CH) {
+            // stage sources c0 .. c0 + CH - 1 (zero rows past the end)
+            const int sidx = c0 + lane;
+            float4 xs = ci, ws = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (sidx < T) {
+                int lo = 0, hi = nmem;  // member a with off[a] <= sidx < off[a+1]
+                while (hi - lo > 1) {
+                    const int mid = (lo + hi) >> 1;
+                    if (s_off[wv][mid] <= sidx) lo = mid; else hi = mid;
+                }
+                const int jj = s_j[wv][lo], q = sidx - s_off[wv][lo];
+                if (m0 + lo < na) {
+                    const float4 cj = h.cent[jj];
+                    xs = h.skelf[(size_t)jj * RMAX + q];
+                    const float4 w = h.wc[(size_t)jj * RMAX + q];
+                    ws = make_float4(w.x, (ci.x - cj.x) * w.x - w.y, (ci.y - cj.y) * w.x - w.z,
+                                     (ci.z - cj.z) * w.x - w.w);
+                } else {
+                    const VT<D> y = ys[h.nb[jj] + q];
+                    xs.x = y.x;
+                    xs.y = y.y;
+                    xs.z = 0.f;
+                    if constexpr (D == 3) xs.z = y.z;
+                    ws = make_float4(1.f, ci.x - y.x, ci.y - y.y, (D == 3) ? ci.z - xs.z : 0.f);
+                }
+            }
+            s_x[wv][lane] = xs;
+            {
+                const v4f wv4 = {ws.x, ws.y, ws.z, ws.w};
+                const v4s hb = bf16x4(wv4);
+                const v4s lb = bf16x4(wv4 - f32x4(hb));
+                const v4s qb = bf16x4(wv4 * wv4);
+#pragma unroll
+                for (int c = 0; c < 4; c++) {
+                    s_wh[wv][c][lane] = hb[c];
+                    s_wl[wv][c][lane] = lb[c];
+                    s_w2[wv][c][lane] = qb[c];
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            const int nsteps = (min(CH, T - c0) + 4 * P - 1) / (4 * P);
+            for (int t = 0; t < nsteps; t++) {
+                const int sl = t * 4 * P + 4 * ph;  // sources sl .. sl+3 (< CH: 4P divides CH)
+                v4f Kf;
+#pragma unroll
+                for (int k = 0; k < 4; k++) {
+                    const float4 q = s_x[wv][sl + k];
+                    Kf[k] = kern32(xk.x - q.x, xk.y - q.y, D == 3 ? xk.z - q.z : 0.f);
+                }
+                const v4s A = bf16x4(Kf), A2 = bf16x4(Kf * Kf);
+                const v4s Bh = *reinterpret_cast<const v4s *>(&s_wh[wv][u][sl]);
+                const v4s Bl = *reinterpret_cast<const v4s *>(&s_wl[wv][u][sl]);
+                const v4s B2 = *reinterpret_cast<const v4s *>(&s_w2[wv][u][sl]);
+                acc = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(A, Bh, acc, 0, 0, 0);
+                acc = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(A, Bl, acc, 0, 0, 0);
+                acc2 = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(A2, B2, acc2, 0, 0, 0);
+            }
+            __builtin_amdgcn_wave_barrier();  // the next round overwrites the staging
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+    // sum the phases: blocks with the same row group are 4G lanes apart
+    for (int o = 4 * G; o < 64; o <<= 1)
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            acc[q] += __shfl_xor(acc[q], o, 64);
+            acc2[q] += __shfl_xor(acc2[q], o, 64);
+        }
+    // lane (block b, column u) now holds z[4g + q][u], q = 0..3
+    bool bad = false;
+#pragma unroll
+    for (int q = 0; q < 4; q++)
+        if (4 * g + q < r && 0.001953125f * sqrtf(fmaxf(acc2[q], 0.f)) > tau * fabsf(acc[q]))
+            bad = true;
+    if (__ballot(bad) != 0) {
+        if (lane == 0) {
+            h.nflag[i] = 1;
+            atomicAdd(&h.cnt[CNT_FALLBACK], 1);
+        }
+        return;
+    }
+    if (ph == 0)
+#pragma unroll
+        for (int q = 0; q < 4; q++) s_z[wv][b][q * 4 + u] = acc[q];
+    __builtin_amdgcn_wave_barrier();
+    if (lane < r) {
+        const float *zr = &s_z[wv][lane >> 2][(lane & 3) * 4];
+        h.zc[(size_t)i * RMAX + lane] = make_float4(zr[0], zr[1], zr[2], zr[3]);
+    }
 }
 
 // Downward sweep for the nodes of one level (>= 2): child c takes its rows of the
@@ -280,15 +454,23 @@ leaf_kernel(H2Dev h, const int *__restrict__ leaves, float *__restrict__ s_out,
     }
 }
 
-int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, hipStream_t st)
+int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, float tau, hipStream_t st)
 {
-    (void)mixed;
     const int g = (n_nodes - 1 + 3) / 4;
     if (g <= 0) return 0;
+    if (mixed) {
+        if (hipMemsetAsync(h.nflag, 0, sizeof(int) * (size_t)n_nodes, st) != hipSuccess ||
+            hipMemsetAsync(h.cnt + CNT_FALLBACK, 0, sizeof(int), st) != hipSuccess)
+            return -1;
+        if (D == 2)
+            hipLaunchKernelGGL(couple_mfma_kernel<2>, dim3(g), dim3(256), 0, st, h, n_nodes, tau);
+        else
+            hipLaunchKernelGGL(couple_mfma_kernel<3>, dim3(g), dim3(256), 0, st, h, n_nodes, tau);
+    }
     if (D == 2)
-        hipLaunchKernelGGL(couple_kernel<2>, dim3(g), dim3(256), 0, st, h, n_nodes);
+        hipLaunchKernelGGL(couple_kernel<2>, dim3(g), dim3(256), 0, st, h, n_nodes, mixed);
     else
-        hipLaunchKernelGGL(couple_kernel<3>, dim3(g), dim3(256), 0, st, h, n_nodes);
+        hipLaunchKernelGGL(couple_kernel<3>, dim3(g), dim3(256), 0, st, h, n_nodes, mixed);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

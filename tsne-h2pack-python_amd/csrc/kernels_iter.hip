@@ -96,7 +96,11 @@ __device__ __forceinline__ double row_epilogue(float ax, float ay, float az, int
 // chunks per lane per batch) with non-temporal loads, so the 8 bytes per nonzero of the
 // stream take a quarter of the load instructions and do not displace the gather records
 // from the caches; elements of a chunk outside the row are masked.
-template <int D, int LANES, int BATCH, bool VEC>
+// ATTR_ONLY: the attractive sums only, written to A (float [n][D], unscaled) for
+// update_kernel; Y64, uY, gains, Yg_next and block_sums are left alone.  This form runs
+// on a second stream next to an H2 rebuild (the rebuild and the attractive force of an
+// iteration both read only the iteration's embedding).
+template <int D, int LANES, int BATCH, bool VEC, bool ATTR_ONLY = false>
 __global__ void __launch_bounds__(256)
 attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
                    const float *__restrict__ pval, const typename Vec<D>::T *__restrict__ Yg,
@@ -104,7 +108,7 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
                    double *__restrict__ uY, double *__restrict__ gains,
                    typename Vec<D>::T *__restrict__ Yg_next, const double *__restrict__ scal,
                    double exag4, double mom, double lr, int rows_per_block,
-                   double *__restrict__ block_sums)
+                   double *__restrict__ block_sums, float *__restrict__ A = nullptr)
 {
     using V = typename Vec<D>::T;
     __shared__ double gsum[(256 / LANES)][3];
@@ -117,8 +121,8 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const int row0 = lblk * rows_per_block;
     const int row1 = min(n, row0 + rows_per_block);
-    const double rep_scale = 4.0 / scal[SLOT_Z];
-    const double m_d = (sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
+    const double rep_scale = ATTR_ONLY ? 0.0 : 4.0 / scal[SLOT_Z];
+    const double m_d = (!ATTR_ONLY && sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
     double my_sum = 0.0;  // lane d of the group: sum of new coordinate d over its rows
 
     for (int i = row0 + grp; i < row1; i += (256 / LANES)) {
@@ -126,7 +130,7 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
         // update operands of this row, fetched now and used after the reduction
         const size_t kd = (size_t)i * D + (sub < D ? sub : 0);
         double f_d = 0.0, u_d = 0.0, g_d = 0.0, y_d = 0.0;
-        if (sub < D) {
+        if (!ATTR_ONLY && sub < D) {
             f_d = (double)F[kd];
             u_d = uY[kd];
             g_d = gains[kd];
@@ -203,9 +207,19 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
                 if constexpr (D == 3) az = fmaf(q, dz, az);
             }
         }
-        my_sum += row_epilogue<D, LANES>(ax, ay, az, sub, kd, f_d, u_d, g_d, y_d, m_d, exag4,
-                                         rep_scale, mom, lr, Y64, uY, gains, Yg_next + i);
+        if constexpr (ATTR_ONLY) {
+            for (int off = LANES / 2; off > 0; off >>= 1) {
+                ax += __shfl_xor(ax, off, 64);
+                ay += __shfl_xor(ay, off, 64);
+                if constexpr (D == 3) az += __shfl_xor(az, off, 64);
+            }
+            if (sub < D) A[kd] = (sub == 0) ? ax : (sub == 1) ? ay : az;
+        } else {
+            my_sum += row_epilogue<D, LANES>(ax, ay, az, sub, kd, f_d, u_d, g_d, y_d, m_d, exag4,
+                                             rep_scale, mom, lr, Y64, uY, gains, Yg_next + i);
+        }
     }
+    if constexpr (ATTR_ONLY) return;
     // per-group sums (lane d holds coordinate d), then groups in order
     if (sub < D) gsum[grp][sub] = my_sum;
     __syncthreads();
@@ -348,6 +362,55 @@ attr_tile_kernel(int n, int ntiles, int tile_rows, const int *__restrict__ row_p
         for (int w = 0; w < (NT / LANES); w++) t += gsum[w][threadIdx.x];
         block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
     }
+}
+
+// Update from a separately computed attractive sum A (the ATTR_ONLY kernel): the same
+// gains/momentum rule, record and centring sums as the fused kernel, one thread per row,
+// blocks over the same row ranges as the fused kernel.
+template <int D>
+__global__ void __launch_bounds__(256)
+update_kernel(int n, const float *__restrict__ A, const float *__restrict__ F,
+              double *__restrict__ Y64, double *__restrict__ uY, double *__restrict__ gains,
+              typename Vec<D>::T *__restrict__ Yg_next, const double *__restrict__ scal,
+              double exag4, double mom, double lr, int rows_per_block,
+              double *__restrict__ block_sums)
+{
+    __shared__ double red[3][256];
+    const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const int row0 = lblk * rows_per_block;
+    const int row1 = min(n, row0 + rows_per_block);
+    const double rep_scale = 4.0 / scal[SLOT_Z];
+    double m[3] = {0.0, 0.0, 0.0}, sum[3] = {0.0, 0.0, 0.0};
+    for (int d = 0; d < D; d++) m[d] = scal[SLOT_MEAN + d];
+    for (int i = row0 + threadIdx.x; i < row1; i += 256) {
+        double yn[3] = {0.0, 0.0, 0.0};
+        for (int d = 0; d < D; d++) {
+            const size_t kd = (size_t)i * D + d;
+            const double g = exag4 * (double)A[kd] - rep_scale * (double)F[kd];
+            double u = uY[kd], gn = gains[kd];
+            gn = (u * g < 0.0) ? gn + 0.2 : gn * 0.8;
+            if (gn < 0.01) gn = 0.01;
+            u = mom * u - lr * (gn * g);
+            yn[d] = (Y64[kd] - m[d]) + u;
+            Y64[kd] = yn[d];
+            uY[kd] = u;
+            gains[kd] = gn;
+            sum[d] += yn[d];
+        }
+        typename Vec<D>::T rec;
+        rec.x = (float)yn[0];
+        rec.y = (float)yn[1];
+        if constexpr (D == 3) { rec.z = (float)yn[2]; rec.w = 0.f; }
+        Yg_next[i] = rec;
+    }
+    for (int d = 0; d < 3; d++) red[d][threadIdx.x] = sum[d];
+    __syncthreads();
+    for (int off = 128; off > 0; off >>= 1) {
+        if (threadIdx.x < off)
+            for (int d = 0; d < D; d++) red[d][threadIdx.x] += red[d][threadIdx.x + off];
+        __syncthreads();
+    }
+    if (threadIdx.x < D) block_sums[(size_t)lblk * 3 + threadIdx.x] = red[threadIdx.x][0];
 }
 
 // Sum of the per-block sums in block order -> new pending mean (SLOT_MEAN..+D-1).
@@ -803,6 +866,37 @@ int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, con
     }
 #undef TSNE_ATTR_ARGS
     if (ev_end) (void)hipEventRecord(ev_end, st);
+    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, nblocks, D, n,
+                       block_sums, (double *)scal);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_attr_only(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
+                     const void *Yg, float *A, int nblocks, hipStream_t st)
+{
+    const int rpb = (n + nblocks - 1) / nblocks;
+    if (D == 2)
+        hipLaunchKernelGGL((attr_update_kernel<2, 16, 3, true, true>), dim3(nblocks), dim3(256), 0, st,
+                           n, row_ptr, col_idx, pval, (const float2 *)Yg, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, rpb, nullptr, A);
+    else
+        hipLaunchKernelGGL((attr_update_kernel<3, 16, 3, true, true>), dim3(nblocks), dim3(256), 0, st,
+                           n, row_ptr, col_idx, pval, (const float4 *)Yg, nullptr, nullptr, nullptr,
+                           nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, rpb, nullptr, A);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_update(int D, int n, const float *A, const float *F, double *Y64, double *uY,
+                  double *gains, void *Yg_next, const double *scal, double exag, double mom,
+                  double lr, double *block_sums, int nblocks, hipStream_t st)
+{
+    const int rpb = (n + nblocks - 1) / nblocks;
+    if (D == 2)
+        hipLaunchKernelGGL(update_kernel<2>, dim3(nblocks), dim3(256), 0, st, n, A, F, Y64, uY,
+                           gains, (float2 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
+    else
+        hipLaunchKernelGGL(update_kernel<3>, dim3(nblocks), dim3(256), 0, st, n, A, F, Y64, uY,
+                           gains, (float4 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
     hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, nblocks, D, n,
                        block_sums, (double *)scal);
     return hipGetLastError() == hipSuccess ? 0 : -1;

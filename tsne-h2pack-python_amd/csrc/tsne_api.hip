@@ -26,6 +26,12 @@ struct tsne_workspace {
     double lr = 200.0;
     sparse_matrix_t *P = nullptr;
     hipStream_t st = nullptr;
+    // second stream: the attractive pass of a rebuild iteration runs on it next to the
+    // rebuild and the product (same default priority as st)
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_y = nullptr, ev_a = nullptr;
+    float *d_A = nullptr;       // attractive sums of a split iteration, n x D
+    int overlap = 1;
     int nnz = 0;
     int *d_rowptr = nullptr, *d_col = nullptr;
     float *d_val32 = nullptr;
@@ -147,8 +153,11 @@ void free_device(tsne_workspace *ws)
     dfree(ws->d_rowptr, ws->st); dfree(ws->d_col, ws->st); dfree(ws->d_val32, ws->st); dfree(ws->d_val64, ws->st);
     dfree(ws->d_Y64, ws->st); dfree(ws->d_uY, ws->st); dfree(ws->d_gains, ws->st);
     dfree(ws->d_Yg[0], ws->st); dfree(ws->d_Yg[1], ws->st);
-    dfree(ws->d_s, ws->st); dfree(ws->d_F, ws->st); dfree(ws->d_scal, ws->st);
+    dfree(ws->d_s, ws->st); dfree(ws->d_F, ws->st); dfree(ws->d_scal, ws->st); dfree(ws->d_A, ws->st);
     dfree(ws->d_block_sums, ws->st); dfree(ws->d_partial, ws->st); dfree(ws->d_rowkl, ws->st);
+    if (ws->st2) { (void)hipStreamSynchronize(ws->st2); (void)hipStreamDestroy(ws->st2); ws->st2 = nullptr; }
+    if (ws->ev_y) { (void)hipEventDestroy(ws->ev_y); ws->ev_y = nullptr; }
+    if (ws->ev_a) { (void)hipEventDestroy(ws->ev_a); ws->ev_a = nullptr; }
     if (ws->st) { (void)hipStreamDestroy(ws->st); ws->st = nullptr; }
 }
 
@@ -498,7 +507,16 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         ws->tile_nt = (t && atoi(t) == 1024) ? 1024 : 256;
     }
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
+    {
+        // TSNE_OVERLAP=0: no second stream, every attractive pass fused with its update
+        const char *e = getenv("TSNE_OVERLAP");
+        ws->overlap = !(e && e[0] == '0');
+    }
     bool ok = hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
+    if (ok && ws->overlap)
+        ok = hipStreamCreateWithFlags(&ws->st2, hipStreamNonBlocking) == hipSuccess &&
+             hipEventCreateWithFlags(&ws->ev_y, hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&ws->ev_a, hipEventDisableTiming) == hipSuccess;
     if (!ok) {
         fprintf(stderr, "tsne_init_with_P: no HIP device available\n");
         delete ws;
@@ -513,7 +531,8 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
          dalloc(&ws->d_s, n, ws->st) == 0 && dalloc(&ws->d_F, (size_t)n * D, ws->st) == 0 &&
          dalloc(&ws->d_scal, SLOT_COUNT, ws->st) == 0 &&
          dalloc(&ws->d_block_sums, (size_t)ws->nblocks_upd * 3, ws->st) == 0 &&
-         dalloc(&ws->d_partial, ws->nblocks_red, ws->st) == 0 && dalloc(&ws->d_rowkl, n, ws->st) == 0;
+         dalloc(&ws->d_partial, ws->nblocks_red, ws->st) == 0 && dalloc(&ws->d_rowkl, n, ws->st) == 0 &&
+         (!ws->overlap || dalloc(&ws->d_A, (size_t)n * D, ws->st) == 0);
     if (ok) {
         std::vector<float> v32(nnz > 0 ? nnz : 1);
         for (int e = 0; e < nnz; e++) v32[e] = (float)P->values[e];
@@ -605,9 +624,11 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
     // synchronisation, so the run itself is not slowed by host round trips
     std::vector<hipEvent_t> kev;
     std::vector<char> ktiled;  // which timed launches ran the tiled kernel
+    std::vector<char> kused;   // which event pairs were recorded (fused launches only)
     if (ws->timing == 2 && it_end > it_begin) {
         kev.resize(2 * (size_t)(it_end - it_begin), nullptr);
         ktiled.assign((size_t)(it_end - it_begin), 0);
+        kused.assign((size_t)(it_end - it_begin), 0);
         for (auto &e : kev)
             if (hipEventCreate(&e) != hipSuccess) e = nullptr;
     }
@@ -639,6 +660,9 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
             lr = ws->lr * std::pow(0.995, it - pp.momentum_switch_iter);
         if (timing) (void)hipEventRecord(ev[0], ws->st);
         int rebuilt = (it == 0);
+        // split: this iteration's attractive sums go to the second stream while the
+        // rebuild and the product run here; the update waits for both
+        bool split = false;
         if (!direct && rebuild_iter(ws, it, reuse)) {
             const int nro = (int)ws->reorder_iters.size();
             if (ws->next_reorder < nro && it >= ws->reorder_iters[ws->next_reorder] && ws->h2 &&
@@ -647,6 +671,15 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
                     ws->next_reorder++;
                 if (renumber(ws) < 0) return -1;
                 if (ws->next_reorder >= nro) free_renumber_buffers(ws);  // last one
+            }
+            if (ws->st2 && ws->tiles.ntiles == 0) {
+                split = true;
+                TSNE_HIP_CHECK(hipEventRecord(ws->ev_y, ws->st));
+                TSNE_HIP_CHECK(hipStreamWaitEvent(ws->st2, ws->ev_y, 0));
+                if (launch_attr_only(D, ws->n, ws->d_rowptr, ws->d_col, ws->d_val32,
+                                     ws->d_Yg[ws->cur], ws->d_A, ws->nblocks_upd, ws->st2) != 0)
+                    return -1;
+                TSNE_HIP_CHECK(hipEventRecord(ws->ev_a, ws->st2));
             }
             if (rebuild(ws) != 0) {
                 fprintf(stderr, "ERROR: Failed to rebuild H2 matrix at iteration %d\n", it);
@@ -671,8 +704,12 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         const int nxt = ws->cur ^ 1;
         hipEvent_t kb = kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin)];
         hipEvent_t ke = kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin) + 1];
+        if (split) TSNE_HIP_CHECK(hipStreamWaitEvent(ws->st, ws->ev_a, 0));
         const int arc =
-            ws->tiles.ntiles > 0
+            split ? launch_update(D, ws->n, ws->d_A, ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains,
+                                  ws->d_Yg[nxt], ws->d_scal, exag, mom, lr, ws->d_block_sums,
+                                  ws->nblocks_upd, ws->st)
+            : ws->tiles.ntiles > 0
                 ? launch_attr_tile(D, ws->n, ws->tiles, ws->tile_nt, ws->d_rowptr, ws->d_col, ws->d_val32,
                                    ws->d_Yg[ws->cur], ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains,
                                    ws->d_Yg[nxt], ws->d_scal, exag, mom, lr, ws->d_block_sums,
@@ -682,7 +719,10 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
                                      ws->d_Yg[nxt], ws->d_scal, exag, mom, lr, ws->d_block_sums,
                                      ws->nblocks_upd, ws->st, kb, ke);
         if (arc != 0) return -1;
-        if (!ktiled.empty()) ktiled[(size_t)(it - it_begin)] = ws->tiles.ntiles > 0;
+        if (!ktiled.empty()) {
+            ktiled[(size_t)(it - it_begin)] = ws->tiles.ntiles > 0;
+            kused[(size_t)(it - it_begin)] = !split;
+        }
         ws->cur = nxt;
         if (timing) {
             (void)hipEventRecord(ev[4], ws->st);
@@ -717,7 +757,8 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
     }
     for (size_t k = 0; k + 1 < kev.size(); k += 2) {
         float ms = 0.f;
-        if (kev[k] && kev[k + 1] && hipEventElapsedTime(&ms, kev[k], kev[k + 1]) == hipSuccess) {
+        if (kused[k / 2] && kev[k] && kev[k + 1] &&
+            hipEventElapsedTime(&ms, kev[k], kev[k + 1]) == hipSuccess) {
             ws->t_attr_kernel += ms;
             ws->n_attr_kernel++;
             if (ktiled[k / 2]) {
