@@ -3,6 +3,7 @@
 // pair lists, per-level ID + upward pass) and the product (coupling, downward sweep,
 // leaves).  See h2_internal.h for the layout.
 #include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -43,6 +44,12 @@ struct H2 {
 };
 
 namespace {
+
+// Radix sorts of the Morton keys and pair lists: always the onesweep algorithm (the
+// library default sorts up to 2^20 items with a block sort plus merge passes, which at
+// these sizes costs several times more than the 6-8 onesweep digit passes).
+using SortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
+                                           rocprim::default_config, 0>;
 
 // Stream-ordered device memory on the structure's stream (see the workspace allocator).
 template <typename T> bool dmalloc(T **p, size_t n, hipStream_t st)
@@ -124,12 +131,12 @@ bool alloc_sized(H2 *h)
     if (!ok) return false;
     // temporary storage for the library sorts / scans (largest of the three uses)
     size_t b1 = 0, b2 = 0, b3 = 0;
-    hipcub::DoubleBuffer<uint64_t> kb(d.keys, h->keys_alt);
-    hipcub::DoubleBuffer<int> vb(d.perm, h->perm_alt);
-    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, b1, kb, vb, h->n, 0, 64, h->st);
+    rocprim::double_buffer<uint64_t> kb(d.keys, h->keys_alt);
+    rocprim::double_buffer<int> vb(d.perm, h->perm_alt);
+    (void)rocprim::radix_sort_pairs<SortCfg>(nullptr, b1, kb, vb, h->n, 0, 64, h->st);
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b2, d.childcnt, d.childoff, (int)nc, h->st);
-    hipcub::DoubleBuffer<uint64_t> pb(d.adm, d.sort_tmp);
-    (void)hipcub::DeviceRadixSort::SortKeys(nullptr, b3, pb, maxpair, 0, 64, h->st);
+    rocprim::double_buffer<uint64_t> pb(d.adm, d.sort_tmp);
+    (void)rocprim::radix_sort_keys<SortCfg>(nullptr, b3, pb, maxpair, 0, 64, h->st);
     h->cub_bytes = std::max({b1, b2, b3});
     return hipMallocAsync(&h->cub_tmp, h->cub_bytes, h->st) == hipSuccess;
 }
@@ -143,11 +150,11 @@ int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end)
     if (n <= 0) return 0;
     int bits = 32;
     while ((1LL << (bits - 32)) < d.node_cap) bits++;
-    hipcub::DoubleBuffer<uint64_t> db(buf, d.sort_tmp);
+    rocprim::double_buffer<uint64_t> db(buf, d.sort_tmp);
     size_t bytes = h->cub_bytes;
-    if (hipcub::DeviceRadixSort::SortKeys(h->cub_tmp, bytes, db, n, 0, bits, h->st) != hipSuccess)
+    if (rocprim::radix_sort_keys<SortCfg>(h->cub_tmp, bytes, db, n, 0, bits, h->st) != hipSuccess)
         return -1;
-    if (db.Current() != buf) {  // the sorted list now lives in the scratch buffer:
+    if (db.current() != buf) {  // the sorted list now lives in the scratch buffer:
         std::swap(buf, d.sort_tmp);  // exchange them (both hold maxpair entries)
     }
     return launch_pair_offsets(buf, n, beg, end, h->st);
@@ -166,15 +173,15 @@ int build_once(H2 *h, const void *Yg)
     if (launch_root_box(D, Yg, h->n, d, h->part, h->nblk, st)) return -1;
     if (launch_morton(D, Yg, d, st)) return -1;
     {
-        hipcub::DoubleBuffer<uint64_t> kb(d.keys, h->keys_alt);
-        hipcub::DoubleBuffer<int> vb(d.perm, h->perm_alt);
+        rocprim::double_buffer<uint64_t> kb(d.keys, h->keys_alt);
+        rocprim::double_buffer<int> vb(d.perm, h->perm_alt);
         size_t bytes = h->cub_bytes;
         const int endbit = D * (D == 2 ? H2Cfg<2>::B : H2Cfg<3>::B);
-        if (hipcub::DeviceRadixSort::SortPairs(h->cub_tmp, bytes, kb, vb, h->n, 0, endbit, st) !=
+        if (rocprim::radix_sort_pairs<SortCfg>(h->cub_tmp, bytes, kb, vb, h->n, 0, endbit, st) !=
             hipSuccess)
             return -1;
-        if (kb.Current() != d.keys) std::swap(d.keys, h->keys_alt);
-        if (vb.Current() != d.perm) std::swap(d.perm, h->perm_alt);
+        if (kb.current() != d.keys) std::swap(d.keys, h->keys_alt);
+        if (vb.current() != d.perm) std::swap(d.perm, h->perm_alt);
     }
     if (launch_gather_sorted(D, Yg, d, st)) return -1;
     if (launch_init_root(d, st)) return -1;
@@ -203,7 +210,8 @@ int build_once(H2 *h, const void *Yg)
         }
         if (launch_split_write(D, d, lb, le, l, st)) return -1;
         const int pend_out = pend_in ^ 1;
-        if (hipMemsetAsync(d.cnt + CNT_PEND0 + pend_out, 0, sizeof(int), st) != hipSuccess) return -1;
+        if (hipMemsetAsync(d.cnt + CNT_PEND0 + pend_out * kCntSlot, 0, sizeof(int), st) != hipSuccess)
+            return -1;
         if (launch_lists(D, d, le, le + T, l + 1, pend_out, st)) return -1;
         if (launch_pending(D, d, n_pend_in, pend_in, pend_out, st)) return -1;
         lb = le;
@@ -227,9 +235,9 @@ int build_once(H2 *h, const void *Yg)
     h->n_w = h->pin[CNT_W];
     h->n_near = h->pin[CNT_NEAR];
     h->n_leaves = h->pin[CNT_LEAF];
-    if (sort_pairs_list(h, d.adm, h->n_adm, d.adm_beg, d.adm_end) ||
-        sort_pairs_list(h, d.xl, h->n_x, d.x_beg, d.x_end) ||
-        sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end) ||
+    // ADM and X ranges were written per node by the list kernel; NEAR and W pairs are
+    // appended by several levels and are grouped by target here
+    if (sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end) ||
         sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end))
         return -1;
     // ID + upward: every leaf at once (they need nothing from other nodes), then the
@@ -278,7 +286,7 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
               dmalloc(&h->perm_alt, n, h->st) && dmalloc((char **)&d.Ys, (size_t)n * vb, h->st) &&
               dmalloc(&d.root, 1, h->st) && dmalloc(&d.flags, FLAG_COUNT, h->st) && dmalloc(&d.cnt, CNT_COUNT, h->st) &&
               dmalloc(&h->part, (size_t)h->nblk * 7, h->st) &&
-              hipHostMalloc((void **)&h->pin, sizeof(int) * 64,
+              hipHostMalloc((void **)&h->pin, sizeof(int) * CNT_COUNT,
                             hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
               hipHostGetDevicePointer((void **)&h->pin_dev, h->pin, 0) == hipSuccess;
     if (ok) {
@@ -367,23 +375,23 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     st->n_adm_pairs = h->n_adm + h->n_x + h->n_w;
     st->n_near_pairs = h->n_near;
     st->near_point_pairs = -1;
-    st->rank_max = h->hflags[FLAG_RANK_MAX];
-    st->rank_mean = h->n_nodes > 1 ? (double)h->hflags[FLAG_RANK_SUM] / (h->n_nodes - 1) : 0.0;
-    st->proxy_max = h->hflags[FLAG_PROXY_MAX];
+    st->rank_max = 0;
+    st->rank_mean = 0.0;
+    st->proxy_max = 0;
     st->mixed_fallback_nodes = 0;
     if (!h->usable || h->n_nodes <= 0) return 0;
-    if (hipMemcpyAsync(&st->mixed_fallback_nodes, h->d.cnt + CNT_FALLBACK, sizeof(int),
-                       hipMemcpyDeviceToHost, h->st) != hipSuccess)
-        return -1;
     // flop counts of one product from the built structure (host copies; diagnostics only)
     const int nn = h->n_nodes;
-    std::vector<int> nb(nn), ne(nn), rk(nn), c0(nn), par(nn);
+    std::vector<int> nb(nn), ne(nn), rk(nn), c0(nn), par(nn), fl(nn);
+    RootInfo root{};
     std::vector<uint64_t> adm(h->n_adm), xl(h->n_x), wl(h->n_w), nl(h->n_near);
     bool ok = hipMemcpyAsync(nb.data(), h->d.nb, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
               hipMemcpyAsync(ne.data(), h->d.ne, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
               hipMemcpyAsync(rk.data(), h->d.rank, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
               hipMemcpyAsync(c0.data(), h->d.child0, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
-              hipMemcpyAsync(par.data(), h->d.parent, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess;
+              hipMemcpyAsync(par.data(), h->d.parent, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+              hipMemcpyAsync(fl.data(), h->d.nflag, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+              hipMemcpyAsync(&root, h->d.root, sizeof(RootInfo), hipMemcpyDeviceToHost, h->st) == hipSuccess;
     auto cp = [&](std::vector<uint64_t> &v, const uint64_t *src) {
         return v.empty() || hipMemcpyAsync(v.data(), src, v.size() * sizeof(uint64_t),
                                            hipMemcpyDeviceToHost, h->st) == hipSuccess;
@@ -391,6 +399,22 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     ok = ok && cp(adm, h->d.adm) && cp(xl, h->d.xl) && cp(wl, h->d.wl) && cp(nl, h->d.nl) &&
          hipStreamSynchronize(h->st) == hipSuccess;
     if (!ok) return -1;
+    // ranks of the non-root nodes, proxies of the levels in use, mixed-product fallbacks
+    long long rsum = 0;
+    for (int i = 1; i < nn; i++) {
+        st->rank_max = std::max(st->rank_max, rk[i]);
+        rsum += rk[i];
+        st->mixed_fallback_nodes += fl[i] != 0;
+    }
+    st->rank_mean = nn > 1 ? (double)rsum / (nn - 1) : 0.0;
+    {
+        const ProxyTable &pt = proxy_table(h->D, h->ptol, 1.0, pmax_of(h->D));
+        const int nlev = (int)h->lvl.size() - 1;
+        for (int l = 1; l < nlev; l++) {
+            const int kq = std::min(std::max(root.k0 - l, 0), kProxyLevels - 1);
+            if (kq < (int)pt.cnt.size()) st->proxy_max = std::max(st->proxy_max, pt.cnt[kq]);
+        }
+    }
     const double D = h->D;
     auto tgt = [](uint64_t k) { return (int)(k >> 32); };
     auto src = [](uint64_t k) { return (int)(uint32_t)k; };

@@ -112,7 +112,14 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     // U of the node: a leaf's rows are its points, so its block sits at its first sorted
     // point (m x RMAX floats); an internal node reserves m x RMAX floats after the n x RMAX
     // of the leaves (the reservation is issued now, its latency hidden by the ID)
-    if (t == 0) s_base = leafn ? nb * RMAX : h.n * RMAX + atomicAdd(&h.cnt[CNT_POOL], m * RMAX);
+    // (region blockIdx mod kPoolParts of the internal pool, so that the reservations of
+    // one launch spread over kPoolParts counters)
+    const int part = blockIdx.x % kPoolParts;
+    const long long part_cap = (h.pool_cap - (long long)h.n * RMAX) / kPoolParts;
+    if (t == 0)
+        s_base = leafn ? nb * RMAX
+                       : (int)((long long)h.n * RMAX + part * part_cap +
+                               atomicAdd(&h.cnt[CNT_POOL + part * kCntSlot], m * RMAX));
     const double cd[3] = {(double)c.x, (double)c.y, (double)c.z};
 
     // ---- candidates and the rows of X
@@ -149,7 +156,8 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             // single child: same skeleton, U = I, coefficients moved to this centre
             __syncthreads();
             const int base = s_base;
-            const bool fits = (long long)base + (long long)m * RMAX <= h.pool_cap;
+            const bool fits = (long long)base + (long long)m * RMAX <=
+                              (long long)h.n * RMAX + (part + 1) * part_cap;
             if (t == 0) {
                 if (!fits) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
                 h.rank[node] = m;
@@ -369,12 +377,10 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     // ---- store U (pool), skeleton, and the upward coefficients
     if (t == 0) {
         const int base = s_base;
-        if ((long long)base + (long long)m * RMAX > h.pool_cap) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
+        if (!leafn && (long long)base + (long long)m * RMAX > (long long)h.n * RMAX + (part + 1) * part_cap)
+            atomicOr(&h.flags[FLAG_POOL_OVF], 1);
         h.rank[node] = r;
         h.uoff[node] = base;
-        atomicMax(&h.flags[FLAG_RANK_MAX], r);
-        atomicAdd(&h.flags[FLAG_RANK_SUM], r);
-        atomicMax(&h.flags[FLAG_PROXY_MAX], p);
     }
     if (kk >= 0) {
         for (int d = 0; d < D; d++) h.skel[((size_t)node * RMAX + kk) * D + d] = my[d];
@@ -383,7 +389,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     }
     __syncthreads();
     const int base = s_base;
-    if ((long long)base + (long long)m * RMAX <= h.pool_cap)
+    if (leafn || (long long)base + (long long)m * RMAX <= (long long)h.n * RMAX + (part + 1) * part_cap)
         for (int e = t; e < m * r; e += NT) h.upool[(size_t)base + e] = s_U[e / r][e % r];
     if (t < r) {
         float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -411,20 +417,26 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int m_lo, double tol
 }
 
 // Nodes with at most 64 candidates run in single-wavefront workgroups (their barriers
-// are free), larger ones in 128 threads (2D); 3D uses 384 threads.
+// are free), larger ones in 128 threads (2D); 3D uses 384 threads.  A level of at most
+// kOneClassLevel nodes (the top of the tree, where every launch is one node's latency)
+// runs all its nodes in one 128-thread launch instead of two.
+constexpr int kOneClassLevel = 512;
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
                  int pq, hipStream_t st)
 {
     if (count <= 0) return 0;
     if (D == 2) {
+        const bool one = count <= kOneClassLevel;
         if (pq <= 32) {
-            hipLaunchKernelGGL((id_up_kernel<2, 64, 32>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
+            if (!one)
+                hipLaunchKernelGGL((id_up_kernel<2, 64, 32>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
             hipLaunchKernelGGL((id_up_kernel<2, 128, 32>), dim3(count), dim3(128), 0, st, h, list, lb,
-                               64, tol);
+                               one ? 0 : 64, tol);
         } else {
-            hipLaunchKernelGGL((id_up_kernel<2, 64, 48>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
+            if (!one)
+                hipLaunchKernelGGL((id_up_kernel<2, 64, 48>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
             hipLaunchKernelGGL((id_up_kernel<2, 128, 48>), dim3(count), dim3(128), 0, st, h, list, lb,
-                               64, tol);
+                               one ? 0 : 64, tol);
         }
     } else {
         hipLaunchKernelGGL((id_up_kernel<3, 384, 96>), dim3(count), dim3(384), 0, st, h, list, lb, 0,

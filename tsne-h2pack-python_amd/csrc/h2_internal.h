@@ -51,16 +51,19 @@ enum H2Flag {
     FLAG_BIG_LEAF = 3,   // a leaf at the maximum depth holds more than kLeafCap points
     FLAG_NONFINITE = 4,  // non-finite coordinate
     FLAG_RANK_CAP = 5,   // an ID stopped at RMAX before reaching the tolerance
-    FLAG_RANK_MAX = 6,   // statistics: maximum rank
-    FLAG_RANK_SUM = 7,   // statistics: sum of ranks (low 31 bits)
-    FLAG_PROXY_MAX = 8,  // statistics: maximum proxies used
     FLAG_COUNT = 16
 };
 
-// Counters of the pair buffers (device int array H2Dev::cnt).
-enum H2Count { CNT_ADM = 0, CNT_X, CNT_W, CNT_NEAR, CNT_PEND0, CNT_PEND1, CNT_LEAF, CNT_POOL,
-               CNT_FALLBACK,  // nodes whose BF16 coupling was redone in FP32 (last product)
-               CNT_COUNT = 16 };
+// Counters of the pair buffers and the U pool (device int array H2Dev::cnt).  Device-wide
+// atomics on one cache line serialise across the whole GPU, so every counter has a
+// 128-byte line of its own (kCntSlot ints apart), and the internal-node U pool is split
+// into kPoolParts regions, each with its own counter.
+constexpr int kCntSlot = 32;
+constexpr int kPoolParts = 8;
+enum H2Count { CNT_ADM = 0, CNT_X = kCntSlot, CNT_W = 2 * kCntSlot, CNT_NEAR = 3 * kCntSlot,
+               CNT_PEND0 = 4 * kCntSlot, CNT_PEND1 = 5 * kCntSlot, CNT_LEAF = 6 * kCntSlot,
+               CNT_POOL = 7 * kCntSlot,  // kPoolParts counters, one line each
+               CNT_COUNT = (7 + kPoolParts) * kCntSlot };
 
 struct RootInfo {
     double c[3];   // root box centre

@@ -255,10 +255,7 @@ couple_mfma_kernel(H2Dev h, int n_nodes, float tau)
         if (4 * g + q < r && 0.001953125f * sqrtf(fmaxf(acc2[q], 0.f)) > tau * fabsf(acc[q]))
             bad = true;
     if (__ballot(bad) != 0) {
-        if (lane == 0) {
-            h.nflag[i] = 1;
-            atomicAdd(&h.cnt[CNT_FALLBACK], 1);
-        }
+        if (lane == 0) h.nflag[i] = 1;
         return;
     }
     if (ph == 0)
@@ -459,8 +456,7 @@ int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, float tau, hipS
     const int g = (n_nodes - 1 + 3) / 4;
     if (g <= 0) return 0;
     if (mixed) {
-        if (hipMemsetAsync(h.nflag, 0, sizeof(int) * (size_t)n_nodes, st) != hipSuccess ||
-            hipMemsetAsync(h.cnt + CNT_FALLBACK, 0, sizeof(int), st) != hipSuccess)
+        if (hipMemsetAsync(h.nflag, 0, sizeof(int) * (size_t)n_nodes, st) != hipSuccess)
             return -1;
         if (D == 2)
             hipLaunchKernelGGL(couple_mfma_kernel<2>, dim3(g), dim3(256), 0, st, h, n_nodes, tau);

@@ -251,6 +251,7 @@ __global__ void init_root_kernel(H2Dev h, int B)
     h.icoord[0] = h.icoord[1] = h.icoord[2] = 0;
     h.near[0] = 0;
     h.near_cnt[0] = 1;
+    h.adm_beg[0] = h.adm_end[0] = h.x_beg[0] = h.x_end[0] = 0;
     if (h.n <= h.leaf || B == 0) {
         h.nl[0] = 0;
         h.cnt[CNT_NEAR] = 1;
@@ -269,38 +270,62 @@ int launch_init_root(const H2Dev &h, hipStream_t st)
 // digit inside the node's sorted key range, then (after a scan of the child counts)
 // the children are written contiguously after the level.  Leaves are recorded.
 // ---------------------------------------------------------------------------------
+// NCH consecutive lanes per node: lane v >= 1 finds the first sorted point whose
+// level-(l+1) digit is >= v (the NCH - 1 binary searches run side by side instead of one
+// after another, which is what a node of the top levels, with up to n points, waits
+// for); lane 0 then collects the bounds and records the node.
 template <int D>
 __global__ void __launch_bounds__(256)
 split_count_kernel(H2Dev h, int lb, int le, int l)
 {
     constexpr int B = H2Cfg<D>::B, NCH = H2Cfg<D>::NCH;
-    const int i = lb + blockIdx.x * 256 + threadIdx.x;
-    if (i >= le) return;
-    const int b = h.nb[i], e = h.ne[i];
-    int *bd = h.bounds + (size_t)(i - lb) * (NCH + 1);
-    if (e - b <= h.leaf || l >= B) {
-        h.childcnt[i - lb] = 0;
-        h.child0[i] = -1;
-        h.nchild[i] = 0;
-        const int k = atomicAdd(&h.cnt[CNT_LEAF], 1);
-        h.leaves[k] = i;
-        if (e - b > kLeafCap) atomicOr(&h.flags[FLAG_BIG_LEAF], 1);
-        return;
+    static_assert((64 % NCH) == 0, "lanes per node");
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    const int i = lb + t / NCH, v = t % NCH;
+    const bool valid = i < le;
+    int b = 0, e = 0;
+    if (valid) {
+        b = h.nb[i];
+        e = h.ne[i];
     }
-    const int shift = D * (B - l - 1);
-    bd[0] = b;
-    bd[NCH] = e;
-    for (int v = 1; v < NCH; v++) {
+    const bool leaf = valid && (e - b <= h.leaf || l >= B);
+    int bound = b;
+    if (valid && !leaf && v > 0) {
+        const int shift = D * (B - l - 1);
         int lo = b, hi = e;  // first index with digit >= v
         while (lo < hi) {
             const int mid = (lo + hi) >> 1;
             const int dg = (int)((h.keys[mid] >> shift) & (uint64_t)(NCH - 1));
             if (dg < v) lo = mid + 1; else hi = mid;
         }
-        bd[v] = lo;
+        bound = lo;
     }
+    const int g0 = (threadIdx.x & 63) & ~(NCH - 1);  // first lane of this node's group
+    int bd[NCH + 1];
+#pragma unroll
+    for (int q = 0; q < NCH; q++) bd[q] = __shfl(bound, g0 + q, 64);
+    bd[NCH] = e;
+    // the wavefront's new leaves take consecutive slots of the leaf list (one atomic)
+    const bool newleaf = valid && v == 0 && leaf;
+    const unsigned long long ml = __ballot(newleaf);
+    int lbase = 0;
+    if ((threadIdx.x & 63) == 0 && ml) lbase = atomicAdd(&h.cnt[CNT_LEAF], __popcll(ml));
+    lbase = __shfl(lbase, 0, 64);
+    if (!valid || v != 0) return;
+    if (leaf) {
+        h.childcnt[i - lb] = 0;
+        h.child0[i] = -1;
+        h.nchild[i] = 0;
+        h.leaves[lbase + __popcll(ml & ((1ull << (threadIdx.x & 63)) - 1ull))] = i;
+        if (e - b > kLeafCap) atomicOr(&h.flags[FLAG_BIG_LEAF], 1);
+        return;
+    }
+    int *bdp = h.bounds + (size_t)(i - lb) * (NCH + 1);
     int c = 0;
-    for (int v = 0; v < NCH; v++) c += bd[v + 1] > bd[v];
+#pragma unroll
+    for (int q = 0; q <= NCH; q++) bdp[q] = bd[q];
+#pragma unroll
+    for (int q = 0; q < NCH; q++) c += bd[q + 1] > bd[q];
     h.childcnt[i - lb] = c;
 }
 
@@ -341,7 +366,7 @@ split_write_kernel(H2Dev h, int lb, int le, int l)
 
 int launch_split_count(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st)
 {
-    const int g = (le - lb + 255) / 256;
+    const int g = ((le - lb) * (D == 2 ? H2Cfg<2>::NCH : H2Cfg<3>::NCH) + 255) / 256;
     if (D == 2)
         hipLaunchKernelGGL(split_count_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le, l);
     else
@@ -368,13 +393,21 @@ int launch_split_write(int D, const H2Dev &h, int lb, int le, int l, hipStream_t
 // ---------------------------------------------------------------------------------
 // One wavefront per node: lanes enumerate the candidates (children of the parent's
 // near-list members, in member order), classify them, and the near list is compacted
-// in candidate order with ballots; pairs get one atomic per category per wavefront.
+// in candidate order with ballots.  The pair buffers are shared by every node of the
+// level, so their slots come from global counters: the wavefront first classifies all
+// its candidates (the per-round ballot masks are kept in LDS), the workgroup then takes
+// the slots of all its pairs with one atomic per category, and the pairs are written.
+// (Device-wide atomics on a shared counter are serialised across the GPU: one per node
+// and category made this kernel's time grow with the node count.)
 template <int D>
 __global__ void __launch_bounds__(256)
 lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
 {
-    constexpr int B = H2Cfg<D>::B, NEAR_CAP = H2Cfg<D>::NEAR_CAP;
+    constexpr int B = H2Cfg<D>::B, NEAR_CAP = H2Cfg<D>::NEAR_CAP, NCH = H2Cfg<D>::NCH;
+    constexpr int RMAXR = (NEAR_CAP * NCH + 63) / 64;  // candidate rounds at most
     __shared__ int s_off[4][65], s_c0[4][64];
+    __shared__ unsigned long long s_m[4][RMAXR][4];  // ADM, X, NEAR pair, pending masks
+    __shared__ int s_cnt[4][4], s_bas[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = lb + blockIdx.x * 4 + wv;
     const bool valid = i < le;  // uniform over the wavefront
@@ -399,56 +432,107 @@ lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
     if (lane == 0) s_off[wv][0] = 0;
     s_c0[wv][lane] = c0;
     __syncthreads();  // reached by every wavefront of the block
-    if (!valid) return;
-    const bool leaf = (h.ne[i] - h.nb[i]) <= h.leaf || l >= B;
-    int *my = h.near + (size_t)i * NEAR_CAP;
-    int cnt = 0;
-    for (int k0 = 0; k0 < T; k0 += 64) {
-        const int k = k0 + lane;
-        int cand = -1, cat = 0;  // 1 ADM, 2 X, 3 near
+    const bool leaf = valid && ((h.ne[i] - h.nb[i]) <= h.leaf || l >= B);
+    auto candidate = [&](int k) {  // member a with off[a] <= k < off[a+1]
+        int lo = 0, hi = np;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (s_off[wv][mid] <= k) lo = mid; else hi = mid;
+        }
+        return s_c0[wv][lo] + (k - s_off[wv][lo]);
+    };
+    // pass 1: classify (1 ADM, 2 X, 3 near; a leaf's near members become NEAR pairs or
+    // pending pairs), compact the near list, count the pairs of each category
+    int *my = h.near + (size_t)(valid ? i : 0) * NEAR_CAP;
+    int cnt = 0, nA = 0, nX = 0, nN = 0, nP = 0;
+    int rounds = valid ? (T + 63) / 64 : 0;
+    if (rounds > RMAXR) {  // more candidates than NEAR_CAP members can have: overflow
+        if (lane == 0) atomicOr(&h.flags[FLAG_NEAR_OVF], 1);
+        rounds = 0;
+    }
+    for (int r = 0; r < rounds; r++) {
+        const int k = r * 64 + lane;
+        int cand = -1, cat = 0, sub = 0;
         if (k < T) {
-            int lo = 0, hi = np;  // member a with off[a] <= k < off[a+1]
-            while (hi - lo > 1) {
-                const int mid = (lo + hi) >> 1;
-                if (s_off[wv][mid] <= k) lo = mid; else hi = mid;
-            }
-            cand = s_c0[wv][lo] + (k - s_off[wv][lo]);
+            cand = candidate(k);
             const int lc = h.level[cand];
             if (box_gap<D>(h.icoord, i, l, cand, lc) >= 1) cat = (lc == l) ? 1 : 2;
             else cat = 3;
+            if (cat == 3 && leaf)
+                sub = ((h.ne[cand] - h.nb[cand]) <= h.leaf || h.level[cand] >= B) ? 1 : 2;
         }
-        // a leaf's near list is final: its members become NEAR pairs (leaf members) or
-        // pending pairs (members to descend at the next levels)
-        int sub = 0;  // 1 NEAR, 2 pending
-        if (cat == 3 && leaf)
-            sub = ((h.ne[cand] - h.nb[cand]) <= h.leaf || h.level[cand] >= B) ? 1 : 2;
         const unsigned long long mA = __ballot(cat == 1), mX = __ballot(cat == 2),
                                  mN = __ballot(cat == 3), mNL = __ballot(sub == 1),
                                  mP = __ballot(sub == 2);
-        int bA = 0, bX = 0, bN = 0, bP = 0;
-        if (lane == 0) {
-            if (mA) bA = atomicAdd(&h.cnt[CNT_ADM], __popcll(mA));
-            if (mX) bX = atomicAdd(&h.cnt[CNT_X], __popcll(mX));
-            if (mNL) bN = atomicAdd(&h.cnt[CNT_NEAR], __popcll(mNL));
-            if (mP) bP = atomicAdd(&h.cnt[CNT_PEND0 + pend_out], __popcll(mP));
-        }
-        bA = __shfl(bA, 0, 64);
-        bX = __shfl(bX, 0, 64);
-        bN = __shfl(bN, 0, 64);
-        bP = __shfl(bP, 0, 64);
-        if (cat == 1) put_pair(h.adm, bA + __popcll(mA & lt), h.adm_cap, i, cand, h.flags);
-        if (cat == 2) put_pair(h.xl, bX + __popcll(mX & lt), h.x_cap, i, cand, h.flags);
         if (cat == 3) {
             const int pos = cnt + __popcll(mN & lt);
             if (pos < NEAR_CAP) my[pos] = cand;
             else atomicOr(&h.flags[FLAG_NEAR_OVF], 1);
         }
-        if (sub == 1) put_pair(h.nl, bN + __popcll(mNL & lt), h.nl_cap, i, cand, h.flags);
-        if (sub == 2)
-            put_pair(h.pend[pend_out], bP + __popcll(mP & lt), h.pend_cap, i, cand, h.flags);
         cnt += __popcll(mN);
+        if (r < RMAXR && lane == 0) {
+            s_m[wv][r][0] = mA;
+            s_m[wv][r][1] = mX;
+            s_m[wv][r][2] = mNL;
+            s_m[wv][r][3] = mP;
+        }
+        nA += __popcll(mA);
+        nX += __popcll(mX);
+        nN += __popcll(mNL);
+        nP += __popcll(mP);
     }
-    if (lane == 0) h.near_cnt[i] = min(cnt, NEAR_CAP);
+    if (valid && lane == 0) h.near_cnt[i] = min(cnt, NEAR_CAP);
+    // one atomic per category for the whole workgroup
+    if (lane == 0) {
+        s_cnt[wv][0] = nA;
+        s_cnt[wv][1] = nX;
+        s_cnt[wv][2] = nN;
+        s_cnt[wv][3] = nP;
+    }
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int c = threadIdx.x;
+        const int tot = s_cnt[0][c] + s_cnt[1][c] + s_cnt[2][c] + s_cnt[3][c];
+        int *ctr = (c == 0) ? &h.cnt[CNT_ADM] : (c == 1) ? &h.cnt[CNT_X]
+                 : (c == 2) ? &h.cnt[CNT_NEAR] : &h.cnt[CNT_PEND0 + pend_out * kCntSlot];
+        s_bas[c] = tot ? atomicAdd(ctr, tot) : 0;
+    }
+    __syncthreads();
+    if (!valid) return;
+    int bA = s_bas[0], bX = s_bas[1], bN = s_bas[2], bP = s_bas[3];
+    for (int w = 0; w < wv; w++) {
+        bA += s_cnt[w][0];
+        bX += s_cnt[w][1];
+        bN += s_cnt[w][2];
+        bP += s_cnt[w][3];
+    }
+    // this node's ADM and X pairs are contiguous, in candidate order: their ranges are
+    // final (no sort; only the NEAR and W lists, which later levels append to, are sorted)
+    if (lane == 0) {
+        h.adm_beg[i] = bA;
+        h.adm_end[i] = bA + nA;
+        h.x_beg[i] = bX;
+        h.x_end[i] = bX + nX;
+    }
+    // pass 2: write the pairs in candidate order
+    for (int r = 0; r < rounds; r++) {
+        const int k = r * 64 + lane;
+        const unsigned long long mA = s_m[wv][r][0], mX = s_m[wv][r][1], mNL = s_m[wv][r][2],
+                                 mP = s_m[wv][r][3];
+        const unsigned long long me = 1ull << lane;
+        if (k < T && ((mA | mX | mNL | mP) & me)) {
+            const int cand = candidate(k);
+            if (mA & me) put_pair(h.adm, bA + __popcll(mA & lt), h.adm_cap, i, cand, h.flags);
+            if (mX & me) put_pair(h.xl, bX + __popcll(mX & lt), h.x_cap, i, cand, h.flags);
+            if (mNL & me) put_pair(h.nl, bN + __popcll(mNL & lt), h.nl_cap, i, cand, h.flags);
+            if (mP & me)
+                put_pair(h.pend[pend_out], bP + __popcll(mP & lt), h.pend_cap, i, cand, h.flags);
+        }
+        bA += __popcll(mA);
+        bX += __popcll(mX);
+        bN += __popcll(mNL);
+        bP += __popcll(mP);
+    }
 }
 
 // Pending pairs (leaf i, non-leaf c): each child d of c is a W source when i lies at
@@ -479,7 +563,7 @@ pending_kernel(H2Dev h, int n_pend, int pend_in, int pend_out)
     }
     int kw = wave_alloc(&h.cnt[CNT_W], nw);
     int kn = wave_alloc(&h.cnt[CNT_NEAR], nn);
-    int kp = wave_alloc(&h.cnt[CNT_PEND0 + pend_out], np);
+    int kp = wave_alloc(&h.cnt[CNT_PEND0 + pend_out * kCntSlot], np);
     for (int q = 0; q < nc; q++) {
         const int d = c0 + q;
         const int ld = h.level[d];
@@ -522,7 +606,7 @@ __global__ void publish_level_kernel(H2Dev h, int last, int pend_in, int *out)
     if (threadIdx.x != 0) return;
     out[0] = h.childoff[last];
     out[1] = h.childcnt[last];
-    out[2] = h.cnt[CNT_PEND0 + pend_in];
+    out[2] = h.cnt[CNT_PEND0 + pend_in * kCntSlot];
     out[3] = h.flags[FLAG_NONFINITE];
 }
 
