@@ -45,6 +45,9 @@ struct H2 {
 
 namespace {
 
+constexpr int kLevelBatch = 4;   // tree levels built between two host checks
+constexpr int kMaxLevels = 32;   // > H2Cfg<D>::B + 1 for both dimensions
+
 // Radix sorts of the Morton keys and pair lists: always the onesweep algorithm (the
 // library default sorts up to 2^20 items with a block sort plus merge passes, which at
 // these sizes costs several times more than the 6-8 onesweep digit passes).
@@ -185,46 +188,58 @@ int build_once(H2 *h, const void *Yg)
     }
     if (launch_gather_sorted(D, Yg, d, st)) return -1;
     if (launch_init_root(d, st)) return -1;
-    h->lvl.assign({0, 1});
-    int lb = 0, le = 1, l = 0, pend_in = 0, n_pend_in = 0;
-    bool first = true;
+    // Levels are built kLevelBatch at a time without waiting for the host: each level's
+    // kernels take its node range from d.lvl (written by the previous level's
+    // publish_level) and are sized by an upper bound on its node count (NCH times the
+    // previous bound; exact after each check).  Levels past the last one are empty and
+    // cost only their launches.
+    const int NCH = (D == 2) ? H2Cfg<2>::NCH : H2Cfg<3>::NCH;
+    const int maxlev = (D == 2) ? H2Cfg<2>::B + 1 : H2Cfg<3>::B + 1;  // level B holds leaves only
+    int l = 0, pend_in = 0, bound = 1;
     for (;;) {
-        if (launch_split_count(D, d, lb, le, l, st)) return -1;
+        const int pend_out = pend_in ^ 1;
+        if (launch_split_count(D, d, l, bound, st)) return -1;
         size_t bytes = h->cub_bytes;
-        if (hipcub::DeviceScan::ExclusiveSum(h->cub_tmp, bytes, d.childcnt, d.childoff, le - lb, st) !=
+        if (hipcub::DeviceScan::ExclusiveSum(h->cub_tmp, bytes, d.childcnt, d.childoff, bound, st) !=
             hipSuccess)
             return -1;
-        // the level's child count, pending pairs and finiteness flag, written by one
-        // thread straight into the pinned host words (one launch instead of four copies)
-        if (launch_publish_level(d, le - lb - 1, pend_in, h->pin_dev, st) ||
+        if (launch_publish_level(d, l, pend_out, h->pin_dev, st)) return -1;
+        if (launch_split_write(D, d, l, bound, st)) return -1;
+        const int nbound = (int)std::min<long long>(d.node_cap, (long long)NCH * bound);
+        if (launch_lists(D, d, l + 1, nbound, pend_out, st)) return -1;
+        if (launch_pending(D, d, pend_in, pend_out, st)) return -1;
+        pend_in = pend_out;
+        l++;
+        bound = std::max(nbound, 1);
+        if (l % kLevelBatch == 0 || l >= maxlev) {
+            if (hipStreamSynchronize(st) != hipSuccess) return -1;
+            if (h->pin[1]) return -2;  // non-finite coordinate
+            if (h->pin[2]) {           // node arrays full
+                h->node_f *= 2.0;
+                return 1;
+            }
+            if (h->pin[0] == 0 || l >= maxlev) break;  // level l is empty
+            bound = h->pin[0];                         // level l's exact node count
+        }
+    }
+    {
+        std::vector<int> lv(l + 2);
+        if (hipMemcpyAsync(lv.data(), d.lvl, sizeof(int) * lv.size(), hipMemcpyDeviceToHost, st) !=
+                hipSuccess ||
             hipStreamSynchronize(st) != hipSuccess)
             return -1;
-        if (first && h->pin[3]) return -2;
-        first = false;
-        const int T = h->pin[0] + h->pin[1];
-        n_pend_in = std::min(h->pin[2], d.pend_cap);
-        if (T == 0) break;
-        if ((long long)le + T > d.node_cap) {
-            h->node_f *= 2.0;
-            return 1;
-        }
-        if (launch_split_write(D, d, lb, le, l, st)) return -1;
-        const int pend_out = pend_in ^ 1;
-        if (hipMemsetAsync(d.cnt + CNT_PEND0 + pend_out * kCntSlot, 0, sizeof(int), st) != hipSuccess)
-            return -1;
-        if (launch_lists(D, d, le, le + T, l + 1, pend_out, st)) return -1;
-        if (launch_pending(D, d, n_pend_in, pend_in, pend_out, st)) return -1;
-        lb = le;
-        le += T;
-        l++;
-        h->lvl.push_back(le);
-        pend_in = pend_out;
+        h->lvl.assign({0, 1});
+        for (int k = 1; k + 1 < (int)lv.size() && lv[k + 1] > lv[k]; k++) h->lvl.push_back(lv[k + 1]);
     }
-    h->n_nodes = le;
+    h->n_nodes = h->lvl.back();
     if (hipMemcpyAsync(h->pin, d.cnt, sizeof(int) * CNT_COUNT, hipMemcpyDeviceToHost, st) ||
         hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
         hipStreamSynchronize(st) != hipSuccess)
         return -1;
+    if (h->hflags[FLAG_NODE_OVF]) {
+        h->node_f *= 2.0;
+        return 1;
+    }
     if (h->hflags[FLAG_PAIR_OVF] || h->hflags[FLAG_NEAR_OVF]) {
         if (h->hflags[FLAG_NEAR_OVF]) return -1;  // geometry the fixed near list cannot hold
         h->pair_f *= 2.0;
@@ -285,6 +300,7 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
     bool ok = dmalloc(&d.keys, n, h->st) && dmalloc(&h->keys_alt, n, h->st) && dmalloc(&d.perm, n, h->st) &&
               dmalloc(&h->perm_alt, n, h->st) && dmalloc((char **)&d.Ys, (size_t)n * vb, h->st) &&
               dmalloc(&d.root, 1, h->st) && dmalloc(&d.flags, FLAG_COUNT, h->st) && dmalloc(&d.cnt, CNT_COUNT, h->st) &&
+              dmalloc(&d.lvl, kMaxLevels + 4, h->st) &&
               dmalloc(&h->part, (size_t)h->nblk * 7, h->st) &&
               hipHostMalloc((void **)&h->pin, sizeof(int) * CNT_COUNT,
                             hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
@@ -320,6 +336,7 @@ void h2_destroy(H2 *h)
     H2Dev &d = h->d;
     dfree(d.keys, h->st); dfree(h->keys_alt, h->st); dfree(d.perm, h->st); dfree(h->perm_alt, h->st); dfree(d.Ys, h->st);
     dfree(d.root, h->st); dfree(d.flags, h->st); dfree(d.cnt, h->st); dfree(h->part, h->st);
+    dfree(d.lvl, h->st);
     dfree(h->d_proxy, h->st); dfree(h->d_proxy_cnt, h->st);
     if (h->pin) (void)hipHostFree(h->pin);
     delete h;

@@ -51,6 +51,7 @@ enum H2Flag {
     FLAG_BIG_LEAF = 3,   // a leaf at the maximum depth holds more than kLeafCap points
     FLAG_NONFINITE = 4,  // non-finite coordinate
     FLAG_RANK_CAP = 5,   // an ID stopped at RMAX before reaching the tolerance
+    FLAG_NODE_OVF = 6,   // the node arrays overflowed
     FLAG_COUNT = 16
 };
 
@@ -80,7 +81,8 @@ struct H2Dev {
     int *perm;
     void *Ys;          // Vec<D> sorted coordinates
     RootInfo *root;
-    // nodes
+    // nodes; level l is [lvl[l], lvl[l+1]) (written on the device as the tree grows)
+    int *lvl;
     int node_cap;
     int *nb, *ne, *level, *parent, *child0, *nchild;
     float4 *cent;      // centre (xyz) of each node's box, w = half-side
@@ -113,11 +115,16 @@ int launch_root_box(int D, const void *Yg, int n, const H2Dev &h, double *partia
 int launch_morton(int D, const void *Yg, const H2Dev &h, hipStream_t st);
 int launch_gather_sorted(int D, const void *Yg, const H2Dev &h, hipStream_t st);
 int launch_init_root(const H2Dev &h, hipStream_t st);
-int launch_split_count(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st);
-int launch_split_write(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st);
-int launch_lists(int D, const H2Dev &h, int lb, int le, int l, int pend_out, hipStream_t st);
-int launch_pending(int D, const H2Dev &h, int n_pend, int pend_in, int pend_out, hipStream_t st);
-int launch_publish_level(const H2Dev &h, int last, int pend_in, int *out, hipStream_t st);
+// Level kernels take the level's node range from h.lvl on the device; `bound` (an upper
+// bound on the level's node count, known to the host) sizes the grid.
+int launch_split_count(int D, const H2Dev &h, int l, int bound, hipStream_t st);
+int launch_split_write(int D, const H2Dev &h, int l, int bound, hipStream_t st);
+int launch_lists(int D, const H2Dev &h, int l, int bound, int pend_out, hipStream_t st);
+int launch_pending(int D, const H2Dev &h, int pend_in, int pend_out, hipStream_t st);
+// After the scan of level l's child counts: lvl[l+2], the zeroed pending counter of the
+// next pass, and (pinned host words) out[0] children of level l, out[1] non-finite
+// input, out[2] node overflow.
+int launch_publish_level(const H2Dev &h, int l, int pend_out, int *out, hipStream_t st);
 int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, hipStream_t st);
 
 // h2_id.hip

@@ -4,6 +4,7 @@
 // Reference counterpart: H2P_partition_points / the admissible-pair search inside
 // H2P_build (H2Pack, called at src/tsne_h2pack.c:863 and :908-919) and the root box
 // tsne_calc_root_enbox (src/tsne_h2pack.c:651-698).
+#include <algorithm>
 #include <cmath>
 
 #include "h2_internal.h"
@@ -252,6 +253,8 @@ __global__ void init_root_kernel(H2Dev h, int B)
     h.near[0] = 0;
     h.near_cnt[0] = 1;
     h.adm_beg[0] = h.adm_end[0] = h.x_beg[0] = h.x_end[0] = 0;
+    h.lvl[0] = 0;
+    h.lvl[1] = 1;
     if (h.n <= h.leaf || B == 0) {
         h.nl[0] = 0;
         h.cnt[CNT_NEAR] = 1;
@@ -276,13 +279,17 @@ int launch_init_root(const H2Dev &h, hipStream_t st)
 // for); lane 0 then collects the bounds and records the node.
 template <int D>
 __global__ void __launch_bounds__(256)
-split_count_kernel(H2Dev h, int lb, int le, int l)
+split_count_kernel(H2Dev h, int l, int nbound)
 {
     constexpr int B = H2Cfg<D>::B, NCH = H2Cfg<D>::NCH;
     static_assert((64 % NCH) == 0, "lanes per node");
+    if (h.flags[FLAG_NODE_OVF]) return;  // the build is redone with larger arrays
+    const int lb = h.lvl[l], le = h.lvl[l + 1];
     const int t = blockIdx.x * 256 + threadIdx.x;
     const int i = lb + t / NCH, v = t % NCH;
     const bool valid = i < le;
+    // the scan runs over `nbound` counts: slots past the level's end count no children
+    if (!valid && v == 0 && t / NCH < nbound) h.childcnt[t / NCH] = 0;
     int b = 0, e = 0;
     if (valid) {
         b = h.nb[i];
@@ -331,9 +338,11 @@ split_count_kernel(H2Dev h, int lb, int le, int l)
 
 template <int D>
 __global__ void __launch_bounds__(256)
-split_write_kernel(H2Dev h, int lb, int le, int l)
+split_write_kernel(H2Dev h, int l)
 {
     constexpr int NCH = H2Cfg<D>::NCH;
+    if (h.flags[FLAG_NODE_OVF]) return;
+    const int lb = h.lvl[l], le = h.lvl[l + 1];
     const int i = lb + blockIdx.x * 256 + threadIdx.x;
     if (i >= le) return;
     if (h.childcnt[i - lb] == 0) return;
@@ -345,6 +354,7 @@ split_write_kernel(H2Dev h, int lb, int le, int l)
     for (int v = 0; v < NCH; v++) {
         if (bd[v + 1] <= bd[v]) continue;
         const int ch = base + k++;
+        if (ch >= h.node_cap) continue;  // overflow: flagged by publish_level, rebuilt larger
         h.nb[ch] = bd[v];
         h.ne[ch] = bd[v + 1];
         h.level[ch] = l + 1;
@@ -364,23 +374,23 @@ split_write_kernel(H2Dev h, int lb, int le, int l)
     h.nchild[i] = k;
 }
 
-int launch_split_count(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st)
+int launch_split_count(int D, const H2Dev &h, int l, int bound, hipStream_t st)
 {
-    const int g = ((le - lb) * (D == 2 ? H2Cfg<2>::NCH : H2Cfg<3>::NCH) + 255) / 256;
+    const int g = std::max(1, (bound * (D == 2 ? H2Cfg<2>::NCH : H2Cfg<3>::NCH) + 255) / 256);
     if (D == 2)
-        hipLaunchKernelGGL(split_count_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le, l);
+        hipLaunchKernelGGL(split_count_kernel<2>, dim3(g), dim3(256), 0, st, h, l, bound);
     else
-        hipLaunchKernelGGL(split_count_kernel<3>, dim3(g), dim3(256), 0, st, h, lb, le, l);
+        hipLaunchKernelGGL(split_count_kernel<3>, dim3(g), dim3(256), 0, st, h, l, bound);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_split_write(int D, const H2Dev &h, int lb, int le, int l, hipStream_t st)
+int launch_split_write(int D, const H2Dev &h, int l, int bound, hipStream_t st)
 {
-    const int g = (le - lb + 255) / 256;
+    const int g = std::max(1, (bound + 255) / 256);
     if (D == 2)
-        hipLaunchKernelGGL(split_write_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le, l);
+        hipLaunchKernelGGL(split_write_kernel<2>, dim3(g), dim3(256), 0, st, h, l);
     else
-        hipLaunchKernelGGL(split_write_kernel<3>, dim3(g), dim3(256), 0, st, h, lb, le, l);
+        hipLaunchKernelGGL(split_write_kernel<3>, dim3(g), dim3(256), 0, st, h, l);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -401,8 +411,10 @@ int launch_split_write(int D, const H2Dev &h, int lb, int le, int l, hipStream_t
 // and category made this kernel's time grow with the node count.)
 template <int D>
 __global__ void __launch_bounds__(256)
-lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
+lists_kernel(H2Dev h, int l, int pend_out)
 {
+    if (h.flags[FLAG_NODE_OVF]) return;  // uniform over the grid
+    const int lb = h.lvl[l], le = h.lvl[l + 1];
     constexpr int B = H2Cfg<D>::B, NEAR_CAP = H2Cfg<D>::NEAR_CAP, NCH = H2Cfg<D>::NCH;
     constexpr int RMAXR = (NEAR_CAP * NCH + 63) / 64;  // candidate rounds at most
     __shared__ int s_off[4][65], s_c0[4][64];
@@ -539,9 +551,11 @@ lists_kernel(H2Dev h, int lb, int le, int l, int pend_out)
 // least one box of d away, a NEAR source when d is a leaf, otherwise pending again.
 template <int D>
 __global__ void __launch_bounds__(256)
-pending_kernel(H2Dev h, int n_pend, int pend_in, int pend_out)
+pending_kernel(H2Dev h, int pend_in, int pend_out)
 {
     constexpr int B = H2Cfg<D>::B;
+    if (h.flags[FLAG_NODE_OVF]) return;
+    const int n_pend = min(h.cnt[CNT_PEND0 + pend_in * kCntSlot], h.pend_cap);
     const int k = blockIdx.x * 256 + threadIdx.x;
     const bool valid = k < n_pend;
     int i = 0, c = 0, li = 0, nc = 0, c0 = 0;
@@ -576,43 +590,49 @@ pending_kernel(H2Dev h, int n_pend, int pend_in, int pend_out)
     }
 }
 
-int launch_lists(int D, const H2Dev &h, int lb, int le, int l, int pend_out, hipStream_t st)
+int launch_lists(int D, const H2Dev &h, int l, int bound, int pend_out, hipStream_t st)
 {
-    const int g = (le - lb + 3) / 4;
-    if (g == 0) return 0;
+    const int g = std::max(1, (bound + 3) / 4);
     if (D == 2)
-        hipLaunchKernelGGL(lists_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le, l, pend_out);
+        hipLaunchKernelGGL(lists_kernel<2>, dim3(g), dim3(256), 0, st, h, l, pend_out);
     else
-        hipLaunchKernelGGL(lists_kernel<3>, dim3(g), dim3(256), 0, st, h, lb, le, l, pend_out);
+        hipLaunchKernelGGL(lists_kernel<3>, dim3(g), dim3(256), 0, st, h, l, pend_out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_pending(int D, const H2Dev &h, int n_pend, int pend_in, int pend_out, hipStream_t st)
+// Grid over the whole pending buffer: the live count is on the device.
+int launch_pending(int D, const H2Dev &h, int pend_in, int pend_out, hipStream_t st)
 {
-    if (n_pend <= 0) return 0;
-    const int g = (n_pend + 255) / 256;
+    const int g = std::max(1, (h.pend_cap + 255) / 256);
     if (D == 2)
-        hipLaunchKernelGGL(pending_kernel<2>, dim3(g), dim3(256), 0, st, h, n_pend, pend_in, pend_out);
+        hipLaunchKernelGGL(pending_kernel<2>, dim3(g), dim3(256), 0, st, h, pend_in, pend_out);
     else
-        hipLaunchKernelGGL(pending_kernel<3>, dim3(g), dim3(256), 0, st, h, n_pend, pend_in, pend_out);
+        hipLaunchKernelGGL(pending_kernel<3>, dim3(g), dim3(256), 0, st, h, pend_in, pend_out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-// After the scan of a level's child counts: the total child count (exclusive offset +
-// count of the last node), the pending-pair count of the input buffer and the
-// non-finite flag, into the pinned host words out[0..3].
-__global__ void publish_level_kernel(H2Dev h, int last, int pend_in, int *out)
+__global__ void publish_level_kernel(H2Dev h, int l, int pend_out, int *out)
 {
     if (threadIdx.x != 0) return;
-    out[0] = h.childoff[last];
-    out[1] = h.childcnt[last];
-    out[2] = h.cnt[CNT_PEND0 + pend_in * kCntSlot];
-    out[3] = h.flags[FLAG_NONFINITE];
+    const int lb = h.lvl[l], le = h.lvl[l + 1];
+    const int cnt = le - lb;
+    const int T = (cnt > 0 && !h.flags[FLAG_NODE_OVF]) ? h.childoff[cnt - 1] + h.childcnt[cnt - 1] : 0;
+    int ne = le + T, ovf = h.flags[FLAG_NODE_OVF];
+    if (ne > h.node_cap) {
+        ovf = 1;
+        ne = h.node_cap;
+        h.flags[FLAG_NODE_OVF] = 1;
+    }
+    h.lvl[l + 2] = ne;
+    h.cnt[CNT_PEND0 + pend_out * kCntSlot] = 0;
+    out[0] = T;
+    out[1] = h.flags[FLAG_NONFINITE];
+    out[2] = ovf;
 }
 
-int launch_publish_level(const H2Dev &h, int last, int pend_in, int *out, hipStream_t st)
+int launch_publish_level(const H2Dev &h, int l, int pend_out, int *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(publish_level_kernel, dim3(1), dim3(64), 0, st, h, last, pend_in, out);
+    hipLaunchKernelGGL(publish_level_kernel, dim3(1), dim3(64), 0, st, h, l, pend_out, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
