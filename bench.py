@@ -522,6 +522,10 @@ def main():
     ap.add_argument("--leaf", type=int, default=100)
     ap.add_argument("--leaf-mixed", type=int, default=0)
     ap.add_argument("--tol", type=float, default=1e-2)
+    ap.add_argument("--k", type=int, default=90, help="neighbours of the exact k-NN graph")
+    ap.add_argument("--perplexity", type=float, default=30.0)
+    ap.add_argument("--clusters", type=int, default=0,
+                    help="Gaussian clusters (default: 100 at N >= 1M, else 20)")
     ap.add_argument("--paths", default="mixed,plain")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--save-inputs", default="", help="write the prepared P here and exit")
@@ -552,13 +556,13 @@ def main():
     from tsne_h2pack._backend import get_backend
     be = get_backend()
     n, dim = a.n, a.dim
-    clusters = 100 if n >= 1_000_000 else 20
+    clusters = a.clusters if a.clusters > 0 else (100 if n >= 1_000_000 else 20)
     if a.load_inputs:
         z = np.load(a.load_inputs)
         csr = (z["row_ptr"], z["col_idx"], z["values"])
         assert csr[0].size == n + 1
     else:
-        csr = make_inputs(n, 50, clusters, 90, 30.0)
+        csr = make_inputs(n, 50, clusters, a.k, a.perplexity)
     if a.save_inputs:
         np.savez(a.save_inputs, row_ptr=csr[0], col_idx=csr[1], values=csr[2])
         return
@@ -578,8 +582,9 @@ def main():
         "warmup": a.warmup, "ms_per_step": 1e3 / head["its"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded Gaussian mixture, exact k-NN on GPU)",
-        "config": {"workload": f"tsne-{dim}d N={n} D=50 k=90 perplexity=30 h2_tol={a.tol} "
-                               f"rebuild 3/10 leaf {a.leaf}", "n": n, "dim": dim, "nnz_P": nnz,
+        "config": {"workload": f"tsne-{dim}d N={n} D=50 clusters={clusters} k={a.k} "
+                               f"perplexity={a.perplexity:g} h2_tol={a.tol} rebuild 3/10 "
+                               f"leaf {a.leaf}", "n": n, "dim": dim, "nnz_P": nnz,
                    "paths": a.paths},
         "paths": res,
         "roofline": roof,

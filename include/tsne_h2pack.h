@@ -169,8 +169,9 @@ int tsne_exact_repulsive_rows(tsne_workspace_t *workspace, const int *rows, int 
 
 /* Statistics of the last H2 build: levels, nodes, leaves, admissible and near pairs,
  * maximum and mean skeleton rank, proxy points of the finest level, flop counts of one
- * product; and of the last mixed-precision product, the nodes whose BF16 coupling was
- * redone in FP32.  0 on success, -1 if no H2 structure exists. */
+ * product; of the last mixed-precision product, the nodes whose BF16 coupling was redone
+ * in FP32; and the number of builds so far that could not be used (the product then
+ * takes the direct sum).  0 on success, -1 if no H2 structure exists. */
 typedef struct {
     int levels, n_nodes, n_leaves, leaf_points_max;
     long long n_adm_pairs, n_near_pairs, near_point_pairs;
@@ -179,6 +180,7 @@ typedef struct {
     int proxy_max;
     double far_flops, near_flops;
     int mixed_fallback_nodes;
+    int direct_fallbacks;  /* builds so far whose product used the direct sum instead */
 } tsne_h2_stats_t;
 int tsne_h2_stats(tsne_workspace_t *workspace, tsne_h2_stats_t *stats);
 

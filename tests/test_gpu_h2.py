@@ -334,3 +334,33 @@ def test_overlapped_attractive_pass_matches_fused(backend, problem):
     rel = np.abs(out["0"][0] - out["1"][0]).max() / np.abs(out["0"][0]).max()
     _say(f"  rel diff {rel:.2e}, KL {out['0'][1]:.8f} / {out['1'][1]:.8f}")
     assert rel < 1e-6
+
+
+def test_oversized_deepest_leaf_3d(backend, problem):
+    """A 3D embedding whose extent (outliers at 1e4) leaves the deepest octree level
+    with boxes of ~0.03 while 300 points sit within 0.01 of each other: that leaf holds
+    more points than one chunk of the leaf kernel.  The product must still use the H2
+    structure (no direct-sum fallback) and meet the 3D bound against FP64."""
+    _say("oversized deepest leaf 3d")
+    _, _, P = problem
+    n = P.shape[0]
+    rng = np.random.default_rng(8)
+    Y0 = _embedding(n, 3, 3.0, seed=8).T.copy()
+    Y0[:300] = 5.0 + rng.uniform(0.0, 0.01, size=(300, 3))
+    Y0[300:304] = np.array([[1e4, 0, 0], [-1e4, 0, 0], [0, 1e4, 0], [0, 0, -1e4]])
+    Y0 = Y0.T.copy()
+    ws = _ws(backend, P, Y0, 3, 3e-2)
+    try:
+        g, Z = _grad(backend, ws, 3, n, 1.0, 0)
+        st = H2Stats()
+        assert backend.lib.tsne_h2_stats(ws, ctypes.byref(st)) == 0
+        rep64, Z64, _, _ = oracle.repulsive_exact(Y0)
+        g64 = oracle.attractive(P, Y0, 1.0) - rep64
+        err = np.linalg.norm(g - g64) / np.linalg.norm(g64)
+        _say(f"  levels {st.levels} direct fallbacks {st.direct_fallbacks} grad err {err:.2e} "
+             f"Z {Z:.6e}/{Z64:.6e}")
+        assert st.direct_fallbacks == 0
+        assert st.levels >= 20
+        assert err <= 3e-2
+    finally:
+        backend.lib.tsne_destroy(ws)

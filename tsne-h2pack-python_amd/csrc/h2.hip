@@ -8,6 +8,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <vector>
@@ -39,6 +40,8 @@ struct H2 {
     int usable = 0;
     int pq = 0;  // proxy slots the ID processes: the table's largest count, rounded up
     int hflags[FLAG_COUNT] = {0};
+    int fallbacks = 0;  // builds that left the product to the direct sum, and why (first)
+    int last_mixed = 0; // the last product was the mixed-precision one
     // growth factors (grown on overflow)
     double node_f = 8.0, pair_f = 1.0, pool_f = 1.0;
 };
@@ -48,11 +51,30 @@ namespace {
 constexpr int kLevelBatch = 4;   // tree levels built between two host checks
 constexpr int kMaxLevels = 32;   // > H2Cfg<D>::B + 1 for both dimensions
 
-// Radix sorts of the Morton keys and pair lists: always the onesweep algorithm (the
-// library default sorts up to 2^20 items with a block sort plus merge passes, which at
-// these sizes costs several times more than the 6-8 onesweep digit passes).
+// Radix sorts of the Morton keys and pair lists.  From kOnesweepMin items on, always the
+// onesweep algorithm: the library default sorts up to 2^20 items with a block sort plus
+// merge passes, which near 10^6 items costs several times more than the 6-8 onesweep
+// digit passes.  Smaller sorts keep the default, which needs fewer launches (the small
+// jobs of a job server are bound by launch latency, not bandwidth).
 using SortCfg = rocprim::radix_sort_config<rocprim::default_config, rocprim::default_config,
                                            rocprim::default_config, 0>;
+constexpr int kOnesweepMin = 1 << 18;
+
+template <typename V>
+hipError_t sort_pairs64(void *tmp, size_t &bytes, rocprim::double_buffer<uint64_t> &k,
+                        rocprim::double_buffer<V> &v, int n, int endbit, hipStream_t st)
+{
+    return n >= kOnesweepMin
+               ? rocprim::radix_sort_pairs<SortCfg>(tmp, bytes, k, v, n, 0, endbit, st)
+               : rocprim::radix_sort_pairs(tmp, bytes, k, v, n, 0, endbit, st);
+}
+
+hipError_t sort_keys64(void *tmp, size_t &bytes, rocprim::double_buffer<uint64_t> &k, int n,
+                       int endbit, hipStream_t st)
+{
+    return n >= kOnesweepMin ? rocprim::radix_sort_keys<SortCfg>(tmp, bytes, k, n, 0, endbit, st)
+                             : rocprim::radix_sort_keys(tmp, bytes, k, n, 0, endbit, st);
+}
 
 // Stream-ordered device memory on the structure's stream (see the workspace allocator).
 template <typename T> bool dmalloc(T **p, size_t n, hipStream_t st)
@@ -136,10 +158,16 @@ bool alloc_sized(H2 *h)
     size_t b1 = 0, b2 = 0, b3 = 0;
     rocprim::double_buffer<uint64_t> kb(d.keys, h->keys_alt);
     rocprim::double_buffer<int> vb(d.perm, h->perm_alt);
+    // temporary storage: the larger of the two algorithms at the sizes in use
+    size_t b1a = 0, b3a = 0;
     (void)rocprim::radix_sort_pairs<SortCfg>(nullptr, b1, kb, vb, h->n, 0, 64, h->st);
+    (void)rocprim::radix_sort_pairs(nullptr, b1a, kb, vb, h->n, 0, 64, h->st);
+    b1 = std::max(b1, b1a);
     (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b2, d.childcnt, d.childoff, (int)nc, h->st);
     rocprim::double_buffer<uint64_t> pb(d.adm, d.sort_tmp);
     (void)rocprim::radix_sort_keys<SortCfg>(nullptr, b3, pb, maxpair, 0, 64, h->st);
+    (void)rocprim::radix_sort_keys(nullptr, b3a, pb, maxpair, 0, 64, h->st);
+    b3 = std::max(b3, b3a);
     h->cub_bytes = std::max({b1, b2, b3});
     return hipMallocAsync(&h->cub_tmp, h->cub_bytes, h->st) == hipSuccess;
 }
@@ -155,7 +183,7 @@ int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end)
     while ((1LL << (bits - 32)) < d.node_cap) bits++;
     rocprim::double_buffer<uint64_t> db(buf, d.sort_tmp);
     size_t bytes = h->cub_bytes;
-    if (rocprim::radix_sort_keys<SortCfg>(h->cub_tmp, bytes, db, n, 0, bits, h->st) != hipSuccess)
+    if (sort_keys64(h->cub_tmp, bytes, db, n, bits, h->st) != hipSuccess)
         return -1;
     if (db.current() != buf) {  // the sorted list now lives in the scratch buffer:
         std::swap(buf, d.sort_tmp);  // exchange them (both hold maxpair entries)
@@ -180,8 +208,7 @@ int build_once(H2 *h, const void *Yg)
         rocprim::double_buffer<int> vb(d.perm, h->perm_alt);
         size_t bytes = h->cub_bytes;
         const int endbit = D * (D == 2 ? H2Cfg<2>::B : H2Cfg<3>::B);
-        if (rocprim::radix_sort_pairs<SortCfg>(h->cub_tmp, bytes, kb, vb, h->n, 0, endbit, st) !=
-            hipSuccess)
+        if (sort_pairs64(h->cub_tmp, bytes, kb, vb, h->n, endbit, st) != hipSuccess)
             return -1;
         if (kb.current() != d.keys) std::swap(d.keys, h->keys_alt);
         if (vb.current() != d.perm) std::swap(d.perm, h->perm_alt);
@@ -346,11 +373,14 @@ int h2_build(H2 *h, const void *Yg, int *nonfinite)
 {
     *nonfinite = 0;
     h->usable = 0;
+    const char *why = "capacity retries exhausted";
     for (int attempt = 0; attempt < 6; attempt++) {
         const int rc = build_once(h, Yg);
         if (rc == 0) {
             h->usable = !h->hflags[FLAG_BIG_LEAF];
-            return 0;
+            if (h->usable) return 0;
+            why = "a leaf at the deepest level holds more points than a leaf can";
+            break;
         }
         if (rc == -2) {
             *nonfinite = 1;
@@ -359,9 +389,50 @@ int h2_build(H2 *h, const void *Yg, int *nonfinite)
         if (rc < 0) {
             // the structure cannot be built (e.g. a near list beyond its capacity):
             // the caller falls back to the direct sum
-            return hipGetLastError() == hipSuccess ? 0 : -1;
+            if (hipGetLastError() != hipSuccess) return -1;
+            why = h->hflags[FLAG_NEAR_OVF] ? "a near list over its capacity" : "a device error";
+            break;
         }
         if (!alloc_sized(h)) return -1;
+    }
+    if (h->fallbacks++ == 0) {
+        fprintf(stderr, "tsne_h2pack: H2 structure not usable (%s); this product and any like it "
+                        "use the direct O(N^2) sum\n", why);
+        if (getenv("TSNE_H2_DEBUG") && h->hflags[FLAG_BIG_LEAF]) {
+            // the largest leaf: its level, size and the extent of its points
+            const int nn = h->n_nodes;
+            std::vector<int> nb(nn), ne(nn), lv(nn), c0(nn);
+            RootInfo root{};
+            (void)hipMemcpy(nb.data(), h->d.nb, nn * sizeof(int), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(ne.data(), h->d.ne, nn * sizeof(int), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(lv.data(), h->d.level, nn * sizeof(int), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(c0.data(), h->d.child0, nn * sizeof(int), hipMemcpyDeviceToHost);
+            (void)hipMemcpy(&root, h->d.root, sizeof(RootInfo), hipMemcpyDeviceToHost);
+            int best = -1, nbig = 0;
+            for (int i = 0; i < nn; i++)
+                if (c0[i] < 0) {
+                    if (best < 0 || ne[i] - nb[i] > ne[best] - nb[best]) best = i;
+                    nbig += ne[i] - nb[i] > (h->D == 2 ? H2Cfg<2>::LEAF_CAP : H2Cfg<3>::LEAF_CAP);
+                }
+            if (best < 0) best = 0;
+            fprintf(stderr, "  nodes %d levels %d, leaves over the cap: %d\n", nn,
+                    (int)h->lvl.size() - 1, nbig);
+            const int m = ne[best] - nb[best];
+            const int vs = h->D == 2 ? 2 : 4;
+            std::vector<float> y((size_t)m * vs);
+            (void)hipMemcpy(y.data(), (const float *)h->d.Ys + (size_t)nb[best] * vs,
+                            y.size() * sizeof(float), hipMemcpyDeviceToHost);
+            float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+            for (int k = 0; k < m; k++)
+                for (int q = 0; q < h->D; q++) {
+                    lo[q] = std::min(lo[q], y[(size_t)k * vs + q]);
+                    hi[q] = std::max(hi[q], y[(size_t)k * vs + q]);
+                }
+            fprintf(stderr, "  root H0 %.6g centre (%.6g %.6g %.6g); leaf %d level %d points %d, "
+                            "extent %.3g %.3g %.3g at (%.6g %.6g %.6g)\n",
+                    root.H0, root.c[0], root.c[1], root.c[2], best, lv[best], m, hi[0] - lo[0],
+                    hi[1] - lo[1], hi[2] - lo[2], lo[0], lo[1], lo[2]);
+        }
     }
     return 0;
 }
@@ -371,6 +442,7 @@ int h2_product(H2 *h, const void *Yg, float *s, float *F, int mixed)
     (void)Yg;
     if (!h->usable) return -1;
     H2Dev &d = h->d;
+    h->last_mixed = mixed;
     if (launch_couple(h->D, d, h->n_nodes, mixed, (float)h->tol, h->st)) return -1;
     const int nlev = (int)h->lvl.size() - 1;
     for (int lv = 2; lv < nlev; lv++)
@@ -396,6 +468,7 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     st->rank_mean = 0.0;
     st->proxy_max = 0;
     st->mixed_fallback_nodes = 0;
+    st->direct_fallbacks = h->fallbacks;
     if (!h->usable || h->n_nodes <= 0) return 0;
     // flop counts of one product from the built structure (host copies; diagnostics only)
     const int nn = h->n_nodes;
@@ -421,7 +494,7 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     for (int i = 1; i < nn; i++) {
         st->rank_max = std::max(st->rank_max, rk[i]);
         rsum += rk[i];
-        st->mixed_fallback_nodes += fl[i] != 0;
+        if (h->last_mixed) st->mixed_fallback_nodes += fl[i] != 0;
     }
     st->rank_mean = nn > 1 ? (double)rsum / (nn - 1) : 0.0;
     {

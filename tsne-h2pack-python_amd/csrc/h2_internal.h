@@ -31,24 +31,29 @@ template <> struct H2Cfg<2> {
     static constexpr int PMAX = 48;      // maximum proxy points
     static constexpr int MCAP = 128;     // maximum ID candidates (NCH * RMAX, leaf cap)
     static constexpr int NEAR_CAP = 48;  // near-list slots per node
+    static constexpr int LEAF_CAP = 128; // points of a leaf at the deepest level (= MCAP)
 };
 template <> struct H2Cfg<3> {
-    static constexpr int B = 20;
+    static constexpr int B = 21;          // 63 key bits
     static constexpr int NCH = 8;
     static constexpr int RMAX = 48;
     static constexpr int PMAX = 96;
     static constexpr int MCAP = 384;
     static constexpr int NEAR_CAP = 96;
+    static constexpr int LEAF_CAP = 384;
 };
 
-constexpr int kLeafCap = 128;  // points per leaf handled by the leaf kernel
+// Leaf size limit of the build parameter, and the chunk of target / source points the
+// leaf kernel holds at a time.  A leaf at the deepest level can hold more (the points
+// of a box too small to split further), up to H2Cfg<D>::LEAF_CAP.
+constexpr int kLeafCap = 128;
 
 // Error / status flags (device int array H2Dev::flags).
 enum H2Flag {
     FLAG_NEAR_OVF = 0,   // a node's near list exceeded NEAR_CAP
     FLAG_PAIR_OVF = 1,   // a pair buffer (ADM/X/W/NEAR/pending) overflowed
     FLAG_POOL_OVF = 2,   // the U pool overflowed
-    FLAG_BIG_LEAF = 3,   // a leaf at the maximum depth holds more than kLeafCap points
+    FLAG_BIG_LEAF = 3,   // a leaf at the maximum depth holds more than LEAF_CAP points
     FLAG_NONFINITE = 4,  // non-finite coordinate
     FLAG_RANK_CAP = 5,   // an ID stopped at RMAX before reaching the tolerance
     FLAG_NODE_OVF = 6,   // the node arrays overflowed

@@ -43,57 +43,111 @@ __device__ __forceinline__ float4 group_sum(float4 v, int RS)
 
 }  // namespace
 
-// One wavefront per target node (all non-root nodes); lanes are (group g, row k): row k
-// of the node's skeleton, groups split the source list.
-// only_flagged: recompute just the nodes the mixed kernel flagged.
+// Sources of the coupling of target node i: the skeleton points of its ADM nodes, then
+// the points of its X leaves.  Source s carries its coordinates and its row of W':
+// [w0, (c_i - c_j) w0 - wY] for skeleton point q of ADM node j, [1, c_i - y] for point y
+// of an X leaf.  A wavefront walks the members in rounds of up to 64 (one per lane; their
+// source counts scanned into off[0..64]) and stages 64 sources at a time into LDS.
+template <int D>
+__device__ __forceinline__ int member_round(const H2Dev &h, int m0, int na, int nm,
+                                            int a0, int x0, int lane, int *off, int *mj)
+{
+    const int m = m0 + lane;
+    int sz = 0, j = -1;
+    if (m < nm) {
+        j = (m < na) ? (int)(uint32_t)h.adm[a0 + m] : (int)(uint32_t)h.xl[x0 + m - na];
+        sz = (m < na) ? h.rank[j] : (h.ne[j] - h.nb[j]);
+    }
+    int x = sz;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o, 64);
+        if (lane >= o) x += y;
+    }
+    off[lane + 1] = x;
+    if (lane == 0) off[0] = 0;
+    mj[lane] = j;
+    __builtin_amdgcn_wave_barrier();
+    return __shfl(x, 63, 64);
+}
+
+template <int D>
+__device__ __forceinline__ void stage_source(const H2Dev &h, float4 ci, int m0, int na,
+                                             int nmem, int sidx, int T, const int *off,
+                                             const int *mj, float4 &xs, float4 &ws)
+{
+    constexpr int RMAX = H2Cfg<D>::RMAX;
+    xs = ci;
+    ws = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (sidx >= T) return;
+    int lo = 0, hi = nmem;  // member a with off[a] <= sidx < off[a+1]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= sidx) lo = mid; else hi = mid;
+    }
+    const int jj = mj[lo], q = sidx - off[lo];
+    if (m0 + lo < na) {
+        const float4 cj = h.cent[jj];
+        xs = h.skelf[(size_t)jj * RMAX + q];
+        const float4 w = h.wc[(size_t)jj * RMAX + q];
+        ws = make_float4(w.x, (ci.x - cj.x) * w.x - w.y, (ci.y - cj.y) * w.x - w.z,
+                         (ci.z - cj.z) * w.x - w.w);
+    } else {
+        const VT<D> y = ((const VT<D> *)h.Ys)[h.nb[jj] + q];
+        xs.x = y.x;
+        xs.y = y.y;
+        xs.z = 0.f;
+        if constexpr (D == 3) xs.z = y.z;
+        ws = make_float4(1.f, ci.x - y.x, ci.y - y.y, (D == 3) ? ci.z - xs.z : 0.f);
+    }
+}
+
+// FP32 coupling: one wavefront per target node (all non-root nodes); lanes are (group g,
+// row k): row k of the node's skeleton, the groups split each staged round of 64
+// sources, which every lane reads from LDS.  only_flagged: just the nodes the mixed
+// kernel flagged.
 template <int D>
 __global__ void __launch_bounds__(256)
 couple_kernel(H2Dev h, int n_nodes, int only_flagged)
 {
     constexpr int RMAX = H2Cfg<D>::RMAX;
-    const int lane = threadIdx.x & 63;
-    const int i = 1 + blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n_nodes) return;
+    constexpr int CH = 64;
+    __shared__ float4 s_x[4][CH], s_w[4][CH];
+    __shared__ int s_off[4][65], s_j[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = 1 + blockIdx.x * 4 + wv;
+    if (i >= n_nodes) return;  // per wavefront: only wavefront-level LDS traffic below
     if (only_flagged && h.nflag[i] == 0) return;
     const int r = h.rank[i];
+    if (r <= 0) return;
     const int RS = rank_stride(r), G = 64 / RS;
     const int k = lane % RS, g = lane / RS;
     const float4 ci = h.cent[i];
-    const float4 xk = (k < r) ? h.skelf[(size_t)i * RMAX + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+    const float4 xk = (k < r) ? h.skelf[(size_t)i * RMAX + k] : ci;
     float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-    const int a0 = h.adm_beg[i], a1 = h.adm_end[i];
-    for (int e = a0 + g; e < a1; e += G) {
-        const int j = (int)(uint32_t)h.adm[e];
-        const int rj = h.rank[j];
-        const float4 cj = h.cent[j];
-        const float dcx = ci.x - cj.x, dcy = ci.y - cj.y, dcz = ci.z - cj.z;
-        const float4 *sk = h.skelf + (size_t)j * RMAX;
-        const float4 *wj = h.wc + (size_t)j * RMAX;
-        for (int q = 0; q < rj; q++) {
-            const float4 xl = sk[q];
-            const float4 w = wj[q];
-            const float K = kern32(xk.x - xl.x, xk.y - xl.y, D == 3 ? xk.z - xl.z : 0.f);
-            const float a = K * w.x;
-            acc.x += a;
-            acc.y = fmaf(dcx, a, fmaf(-K, w.y, acc.y));
-            acc.z = fmaf(dcy, a, fmaf(-K, w.z, acc.z));
-            if constexpr (D == 3) acc.w = fmaf(dcz, a, fmaf(-K, w.w, acc.w));
+    const int a0 = h.adm_beg[i], a1 = h.adm_end[i], x0 = h.x_beg[i], x1 = h.x_end[i];
+    const int na = a1 - a0, nm = na + (x1 - x0);
+    for (int m0 = 0; m0 < nm; m0 += 64) {
+        const int T = member_round<D>(h, m0, na, nm, a0, x0, lane, s_off[wv], s_j[wv]);
+        const int nmem = min(64, nm - m0);
+        for (int c0 = 0; c0 < T; c0 += CH) {
+            float4 xs, ws;
+            stage_source<D>(h, ci, m0, na, nmem, c0 + lane, T, s_off[wv], s_j[wv], xs, ws);
+            s_x[wv][lane] = xs;
+            s_w[wv][lane] = ws;
+            __builtin_amdgcn_wave_barrier();
+            const int cn = min(CH, T - c0);
+            for (int sl = g; sl < cn; sl += G) {
+                const float4 q = s_x[wv][sl];
+                const float4 w = s_w[wv][sl];
+                const float K = kern32(xk.x - q.x, xk.y - q.y, D == 3 ? xk.z - q.z : 0.f);
+                acc.x = fmaf(K, w.x, acc.x);
+                acc.y = fmaf(K, w.y, acc.y);
+                acc.z = fmaf(K, w.z, acc.z);
+                if constexpr (D == 3) acc.w = fmaf(K, w.w, acc.w);
+            }
+            __builtin_amdgcn_wave_barrier();  // the next round overwrites the staging
         }
-    }
-    const int x0 = h.x_beg[i], x1 = h.x_end[i];
-    for (int e = x0 + g; e < x1; e += G) {
-        const int j = (int)(uint32_t)h.xl[e];
-        const VT<D> *ys = (const VT<D> *)h.Ys;
-        for (int u = h.nb[j]; u < h.ne[j]; u++) {
-            const VT<D> y = ys[u];
-            float dz = 0.f;
-            if constexpr (D == 3) dz = xk.z - y.z;
-            const float K = kern32(xk.x - y.x, xk.y - y.y, dz);
-            acc.x += K;
-            acc.y = fmaf(K, ci.x - y.x, acc.y);
-            acc.z = fmaf(K, ci.y - y.y, acc.z);
-            if constexpr (D == 3) acc.w = fmaf(K, ci.z - y.z, acc.w);
-        }
+        __builtin_amdgcn_wave_barrier();
     }
     acc = group_sum(acc, RS);
     if (g == 0 && k < r) h.zc[(size_t)i * RMAX + k] = acc;
@@ -157,55 +211,16 @@ couple_mfma_kernel(H2Dev h, int n_nodes, float tau)
     const float4 ci = h.cent[i];
     const int tr = 4 * g + u;
     const float4 xk = (tr < r) ? h.skelf[(size_t)i * RMAX + tr] : ci;
-    const VT<D> *ys = (const VT<D> *)h.Ys;
     v4f acc = {0.f, 0.f, 0.f, 0.f}, acc2 = {0.f, 0.f, 0.f, 0.f};
     const int a0 = h.adm_beg[i], a1 = h.adm_end[i], x0 = h.x_beg[i], x1 = h.x_end[i];
     const int na = a1 - a0, nm = na + (x1 - x0);
     for (int m0 = 0; m0 < nm; m0 += 64) {
-        // this round's members (one per lane) and their source counts, scanned
-        const int m = m0 + lane;
-        int sz = 0, j = -1;
-        if (m < nm) {
-            j = (m < na) ? (int)(uint32_t)h.adm[a0 + m] : (int)(uint32_t)h.xl[x0 + m - na];
-            sz = (m < na) ? h.rank[j] : (h.ne[j] - h.nb[j]);
-        }
-        int x = sz;
-        for (int off = 1; off < 64; off <<= 1) {
-            const int y = __shfl_up(x, off, 64);
-            if (lane >= off) x += y;
-        }
-        const int T = __shfl(x, 63, 64);
+        const int T = member_round<D>(h, m0, na, nm, a0, x0, lane, s_off[wv], s_j[wv]);
         const int nmem = min(64, nm - m0);
-        s_off[wv][lane + 1] = x;
-        if (lane == 0) s_off[wv][0] = 0;
-        s_j[wv][lane] = j;
-        __builtin_amdgcn_wave_barrier();
         for (int c0 = 0; c0 < T; c0 += CH) {
             // stage sources c0 .. c0 + CH - 1 (zero rows past the end)
-            const int sidx = c0 + lane;
-            float4 xs = ci, ws = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (sidx < T) {
-                int lo = 0, hi = nmem;  // member a with off[a] <= sidx < off[a+1]
-                while (hi - lo > 1) {
-                    const int mid = (lo + hi) >> 1;
-                    if (s_off[wv][mid] <= sidx) lo = mid; else hi = mid;
-                }
-                const int jj = s_j[wv][lo], q = sidx - s_off[wv][lo];
-                if (m0 + lo < na) {
-                    const float4 cj = h.cent[jj];
-                    xs = h.skelf[(size_t)jj * RMAX + q];
-                    const float4 w = h.wc[(size_t)jj * RMAX + q];
-                    ws = make_float4(w.x, (ci.x - cj.x) * w.x - w.y, (ci.y - cj.y) * w.x - w.z,
-                                     (ci.z - cj.z) * w.x - w.w);
-                } else {
-                    const VT<D> y = ys[h.nb[jj] + q];
-                    xs.x = y.x;
-                    xs.y = y.y;
-                    xs.z = 0.f;
-                    if constexpr (D == 3) xs.z = y.z;
-                    ws = make_float4(1.f, ci.x - y.x, ci.y - y.y, (D == 3) ? ci.z - xs.z : 0.f);
-                }
-            }
+            float4 xs, ws;
+            stage_source<D>(h, ci, m0, na, nmem, c0 + lane, T, s_off[wv], s_j[wv], xs, ws);
             s_x[wv][lane] = xs;
             {
                 const v4f wv4 = {ws.x, ws.y, ws.z, ws.w};
@@ -326,11 +341,15 @@ leaf_kernel(H2Dev h, const int *__restrict__ leaves, float *__restrict__ s_out,
     const int lane = threadIdx.x;
     const int i = leaves[blockIdx.x];
     const int b = h.nb[i];
-    const int m = min(h.ne[i] - b, kLeafCap);
+    const int mtot = min(h.ne[i] - b, H2Cfg<D>::LEAF_CAP);
     const VT<D> *ys = (const VT<D> *)h.Ys;
     const float4 ci = h.cent[i];
-    const bool two = m > 64;
-    const int t0 = lane, t1 = lane + 64;
+    // targets in chunks of kLeafCap points (one chunk but for the oversized leaves of
+    // the deepest level); each chunk walks the near and W lists
+    for (int tb = 0; tb < mtot; tb += kLeafCap) {
+    const int m = min(mtot - tb, kLeafCap) + tb;  // targets tb .. m-1 of this chunk
+    const bool two = m - tb > 64;
+    const int t0 = tb + lane, t1 = tb + lane + 64;
     VT<D> y0 = ys[b + min(t0, m - 1)];
     VT<D> y1 = ys[b + min(t1, m - 1)];
     float s0 = 0.f, s1 = 0.f;
@@ -364,8 +383,10 @@ leaf_kernel(H2Dev h, const int *__restrict__ leaves, float *__restrict__ s_out,
     // near field: points <- points
     for (int e = h.nl_beg[i]; e < h.nl_end[i]; e++) {
         const int j = (int)(uint32_t)h.nl[e];
-        const int bj = h.nb[j];
-        const int mj = min(h.ne[j] - bj, kLeafCap);
+        const int mjt = min(h.ne[j] - h.nb[j], H2Cfg<D>::LEAF_CAP);
+        for (int sb = 0; sb < mjt; sb += kLeafCap) {
+        const int bj = h.nb[j] + sb;
+        const int mj = min(mjt - sb, kLeafCap);
         __syncthreads();
         for (int u = lane; u < mj; u += 64) s_src[u] = ys[bj + u];
         __syncthreads();
@@ -399,6 +420,7 @@ leaf_kernel(H2Dev h, const int *__restrict__ leaves, float *__restrict__ s_out,
                 if constexpr (D == 3) { f0z = fmaf(K0, dz0, f0z); f1z = fmaf(K1, dz1, f1z); }
             }
         }
+        }  // source chunks
     }
     // W: points <- skeleton of a finer node
     for (int e = h.w_beg[i]; e < h.w_end[i]; e++) {
@@ -449,6 +471,7 @@ leaf_kernel(H2Dev h, const int *__restrict__ leaves, float *__restrict__ s_out,
         F_out[(size_t)o * D + 1] = f1y;
         if constexpr (D == 3) F_out[(size_t)o * D + 2] = f1z;
     }
+    }  // target chunks
 }
 
 int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, float tau, hipStream_t st)

@@ -324,7 +324,7 @@ split_count_kernel(H2Dev h, int l, int nbound)
         h.child0[i] = -1;
         h.nchild[i] = 0;
         h.leaves[lbase + __popcll(ml & ((1ull << (threadIdx.x & 63)) - 1ull))] = i;
-        if (e - b > kLeafCap) atomicOr(&h.flags[FLAG_BIG_LEAF], 1);
+        if (e - b > H2Cfg<D>::LEAF_CAP) atomicOr(&h.flags[FLAG_BIG_LEAF], 1);
         return;
     }
     int *bdp = h.bounds + (size_t)(i - lb) * (NCH + 1);

@@ -508,9 +508,11 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
     }
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
     {
-        // TSNE_OVERLAP=0: no second stream, every attractive pass fused with its update
+        // the attractive pass of rebuild iterations on a second stream: from 2^18 points
+        // on (below, the two extra launches and the event waits cost more than the
+        // overlap saves); TSNE_OVERLAP=0 / 1 turns it off / on for any size
         const char *e = getenv("TSNE_OVERLAP");
-        ws->overlap = !(e && e[0] == '0');
+        ws->overlap = e ? (e[0] != '0') : (n_point >= (1 << 18));
     }
     bool ok = hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
     if (ok && ws->overlap)

@@ -84,6 +84,7 @@ class H2Stats(ctypes.Structure):
         ('rank_max', ctypes.c_int), ('rank_mean', ctypes.c_double),
         ('proxy_max', ctypes.c_int), ('far_flops', ctypes.c_double),
         ('near_flops', ctypes.c_double), ('mixed_fallback_nodes', ctypes.c_int),
+        ('direct_fallbacks', ctypes.c_int),
     ]
 
 
