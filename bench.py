@@ -77,8 +77,9 @@ def _child(args):
 
 
 def make_inputs(n, dim_in, n_clusters, k, perplexity, seed=0):
-    """Seeded Gaussian mixture -> exact k-NN (torch, child process) -> conditional P (the
-    library's perplexity kernel) -> symmetric P (torch, child process).
+    """Seeded Gaussian mixture -> exact k-NN (torch, child process) -> joint P on the GPU
+    (the library's perplexity kernel, symmetrisation and normalisation, equal bit for bit
+    to the reference's scipy arithmetic).
     Returns (row_ptr, col_idx, values) int32/int32/float64 of the joint P."""
     import tempfile
     from tsne_h2pack._backend import get_backend
@@ -91,17 +92,12 @@ def make_inputs(n, dim_in, n_clusters, k, perplexity, seed=0):
     dist = np.load(out + "_dist.npy").astype(np.float64)
     indptr = np.arange(0, (n + 1) * k, k, dtype=np.int32)
     t0 = time.time()
-    Pc = get_backend().conditional_P(np.ascontiguousarray(dist.ravel()),
+    P = get_backend().joint_P_device(np.ascontiguousarray(dist.ravel()),
                                      np.ascontiguousarray(idx.ravel().astype(np.int32)), indptr,
                                      perplexity)
-    log(f"conditional P on the GPU (library kernel): {time.time() - t0:.2f}s")
-    # symmetrise and normalise with scipy, as the reference does (_backend.py:709-718)
-    t0 = time.time()
-    P = Pc + Pc.T
-    P.data /= P.sum()
     csr = (P.indptr.astype(np.int32), P.indices.astype(np.int32), P.data.astype(np.float64))
-    del Pc, P
-    log(f"symmetric P (scipy) nnz={csr[1].size} ({time.time() - t0:.1f}s)")
+    del P
+    log(f"joint P on the GPU (library kernels) nnz={csr[1].size}: {time.time() - t0:.2f}s")
     import shutil
     shutil.rmtree(tmp, ignore_errors=True)
     return csr
@@ -360,21 +356,7 @@ def _torch_knn_many(spec_file, out_dir):
         _torch_knn(sp["n"], 50, sp["clusters"], 90, sp["seed"], os.path.join(out_dir, f"j{j}"))
 
 
-def _symmetrise_job(args):
-    """Pool worker: joint P of one job from its conditional P files."""
-    path, n = args
-    from scipy.sparse import csr_matrix
-    z = np.load(path)
-    Pc = csr_matrix((z["p"], z["idx"], z["ptr"]), shape=(n, n))
-    P = Pc + Pc.T
-    P.data /= P.sum()
-    np.savez(path.replace("_pc.npz", "_P.npz"), ptr=P.indptr.astype(np.int32),
-             idx=P.indices.astype(np.int32), val=P.data)
-    return path
-
-
 def serving_inputs(specs, tmp):
-    import multiprocessing as mp
     from tsne_h2pack._backend import get_backend
     spec_file = os.path.join(tmp, "specs.json")
     with open(spec_file, "w") as f:
@@ -390,18 +372,11 @@ def serving_inputs(specs, tmp):
         dist = np.load(base + "_dist.npy").astype(np.float64)
         n, k = idx.shape
         indptr = np.arange(0, (n + 1) * k, k, dtype=np.int32)
-        Pc = be.conditional_P(np.ascontiguousarray(dist.ravel()),
+        P = be.joint_P_device(np.ascontiguousarray(dist.ravel()),
                               np.ascontiguousarray(idx.ravel().astype(np.int32)), indptr, 30.0)
-        np.savez(base + "_pc.npz", p=Pc.data, idx=Pc.indices.astype(np.int32),
-                 ptr=Pc.indptr.astype(np.int32))
-    log(f"serving: conditional P (library kernel) in {time.time() - t0:.1f}s")
-    t0 = time.time()
-    ctx = mp.get_context("spawn")
-    with ctx.Pool(min(16, os.cpu_count() or 1)) as pool:
-        list(pool.imap_unordered(_symmetrise_job,
-                                 [(os.path.join(tmp, f"j{j}_pc.npz"), sp["n"])
-                                  for j, sp in enumerate(specs)]))
-    log(f"serving: symmetric P in {time.time() - t0:.1f}s")
+        np.savez(base + "_P.npz", ptr=P.indptr.astype(np.int32), idx=P.indices.astype(np.int32),
+                 val=P.data)
+    log(f"serving: joint P of every job on the GPU (library kernels) in {time.time() - t0:.1f}s")
 
 
 def _load_job_P(tmp, j, n):

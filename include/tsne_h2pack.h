@@ -221,6 +221,21 @@ int tsne_attr_kernel_time_tiled(tsne_workspace_t *workspace, double *total_ms, i
 int tsne_attr_tile_stats(tsne_workspace_t *workspace, int *tile_rows, int *ntiles, int *umax,
                          long long *ucount, int *n_global);
 
+/* [add] Joint P from k-NN distances on the device: the conditional P of
+ * compute_P_from_knn_distances, then P = Pc + Pc^T and P /= sum(P) -- the symmetrisation
+ * the reference does on the host with scipy (tsne_h2pack/_backend.py:695-718), bit for
+ * bit: entries summing to exactly 0 are dropped, the sum is scipy's (rows summed in
+ * order, then numpy's blocked pairwise sum).  Rows must have increasing columns in
+ * [0, n) and no self entries (the reference's neighbour indexing).  Returns a matrix from
+ * sparse_matrix_create (release with sparse_matrix_destroy) or NULL. */
+sparse_matrix_t *tsne_joint_P_from_knn_distances(const double *knn_distances,
+                                                 const int *knn_indices, const int *knn_indptr,
+                                                 int n_samples, int nnz, double perplexity);
+
+/* [add] numpy's float64 sum of n contiguous values (blocks of 8192 summed pairwise, in
+ * order), the normalisation the joint P uses. */
+double tsne_numpy_sum(const double *values, long long n);
+
 /* ------------------------------------------------------------------------------------
  * [add] Job server: continuous batching of independent jobs on one GPU.  No counterpart
  * in the reference (one job per process there).  Jobs are admitted in arrival order

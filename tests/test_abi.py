@@ -110,3 +110,20 @@ def test_job_footprint_and_server_without_device():
     import os
     if not os.path.exists("/dev/kfd"):
         assert not be.lib.tsne_server_create(4, 0)
+
+
+def test_numpy_sum_replica_is_bit_exact():
+    """tsne_numpy_sum (host code of the library, no device needed) reproduces numpy's
+    float64 sum bit for bit: blocks of 8192 summed pairwise, in order.  The joint P's
+    normalisation depends on it."""
+    import ctypes
+
+    import numpy as np
+
+    from tsne_h2pack._backend import get_backend
+    lib = get_backend().lib
+    rng = np.random.default_rng(11)
+    for n in [1, 2, 7, 8, 9, 127, 128, 129, 1000, 8191, 8192, 8193, 10000, 65537, 250001]:
+        a = rng.random(n) * rng.choice([1e-9, 1.0, 1e7], n)
+        got = lib.tsne_numpy_sum(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+        assert got == np.sum(a), n

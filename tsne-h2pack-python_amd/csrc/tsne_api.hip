@@ -12,6 +12,7 @@
 #include "api_internal.h"
 #include "common.h"
 #include "h2.h"
+#include "joint_p.h"
 #include "kernels.h"
 #include "renumber.h"
 #include "tiles.h"
@@ -418,6 +419,70 @@ sparse_matrix_t *compute_P_from_knn_distances(const double *knn_distances, const
         return nullptr;
     }
     return P;
+}
+
+sparse_matrix_t *tsne_joint_P_from_knn_distances(const double *knn_distances,
+                                                 const int *knn_indices, const int *knn_indptr,
+                                                 int n_samples, int nnz, double perplexity)
+{
+    bool args_ok = knn_distances && knn_indices && knn_indptr && n_samples > 0 && nnz >= 0 &&
+                   knn_indptr[0] == 0 && knn_indptr[n_samples] == nnz && perplexity > 0.0;
+    for (int i = 0; args_ok && i < n_samples; i++) {
+        args_ok = knn_indptr[i] <= knn_indptr[i + 1];
+        for (int e = knn_indptr[i]; args_ok && e < knn_indptr[i + 1]; e++)
+            args_ok = knn_indices[e] >= 0 && knn_indices[e] < n_samples &&
+                      (e == knn_indptr[i] || knn_indices[e] > knn_indices[e - 1]);
+    }
+    if (!args_ok) {
+        fprintf(stderr, "tsne_joint_P_from_knn_distances: invalid arguments (rows must have "
+                        "increasing columns in [0, n))\n");
+        return nullptr;
+    }
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        fprintf(stderr, "tsne_joint_P_from_knn_distances: no HIP device available\n");
+        return nullptr;
+    }
+    double *d_dist = nullptr, *d_p = nullptr;
+    int *d_ptr = nullptr, *d_col = nullptr;
+    JointP J;
+    sparse_matrix_t *P = nullptr;
+    bool ok = dalloc(&d_dist, nnz, st) == 0 && dalloc(&d_p, nnz, st) == 0 &&
+              dalloc(&d_ptr, (size_t)n_samples + 1, st) == 0 && dalloc(&d_col, nnz, st) == 0 &&
+              hipMemcpyAsync(d_dist, knn_distances, (size_t)nnz * sizeof(double),
+                             hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_col, knn_indices, (size_t)nnz * sizeof(int),
+                             hipMemcpyHostToDevice, st) == hipSuccess &&
+              hipMemcpyAsync(d_ptr, knn_indptr, ((size_t)n_samples + 1) * sizeof(int),
+                             hipMemcpyHostToDevice, st) == hipSuccess &&
+              (nnz == 0 || launch_perplexity(n_samples, d_dist, d_ptr, perplexity, d_p, st) == 0) &&
+              joint_P_device(n_samples, nnz, d_ptr, d_col, d_p, &J, st) == 0;
+    if (ok) {
+        P = sparse_matrix_create(n_samples, n_samples, J.nnz);
+        ok = P && hipMemcpyAsync(P->row_ptr, J.rp, ((size_t)n_samples + 1) * sizeof(int),
+                                 hipMemcpyDeviceToHost, st) == hipSuccess &&
+             (J.nnz == 0 ||
+              (hipMemcpyAsync(P->col_idx, J.col, (size_t)J.nnz * sizeof(int),
+                              hipMemcpyDeviceToHost, st) == hipSuccess &&
+               hipMemcpyAsync(P->values, J.val, (size_t)J.nnz * sizeof(double),
+                              hipMemcpyDeviceToHost, st) == hipSuccess)) &&
+             hipStreamSynchronize(st) == hipSuccess;
+    }
+    dfree(d_dist, st); dfree(d_p, st); dfree(d_ptr, st); dfree(d_col, st);
+    dfree(J.rp, st); dfree(J.col, st); dfree(J.val, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (!ok) {
+        fprintf(stderr, "tsne_joint_P_from_knn_distances: device failure\n");
+        if (P) sparse_matrix_destroy(P);
+        return nullptr;
+    }
+    return P;
+}
+
+double tsne_numpy_sum(const double *values, long long n)
+{
+    return (values && n > 0) ? numpy_sum(values, n) : 0.0;
 }
 
 tsne_workspace_t *tsne_init(const double *X, int n_point, int pt_dim, int low_dim,

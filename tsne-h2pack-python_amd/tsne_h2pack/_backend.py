@@ -249,16 +249,37 @@ class TSNEBackend:
 
     @staticmethod
     def symmetrize(P_cond):
-        """P = (P + P^T) / sum (reference _backend.py:709-718, scipy arithmetic)."""
+        """P = (P + P^T) / sum on the host (reference _backend.py:709-718, scipy
+        arithmetic); joint_P does the same on the device."""
         P = P_cond + P_cond.T
         total = P.sum()
         if total > 0:
             P.data /= total
         return P
 
+    def joint_P_device(self, data, indices, indptr, perplexity):
+        """tsne_joint_P_from_knn_distances: conditional P, symmetrisation and
+        normalisation on the GPU, equal bit for bit to symmetrize(conditional_P(...))."""
+        from scipy.sparse import csr_matrix
+        n = indptr.size - 1
+        ptr = self._lib.tsne_joint_P_from_knn_distances(
+            data.ctypes.data_as(ctypes.POINTER(DTYPE)),
+            indices.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            indptr.ctypes.data_as(ctypes.POINTER(ctypes.c_int)),
+            n, int(indptr[-1]), ctypes.c_double(perplexity))
+        if not ptr:
+            raise RuntimeError("tsne_joint_P_from_knn_distances returned NULL")
+        m = ptr.contents
+        rp = np.ctypeslib.as_array(m.row_ptr, shape=(m.n_rows + 1,)).copy()
+        ci = np.ctypeslib.as_array(m.col_idx, shape=(max(m.nnz, 1),))[:m.nnz].copy()
+        va = np.ctypeslib.as_array(m.values, shape=(max(m.nnz, 1),))[:m.nnz].copy()
+        self._lib.sparse_matrix_destroy(ptr)
+        return csr_matrix((va, ci, rp), shape=(n, n))
+
     def joint_P(self, knn_distances_csr, perplexity, num_threads=1):
+        """Neighbour indexing on the host, then the joint P on the device."""
         data, indices, indptr = self.prepare_knn_csr(knn_distances_csr)
-        return self.symmetrize(self.conditional_P(data, indices, indptr, perplexity, num_threads))
+        return self.joint_P_device(data, indices, indptr, perplexity)
 
     def init_workspace(self, P, n_components, params, init_Y=None, n_features=0):
         """tsne_init_with_P with a library-owned copy of P; returns the workspace."""
@@ -389,6 +410,12 @@ def configure_signatures(lib):
     lib.tsne_attr_kernel_time.restype = ctypes.c_int
     lib.tsne_attr_kernel_time_tiled.argtypes = [c_ws, dp, ip]
     lib.tsne_attr_kernel_time_tiled.restype = ctypes.c_int
+    lib.tsne_joint_P_from_knn_distances.argtypes = [
+        ctypes.POINTER(DTYPE), ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+        ctypes.c_int, ctypes.c_int, ctypes.c_double]
+    lib.tsne_joint_P_from_knn_distances.restype = c_sm
+    lib.tsne_numpy_sum.argtypes = [ctypes.POINTER(DTYPE), ctypes.c_longlong]
+    lib.tsne_numpy_sum.restype = ctypes.c_double
     lib.tsne_attr_tile_stats.argtypes = [c_ws, ip, ip, ip, ctypes.POINTER(ctypes.c_longlong), ip]
     lib.tsne_attr_tile_stats.restype = ctypes.c_int
     # job server
