@@ -487,6 +487,149 @@ def serving_bench(n_jobs, slots_list, seq_sample, load=0.7):
     }
 
 
+def mixture_points(n, dim_in, n_clusters, seed):
+    """The seeded Gaussian mixture of the k-NN inputs (same draws as _torch_knn)."""
+    rng = np.random.default_rng(seed)
+    centers = rng.normal(scale=5.0, size=(n_clusters, dim_in))
+    w = rng.dirichlet(np.ones(n_clusters))
+    std = rng.uniform(0.5, 2.0, size=n_clusters)
+    labels = rng.choice(n_clusters, size=n, p=w)
+    return centers[labels] + rng.normal(size=(n, dim_in)) * std[labels, None]
+
+
+def knn_bench(n, dim_in, k, clusters, reps, cpu_rows):
+    """Exact k-NN graph on the device (tsne_knn_exact): float64 mixture points, k
+    neighbours.  value = seconds of device work per graph (HIP events from the first
+    preparation kernel to the last kernel, inputs resident); the call's wall time (host
+    validation, PCIe copies both ways) is reported beside it.  Roofline: the MFMA
+    screening kernel, algorithmic flops 2 * 3 * n^2 * D (three BF16 products per pair)."""
+    from tsne_h2pack._backend import get_backend
+    be = get_backend()
+    X = mixture_points(n, dim_in, clusters, 0)
+    be.knn_exact(X[:4096], k)  # load the code objects
+    dev, wall, scr = [], [], []
+    for r in range(reps):
+        t0 = time.perf_counter()
+        idx, dist, st = be.knn_exact(X, k)
+        wall.append(time.perf_counter() - t0)
+        dev.append(st.device_ms * 1e-3)
+        scr.append(st.screen_ms * 1e-3)
+        log(f"knn rep {r}: device {st.device_ms:.1f} ms (screening {st.screen_ms:.1f} ms), "
+            f"wall {wall[-1]:.2f} s, uncertain rows {st.uncertain_rows}")
+    d_s, s_s = float(np.median(dev)), float(np.median(scr))
+    alg = 2.0 * 3.0 * float(n) * float(n) * dim_in
+    ach = alg / s_s / 1e12
+    # spot check of the result against the oracle on a few rows
+    import oracle
+    rows = np.random.default_rng(1).choice(n, size=8, replace=False)
+    ok = True
+    for q in rows:
+        d2 = ((X - X[q]) ** 2).sum(1)
+        order = np.lexsort((np.arange(n), d2))[:k + 1]
+        want = [c for c in order if c != q][:k]
+        ok &= list(idx[q]) == want
+    cpu = None
+    if cpu_rows > 0:
+        sub = np.ascontiguousarray(X[:cpu_rows])
+        t0 = time.perf_counter()
+        oracle.knn_exact(sub, k)
+        t = time.perf_counter() - t0
+        # the oracle is O(n^2 D + n^2 log n): scale the n_s x n_s run by (n / n_s)^2
+        est = t * (n / cpu_rows) ** 2
+        cpu = {"value": est, "unit": "s", "cores": oracle.num_threads(), "kind": "port",
+               "sample": f"oracle FP64 brute force on the first {cpu_rows} points "
+                         f"({t:.1f}s), scaled by (N/n)^2"}
+    return {
+        "metric": f"exact k-NN graph seconds at N={n} D={dim_in} k={k}",
+        "value": d_s, "unit": "s", "n_gpus": 1, "steps": reps, "warmup": 1,
+        "ms_per_step": d_s * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16x3/f64",
+        "data": "synthetic (seeded Gaussian mixture)",
+        "config": {"workload": f"knn N={n} D={dim_in} k={k} clusters={clusters}", "n": n,
+                   "dim_in": dim_in, "k": k},
+        "wall_s": float(np.median(wall)), "screen_s": s_s,
+        "uncertain_rows": int(st.uncertain_rows), "survivors": int(st.survivors),
+        "spot_check_vs_bruteforce": bool(ok),
+        "roofline": {"bound": "mfma", "achieved": ach, "peak": 2500.0, "unit": "TFLOP/s",
+                     "frac": ach / 2500.0, "traffic": None, "kernel": "knn_screen_kernel",
+                     "algorithmic_flops": alg, "executed_flops": int(st.screen_flops)},
+        "cpu_baseline": cpu,
+    }
+
+
+def pipeline_bench(n, dim, k, perplexity, clusters, cpu_quality):
+    """Whole path on the device for one dataset: exact k-NN graph (tsne_knn_exact), the
+    neighbour indexing of the reference (columns sorted per row, host), joint P
+    (tsne_joint_P_from_knn_distances), the 1000-iteration optimisation (plain), and the
+    quality score of the result (tsne_quality_score, the reference's scorer with k=10,
+    5000 queries, 1000 anchors).  Seconds per stage, wall clock around each call."""
+    from tsne_h2pack._backend import get_backend
+    from tsne_h2pack.quality import score
+    be = get_backend()
+    X = mixture_points(n, 50, clusters, 0)
+    be.knn_exact(X[:4096], k)  # load the code objects
+    stages = {}
+    t0 = time.perf_counter()
+    idx, dist, kst = be.knn_exact(X, k)
+    stages["knn_s"] = time.perf_counter() - t0
+    stages["knn_device_s"] = kst.device_ms * 1e-3
+    t0 = time.perf_counter()
+    order = np.argsort(idx, axis=1, kind="stable")
+    cols = np.ascontiguousarray(np.take_along_axis(idx, order, 1)).ravel()
+    dd = np.ascontiguousarray(np.take_along_axis(dist, order, 1), dtype=np.float64).ravel()
+    indptr = np.arange(0, (n + 1) * k, k, dtype=np.int32)
+    stages["indexing_s"] = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    P = be.joint_P_device(dd, cols.astype(np.int32), indptr, perplexity)
+    stages["joint_P_s"] = time.perf_counter() - t0
+    csr = (P.indptr.astype(np.int32), P.indices.astype(np.int32), P.data)
+    t0 = time.perf_counter()
+    ws = make_workspace(be, csr, n, dim, "plain", 1000, 100, 1e-2)
+    stages["init_s"] = time.perf_counter() - t0
+    try:
+        t0 = time.perf_counter()
+        assert be.lib.tsne_run(ws) == 0
+        be.lib.tsne_synchronize(ws)
+        stages["optimise_s"] = time.perf_counter() - t0
+        Y = np.zeros((dim, n))
+        be.lib.tsne_get_embedding(ws, Y)
+        kl_lib = float(be.lib.compute_kl_divergence(ws))
+    finally:
+        be.lib.tsne_destroy(ws)
+    Y = Y.T.copy()
+    t0 = time.perf_counter()
+    q = score(Y, idx[:, :10], P, k=10)
+    stages["quality_s"] = time.perf_counter() - t0
+    total = sum(v for key, v in stages.items() if key != "knn_device_s")
+    cpu = None
+    if cpu_quality:
+        import oracle
+        r = np.random.default_rng(0)
+        qs = r.choice(n, size=5000, replace=False)
+        an = r.choice(n, size=1000, replace=False)
+        t0 = time.perf_counter()
+        qo = oracle.quality_score(Y, idx[:, :10], qs, P, an, k=10)
+        cpu = {"value": time.perf_counter() - t0, "unit": "s", "cores": 1, "kind": "port",
+               "sample": "the oracle's restatement of the reference's scorer (numpy + sklearn "
+                         "KDTree) on the same embedding, for the quality stage only",
+               "agrees": bool(qo["knn_preservation"] == q["knn_preservation"]
+                              and abs(qo["kl"] - q["kl"]) < 1e-9 * abs(qo["kl"]))}
+    log(f"pipeline stages {json.dumps(stages)} quality {q}")
+    return {
+        "metric": f"end-to-end seconds at N={n} {dim}D (exact k-NN k={k} + joint P + 1000 "
+                  f"iterations + quality score)",
+        "value": total, "unit": "s", "n_gpus": 1, "steps": 1000, "warmup": 0,
+        "ms_per_step": None, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32/f64",
+        "data": "synthetic (seeded Gaussian mixture, 50 features)",
+        "config": {"workload": f"pipeline N={n} D=50 {dim}D k={k} perplexity={perplexity:g} "
+                               f"clusters={clusters}", "n": n, "dim": dim, "k": k},
+        "stages": stages, "quality": q, "kl_library": kl_lib,
+        "knn_uncertain_rows": int(kst.uncertain_rows), "nnz_P": int(P.nnz),
+        "cpu_baseline": cpu,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -513,6 +656,13 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for this process (serving; 0 leaves the default)")
     ap.add_argument("--seq-sample", type=int, default=32)
+    ap.add_argument("--knn", action="store_true",
+                    help="exact k-NN graph on the GPU (seconds per graph) instead of the "
+                         "iteration rate")
+    ap.add_argument("--pipeline", action="store_true",
+                    help="k-NN + joint P + optimisation + quality score, seconds per stage")
+    ap.add_argument("--knn-cpu-rows", type=int, default=20000,
+                    help="points of the oracle's CPU baseline run for --knn (0: none)")
     ap.add_argument("--tile-rows", type=int, default=-1,
                     help="rows per attractive tile (0: per-row kernel only; default: library's)")
     a = ap.parse_args()
@@ -522,6 +672,16 @@ def main():
         print("bench.py measures one GPU: this build has no multi-GPU path "
               "(the reference is single-process; see DESIGN.md on sharding)", file=sys.stderr)
         sys.exit(2)
+    if a.pipeline:
+        clusters = a.clusters if a.clusters > 0 else (100 if a.n >= 1_000_000 else 20)
+        print(json.dumps(pipeline_bench(a.n, a.dim, a.k, a.perplexity, clusters,
+                                        not a.no_cpu_baseline)), flush=True)
+        return
+    if a.knn:
+        clusters = a.clusters if a.clusters > 0 else (100 if a.n >= 1_000_000 else 20)
+        print(json.dumps(knn_bench(a.n, 50, a.k, clusters, max(1, min(a.steps, 5)),
+                                   0 if a.no_cpu_baseline else a.knn_cpu_rows)), flush=True)
+        return
     if a.serving:
         if a.hw_queues:
             os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(a.hw_queues, 1), 16))

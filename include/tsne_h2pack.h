@@ -236,6 +236,50 @@ sparse_matrix_t *tsne_joint_P_from_knn_distances(const double *knn_distances,
  * order), the normalisation the joint P uses. */
 double tsne_numpy_sum(const double *values, long long n);
 
+/* [add] Exact Euclidean k-NN graph on the device.  Replaces the reference's exact search
+ * (tsne_h2pack/_knn_methods.py:71-119, ExactKNN: sklearn NearestNeighbors(algorithm=
+ * 'brute').kneighbors_graph(mode='distance'), data cast to float32).  X is n_samples x
+ * n_features float64, row-major.  Row i of indices / distances (n_samples x k) holds the k
+ * nearest other points in increasing distance (ties by index) and their float32 L2
+ * distances; the point itself is removed as sklearn removes it (or the nearest entry when
+ * more than k duplicates precede it).  Neighbour sets are exact in FP64: a BF16-split MFMA
+ * screening keeps candidates within a rigorous error bound, FP64 ranks them, and rows the
+ * bound cannot settle are re-ranked in FP64.  1 <= k < n_samples, k <= 900; X finite.
+ * stats may be NULL.  0 on success, -1 otherwise (message on stderr). */
+typedef struct {
+    int survivors;       /* candidates per point kept from the screening */
+    int uncertain_rows;  /* rows re-ranked by the FP64 fallback */
+    int overflow_rows;   /* of those, rows ranked by a full sort of their distances */
+    double eps;          /* screening error bound relative to |x|^2 + |y|^2 (centred) */
+    double device_ms;    /* device time, first preparation kernel to the last kernel */
+    double screen_ms;    /* of which the MFMA screening launches */
+    long long screen_flops;  /* BF16 MFMA flops of the screening */
+} tsne_knn_stats_t;
+int tsne_knn_exact(const double *X, int n_samples, int n_features, int k, int *indices,
+                   float *distances, tsne_knn_stats_t *stats);
+
+/* [add] Embedding quality on a common footing, on the device: the reference's score()
+ * (benchmarks/quality.py:56-81).  Y: n x dim float64 row-major embedding; hd_idx: n x hd_k
+ * high-dimensional neighbours per point in increasing distance (the shared k-NN graph, self
+ * removed); queries / anchors: point indices (the reference draws them with
+ * numpy default_rng(0).choice); P: the common joint P (CSR, int32 row pointers).
+ *   knn_preservation = mean over queries of |hd[i, :k] & ld(i)| / k, ld(i) = the k+1
+ *     nearest embedding points of y_i (exact, FP64; ties by index) without the first;
+ *   Z = n * mean over anchors a of sum_{j != a} 1 / (1 + |y_a - y_j|^2);
+ *   kl = sum p log p + sum p log1p(|y_i - y_j|^2) + log Z over the nonzeros of P;
+ *   span = the largest coordinate range.
+ * 1 <= k <= 15, k <= hd_k.  0 on success, -1 otherwise (message on stderr). */
+typedef struct {
+    double kl;
+    double knn_preservation;
+    double Z;
+    double span;
+} tsne_quality_t;
+int tsne_quality_score(const double *Y, int n, int dim, const int *hd_idx, int hd_k, int k,
+                       const int *queries, int n_queries, const int *P_row_ptr,
+                       const int *P_col_idx, const double *P_values, const int *anchors,
+                       int n_anchors, tsne_quality_t *out);
+
 /* ------------------------------------------------------------------------------------
  * [add] Job server: continuous batching of independent jobs on one GPU.  No counterpart
  * in the reference (one job per process there).  Jobs are admitted in arrival order

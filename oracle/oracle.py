@@ -60,6 +60,10 @@ def lib():
                                  ctypes.c_double, ctypes.c_int, ctypes.c_double,
                                  ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.oracle_run.restype = ctypes.c_double
+        L.oracle_knn_exact.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _ip,
+                                       np.ctypeslib.ndpointer(dtype=np.float32,
+                                                              flags="C_CONTIGUOUS")]
+        L.oracle_knn_exact.restype = None
         L.oracle_num_threads.argtypes = []
         L.oracle_num_threads.restype = ctypes.c_int
         _lib = L
@@ -158,3 +162,39 @@ def run(P, Y0, n_iter=1000, lr=None, mom0=0.5, mom1=0.8, mom_switch=250, exag=12
 
 def num_threads():
     return lib().oracle_num_threads()
+
+
+def knn_exact(X, k):
+    """Exact k-NN graph rows (indices (n, k) int32, float32 distances), sklearn's
+    kneighbors_graph(mode='distance') order: increasing distance, ties by index."""
+    X = np.ascontiguousarray(X, dtype=np.float64)
+    n, d = X.shape
+    idx = np.empty((n, k), dtype=np.int32)
+    dist = np.empty((n, k), dtype=np.float32)
+    lib().oracle_knn_exact(X, n, d, k, idx, dist)
+    return idx, dist
+
+
+def quality_score(Y, hd, queries, P, anchors, k=10):
+    """The reference's common-footing score (benchmarks/quality.py:56-81) with its random
+    draws passed in: kNN-preservation@k against the high-dimensional neighbour rows hd
+    (distance order), sklearn KDTree in the embedding (k+1 nearest, first column dropped);
+    KL = sum p log p + sum p log1p(d^2) + log Z with Z = n * mean anchor row sum.
+    Returns dict(kl, knn_preservation, Z, span)."""
+    from sklearn.neighbors import KDTree
+    Y = np.asarray(Y, dtype=np.float64)
+    n = Y.shape[0]
+    ld = KDTree(Y).query(Y[queries], k=k + 1, return_distance=False)[:, 1:]
+    preserved = np.mean([len(set(hd[i, :k]) & set(ld[r])) for r, i in enumerate(queries)]) / k
+    indptr, indices, p = P.indptr, P.indices, P.data
+    rows = np.repeat(np.arange(n), np.diff(indptr))
+    d2 = np.sum((Y[rows] - Y[indices]) ** 2, axis=1)
+    row_sums = np.empty(len(anchors))
+    for a, i in enumerate(anchors):
+        w = 1.0 / (1.0 + np.sum((Y - Y[i]) ** 2, axis=1))
+        w[i] = 0.0
+        row_sums[a] = w.sum()
+    Z = n * row_sums.mean()
+    kl = float(np.sum(p * np.log(np.maximum(p, 1e-300))) + np.sum(p * np.log1p(d2)) + np.log(Z))
+    return dict(kl=kl, knn_preservation=float(preserved), Z=float(Z),
+                span=float(np.ptp(Y, axis=0).max()))

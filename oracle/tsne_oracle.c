@@ -22,6 +22,9 @@
  *   oracle_run              schedule constants (292 H2 products per 1000 iterations),
  *                           the pins above composed.
  *   oracle_init_random      glibc srand/rand Box-Muller, value checks on N(0,1e-4) moments.
+ *   oracle_knn_exact        sklearn NearestNeighbors(algorithm='brute').kneighbors_graph(
+ *                           mode='distance') (the reference's ExactKNN) on fixtures with
+ *                           and without duplicate points; brute force on tiny inputs.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -330,4 +333,50 @@ int oracle_num_threads(void)
 #else
     return 1;
 #endif
+}
+
+/* Exact k-NN graph, the reference's ExactKNN (tsne_h2pack/_knn_methods.py:71-119):
+ * sklearn's brute-force kneighbors_graph(mode='distance') on the training points, rows in
+ * increasing distance.  sklearn searches k+1 neighbours and removes the point itself, or
+ * the first entry when the point is not among them (more than k duplicates at distance 0).
+ * Squared distances are FP64 sums of squared differences in feature order (fma); ties
+ * are ordered by index.  Distances are returned as float32 sqrt(d2), the reference's
+ * cast.  O(n^2 D) -- small inputs only. */
+static int knn_cmp(const void *a, const void *b)
+{
+    const double *x = (const double *)a, *y = (const double *)b;
+    if (x[0] != y[0]) return x[0] < y[0] ? -1 : 1;
+    return x[1] < y[1] ? -1 : (x[1] > y[1]);
+}
+
+void oracle_knn_exact(const double *X, int n, int D, int k, int *idx, float *dist)
+{
+#pragma omp parallel
+    {
+        double *pairs = (double *)malloc(sizeof(double) * 2 * (size_t)n);
+#pragma omp for schedule(dynamic, 16)
+        for (int q = 0; q < n; q++) {
+            for (int c = 0; c < n; c++) {
+                double s = 0.0;
+                for (int f = 0; f < D; f++) {
+                    const double t = X[(size_t)q * D + f] - X[(size_t)c * D + f];
+                    s = fma(t, t, s);
+                }
+                pairs[2 * c] = s;
+                pairs[2 * c + 1] = (double)c;
+            }
+            qsort(pairs, (size_t)n, 2 * sizeof(double), knn_cmp);
+            int drop = 0;
+            for (int r = 0; r <= k; r++)
+                if ((int)pairs[2 * r + 1] == q) { drop = r; break; }
+            int o = 0;
+            for (int r = 0; r <= k; r++) {
+                if (r == drop) continue;
+                idx[(size_t)q * k + o] = (int)pairs[2 * r + 1];
+                dist[(size_t)q * k + o] = (float)sqrt(pairs[2 * r]);
+                o++;
+            }
+        }
+        free(pairs);
+    }
 }

@@ -10,6 +10,11 @@ library.  Run from the repository root: python tests/golden/make_golden.py
                         reference tests/test_perplexity_calibration.py:42-45), the beta
                         that solves H(beta) = ln(30) per row (brentq, 1e-14), and
                         sklearn's _binary_search_perplexity P.
+  golden_knn.npz        the reference's exact k-NN graph call (NearestNeighbors(
+                        algorithm='brute').kneighbors_graph(mode='distance'), float32
+                        data; reference _knn_methods.py:71-119) on a seeded 50-D
+                        Gaussian mixture (n=1000, k=60) and on data with duplicate
+                        points (n=240, k=6).
 """
 import os
 
@@ -77,7 +82,25 @@ def perplexity_fixture():
                         P_sklearn=P_sk)
 
 
+def knn_fixture():
+    from sklearn.neighbors import NearestNeighbors
+    rng = np.random.default_rng(7)
+    centers = rng.normal(scale=5.0, size=(8, 50))
+    lab = rng.integers(0, 8, 1000)
+    X = centers[lab] + rng.normal(size=(1000, 50)) * rng.uniform(0.5, 2.0, 8)[lab, None]
+    Xd = np.vstack([np.repeat(rng.normal(size=(50, 5)), 4, axis=0), rng.normal(size=(40, 5))])
+    out = {}
+    for name, A, k in (("mix", X, 60), ("dup", Xd, 6)):
+        G = NearestNeighbors(n_neighbors=k, algorithm='brute').fit(A).kneighbors_graph(
+            mode='distance')
+        out[f"{name}_X"] = A
+        out[f"{name}_idx"] = G.indices.reshape(A.shape[0], k).astype(np.int32)
+        out[f"{name}_dist"] = G.data.reshape(A.shape[0], k).astype(np.float32)
+    np.savez_compressed(os.path.join(HERE, "golden_knn.npz"), **out)
+
+
 if __name__ == "__main__":
+    knn_fixture()
     gradient_fixture()
     update_fixture()
     perplexity_fixture()

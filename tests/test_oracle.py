@@ -1,5 +1,7 @@
 """Pins of the CPU oracle (oracle/tsne_oracle.c) against independent implementations,
 the reference's own test fixtures and closed forms.  CPU only."""
+import os
+
 import numpy as np
 import pytest
 from scipy.sparse import csr_matrix
@@ -188,3 +190,98 @@ def test_run_small_decreases_kl():
     _, kl_short = oracle.run(P, Y0, n_iter=300)
     _, kl_long = oracle.run(P, Y0, n_iter=600)
     assert kl_long < kl_short
+
+
+# ---------------------------------------------------------------- exact k-NN graph
+
+def _knn_golden():
+    return np.load(os.path.join(os.path.dirname(__file__), "golden", "golden_knn.npz"))
+
+
+def test_knn_matches_sklearn_brute_fixture():
+    """oracle_knn_exact == the reference's ExactKNN call (sklearn brute kneighbors_graph,
+    float32 data) on a 50-D mixture: same neighbours in the same order, same float32
+    distances."""
+    z = _knn_golden()
+    idx, dist = oracle.knn_exact(z["mix_X"], z["mix_idx"].shape[1])
+    assert np.array_equal(idx, z["mix_idx"])
+    assert np.array_equal(dist, z["mix_dist"])
+
+
+def test_knn_duplicates_match_sklearn_up_to_tie_order():
+    """With duplicate points sklearn orders equal distances arbitrarily; the oracle orders
+    them by index.  Distances agree exactly, every listed neighbour is at its listed
+    distance, no row lists itself, and rows whose boundary is not tied hold the same set."""
+    z = _knn_golden()
+    X, gi, gd = z["dup_X"], z["dup_idx"], z["dup_dist"]
+    n, k = gi.shape
+    idx, dist = oracle.knn_exact(X, k)
+    assert np.array_equal(dist, gd)
+    true = np.sqrt(((X[:, None, :] - X[idx]) ** 2).sum(-1)).astype(np.float32)
+    assert np.allclose(true, dist, rtol=1e-6, atol=1e-7)
+    assert not (idx == np.arange(n)[:, None]).any()
+    full = np.sort(np.sqrt(((X[:, None, :] - X[None, :, :]) ** 2).sum(-1)), axis=1)
+    untied = full[:, k] != full[:, k + 1]  # column 0 is the point itself (distance 0)
+    for i in np.nonzero(untied)[0]:
+        assert set(idx[i]) == set(gi[i])
+
+
+def test_knn_brute_force_tiny():
+    """Pure-Python brute force on 13 points (ties by index, self removed)."""
+    rng = np.random.default_rng(3)
+    X = rng.integers(0, 4, size=(13, 3)).astype(np.float64)  # many exact ties
+    k = 5
+    idx, dist = oracle.knn_exact(X, k)
+    for q in range(13):
+        cand = sorted((sum((X[q, f] - X[c, f]) ** 2 for f in range(3)), c) for c in range(13))
+        cand = cand[:k + 1]
+        drop = next((r for r, (_, c) in enumerate(cand) if c == q), 0)
+        want = [c for r, (_, c) in enumerate(cand) if r != drop]
+        assert idx[q].tolist() == want
+        assert np.allclose(dist[q], [np.sqrt(d) for r, (d, _) in enumerate(cand) if r != drop])
+
+
+# ---------------------------------------------------------------- quality score
+
+def _quality_problem(n=600, seed=2):
+    from datasets import gaussian_mixture, joint_P_oracle, knn_graph
+    X, _ = gaussian_mixture(n, dim=10, n_clusters=4, seed=seed)
+    P = joint_P_oracle(knn_graph(X, 30), 10.0)
+    hd, _ = oracle.knn_exact(X, 10)
+    Y = np.random.default_rng(seed).normal(scale=5.0, size=(n, 2))
+    return X, P, hd, Y
+
+
+def test_quality_with_every_anchor_is_the_exact_kl():
+    """Anchors = every row: Z is the exact sum_{i != j} w_ij (oracle_repulsive_exact) and
+    the KL is the exact KL(P || Q) (oracle_kl, pinned to sklearn), since sum p = 1."""
+    X, P, hd, Y = _quality_problem()
+    n = Y.shape[0]
+    q = oracle.quality_score(Y, hd, np.arange(50), P, np.arange(n))
+    _, Z, _, _ = oracle.repulsive_exact(Y.T.copy())
+    assert abs(q["Z"] / Z - 1) < 1e-12
+    assert abs(q["kl"] - oracle.kl(P, Y.T.copy(), Z)) < 1e-10 * max(1.0, abs(q["kl"]))
+
+
+def test_quality_preservation_limits():
+    """The data itself as the embedding (2-D data) preserves every neighbour; a random
+    relabelling preserves about k/n."""
+    rng = np.random.default_rng(4)
+    X = rng.normal(size=(500, 2))
+    hd, _ = oracle.knn_exact(X, 10)
+    P = csr_matrix(np.full((500, 500), 1.0 / 500 ** 2))
+    q = oracle.quality_score(X, hd, np.arange(500), P, np.arange(10))
+    assert q["knn_preservation"] == 1.0
+    q2 = oracle.quality_score(X[rng.permutation(500)], hd, np.arange(500), P, np.arange(10))
+    assert q2["knn_preservation"] < 0.1
+
+
+def test_quality_kdtree_neighbours_are_exact():
+    """The KDTree query of the scorer equals the brute-force k+1 nearest (no ties)."""
+    from sklearn.neighbors import KDTree
+    Y = np.random.default_rng(9).normal(size=(400, 2))
+    ld = KDTree(Y).query(Y, k=11, return_distance=False)
+    idx, _ = oracle.knn_exact(Y, 10)
+    assert np.array_equal(ld[:, 0], np.arange(400))
+    for i in range(400):
+        assert set(ld[i, 1:]) == set(idx[i])

@@ -88,6 +88,24 @@ class H2Stats(ctypes.Structure):
     ]
 
 
+class KnnStats(ctypes.Structure):
+    """tsne_knn_stats_t."""
+    _fields_ = [
+        ('survivors', ctypes.c_int), ('uncertain_rows', ctypes.c_int),
+        ('overflow_rows', ctypes.c_int), ('eps', ctypes.c_double),
+        ('device_ms', ctypes.c_double), ('screen_ms', ctypes.c_double),
+        ('screen_flops', ctypes.c_longlong),
+    ]
+
+
+class Quality(ctypes.Structure):
+    """tsne_quality_t."""
+    _fields_ = [
+        ('kl', ctypes.c_double), ('knn_preservation', ctypes.c_double),
+        ('Z', ctypes.c_double), ('span', ctypes.c_double),
+    ]
+
+
 class JobInfo(ctypes.Structure):
     """tsne_job_info_t."""
     _fields_ = [
@@ -276,6 +294,46 @@ class TSNEBackend:
         self._lib.sparse_matrix_destroy(ptr)
         return csr_matrix((va, ci, rp), shape=(n, n))
 
+    def knn_exact(self, X, k):
+        """tsne_knn_exact: exact Euclidean k-NN on the GPU.  Returns (indices (n, k) int32,
+        distances (n, k) float32, KnnStats); rows in increasing distance, the point itself
+        removed as sklearn's kneighbors_graph removes it."""
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        if X.ndim != 2:
+            raise ValueError("X must be 2-D (n_samples, n_features)")
+        n, d = X.shape
+        idx = np.empty((n, k), dtype=np.int32)
+        dist = np.empty((n, k), dtype=np.float32)
+        st = KnnStats()
+        rc = self._lib.tsne_knn_exact(X, n, d, int(k), idx, dist, ctypes.byref(st))
+        if rc != 0:
+            raise RuntimeError("tsne_knn_exact failed (see stderr)")
+        return idx, dist, st
+
+    def quality_score(self, Y, hd, queries, P, anchors, k=10):
+        """tsne_quality_score: the reference's common-footing quality score on the GPU
+        (benchmarks/quality.py:56-81) with its random draws passed in.  Y (n, dim); hd (n,
+        >= k) high-dimensional neighbours in distance order; P scipy CSR.  Returns dict(kl,
+        knn_preservation, Z, span)."""
+        Y = np.ascontiguousarray(Y, dtype=np.float64)
+        n, dim = Y.shape
+        hd = np.ascontiguousarray(hd, dtype=np.int32)
+        q = np.ascontiguousarray(queries, dtype=np.int32)
+        an = np.ascontiguousarray(anchors, dtype=np.int32)
+        rp = np.ascontiguousarray(P.indptr, dtype=np.int32)
+        ci = np.ascontiguousarray(P.indices, dtype=np.int32)
+        va = np.ascontiguousarray(P.data, dtype=np.float64)
+        out = Quality()
+        i32 = ctypes.POINTER(ctypes.c_int)
+        rc = self._lib.tsne_quality_score(
+            Y.ctypes.data_as(ctypes.POINTER(DTYPE)), n, dim, hd.ctypes.data_as(i32), hd.shape[1],
+            int(k), q.ctypes.data_as(i32), q.size, rp.ctypes.data_as(i32), ci.ctypes.data_as(i32),
+            va.ctypes.data_as(ctypes.POINTER(DTYPE)), an.ctypes.data_as(i32), an.size,
+            ctypes.byref(out))
+        if rc != 0:
+            raise RuntimeError("tsne_quality_score failed (see stderr)")
+        return dict(kl=out.kl, knn_preservation=out.knn_preservation, Z=out.Z, span=out.span)
+
     def joint_P(self, knn_distances_csr, perplexity, num_threads=1):
         """Neighbour indexing on the host, then the joint P on the device."""
         data, indices, indptr = self.prepare_knn_csr(knn_distances_csr)
@@ -418,6 +476,18 @@ def configure_signatures(lib):
     lib.tsne_numpy_sum.restype = ctypes.c_double
     lib.tsne_attr_tile_stats.argtypes = [c_ws, ip, ip, ip, ctypes.POINTER(ctypes.c_longlong), ip]
     lib.tsne_attr_tile_stats.restype = ctypes.c_int
+    lib.tsne_knn_exact.argtypes = [
+        f64c, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        np.ctypeslib.ndpointer(dtype=np.int32, flags='C_CONTIGUOUS'),
+        np.ctypeslib.ndpointer(dtype=np.float32, flags='C_CONTIGUOUS'),
+        ctypes.POINTER(KnnStats)]
+    lib.tsne_knn_exact.restype = ctypes.c_int
+    i32p = ctypes.POINTER(ctypes.c_int)
+    lib.tsne_quality_score.argtypes = [
+        ctypes.POINTER(DTYPE), ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, ctypes.c_int,
+        i32p, ctypes.c_int, i32p, i32p, ctypes.POINTER(DTYPE), i32p, ctypes.c_int,
+        ctypes.POINTER(Quality)]
+    lib.tsne_quality_score.restype = ctypes.c_int
     # job server
     c_srv = ctypes.c_void_p
     lp = ctypes.POINTER(ctypes.c_longlong)

@@ -64,7 +64,29 @@ class PyNNDescentKNN(ExactKNN):
         return G
 
 
-_METHODS = {'exact': ExactKNN, 'pynndescent': PyNNDescentKNN, 'balltree': ExactKNN}
+class GPUExactKNN:
+    """Exact Euclidean k-NN on the GPU (tsne_knn_exact): the same graph as ExactKNN --
+    sklearn's brute-force kneighbors_graph(mode='distance') with float32 distances
+    (reference _knn_methods.py:71-119) -- with neighbour sets exact in FP64."""
+
+    def compute(self, X, k, n_jobs=1, verbose=False, **kwargs):
+        from scipy.sparse import csr_matrix
+        from ._backend import get_backend
+        t0 = time.time()
+        X = np.asarray(X, dtype=np.float64)
+        n = X.shape[0]
+        idx, dist, st = get_backend().knn_exact(X, k)
+        indptr = np.arange(0, (n + 1) * k, k, dtype=np.int64)
+        G = csr_matrix((dist.ravel(), idx.ravel(), indptr), shape=(n, n))
+        G.knn_time = time.time() - t0
+        if verbose:
+            print(f"[t-SNE kNN] GPU exact search: {G.knn_time:.3f}s "
+                  f"({st.uncertain_rows} rows re-ranked in FP64)")
+        return G
+
+
+_METHODS = {'exact': ExactKNN, 'pynndescent': PyNNDescentKNN, 'balltree': ExactKNN,
+            'gpu': GPUExactKNN, 'exact_gpu': GPUExactKNN}
 
 
 def get_knn_method(name):
