@@ -11,16 +11,20 @@
 // The neighbour sets are exact in FP64.  Three steps:
 //
 //  1. Screening on MFMA.  Points are centred (FP64 mean) and split into two BF16 terms,
-//     x = xh + xl + O(2^-16 |x|).  d^2 = |x|^2 + |y|^2 - 2 x.y with x.y taken from
-//     xh.yh + xh.yl + xl.yh on v_mfma_f32_16x16x32_bf16 (FP32 accumulation).  Every
-//     estimate is within eps * (|x|^2 + |y|^2) of the exact d^2 (eps from the error
-//     analysis in knn_eps below); the screening key of candidate y is the estimate minus
-//     eps |y|^2, so key - eps |x|^2 <= d^2 <= key + eps |x|^2.
-//     One workgroup owns 256 queries (64 per wave, the operands of 4 x 4 MFMA tiles held
+//     x = xh + xl + O(2^-16 |x|).  d^2 = |x|^2 + |y|^2 - 2 x.y with x.y taken as
+//     xh.yh + xh.yl + xl.yh, one BF16 dot product of packed rows [xh | xh | xl] and
+//     [yh | yl | yh] (3D features, zero padded to a multiple of 32) on
+//     v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Every estimate is within
+//     eps * (|x|^2 + |y|^2) of the exact d^2 (eps from the error analysis in knn_eps
+//     below); the screening key of candidate y is the estimate minus eps |y|^2, so
+//     key - eps |x|^2 <= d^2 <= key + eps |x|^2.
+//     One workgroup owns 128 queries (32 per wave, the operands of 2 x 4 MFMA tiles held
 //     in registers) and streams every candidate through LDS in chunks of 64.  Each query
 //     keeps a threshold and a buffer in HBM: candidates whose key is below the threshold
-//     are appended; a buffer that fills is cut to its L smallest keys (radix select over
-//     the key bits by the owning wave), and the threshold drops to the L-th key.
+//     are appended (the test is two VALU operations per pair; keys are formed only for
+//     the rare chunks where some pair passes); a buffer that fills is cut to its L
+//     smallest keys (radix select over the key bits by the owning wave), and the
+//     threshold drops to the L-th key.
 //  2. Refinement in FP64.  The L survivors of each query get exact FP64 distances
 //     (differences, fixed fma order); the k+1 smallest, ordered by (distance, index),
 //     give the row.  The row is certain when its (k+1)-th exact d^2 is below
@@ -63,14 +67,14 @@ struct DEntry {
     int idx;
 };
 
-// Rigorous bound of |estimate - exact d^2| / (|x|^2 + |y|^2) for K padded features:
-// representation x - (xh + xl) <= 2^-16 |x| per component (two BF16 roundings) gives
-// 2^-14 through the difference and 2^-16 through the dropped xl.yl; FP32 summation of
-// the 3K exact products <= 3K * 2^-24 * 1.02 |x||y|; norms and the final combination add
-// a few 2^-24.  A 10% margin covers the second-order terms.
-double knn_eps(int kpad)
+// Rigorous bound of |estimate - exact d^2| / (|x|^2 + |y|^2) for a packed dot product of
+// K terms (K = 3 D padded): representation x - (xh + xl) <= 2^-16 |x| per component (two
+// BF16 roundings) gives 2^-14 through the difference and 2^-16 through the dropped xl.yl;
+// FP32 summation of the K exact products <= K * 2^-24 * 1.02 |x||y|; norms and the final
+// combination add a few 2^-24.  A 10% margin covers the second-order terms.
+double knn_eps(int kterms)
 {
-    return 1.1 * (std::ldexp(1.0, -14) + std::ldexp(1.0, -16) + 3.1 * kpad * std::ldexp(1.0, -24) +
+    return 1.1 * (std::ldexp(1.0, -14) + std::ldexp(1.0, -16) + 1.05 * kterms * std::ldexp(1.0, -24) +
                   8.0 * std::ldexp(1.0, -24));
 }
 
@@ -111,32 +115,49 @@ col_mean_kernel(const double *__restrict__ part, int nchunk, int n, int D, doubl
     mean[f] = s / (double)n;
 }
 
-// Row i of Xs: per K block of 32*KS features, KS*32 high parts then KS*32 low parts (BF16);
-// rows >= n and features >= D are zero.  nq = |xh + xl|^2, nkey = nq * (1 - eps) (FP32).
+// Packed BF16 rows of point i (K = 32 * KT * nkb terms each, zero padded):
+// A (queries) = [xh | xh | xl], B (candidates) = [xh | xl | xh], so that A_i . B_j =
+// xh_i.yh_j + xh_i.yl_j + xl_i.yh_j.  nq = |xh + xl|^2 and nkey = nq * (1 - eps) in FP32;
+// rows >= n are zero with nkey = +inf (padding candidates never pass a threshold).
 __global__ void __launch_bounds__(256)
 split_kernel(const double *__restrict__ X, const double *__restrict__ mean, int n, int npad, int D,
-             int KS, int nkb, float eps, __bf16 *__restrict__ Xs, float *__restrict__ nq,
-             float *__restrict__ nkey)
+             int K, float eps, __bf16 *__restrict__ XA, __bf16 *__restrict__ XB,
+             float *__restrict__ nq, float *__restrict__ nkey)
 {
     const int i = blockIdx.x;
     if (i >= npad) return;
-    const int seg = 64 * KS, rowlen = seg * nkb, kpad = 32 * KS * nkb;
     __shared__ double red[256];
+    __shared__ float sh[1024], sl[1024];  // D <= 1024 per pass
     double s = 0.0;
-    for (int f = threadIdx.x; f < kpad; f += 256) {
-        float hf = 0.f, lf = 0.f;
-        if (i < n && f < D) {
-            const double x = X[(size_t)i * D + f] - mean[f];
-            const __bf16 h = (__bf16)(float)x;
-            hf = (float)h;
-            const __bf16 lo = (__bf16)(float)(x - (double)hf);
-            lf = (float)lo;
+    for (int f0 = 0; f0 < D; f0 += 1024) {
+        const int nf = min(1024, D - f0);
+        for (int f = threadIdx.x; f < nf; f += 256) {
+            float hf = 0.f, lf = 0.f;
+            if (i < n) {
+                const double x = X[(size_t)i * D + f0 + f] - mean[f0 + f];
+                hf = (float)(__bf16)(float)x;
+                lf = (float)(__bf16)(float)(x - (double)hf);
+            }
+            sh[f] = hf;
+            sl[f] = lf;
+            const double r = (double)hf + (double)lf;
+            s = fma(r, r, s);
         }
-        const int kb = f / (32 * KS), kk = f % (32 * KS);
-        Xs[(size_t)i * rowlen + kb * seg + kk] = (__bf16)hf;
-        Xs[(size_t)i * rowlen + kb * seg + 32 * KS + kk] = (__bf16)lf;
-        const double r = (double)hf + (double)lf;
-        s = fma(r, r, s);
+        __syncthreads();
+        for (int f = threadIdx.x; f < nf; f += 256) {
+            const size_t row = (size_t)i * K;
+            XA[row + f0 + f] = (__bf16)sh[f];
+            XA[row + D + f0 + f] = (__bf16)sh[f];
+            XA[row + 2 * D + f0 + f] = (__bf16)sl[f];
+            XB[row + f0 + f] = (__bf16)sh[f];
+            XB[row + D + f0 + f] = (__bf16)sl[f];
+            XB[row + 2 * D + f0 + f] = (__bf16)sh[f];
+        }
+        __syncthreads();
+    }
+    for (int f = 3 * D + threadIdx.x; f < K; f += 256) {
+        XA[(size_t)i * K + f] = (__bf16)0.f;
+        XB[(size_t)i * K + f] = (__bf16)0.f;
     }
     red[threadIdx.x] = s;
     __syncthreads();
@@ -146,7 +167,7 @@ split_kernel(const double *__restrict__ X, const double *__restrict__ mean, int 
     }
     if (threadIdx.x == 0) {
         nq[i] = (float)red[0];
-        nkey[i] = (float)(red[0] * (1.0 - (double)eps));
+        nkey[i] = (i < n) ? (float)(red[0] * (1.0 - (double)eps)) : INFINITY;
     }
 }
 
@@ -249,16 +270,18 @@ __device__ __forceinline__ float wave_select_cut(KEntry *b, int m, int L, int *_
 
 // ---------------------------------------------------------------- screening (MFMA)
 
-// Workgroup b owns queries q0 + 256 b .. +255 (wave w: +64 w .. +63).  Buffers: entry
+// Workgroup b owns queries q0 + 128 b .. +127 (wave w: +32 w .. +31).  Buffers: entry
 // e of local query j at buf[j * cap + e].  On exit cnt[q] <= L entries and thr[q] hold
-// the survivors and the final threshold (+inf when nothing was ever cut).
-template <int KS>
+// the survivors and the final threshold (+inf when nothing was ever cut).  KT: MFMA
+// k-steps (32 terms each) per K block; the packed rows hold nkb blocks.
+template <int KT>
 __global__ void __launch_bounds__(256)
-knn_screen_kernel(const __bf16 *__restrict__ Xs, const float *__restrict__ nq,
-                  const float *__restrict__ nkey, int n, int nkb, int q0, int L, int cap,
-                  KEntry *__restrict__ buf, int *__restrict__ cnt_out, float *__restrict__ thr_out)
+knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
+                  const float *__restrict__ nq, const float *__restrict__ nkey, int n, int nkb,
+                  int q0, int L, int cap, KEntry *__restrict__ buf, int *__restrict__ cnt_out,
+                  float *__restrict__ thr_out)
 {
-    constexpr int SEG = 64 * KS;       // bf16 per candidate per K block (hi + lo)
+    constexpr int SEG = 32 * KT;       // bf16 per row per K block
     constexpr int LROW = SEG + 8;      // LDS row (16-byte pad: conflict-free b128 reads)
     constexpr int UNITS = SEG / 8;     // 16-byte units per candidate per K block
     constexpr int PER_T = kCC * UNITS / 256;  // 16-byte units each thread stages
@@ -281,18 +304,15 @@ knn_screen_kernel(const __bf16 *__restrict__ Xs, const float *__restrict__ nq,
         thr_lds[w][l] = INFINITY;
     }
 
-    // query operands: row rb*16 + l%16, features s*32 + (l/16)*8 .. +7 of the block
-    bf16x8 ah[kRB][KS], al[kRB][KS];
-    float nqr[kRB][4], thr[kRB][4];
+    // query operands: row rb*16 + l%16, terms s*32 + (l/16)*8 .. +7 of the block
+    bf16x8 a[kRB][KT];
+    float nqr[kRB][4], tq[kRB][4];  // tq = threshold - |x|^2: the pass test is t < tq
     auto load_a = [&](int kb) {
 #pragma unroll
         for (int rb = 0; rb < kRB; rb++) {
-            const __bf16 *p = Xs + (size_t)(qw0 + rb * 16 + (l & 15)) * rowlen + kb * SEG + (l >> 4) * 8;
+            const __bf16 *p = XA + (size_t)(qw0 + rb * 16 + (l & 15)) * rowlen + kb * SEG + (l >> 4) * 8;
 #pragma unroll
-            for (int s = 0; s < KS; s++) {
-                ah[rb][s] = *(const bf16x8 *)(p + s * 32);
-                al[rb][s] = *(const bf16x8 *)(p + KS * 32 + s * 32);
-            }
+            for (int ks = 0; ks < KT; ks++) a[rb][ks] = *(const bf16x8 *)(p + ks * 32);
         }
     };
 #pragma unroll
@@ -300,7 +320,7 @@ knn_screen_kernel(const __bf16 *__restrict__ Xs, const float *__restrict__ nq,
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             nqr[rb][i] = nq[qw0 + rb * 16 + (l >> 4) * 4 + i];
-            thr[rb][i] = INFINITY;
+            tq[rb][i] = INFINITY;
         }
     if (nkb == 1) load_a(0);
 
@@ -312,7 +332,7 @@ knn_screen_kernel(const __bf16 *__restrict__ Xs, const float *__restrict__ nq,
 #pragma unroll
         for (int r = 0; r < PER_T; r++) {
             const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
-            pre[r] = *(const uint4 *)(Xs + (size_t)(c0 + cand) * rowlen + kb * SEG + part * 8);
+            pre[r] = *(const uint4 *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + part * 8);
         }
         if (t < kCC) pre_nk = nkey[c0 + t];
     };
@@ -341,57 +361,50 @@ knn_screen_kernel(const __bf16 *__restrict__ Xs, const float *__restrict__ nq,
                 for (int cb = 0; cb < 4; cb++) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
         if (nkb > 1) load_a(kb);
-        // 16 independent accumulators between two products on the same one
 #pragma unroll
-        for (int ks = 0; ks < KS; ks++) {
-            bf16x8 bh[4], bl[4];
+        for (int ks = 0; ks < KT; ks++) {
+            bf16x8 bv[4];
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++) {
-                const __bf16 *bp = &Bs[b][(cb * 16 + (l & 15)) * LROW + (l >> 4) * 8 + ks * 32];
-                bh[cb] = *(const bf16x8 *)bp;
-                bl[cb] = *(const bf16x8 *)(bp + KS * 32);
-            }
+            for (int cb = 0; cb < 4; cb++)
+                bv[cb] = *(const bf16x8 *)&Bs[b][(cb * 16 + (l & 15)) * LROW + (l >> 4) * 8 + ks * 32];
 #pragma unroll
             for (int cb = 0; cb < 4; cb++)
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++)
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb][ks], bh[cb], acc[rb][cb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-                for (int rb = 0; rb < kRB; rb++)
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[rb][ks], bl[cb], acc[rb][cb], 0, 0, 0);
-#pragma unroll
-            for (int cb = 0; cb < 4; cb++)
-#pragma unroll
-                for (int rb = 0; rb < kRB; rb++)
-                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[rb][ks], bh[cb], acc[rb][cb], 0, 0, 0);
+                    acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][ks], bv[cb], acc[rb][cb], 0, 0, 0);
         }
         if (kb == nkb - 1) {
-            // D layout: lane l holds column l%16, rows (l/16)*4 + i of each 16 x 16 tile
+            // D layout: lane l holds column l%16, rows (l/16)*4 + i of each 16 x 16 tile.
+            // Fast test per pair: |y|^2(1-eps) - 2 x.y < thr - |x|^2 (padding: +inf).
             const int c0 = (s / nkb) * kCC;
-            bool appended = false;
+            float nk[4];
+            bool any = false;
 #pragma unroll
             for (int cb = 0; cb < 4; cb++) {
-                const int cl = cb * 16 + (l & 15), c = c0 + cl;
-                const float nk = nk_lds[b][cl];
-                const bool valid = c < n;
+                nk[cb] = nk_lds[b][cb * 16 + (l & 15)];
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) {
-                        float key = fmaf(-2.f, acc[rb][cb][i], nqr[rb][i] + nk);
-                        key = key > 0.f ? key : 0.f;
-                        if (valid && key < thr[rb][i]) {
-                            const int j = rb * 16 + (l >> 4) * 4 + i;
-                            const int pos = atomicAdd(&cnt_lds[w][j], 1);
-                            wbuf[(size_t)j * cap + pos] = KEntry{key, c};
-                            appended = true;
-                        }
-                    }
+                    for (int i = 0; i < 4; i++) any |= fmaf(-2.f, acc[rb][cb][i], nk[cb]) < tq[rb][i];
             }
-            // cut every buffer that could overflow in the next chunk (cap >= L + 64)
-            if (__ballot(appended)) {
+            if (__ballot(any)) {
+#pragma unroll
+                for (int cb = 0; cb < 4; cb++) {
+                    const int c = c0 + cb * 16 + (l & 15);
+#pragma unroll
+                    for (int rb = 0; rb < kRB; rb++)
+#pragma unroll
+                        for (int i = 0; i < 4; i++) {
+                            if (fmaf(-2.f, acc[rb][cb][i], nk[cb]) < tq[rb][i]) {
+                                float key = fmaf(-2.f, acc[rb][cb][i], nqr[rb][i] + nk[cb]);
+                                key = key > 0.f ? key : 0.f;
+                                const int j = rb * 16 + (l >> 4) * 4 + i;
+                                const int pos = atomicAdd(&cnt_lds[w][j], 1);
+                                wbuf[(size_t)j * cap + pos] = KEntry{key, c};
+                            }
+                        }
+                }
+                // cut every buffer that could overflow in the next chunk (cap >= L + 64)
                 __threadfence_block();
                 wave_sync();
                 unsigned long long full = __ballot(l < kQW && cnt_lds[w][l] > cap - kCC);
@@ -410,7 +423,8 @@ knn_screen_kernel(const __bf16 *__restrict__ Xs, const float *__restrict__ nq,
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) thr[rb][i] = thr_lds[w][rb * 16 + (l >> 4) * 4 + i];
+                    for (int i = 0; i < 4; i++)
+                        tq[rb][i] = thr_lds[w][rb * 16 + (l >> 4) * 4 + i] - nqr[rb][i];
             }
         }
         if (more) stash(b ^ 1);
@@ -418,6 +432,7 @@ knn_screen_kernel(const __bf16 *__restrict__ Xs, const float *__restrict__ nq,
     }
     // final cut to L survivors
     __threadfence_block();
+    wave_sync();
     unsigned long long over = __ballot(l < kQW && cnt_lds[w][l] > L);
     while (over) {
         const int j = __ffsll((long long)over) - 1;
@@ -507,7 +522,10 @@ knn_refine_kernel(const double *__restrict__ X, int n, int D, int k, int q0, int
     if (l == 0) {
         bound[q] = dk1;
         const float T = thr[q];
-        const bool certain = isinf(T) || dk1 < (double)T - eps * (double)nq[q] * 1.0001;
+        // rejected candidates have d^2 >= T - eps |x|^2 up to the FP32 rounding of the
+        // pass test (a few 2^-24 of |x|^2 + T)
+        const double nqd = (double)nq[q], Td = (double)T;
+        const bool certain = isinf(T) || dk1 < Td - eps * nqd * 1.0001 - std::ldexp(1.0, -20) * (nqd + Td);
         if (!certain) unc[atomicAdd(n_unc, 1)] = q;
     }
 }
@@ -593,12 +611,13 @@ template <typename T> void kfree(T *&p, hipStream_t st)
     p = nullptr;
 }
 
-template <int KS>
-void launch_screen(int grid, hipStream_t st, const __bf16 *Xs, const float *nq, const float *nkey, int n,
-                   int nkb, int q0, int L, int cap, KEntry *buf, int *cnt, float *thr)
+template <int KT>
+void launch_screen(int grid, hipStream_t st, const __bf16 *XA, const __bf16 *XB, const float *nq,
+                   const float *nkey, int n, int nkb, int q0, int L, int cap, KEntry *buf, int *cnt,
+                   float *thr)
 {
-    hipLaunchKernelGGL(knn_screen_kernel<KS>, dim3(grid), dim3(256), 0, st, Xs, nq, nkey, n, nkb, q0, L,
-                       cap, buf, cnt, thr);
+    hipLaunchKernelGGL(knn_screen_kernel<KT>, dim3(grid), dim3(256), 0, st, XA, XB, nq, nkey, n, nkb, q0,
+                       L, cap, buf, cnt, thr);
 }
 
 // Host check of the survivor buffers of one screening launch (TSNE_KNN_DEBUG): counts,
@@ -651,10 +670,11 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
     KnnStats S{};
     const int L = knn_survivors(k);
     if (n < 2 || D < 1 || k < 1 || k >= n || L > kLMax) return -2;
-    const int kpad32 = (D + 31) / 32 * 32;
-    const int KS = kpad32 <= 128 ? kpad32 / 32 : 4;
-    const int nkb = (kpad32 + 32 * KS - 1) / (32 * KS);
-    const int kpad = 32 * KS * nkb;
+    // packed rows of 3 D terms: nkb K blocks of KT <= 8 MFMA k-steps
+    const int ksteps = (3 * D + 31) / 32;
+    const int nkb = (ksteps + 7) / 8;
+    const int KT = (ksteps + nkb - 1) / nkb;
+    const int kpad = 32 * KT * nkb;
     const int npad = (n + kQB - 1) / kQB * kQB;
     const int cap = std::max(4 * L, L + 2 * kCC);
     const double eps = knn_eps(kpad);
@@ -664,7 +684,7 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
     const int nchunk = (n + 1023) / 1024;
 
     double *d_part = nullptr, *d_mean = nullptr, *d_bound = nullptr;
-    __bf16 *d_Xs = nullptr;
+    __bf16 *d_XA = nullptr, *d_XB = nullptr;
     float *d_nq = nullptr, *d_nkey = nullptr, *d_thr = nullptr;
     int *d_cnt = nullptr, *d_ctr = nullptr, *d_unc = nullptr, *d_ovf = nullptr;
     KEntry *d_buf = nullptr;
@@ -672,7 +692,7 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
     const long long nchunk_c = (n + kCC - 1) / kCC;
     int rc = -1;
     if (kalloc(&d_part, (size_t)nchunk * D, st) || kalloc(&d_mean, D, st) ||
-        kalloc(&d_Xs, (size_t)npad * 2 * kpad, st) || kalloc(&d_nq, npad, st) || kalloc(&d_nkey, npad, st) ||
+        kalloc(&d_XA, (size_t)npad * kpad, st) || kalloc(&d_XB, (size_t)npad * kpad, st) || kalloc(&d_nq, npad, st) || kalloc(&d_nkey, npad, st) ||
         kalloc(&d_thr, n, st) || kalloc(&d_cnt, n, st) || kalloc(&d_bound, n, st) || kalloc(&d_ctr, 64, st) ||
         kalloc(&d_unc, n, st) || kalloc(&d_ovf, n, st) || kalloc(&d_buf, (size_t)qbatch * cap, st))
         goto done;
@@ -682,17 +702,22 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
     (void)hipEventRecord(ev[0], st);
     hipLaunchKernelGGL(col_partial_kernel, dim3(nchunk), dim3(256), 0, st, d_X, n, D, d_part);
     hipLaunchKernelGGL(col_mean_kernel, dim3((D + 255) / 256), dim3(256), 0, st, d_part, nchunk, n, D, d_mean);
-    hipLaunchKernelGGL(split_kernel, dim3(npad), dim3(256), 0, st, d_X, d_mean, n, npad, D, KS, nkb,
-                       (float)eps, d_Xs, d_nq, d_nkey);
+    hipLaunchKernelGGL(split_kernel, dim3(npad), dim3(256), 0, st, d_X, d_mean, n, npad, D, kpad,
+                       (float)eps, d_XA, d_XB, d_nq, d_nkey);
     for (int q0 = 0; q0 < npad; q0 += qbatch) {
         const int nqb = std::min(qbatch, npad - q0);
         const int grid = nqb / kQB;
         (void)hipEventRecord(ev[2], st);
-        switch (KS) {
-        case 1: launch_screen<1>(grid, st, d_Xs, d_nq, d_nkey, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); break;
-        case 2: launch_screen<2>(grid, st, d_Xs, d_nq, d_nkey, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); break;
-        case 3: launch_screen<3>(grid, st, d_Xs, d_nq, d_nkey, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); break;
-        default: launch_screen<4>(grid, st, d_Xs, d_nq, d_nkey, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); break;
+        switch (KT) {
+#define KNN_SCREEN_CASE(v)                                                                          \
+    case v:                                                                                         \
+        launch_screen<v>(grid, st, d_XA, d_XB, d_nq, d_nkey, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); \
+        break;
+            KNN_SCREEN_CASE(1) KNN_SCREEN_CASE(2) KNN_SCREEN_CASE(3) KNN_SCREEN_CASE(4)
+            KNN_SCREEN_CASE(5) KNN_SCREEN_CASE(6) KNN_SCREEN_CASE(7)
+            default: launch_screen<8>(grid, st, d_XA, d_XB, d_nq, d_nkey, n, nkb, q0, L, cap, d_buf, d_cnt,
+                                      d_thr);
+#undef KNN_SCREEN_CASE
         }
         (void)hipEventRecord(ev[3], st);
         {
@@ -774,11 +799,11 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
     }
     S.survivors = L;
     S.eps = eps;
-    S.screen_flops = 2LL * 3LL * kpad * (long long)npad * (long long)(nchunk_c * kCC);
+    S.screen_flops = 2LL * kpad * (long long)npad * (long long)(nchunk_c * kCC);
 done:
     for (int e = 0; e < 4; e++)
         if (ev[e]) (void)hipEventDestroy(ev[e]);
-    kfree(d_part, st); kfree(d_mean, st); kfree(d_bound, st); kfree(d_Xs, st); kfree(d_nq, st);
+    kfree(d_part, st); kfree(d_mean, st); kfree(d_bound, st); kfree(d_XA, st); kfree(d_XB, st); kfree(d_nq, st);
     kfree(d_nkey, st); kfree(d_thr, st); kfree(d_cnt, st); kfree(d_ctr, st); kfree(d_unc, st);
     kfree(d_ovf, st); kfree(d_buf, st);
     if (stats) *stats = S;
