@@ -274,7 +274,7 @@ __device__ __forceinline__ float wave_select_cut(KEntry *b, int m, int L, int *_
 // e of local query j at buf[j * cap + e].  On exit cnt[q] <= L entries and thr[q] hold
 // the survivors and the final threshold (+inf when nothing was ever cut).  KT: MFMA
 // k-steps (32 terms each) per K block; the packed rows hold nkb blocks.
-template <int KT>
+template <int KT, bool DMA>
 __global__ void __launch_bounds__(256)
 knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
                   const float *__restrict__ nq, const float *__restrict__ nkey, int n, int nkb,
@@ -286,7 +286,10 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
     constexpr int UNITS = SEG / 8;     // 16-byte units per candidate per K block
     constexpr int PER_T = kCC * UNITS / 256;  // 16-byte units each thread stages
     static_assert(PER_T * 256 == kCC * UNITS, "chunk staging");
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][kCC * LROW];
+    // direct-to-LDS staging: rounds of 256 slots of 16 bytes (row stride UNITS + 1 slots)
+    constexpr int ROUNDS = (kCC * (UNITS + 1) + 255) / 256;
+    constexpr int BSZ = DMA ? ROUNDS * 256 * 8 : kCC * LROW;
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BSZ];
     __shared__ float nk_lds[2][kCC];
     __shared__ int cnt_lds[4][kQW];
     __shared__ float thr_lds[4][kQW];
@@ -325,27 +328,52 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
     if (nkb == 1) load_a(0);
 
     // stage s = chunk s / nkb, K block s % nkb
-    uint4 pre[PER_T];
+    uint4 pre[DMA ? 1 : PER_T];
     float pre_nk = 0.f;
     auto fetch = [&](int s) {
         const int c0 = (s / nkb) * kCC, kb = s % nkb;
+        if constexpr (DMA) {
+            // slot r*256 + 64w + l of the buffer holds unit u of candidate cand (slot =
+            // cand * (UNITS + 1) + u); padding slots fetch a valid unit that is never read
+            const int b = (s & 1);
 #pragma unroll
-        for (int r = 0; r < PER_T; r++) {
-            const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
-            pre[r] = *(const uint4 *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + part * 8);
+            for (int r = 0; r < ROUNDS; r++) {
+                const int slot = r * 256 + w * 64 + l;
+                int cand = slot / (UNITS + 1), u = slot % (UNITS + 1);
+                if (cand >= kCC || u >= UNITS) {
+                    cand = 0;
+                    u = 0;
+                }
+                __builtin_amdgcn_global_load_lds(
+                    (const void *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + u * 8),
+                    (void __attribute__((address_space(3))) *)&Bs[b][(r * 256 + w * 64) * 8], 16, 0, 0);
+            }
+            if (w == 0)
+                __builtin_amdgcn_global_load_lds((const void *)(nkey + c0 + l),
+                                                 (void __attribute__((address_space(3))) *)&nk_lds[b][0], 4,
+                                                 0, 0);
+        } else {
+#pragma unroll
+            for (int r = 0; r < PER_T; r++) {
+                const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
+                pre[r] = *(const uint4 *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + part * 8);
+            }
+            if (t < kCC) pre_nk = nkey[c0 + t];
         }
-        if (t < kCC) pre_nk = nkey[c0 + t];
     };
     auto stash = [&](int b) {
+        if constexpr (!DMA) {
 #pragma unroll
-        for (int r = 0; r < PER_T; r++) {
-            const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
-            *(uint4 *)(&Bs[b][cand * LROW + part * 8]) = pre[r];
+            for (int r = 0; r < PER_T; r++) {
+                const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
+                *(uint4 *)(&Bs[b][cand * LROW + part * 8]) = pre[r];
+            }
+            if (t < kCC) nk_lds[b][t] = pre_nk;
         }
-        if (t < kCC) nk_lds[b][t] = pre_nk;
     };
     fetch(0);
     stash(0);
+    if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     f32x4 acc[kRB][4];
@@ -428,6 +456,7 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
             }
         }
         if (more) stash(b ^ 1);
+        if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS loads have landed
         __syncthreads();
     }
     // final cut to L survivors
@@ -616,8 +645,13 @@ void launch_screen(int grid, hipStream_t st, const __bf16 *XA, const __bf16 *XB,
                    const float *nkey, int n, int nkb, int q0, int L, int cap, KEntry *buf, int *cnt,
                    float *thr)
 {
-    hipLaunchKernelGGL(knn_screen_kernel<KT>, dim3(grid), dim3(256), 0, st, XA, XB, nq, nkey, n, nkb, q0,
-                       L, cap, buf, cnt, thr);
+    static const bool dma = !getenv("TSNE_KNN_DMA") || atoi(getenv("TSNE_KNN_DMA")) != 0;
+    if (dma)
+        hipLaunchKernelGGL((knn_screen_kernel<KT, true>), dim3(grid), dim3(256), 0, st, XA, XB, nq, nkey, n,
+                           nkb, q0, L, cap, buf, cnt, thr);
+    else
+        hipLaunchKernelGGL((knn_screen_kernel<KT, false>), dim3(grid), dim3(256), 0, st, XA, XB, nq, nkey, n,
+                           nkb, q0, L, cap, buf, cnt, thr);
 }
 
 // Host check of the survivor buffers of one screening launch (TSNE_KNN_DEBUG): counts,
