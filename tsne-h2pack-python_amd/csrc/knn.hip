@@ -289,8 +289,10 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
     // direct-to-LDS staging: rounds of 256 slots of 16 bytes (row stride UNITS + 1 slots)
     constexpr int ROUNDS = (kCC * (UNITS + 1) + 255) / 256;
     constexpr int BSZ = DMA ? ROUNDS * 256 * 8 : kCC * LROW;
-    __shared__ __attribute__((aligned(16))) __bf16 Bs[2][BSZ];
-    __shared__ float nk_lds[2][kCC];
+    // DMA: three buffers, chunks fetched two stages ahead; registers: two buffers
+    constexpr int NBUF = DMA ? 3 : 2;
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[NBUF][BSZ];
+    __shared__ float nk_lds[NBUF][4][kCC];  // one copy per wave (every wave issues ROUNDS + 1 loads)
     __shared__ int cnt_lds[4][kQW];
     __shared__ float thr_lds[4][kQW];
     __shared__ int hist[4][256];
@@ -335,7 +337,7 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
         if constexpr (DMA) {
             // slot r*256 + 64w + l of the buffer holds unit u of candidate cand (slot =
             // cand * (UNITS + 1) + u); padding slots fetch a valid unit that is never read
-            const int b = (s & 1);
+            const int b = s % NBUF;
 #pragma unroll
             for (int r = 0; r < ROUNDS; r++) {
                 const int slot = r * 256 + w * 64 + l;
@@ -348,10 +350,9 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
                     (const void *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + u * 8),
                     (void __attribute__((address_space(3))) *)&Bs[b][(r * 256 + w * 64) * 8], 16, 0, 0);
             }
-            if (w == 0)
-                __builtin_amdgcn_global_load_lds((const void *)(nkey + c0 + l),
-                                                 (void __attribute__((address_space(3))) *)&nk_lds[b][0], 4,
-                                                 0, 0);
+            __builtin_amdgcn_global_load_lds((const void *)(nkey + c0 + l),
+                                             (void __attribute__((address_space(3))) *)&nk_lds[b][w][0], 4, 0,
+                                             0);
         } else {
 #pragma unroll
             for (int r = 0; r < PER_T; r++) {
@@ -368,19 +369,26 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
                 const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
                 *(uint4 *)(&Bs[b][cand * LROW + part * 8]) = pre[r];
             }
-            if (t < kCC) nk_lds[b][t] = pre_nk;
+            if (t < kCC) nk_lds[b][0][t] = pre_nk;
         }
     };
     fetch(0);
     stash(0);
-    if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0);
+    if constexpr (DMA) {
+        if (nstage > 1) fetch(1);
+        __builtin_amdgcn_s_waitcnt(0);
+    }
     __syncthreads();
 
     f32x4 acc[kRB][4];
     for (int s = 0; s < nstage; s++) {
-        const int b = s & 1;
+        const int b = s % NBUF;
         const bool more = s + 1 < nstage;
-        if (more) fetch(s + 1);
+        if constexpr (DMA) {
+            if (s + 2 < nstage) fetch(s + 2);
+        } else {
+            if (more) fetch(s + 1);
+        }
         const int kb = s % nkb;
         if (kb == 0) {
 #pragma unroll
@@ -409,7 +417,7 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
             bool any = false;
 #pragma unroll
             for (int cb = 0; cb < 4; cb++) {
-                nk[cb] = nk_lds[b][cb * 16 + (l & 15)];
+                nk[cb] = nk_lds[b][DMA ? w : 0][cb * 16 + (l & 15)];
 #pragma unroll
                 for (int rb = 0; rb < kRB; rb++)
 #pragma unroll
@@ -456,7 +464,14 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
             }
         }
         if (more) stash(b ^ 1);
-        if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0);  // this wave's LDS loads have landed
+        if constexpr (DMA) {
+            // the loads of stage s + 1 have landed; those of stage s + 2 (ROUNDS + 1, issued
+            // last) may still be in flight.  vmcnt <= ROUNDS + 1, no wait on lgkm / exp.
+            if (s + 2 < nstage)
+                __builtin_amdgcn_s_waitcnt(0xF70 | (ROUNDS + 1));
+            else
+                __builtin_amdgcn_s_waitcnt(0);
+        }
         __syncthreads();
     }
     // final cut to L survivors
