@@ -12,19 +12,19 @@
 //
 //  1. Screening on MFMA.  Points are centred (FP64 mean) and split into two BF16 terms,
 //     x = xh + xl + O(2^-16 |x|).  d^2 = |x|^2 + |y|^2 - 2 x.y with x.y taken as
-//     xh.yh + xh.yl + xl.yh, one BF16 dot product of packed rows [xh | xh | xl] and
-//     [yh | yl | yh] (3D features, zero padded to a multiple of 32) on
-//     v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Every estimate is within
-//     eps * (|x|^2 + |y|^2) of the exact d^2 (eps from the error analysis in knn_eps
-//     below); the screening key of candidate y is the estimate minus eps |y|^2, so
+//     xh.yh + xh.yl + xl.yh: one BF16 dot product of packed rows [xh | xh | xl | 1 1 1] and
+//     [yh | yl | yh | -|y|^2/2 in three BF16 parts] (3D + 3 terms, zero padded to a
+//     multiple of 32) on v_mfma_f32_16x16x32_bf16 with FP32 accumulation.  Every estimate
+//     is within eps * (|x|^2 + |y|^2) of the exact d^2 (eps from the error analysis in
+//     knn_eps below); the screening key of candidate y is the estimate minus eps |y|^2, so
 //     key - eps |x|^2 <= d^2 <= key + eps |x|^2.
 //     One workgroup owns 128 queries (32 per wave, the operands of 2 x 4 MFMA tiles held
-//     in registers) and streams every candidate through LDS in chunks of 64.  Each query
-//     keeps a threshold and a buffer in HBM: candidates whose key is below the threshold
-//     are appended (the test is two VALU operations per pair; keys are formed only for
-//     the rare chunks where some pair passes); a buffer that fills is cut to its L
-//     smallest keys (radix select over the key bits by the owning wave), and the
-//     threshold drops to the L-th key.
+//     in registers) and streams every candidate through LDS in chunks of 64 (direct
+//     global-to-LDS loads, two chunks ahead).  Each query keeps a threshold and a buffer
+//     in HBM: candidates whose key is below the threshold are appended (one compare per
+//     pair against the accumulator; keys are formed only in the rare chunks where some
+//     pair passes); a buffer that fills is cut to its L smallest keys (radix select over
+//     the key bits by the owning wave), and the threshold drops to the L-th key.
 //  2. Refinement in FP64.  The L survivors of each query get exact FP64 distances
 //     (differences, fixed fma order); the k+1 smallest, ordered by (distance, index),
 //     give the row.  The row is certain when its (k+1)-th exact d^2 is below
@@ -50,9 +50,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kRB = 2;         // 16-row MFMA blocks of queries per wave
-constexpr int kQW = 16 * kRB;  // queries per wave
-constexpr int kQB = 4 * kQW;   // queries per workgroup
+constexpr int kQBMax = 256;    // largest query block of a workgroup (4 waves x 4 x 16 rows)
 constexpr int kCC = 64;        // candidates per chunk
 constexpr int kLMax = 1024;    // largest survivor count L
 constexpr int kFCap = 4096;    // fallback: largest number of points within the bound
@@ -70,11 +68,12 @@ struct DEntry {
 // Rigorous bound of |estimate - exact d^2| / (|x|^2 + |y|^2) for a packed dot product of
 // K terms (K = 3 D padded): representation x - (xh + xl) <= 2^-16 |x| per component (two
 // BF16 roundings) gives 2^-14 through the difference and 2^-16 through the dropped xl.yl;
-// FP32 summation of the K exact products <= K * 2^-24 * 1.02 |x||y|; norms and the final
+// FP32 summation of the K exact products (including |y|^2 / 2) <= K * 2^-24 * 1.02
+// (|x||y| + |y|^2 / 2); norms and the final
 // combination add a few 2^-24.  A 10% margin covers the second-order terms.
 double knn_eps(int kterms)
 {
-    return 1.1 * (std::ldexp(1.0, -14) + std::ldexp(1.0, -16) + 1.05 * kterms * std::ldexp(1.0, -24) +
+    return 1.1 * (std::ldexp(1.0, -14) + std::ldexp(1.0, -16) + 1.1 * kterms * std::ldexp(1.0, -24) +
                   8.0 * std::ldexp(1.0, -24));
 }
 
@@ -116,13 +115,12 @@ col_mean_kernel(const double *__restrict__ part, int nchunk, int n, int D, doubl
 }
 
 // Packed BF16 rows of point i (K = 32 * KT * nkb terms each, zero padded):
-// A (queries) = [xh | xh | xl], B (candidates) = [xh | xl | xh], so that A_i . B_j =
-// xh_i.yh_j + xh_i.yl_j + xl_i.yh_j.  nq = |xh + xl|^2 and nkey = nq * (1 - eps) in FP32;
-// rows >= n are zero with nkey = +inf (padding candidates never pass a threshold).
+// A (queries) = [xh | xh | xl | 1 1 1], B (candidates) = [xh | xl | xh | -|y|^2 (1-eps)/2 in
+// three parts], so that A_i . B_j = xh_i.yh_j + xh_i.yl_j + xl_i.yh_j - |y_j|^2 (1-eps)/2 and
+// the screening key |x|^2 + |y|^2 (1-eps) - 2 x.y is |x|^2 - 2 A_i.B_j.  nq = |xh + xl|^2.
 __global__ void __launch_bounds__(256)
 split_kernel(const double *__restrict__ X, const double *__restrict__ mean, int n, int npad, int D,
-             int K, float eps, __bf16 *__restrict__ XA, __bf16 *__restrict__ XB,
-             float *__restrict__ nq, float *__restrict__ nkey)
+             int K, float eps, __bf16 *__restrict__ XA, __bf16 *__restrict__ XB, float *__restrict__ nq)
 {
     const int i = blockIdx.x;
     if (i >= npad) return;
@@ -155,7 +153,7 @@ split_kernel(const double *__restrict__ X, const double *__restrict__ mean, int 
         }
         __syncthreads();
     }
-    for (int f = 3 * D + threadIdx.x; f < K; f += 256) {
+    for (int f = 3 * D + 3 + threadIdx.x; f < K; f += 256) {
         XA[(size_t)i * K + f] = (__bf16)0.f;
         XB[(size_t)i * K + f] = (__bf16)0.f;
     }
@@ -167,7 +165,19 @@ split_kernel(const double *__restrict__ X, const double *__restrict__ mean, int 
     }
     if (threadIdx.x == 0) {
         nq[i] = (float)red[0];
-        nkey[i] = (i < n) ? (float)(red[0] * (1.0 - (double)eps)) : INFINITY;
+        // terms 3D .. 3D+2: A = 1, B = -|y|^2 (1 - eps) / 2 as three BF16 parts, so that the
+        // MFMA sum is x.y - |y|^2 (1 - eps) / 2 (padding rows: -1e38, never below a finite
+        // threshold)
+        const double v = (i < n) ? -0.5 * red[0] * (1.0 - (double)eps) : -1e38;
+        const __bf16 h1 = (__bf16)(float)v;
+        const double r1 = v - (double)(float)h1;
+        const __bf16 h2 = (__bf16)(float)r1;
+        const __bf16 h3 = (__bf16)(float)(r1 - (double)(float)h2);
+        const size_t o3 = (size_t)i * K + 3 * D;
+        XA[o3] = XA[o3 + 1] = XA[o3 + 2] = (__bf16)1.f;
+        XB[o3] = h1;
+        XB[o3 + 1] = h2;
+        XB[o3 + 2] = h3;
     }
 }
 
@@ -270,17 +280,19 @@ __device__ __forceinline__ float wave_select_cut(KEntry *b, int m, int L, int *_
 
 // ---------------------------------------------------------------- screening (MFMA)
 
-// Workgroup b owns queries q0 + 128 b .. +127 (wave w: +32 w .. +31).  Buffers: entry
+// Workgroup b owns queries q0 + QB b .. (wave w: QW of them).  Buffers: entry
 // e of local query j at buf[j * cap + e].  On exit cnt[q] <= L entries and thr[q] hold
 // the survivors and the final threshold (+inf when nothing was ever cut).  KT: MFMA
 // k-steps (32 terms each) per K block; the packed rows hold nkb blocks.
-template <int KT, bool DMA>
+template <int KT, bool DMA, int RB>
 __global__ void __launch_bounds__(256)
 knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
-                  const float *__restrict__ nq, const float *__restrict__ nkey, int n, int nkb,
+                  const float *__restrict__ nq, int n, int nkb,
                   int q0, int L, int cap, KEntry *__restrict__ buf, int *__restrict__ cnt_out,
                   float *__restrict__ thr_out)
 {
+    constexpr int QW = 16 * RB;        // queries per wave (RB 16-row MFMA blocks)
+    constexpr int QB = 4 * QW;         // queries per workgroup
     constexpr int SEG = 32 * KT;       // bf16 per row per K block
     constexpr int LROW = SEG + 8;      // LDS row (16-byte pad: conflict-free b128 reads)
     constexpr int UNITS = SEG / 8;     // 16-byte units per candidate per K block
@@ -292,46 +304,44 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
     // DMA: three buffers, chunks fetched two stages ahead; registers: two buffers
     constexpr int NBUF = DMA ? 3 : 2;
     __shared__ __attribute__((aligned(16))) __bf16 Bs[NBUF][BSZ];
-    __shared__ float nk_lds[NBUF][4][kCC];  // one copy per wave (every wave issues ROUNDS + 1 loads)
-    __shared__ int cnt_lds[4][kQW];
-    __shared__ float thr_lds[4][kQW];
+    __shared__ int cnt_lds[4][QW];
+    __shared__ float thr_lds[4][QW];
     __shared__ int hist[4][256];
 
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
-    const int qw0 = q0 + blockIdx.x * kQB + w * kQW;  // first query of this wave
+    const int qw0 = q0 + blockIdx.x * QB + w * QW;  // first query of this wave
     const int rowlen = SEG * nkb;
     const int nchunk = (n + kCC - 1) / kCC;
     const int nstage = nchunk * nkb;
-    KEntry *const wbuf = buf + (size_t)(blockIdx.x * kQB + w * kQW) * cap;
+    KEntry *const wbuf = buf + (size_t)(blockIdx.x * QB + w * QW) * cap;
 
-    if (l < kQW) {
+    if (l < QW) {
         cnt_lds[w][l] = 0;
         thr_lds[w][l] = INFINITY;
     }
 
     // query operands: row rb*16 + l%16, terms s*32 + (l/16)*8 .. +7 of the block
-    bf16x8 a[kRB][KT];
-    float nqr[kRB][4], tq[kRB][4];  // tq = threshold - |x|^2: the pass test is t < tq
+    bf16x8 a[RB][KT];
+    float nqr[RB][4], htq[RB][4];  // htq = (|x|^2 - threshold) / 2: the pass test is A.B > htq
     auto load_a = [&](int kb) {
 #pragma unroll
-        for (int rb = 0; rb < kRB; rb++) {
+        for (int rb = 0; rb < RB; rb++) {
             const __bf16 *p = XA + (size_t)(qw0 + rb * 16 + (l & 15)) * rowlen + kb * SEG + (l >> 4) * 8;
 #pragma unroll
             for (int ks = 0; ks < KT; ks++) a[rb][ks] = *(const bf16x8 *)(p + ks * 32);
         }
     };
 #pragma unroll
-    for (int rb = 0; rb < kRB; rb++)
+    for (int rb = 0; rb < RB; rb++)
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             nqr[rb][i] = nq[qw0 + rb * 16 + (l >> 4) * 4 + i];
-            tq[rb][i] = INFINITY;
+            htq[rb][i] = -INFINITY;
         }
     if (nkb == 1) load_a(0);
 
     // stage s = chunk s / nkb, K block s % nkb
     uint4 pre[DMA ? 1 : PER_T];
-    float pre_nk = 0.f;
     auto fetch = [&](int s) {
         const int c0 = (s / nkb) * kCC, kb = s % nkb;
         if constexpr (DMA) {
@@ -350,16 +360,12 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
                     (const void *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + u * 8),
                     (void __attribute__((address_space(3))) *)&Bs[b][(r * 256 + w * 64) * 8], 16, 0, 0);
             }
-            __builtin_amdgcn_global_load_lds((const void *)(nkey + c0 + l),
-                                             (void __attribute__((address_space(3))) *)&nk_lds[b][w][0], 4, 0,
-                                             0);
         } else {
 #pragma unroll
             for (int r = 0; r < PER_T; r++) {
                 const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
                 pre[r] = *(const uint4 *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + part * 8);
             }
-            if (t < kCC) pre_nk = nkey[c0 + t];
         }
     };
     auto stash = [&](int b) {
@@ -369,7 +375,6 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
                 const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
                 *(uint4 *)(&Bs[b][cand * LROW + part * 8]) = pre[r];
             }
-            if (t < kCC) nk_lds[b][0][t] = pre_nk;
         }
     };
     fetch(0);
@@ -380,7 +385,7 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
     }
     __syncthreads();
 
-    f32x4 acc[kRB][4];
+    f32x4 acc[RB][4];
     for (int s = 0; s < nstage; s++) {
         const int b = s % NBUF;
         const bool more = s + 1 < nstage;
@@ -392,7 +397,7 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
         const int kb = s % nkb;
         if (kb == 0) {
 #pragma unroll
-            for (int rb = 0; rb < kRB; rb++)
+            for (int rb = 0; rb < RB; rb++)
 #pragma unroll
                 for (int cb = 0; cb < 4; cb++) acc[rb][cb] = f32x4{0.f, 0.f, 0.f, 0.f};
         }
@@ -406,33 +411,30 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
 #pragma unroll
             for (int cb = 0; cb < 4; cb++)
 #pragma unroll
-                for (int rb = 0; rb < kRB; rb++)
+                for (int rb = 0; rb < RB; rb++)
                     acc[rb][cb] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[rb][ks], bv[cb], acc[rb][cb], 0, 0, 0);
         }
         if (kb == nkb - 1) {
             // D layout: lane l holds column l%16, rows (l/16)*4 + i of each 16 x 16 tile.
-            // Fast test per pair: |y|^2(1-eps) - 2 x.y < thr - |x|^2 (padding: +inf).
+            // Pass test per pair: A.B > (|x|^2 - threshold) / 2, one compare.
             const int c0 = (s / nkb) * kCC;
-            float nk[4];
             bool any = false;
 #pragma unroll
-            for (int cb = 0; cb < 4; cb++) {
-                nk[cb] = nk_lds[b][DMA ? w : 0][cb * 16 + (l & 15)];
+            for (int cb = 0; cb < 4; cb++)
 #pragma unroll
-                for (int rb = 0; rb < kRB; rb++)
+                for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) any |= fmaf(-2.f, acc[rb][cb][i], nk[cb]) < tq[rb][i];
-            }
+                    for (int i = 0; i < 4; i++) any |= acc[rb][cb][i] > htq[rb][i];
             if (__ballot(any)) {
 #pragma unroll
                 for (int cb = 0; cb < 4; cb++) {
                     const int c = c0 + cb * 16 + (l & 15);
 #pragma unroll
-                    for (int rb = 0; rb < kRB; rb++)
+                    for (int rb = 0; rb < RB; rb++)
 #pragma unroll
                         for (int i = 0; i < 4; i++) {
-                            if (fmaf(-2.f, acc[rb][cb][i], nk[cb]) < tq[rb][i]) {
-                                float key = fmaf(-2.f, acc[rb][cb][i], nqr[rb][i] + nk[cb]);
+                            if (acc[rb][cb][i] > htq[rb][i] && c < n) {
+                                float key = fmaf(-2.f, acc[rb][cb][i], nqr[rb][i]);
                                 key = key > 0.f ? key : 0.f;
                                 const int j = rb * 16 + (l >> 4) * 4 + i;
                                 const int pos = atomicAdd(&cnt_lds[w][j], 1);
@@ -443,7 +445,7 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
                 // cut every buffer that could overflow in the next chunk (cap >= L + 64)
                 __threadfence_block();
                 wave_sync();
-                unsigned long long full = __ballot(l < kQW && cnt_lds[w][l] > cap - kCC);
+                unsigned long long full = __ballot(l < QW && cnt_lds[w][l] > cap - kCC);
                 while (full) {
                     const int j = __ffsll((long long)full) - 1;
                     full &= full - 1;
@@ -457,18 +459,18 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
                     wave_sync();
                 }
 #pragma unroll
-                for (int rb = 0; rb < kRB; rb++)
+                for (int rb = 0; rb < RB; rb++)
 #pragma unroll
                     for (int i = 0; i < 4; i++)
-                        tq[rb][i] = thr_lds[w][rb * 16 + (l >> 4) * 4 + i] - nqr[rb][i];
+                        htq[rb][i] = 0.5f * (nqr[rb][i] - thr_lds[w][rb * 16 + (l >> 4) * 4 + i]);
             }
         }
         if (more) stash(b ^ 1);
         if constexpr (DMA) {
-            // the loads of stage s + 1 have landed; those of stage s + 2 (ROUNDS + 1, issued
-            // last) may still be in flight.  vmcnt <= ROUNDS + 1, no wait on lgkm / exp.
+            // the loads of stage s + 1 have landed; those of stage s + 2 (ROUNDS, issued
+            // last) may still be in flight.  vmcnt <= ROUNDS, no wait on lgkm / exp.
             if (s + 2 < nstage)
-                __builtin_amdgcn_s_waitcnt(0xF70 | (ROUNDS + 1));
+                __builtin_amdgcn_s_waitcnt(0xF70 | ROUNDS);
             else
                 __builtin_amdgcn_s_waitcnt(0);
         }
@@ -477,7 +479,7 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
     // final cut to L survivors
     __threadfence_block();
     wave_sync();
-    unsigned long long over = __ballot(l < kQW && cnt_lds[w][l] > L);
+    unsigned long long over = __ballot(l < QW && cnt_lds[w][l] > L);
     while (over) {
         const int j = __ffsll((long long)over) - 1;
         over &= over - 1;
@@ -490,7 +492,7 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
         wave_sync();
     }
     const int q = qw0 + l;
-    if (l < kQW && q < n) {
+    if (l < QW && q < n) {
         cnt_out[q] = cnt_lds[w][l];
         thr_out[q] = thr_lds[w][l];
     }
@@ -656,17 +658,25 @@ template <typename T> void kfree(T *&p, hipStream_t st)
 }
 
 template <int KT>
-void launch_screen(int grid, hipStream_t st, const __bf16 *XA, const __bf16 *XB, const float *nq,
-                   const float *nkey, int n, int nkb, int q0, int L, int cap, KEntry *buf, int *cnt,
-                   float *thr)
+void launch_screen(int rb, int nqb, hipStream_t st, const __bf16 *XA, const __bf16 *XB, const float *nq,
+                   int n, int nkb, int q0, int L, int cap, KEntry *buf, int *cnt, float *thr)
 {
     static const bool dma = !getenv("TSNE_KNN_DMA") || atoi(getenv("TSNE_KNN_DMA")) != 0;
-    if (dma)
-        hipLaunchKernelGGL((knn_screen_kernel<KT, true>), dim3(grid), dim3(256), 0, st, XA, XB, nq, nkey, n,
-                           nkb, q0, L, cap, buf, cnt, thr);
-    else
-        hipLaunchKernelGGL((knn_screen_kernel<KT, false>), dim3(grid), dim3(256), 0, st, XA, XB, nq, nkey, n,
-                           nkb, q0, L, cap, buf, cnt, thr);
+    if (rb == 4) {
+        if (dma)
+            hipLaunchKernelGGL((knn_screen_kernel<KT, true, 4>), dim3(nqb / 256), dim3(256), 0, st, XA, XB, nq,
+                               n, nkb, q0, L, cap, buf, cnt, thr);
+        else
+            hipLaunchKernelGGL((knn_screen_kernel<KT, false, 4>), dim3(nqb / 256), dim3(256), 0, st, XA, XB, nq,
+                               n, nkb, q0, L, cap, buf, cnt, thr);
+    } else {
+        if (dma)
+            hipLaunchKernelGGL((knn_screen_kernel<KT, true, 2>), dim3(nqb / 128), dim3(256), 0, st, XA, XB, nq,
+                               n, nkb, q0, L, cap, buf, cnt, thr);
+        else
+            hipLaunchKernelGGL((knn_screen_kernel<KT, false, 2>), dim3(nqb / 128), dim3(256), 0, st, XA, XB, nq,
+                               n, nkb, q0, L, cap, buf, cnt, thr);
+    }
 }
 
 // Host check of the survivor buffers of one screening launch (TSNE_KNN_DEBUG): counts,
@@ -719,29 +729,30 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
     KnnStats S{};
     const int L = knn_survivors(k);
     if (n < 2 || D < 1 || k < 1 || k >= n || L > kLMax) return -2;
-    // packed rows of 3 D terms: nkb K blocks of KT <= 8 MFMA k-steps
-    const int ksteps = (3 * D + 31) / 32;
+    // packed rows of 3 D + 3 terms: nkb K blocks of KT <= 8 MFMA k-steps
+    const int ksteps = (3 * D + 3 + 31) / 32;
     const int nkb = (ksteps + 7) / 8;
     const int KT = (ksteps + nkb - 1) / nkb;
     const int kpad = 32 * KT * nkb;
-    const int npad = (n + kQB - 1) / kQB * kQB;
+    const int npad = (n + kQBMax - 1) / kQBMax * kQBMax;
+    const int rb = (getenv("TSNE_KNN_RB") && atoi(getenv("TSNE_KNN_RB")) == 4) ? 4 : 2;
     const int cap = std::max(4 * L, L + 2 * kCC);
     const double eps = knn_eps(kpad);
     // queries per screening launch: the survivor buffers stay under ~1 GiB
     const long long per_q = (long long)cap * sizeof(KEntry);
-    int qbatch = (int)std::min<long long>(npad, std::max<long long>(kQB, (1ll << 30) / per_q / kQB * kQB));
+    int qbatch = (int)std::min<long long>(npad, std::max<long long>(kQBMax, (1ll << 30) / per_q / kQBMax * kQBMax));
     const int nchunk = (n + 1023) / 1024;
 
     double *d_part = nullptr, *d_mean = nullptr, *d_bound = nullptr;
     __bf16 *d_XA = nullptr, *d_XB = nullptr;
-    float *d_nq = nullptr, *d_nkey = nullptr, *d_thr = nullptr;
+    float *d_nq = nullptr, *d_thr = nullptr;
     int *d_cnt = nullptr, *d_ctr = nullptr, *d_unc = nullptr, *d_ovf = nullptr;
     KEntry *d_buf = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     const long long nchunk_c = (n + kCC - 1) / kCC;
     int rc = -1;
     if (kalloc(&d_part, (size_t)nchunk * D, st) || kalloc(&d_mean, D, st) ||
-        kalloc(&d_XA, (size_t)npad * kpad, st) || kalloc(&d_XB, (size_t)npad * kpad, st) || kalloc(&d_nq, npad, st) || kalloc(&d_nkey, npad, st) ||
+        kalloc(&d_XA, (size_t)npad * kpad, st) || kalloc(&d_XB, (size_t)npad * kpad, st) || kalloc(&d_nq, npad, st) ||
         kalloc(&d_thr, n, st) || kalloc(&d_cnt, n, st) || kalloc(&d_bound, n, st) || kalloc(&d_ctr, 64, st) ||
         kalloc(&d_unc, n, st) || kalloc(&d_ovf, n, st) || kalloc(&d_buf, (size_t)qbatch * cap, st))
         goto done;
@@ -752,20 +763,18 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
     hipLaunchKernelGGL(col_partial_kernel, dim3(nchunk), dim3(256), 0, st, d_X, n, D, d_part);
     hipLaunchKernelGGL(col_mean_kernel, dim3((D + 255) / 256), dim3(256), 0, st, d_part, nchunk, n, D, d_mean);
     hipLaunchKernelGGL(split_kernel, dim3(npad), dim3(256), 0, st, d_X, d_mean, n, npad, D, kpad,
-                       (float)eps, d_XA, d_XB, d_nq, d_nkey);
+                       (float)eps, d_XA, d_XB, d_nq);
     for (int q0 = 0; q0 < npad; q0 += qbatch) {
         const int nqb = std::min(qbatch, npad - q0);
-        const int grid = nqb / kQB;
         (void)hipEventRecord(ev[2], st);
         switch (KT) {
 #define KNN_SCREEN_CASE(v)                                                                          \
     case v:                                                                                         \
-        launch_screen<v>(grid, st, d_XA, d_XB, d_nq, d_nkey, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); \
+        launch_screen<v>(rb, nqb, st, d_XA, d_XB, d_nq, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); \
         break;
             KNN_SCREEN_CASE(1) KNN_SCREEN_CASE(2) KNN_SCREEN_CASE(3) KNN_SCREEN_CASE(4)
             KNN_SCREEN_CASE(5) KNN_SCREEN_CASE(6) KNN_SCREEN_CASE(7)
-            default: launch_screen<8>(grid, st, d_XA, d_XB, d_nq, d_nkey, n, nkb, q0, L, cap, d_buf, d_cnt,
-                                      d_thr);
+            default: launch_screen<8>(rb, nqb, st, d_XA, d_XB, d_nq, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr);
 #undef KNN_SCREEN_CASE
         }
         (void)hipEventRecord(ev[3], st);
@@ -853,7 +862,7 @@ done:
     for (int e = 0; e < 4; e++)
         if (ev[e]) (void)hipEventDestroy(ev[e]);
     kfree(d_part, st); kfree(d_mean, st); kfree(d_bound, st); kfree(d_XA, st); kfree(d_XB, st); kfree(d_nq, st);
-    kfree(d_nkey, st); kfree(d_thr, st); kfree(d_cnt, st); kfree(d_ctr, st); kfree(d_unc, st);
+    kfree(d_thr, st); kfree(d_cnt, st); kfree(d_ctr, st); kfree(d_unc, st);
     kfree(d_ovf, st); kfree(d_buf, st);
     if (stats) *stats = S;
     return rc;
