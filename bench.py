@@ -1,15 +1,17 @@
 """Benchmark of the t-SNE gradient path on one MI355X.
 
-Workload (default): N=1,000,000 points, D=50 float32 from a seeded mixture of 100
-Gaussians (Dirichlet(1) weights, per-cluster std U[0.5, 2], centres N(0, 5^2)); exact
-k=90 graph (computed on the GPU with torch, harness only), perplexity 30, 2D,
-h2_tol 1e-2, rebuild every 3 (10 in early exaggeration), leaf 100.
+Workload (default): N=1,000,000 points, D=50 float64 from a seeded mixture of 100
+Gaussians (Dirichlet(1) weights, per-cluster std U[0.5, 2], centres N(0, 5^2)); the exact
+k=90 graph from the library's own device search (tsne_knn_exact), joint P on the device,
+perplexity 30, 2D, h2_tol 1e-2, rebuild every 3 (10 in early exaggeration), leaf 100.
 
-A step is one optimisation iteration of the library's loop (tsne_run_range) on
-device-resident state.  --warmup W iterations run untimed, then K iterations are
-timed (the schedule follows the absolute iteration index of a W+K iteration run).
-The metric is iterations/s of the mixed-precision H2 path; the plain FP32 path is
-timed the same way and reported beside it.  Prints ONE JSON line.
+A step is one optimisation iteration of the library's loop on device-resident state.
+--warmup W: an untimed job of W iterations first (its end state also gives the gradient
+error after warm-up).  Timed: a fresh job of exactly K iterations, its whole schedule,
+bracketed by device synchronisation.  The metric is iterations/s of the mixed-precision H2
+path; the plain FP32 path is timed the same way and reported beside it.  Prints ONE JSON
+line.  Other modes: --serving (jobs/s, p50/p99), --knn (exact k-NN graph seconds),
+--pipeline (k-NN + joint P + optimisation + quality, seconds per stage).
 """
 import argparse
 import ctypes
@@ -34,72 +36,35 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def _torch_knn(n, dim_in, n_clusters, k, seed, out):
-    """Child process: seeded mixture -> exact k-NN on the GPU with torch.  torch ships its
-    own HIP runtime, so it runs apart from the library's process."""
-    import torch
-    rng = np.random.default_rng(seed)
-    centers = rng.normal(scale=5.0, size=(n_clusters, dim_in))
-    w = rng.dirichlet(np.ones(n_clusters))
-    std = rng.uniform(0.5, 2.0, size=n_clusters)
-    labels = rng.choice(n_clusters, size=n, p=w)
-    X = (centers[labels] + rng.normal(size=(n, dim_in)) * std[labels, None]).astype(np.float32)
-    dev = torch.device("cuda:0")
-    Xt = torch.from_numpy(X).to(dev)
-    sq = (Xt * Xt).sum(1)
-    idx_all = torch.empty((n, k), dtype=torch.int32, device=dev)
-    dst_all = torch.empty((n, k), dtype=torch.float32, device=dev)
-    chunk = 4096
-    t0 = time.time()
-    for s in range(0, n, chunk):
-        e = min(n, s + chunk)
-        d2 = sq[s:e, None] - 2.0 * (Xt[s:e] @ Xt.T) + sq[None, :]
-        d2[torch.arange(e - s, device=dev), torch.arange(s, e, device=dev)] = float("inf")
-        vals, idx = torch.topk(d2, k, dim=1, largest=False, sorted=True)
-        idx_all[s:e] = idx.to(torch.int32)
-        dst_all[s:e] = vals.clamp_min(0).sqrt()
-        if (s // chunk) % 64 == 0:
-            log(f"knn rows {e}/{n} ({time.time() - t0:.1f}s)")
-    torch.cuda.synchronize()
-    idx = idx_all.cpu().numpy()
-    dist = dst_all.cpu().numpy()
-    # CSR rows with sorted columns (reference _backend.py:651-653)
+def knn_csr(be, X, k):
+    """Exact k-NN graph on the GPU (tsne_knn_exact) in the CSR form the joint P takes:
+    columns sorted within each row (the reference's neighbour indexing,
+    _backend.py:651-653), float64 distances."""
+    n = X.shape[0]
+    idx, dist, _ = be.knn_exact(X, k)
     order = np.argsort(idx, axis=1, kind="stable")
-    np.save(out + "_idx.npy", np.take_along_axis(idx, order, 1))
-    np.save(out + "_dist.npy", np.take_along_axis(dist, order, 1))
-
-
-def _child(args):
-    import subprocess
-    r = subprocess.run([sys.executable, os.path.abspath(__file__)] + args)
-    if r.returncode != 0:
-        raise RuntimeError(f"input preparation step {args[0]} failed ({r.returncode})")
+    cols = np.ascontiguousarray(np.take_along_axis(idx, order, 1)).ravel().astype(np.int32)
+    dd = np.ascontiguousarray(np.take_along_axis(dist, order, 1), dtype=np.float64).ravel()
+    return dd, cols, np.arange(0, (n + 1) * k, k, dtype=np.int32)
 
 
 def make_inputs(n, dim_in, n_clusters, k, perplexity, seed=0):
-    """Seeded Gaussian mixture -> exact k-NN (torch, child process) -> joint P on the GPU
-    (the library's perplexity kernel, symmetrisation and normalisation, equal bit for bit
-    to the reference's scipy arithmetic).
+    """Seeded Gaussian mixture -> exact k-NN graph (the library's device search) -> joint P
+    on the GPU (the library's perplexity kernel, symmetrisation and normalisation, equal
+    bit for bit to the reference's scipy arithmetic).
     Returns (row_ptr, col_idx, values) int32/int32/float64 of the joint P."""
-    import tempfile
     from tsne_h2pack._backend import get_backend
-    tmp = tempfile.mkdtemp(prefix="tsne_bench_")
-    out = os.path.join(tmp, "in")
+    be = get_backend()
+    X = mixture_points(n, dim_in, n_clusters, seed)
     t0 = time.time()
-    _child(["--_knn", out, str(n), str(dim_in), str(n_clusters), str(k), str(seed)])
-    log(f"k-NN graph: {time.time() - t0:.1f}s")
-    idx = np.load(out + "_idx.npy")
-    dist = np.load(out + "_dist.npy").astype(np.float64)
-    indptr = np.arange(0, (n + 1) * k, k, dtype=np.int32)
+    dd, cols, indptr = knn_csr(be, X, k)
+    del X
+    log(f"k-NN graph (device, exact): {time.time() - t0:.1f}s")
     t0 = time.time()
-    P = get_backend().joint_P_device(np.ascontiguousarray(dist.ravel()),
-                                     np.ascontiguousarray(idx.ravel().astype(np.int32)), indptr,
-                                     perplexity)
+    P = be.joint_P_device(dd, cols, indptr, perplexity)
     csr = (P.indptr.astype(np.int32), P.indices.astype(np.int32), P.data.astype(np.float64))
     del P
     log(f"joint P on the GPU (library kernels) nnz={csr[1].size}: {time.time() - t0:.2f}s")
-    import shutil
-    shutil.rmtree(tmp, ignore_errors=True)
     return csr
 
 
@@ -348,35 +313,18 @@ def _serving_specs(n_jobs, seed=0):
     return specs
 
 
-def _torch_knn_many(spec_file, out_dir):
-    """Child process: k-NN graphs of every serving job (torch on the GPU)."""
-    with open(spec_file) as f:
-        specs = json.load(f)
-    for j, sp in enumerate(specs):
-        _torch_knn(sp["n"], 50, sp["clusters"], 90, sp["seed"], os.path.join(out_dir, f"j{j}"))
-
-
 def serving_inputs(specs, tmp):
+    """Every job's exact k-NN graph (tsne_knn_exact) and joint P (device), saved to tmp."""
     from tsne_h2pack._backend import get_backend
-    spec_file = os.path.join(tmp, "specs.json")
-    with open(spec_file, "w") as f:
-        json.dump(specs, f)
-    t0 = time.time()
-    _child(["--_knn_many", spec_file, tmp])
-    log(f"serving: k-NN graphs of {len(specs)} jobs in {time.time() - t0:.1f}s")
     be = get_backend()
     t0 = time.time()
     for j, sp in enumerate(specs):
-        base = os.path.join(tmp, f"j{j}")
-        idx = np.load(base + "_idx.npy")
-        dist = np.load(base + "_dist.npy").astype(np.float64)
-        n, k = idx.shape
-        indptr = np.arange(0, (n + 1) * k, k, dtype=np.int32)
-        P = be.joint_P_device(np.ascontiguousarray(dist.ravel()),
-                              np.ascontiguousarray(idx.ravel().astype(np.int32)), indptr, 30.0)
-        np.savez(base + "_P.npz", ptr=P.indptr.astype(np.int32), idx=P.indices.astype(np.int32),
-                 val=P.data)
-    log(f"serving: joint P of every job on the GPU (library kernels) in {time.time() - t0:.1f}s")
+        dd, cols, indptr = knn_csr(be, mixture_points(sp["n"], 50, sp["clusters"], sp["seed"]), 90)
+        P = be.joint_P_device(dd, cols, indptr, 30.0)
+        np.savez(os.path.join(tmp, f"j{j}_P.npz"), ptr=P.indptr.astype(np.int32),
+                 idx=P.indices.astype(np.int32), val=P.data)
+    log(f"serving: k-NN graphs and joint P of {len(specs)} jobs on the GPU in "
+        f"{time.time() - t0:.1f}s")
 
 
 def _load_job_P(tmp, j, n):
@@ -656,6 +604,9 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for this process (serving; 0 leaves the default)")
     ap.add_argument("--seq-sample", type=int, default=32)
+    ap.add_argument("--blocking-sync", action="store_true",
+                    help="serving: host threads block (instead of spinning) while they wait "
+                         "for the GPU (hipDeviceScheduleBlockingSync, set before HIP starts)")
     ap.add_argument("--knn", action="store_true",
                     help="exact k-NN graph on the GPU (seconds per graph) instead of the "
                          "iteration rate")
@@ -683,6 +634,10 @@ def main():
                                    0 if a.no_cpu_baseline else a.knn_cpu_rows)), flush=True)
         return
     if a.serving:
+        if a.blocking_sync:
+            hip = ctypes.CDLL("libamdhip64.so")
+            rc = hip.hipSetDeviceFlags(ctypes.c_uint(4))  # hipDeviceScheduleBlockingSync
+            log(f"serving: hipSetDeviceFlags(BlockingSync) -> {rc}")
         if a.hw_queues:
             os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(a.hw_queues, 1), 16))
         slots = [int(x) for x in str(a.slots).split(",")]
@@ -735,10 +690,5 @@ if __name__ == "__main__":
         copy_probe()
     elif len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
         pmc_summary(sys.argv[2:])
-    elif len(sys.argv) > 1 and sys.argv[1] == "--_knn_many":
-        _torch_knn_many(sys.argv[2], sys.argv[3])
-    elif len(sys.argv) > 1 and sys.argv[1] == "--_knn":
-        _torch_knn(int(sys.argv[3]), int(sys.argv[4]), int(sys.argv[5]), int(sys.argv[6]),
-                   int(sys.argv[7]), sys.argv[2])
     else:
         main()

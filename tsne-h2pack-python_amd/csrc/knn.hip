@@ -18,7 +18,7 @@
 //     is within eps * (|x|^2 + |y|^2) of the exact d^2 (eps from the error analysis in
 //     knn_eps below); the screening key of candidate y is the estimate minus eps |y|^2, so
 //     key - eps |x|^2 <= d^2 <= key + eps |x|^2.
-//     One workgroup owns 128 queries (32 per wave, the operands of 2 x 4 MFMA tiles held
+//     One workgroup owns 128 queries (4 waves of 32, the operands of 2 x 4 MFMA tiles held
 //     in registers) and streams every candidate through LDS in chunks of 64 (direct
 //     global-to-LDS loads, two chunks ahead).  Each query keeps a threshold and a buffer
 //     in HBM: candidates whose key is below the threshold are appended (one compare per
@@ -50,7 +50,7 @@ namespace {
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int kQBMax = 256;    // largest query block of a workgroup (4 waves x 4 x 16 rows)
+constexpr int kQBMax = 256;    // largest query block of a workgroup (8 waves x 2 x 16 rows)
 constexpr int kCC = 64;        // candidates per chunk
 constexpr int kLMax = 1024;    // largest survivor count L
 constexpr int kFCap = 4096;    // fallback: largest number of points within the bound
@@ -284,29 +284,29 @@ __device__ __forceinline__ float wave_select_cut(KEntry *b, int m, int L, int *_
 // e of local query j at buf[j * cap + e].  On exit cnt[q] <= L entries and thr[q] hold
 // the survivors and the final threshold (+inf when nothing was ever cut).  KT: MFMA
 // k-steps (32 terms each) per K block; the packed rows hold nkb blocks.
-template <int KT, bool DMA, int RB>
-__global__ void __launch_bounds__(256)
+template <int KT, bool DMA, int RB, int NW>
+__global__ void __launch_bounds__(64 * NW)
 knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
                   const float *__restrict__ nq, int n, int nkb,
                   int q0, int L, int cap, KEntry *__restrict__ buf, int *__restrict__ cnt_out,
                   float *__restrict__ thr_out)
 {
     constexpr int QW = 16 * RB;        // queries per wave (RB 16-row MFMA blocks)
-    constexpr int QB = 4 * QW;         // queries per workgroup
+    constexpr int QB = NW * QW;        // queries per workgroup (NW waves)
+    constexpr int NT = 64 * NW;
     constexpr int SEG = 32 * KT;       // bf16 per row per K block
     constexpr int LROW = SEG + 8;      // LDS row (16-byte pad: conflict-free b128 reads)
     constexpr int UNITS = SEG / 8;     // 16-byte units per candidate per K block
-    constexpr int PER_T = kCC * UNITS / 256;  // 16-byte units each thread stages
-    static_assert(PER_T * 256 == kCC * UNITS, "chunk staging");
-    // direct-to-LDS staging: rounds of 256 slots of 16 bytes (row stride UNITS + 1 slots)
-    constexpr int ROUNDS = (kCC * (UNITS + 1) + 255) / 256;
-    constexpr int BSZ = DMA ? ROUNDS * 256 * 8 : kCC * LROW;
+    constexpr int PER_T = (kCC * UNITS + NT - 1) / NT;  // 16-byte units each thread stages
+    // direct-to-LDS staging: rounds of NT slots of 16 bytes (row stride UNITS + 1 slots)
+    constexpr int ROUNDS = (kCC * (UNITS + 1) + NT - 1) / NT;
+    constexpr int BSZ = DMA ? ROUNDS * NT * 8 : kCC * LROW;
     // DMA: three buffers, chunks fetched two stages ahead; registers: two buffers
     constexpr int NBUF = DMA ? 3 : 2;
     __shared__ __attribute__((aligned(16))) __bf16 Bs[NBUF][BSZ];
-    __shared__ int cnt_lds[4][QW];
-    __shared__ float thr_lds[4][QW];
-    __shared__ int hist[4][256];
+    __shared__ int cnt_lds[NW][QW];
+    __shared__ float thr_lds[NW][QW];
+    __shared__ int hist[NW][256];
 
     const int t = threadIdx.x, w = t >> 6, l = t & 63;
     const int qw0 = q0 + blockIdx.x * QB + w * QW;  // first query of this wave
@@ -342,29 +342,38 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
 
     // stage s = chunk s / nkb, K block s % nkb
     uint4 pre[DMA ? 1 : PER_T];
+    // DMA: slot r*NT + 64w + l of a buffer holds unit u of candidate cand (slot =
+    // cand * (UNITS + 1) + u); padding slots fetch a valid unit that is never read.  The
+    // per-lane element offsets within a chunk are fixed: computed once.
+    int doff[DMA ? ROUNDS : 1];
+    if constexpr (DMA) {
+#pragma unroll
+        for (int r = 0; r < ROUNDS; r++) {
+            const int slot = r * NT + w * 64 + l;
+            int cand = slot / (UNITS + 1), u = slot % (UNITS + 1);
+            if (cand >= kCC || u >= UNITS) {
+                cand = 0;
+                u = 0;
+            }
+            doff[r] = cand * rowlen + u * 8;
+        }
+    }
     auto fetch = [&](int s) {
         const int c0 = (s / nkb) * kCC, kb = s % nkb;
         if constexpr (DMA) {
-            // slot r*256 + 64w + l of the buffer holds unit u of candidate cand (slot =
-            // cand * (UNITS + 1) + u); padding slots fetch a valid unit that is never read
             const int b = s % NBUF;
+            const __bf16 *base = XB + (size_t)c0 * rowlen + kb * SEG;  // uniform
 #pragma unroll
-            for (int r = 0; r < ROUNDS; r++) {
-                const int slot = r * 256 + w * 64 + l;
-                int cand = slot / (UNITS + 1), u = slot % (UNITS + 1);
-                if (cand >= kCC || u >= UNITS) {
-                    cand = 0;
-                    u = 0;
-                }
+            for (int r = 0; r < ROUNDS; r++)
                 __builtin_amdgcn_global_load_lds(
-                    (const void *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + u * 8),
-                    (void __attribute__((address_space(3))) *)&Bs[b][(r * 256 + w * 64) * 8], 16, 0, 0);
-            }
+                    (const void *)(base + doff[r]),
+                    (void __attribute__((address_space(3))) *)&Bs[b][(r * NT + w * 64) * 8], 16, 0, 0);
         } else {
 #pragma unroll
             for (int r = 0; r < PER_T; r++) {
-                const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
-                pre[r] = *(const uint4 *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + part * 8);
+                const int u = r * NT + t, cand = u / UNITS, part = u % UNITS;
+                if (u < kCC * UNITS)
+                    pre[r] = *(const uint4 *)(XB + (size_t)(c0 + cand) * rowlen + kb * SEG + part * 8);
             }
         }
     };
@@ -372,8 +381,8 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
         if constexpr (!DMA) {
 #pragma unroll
             for (int r = 0; r < PER_T; r++) {
-                const int u = r * 256 + t, cand = u / UNITS, part = u % UNITS;
-                *(uint4 *)(&Bs[b][cand * LROW + part * 8]) = pre[r];
+                const int u = r * NT + t, cand = u / UNITS, part = u % UNITS;
+                if (u < kCC * UNITS) *(uint4 *)(&Bs[b][cand * LROW + part * 8]) = pre[r];
             }
         }
     };
@@ -418,14 +427,14 @@ knn_screen_kernel(const __bf16 *__restrict__ XA, const __bf16 *__restrict__ XB,
             // D layout: lane l holds column l%16, rows (l/16)*4 + i of each 16 x 16 tile.
             // Pass test per pair: A.B > (|x|^2 - threshold) / 2, one compare.
             const int c0 = (s / nkb) * kCC;
-            bool any = false;
+            unsigned long long anym = 0;  // compare results go to scalar masks
 #pragma unroll
             for (int cb = 0; cb < 4; cb++)
 #pragma unroll
                 for (int rb = 0; rb < RB; rb++)
 #pragma unroll
-                    for (int i = 0; i < 4; i++) any |= acc[rb][cb][i] > htq[rb][i];
-            if (__ballot(any)) {
+                    for (int i = 0; i < 4; i++) anym |= __ballot(acc[rb][cb][i] > htq[rb][i]);
+            if (anym) {
 #pragma unroll
                 for (int cb = 0; cb < 4; cb++) {
                     const int c = c0 + cb * 16 + (l & 15);
@@ -657,26 +666,32 @@ template <typename T> void kfree(T *&p, hipStream_t st)
     p = nullptr;
 }
 
+template <int KT, int NW>
+void launch_screen_nw(bool dma, int nqb, hipStream_t st, const __bf16 *XA, const __bf16 *XB, const float *nq,
+                      int n, int nkb, int q0, int L, int cap, KEntry *buf, int *cnt, float *thr)
+{
+    const int grid = nqb / (NW * 32);
+    if (dma)
+        hipLaunchKernelGGL((knn_screen_kernel<KT, true, 2, NW>), dim3(grid), dim3(64 * NW), 0, st, XA, XB, nq,
+                           n, nkb, q0, L, cap, buf, cnt, thr);
+    else
+        hipLaunchKernelGGL((knn_screen_kernel<KT, false, 2, NW>), dim3(grid), dim3(64 * NW), 0, st, XA, XB, nq,
+                           n, nkb, q0, L, cap, buf, cnt, thr);
+}
+
+// 2 row blocks (32 queries) per wave; 4 waves per workgroup (TSNE_KNN_WAVES=8: 8 waves,
+// 256 queries per staged chunk, one workgroup per CU: slower), candidate chunks staged
+// straight into LDS (TSNE_KNN_DMA=0 stages through registers).
 template <int KT>
-void launch_screen(int rb, int nqb, hipStream_t st, const __bf16 *XA, const __bf16 *XB, const float *nq,
-                   int n, int nkb, int q0, int L, int cap, KEntry *buf, int *cnt, float *thr)
+void launch_screen(int nqb, hipStream_t st, const __bf16 *XA, const __bf16 *XB, const float *nq, int n,
+                   int nkb, int q0, int L, int cap, KEntry *buf, int *cnt, float *thr)
 {
     static const bool dma = !getenv("TSNE_KNN_DMA") || atoi(getenv("TSNE_KNN_DMA")) != 0;
-    if (rb == 4) {
-        if (dma)
-            hipLaunchKernelGGL((knn_screen_kernel<KT, true, 4>), dim3(nqb / 256), dim3(256), 0, st, XA, XB, nq,
-                               n, nkb, q0, L, cap, buf, cnt, thr);
-        else
-            hipLaunchKernelGGL((knn_screen_kernel<KT, false, 4>), dim3(nqb / 256), dim3(256), 0, st, XA, XB, nq,
-                               n, nkb, q0, L, cap, buf, cnt, thr);
-    } else {
-        if (dma)
-            hipLaunchKernelGGL((knn_screen_kernel<KT, true, 2>), dim3(nqb / 128), dim3(256), 0, st, XA, XB, nq,
-                               n, nkb, q0, L, cap, buf, cnt, thr);
-        else
-            hipLaunchKernelGGL((knn_screen_kernel<KT, false, 2>), dim3(nqb / 128), dim3(256), 0, st, XA, XB, nq,
-                               n, nkb, q0, L, cap, buf, cnt, thr);
-    }
+    static const int nw = (getenv("TSNE_KNN_WAVES") && atoi(getenv("TSNE_KNN_WAVES")) == 8) ? 8 : 4;
+    if (nw == 4)
+        launch_screen_nw<KT, 4>(dma, nqb, st, XA, XB, nq, n, nkb, q0, L, cap, buf, cnt, thr);
+    else
+        launch_screen_nw<KT, 8>(dma, nqb, st, XA, XB, nq, n, nkb, q0, L, cap, buf, cnt, thr);
 }
 
 // Host check of the survivor buffers of one screening launch (TSNE_KNN_DEBUG): counts,
@@ -735,7 +750,6 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
     const int KT = (ksteps + nkb - 1) / nkb;
     const int kpad = 32 * KT * nkb;
     const int npad = (n + kQBMax - 1) / kQBMax * kQBMax;
-    const int rb = (getenv("TSNE_KNN_RB") && atoi(getenv("TSNE_KNN_RB")) == 4) ? 4 : 2;
     const int cap = std::max(4 * L, L + 2 * kCC);
     const double eps = knn_eps(kpad);
     // queries per screening launch: the survivor buffers stay under ~1 GiB
@@ -770,11 +784,11 @@ int knn_exact_device(const double *d_X, int n, int D, int k, int *d_idx, float *
         switch (KT) {
 #define KNN_SCREEN_CASE(v)                                                                          \
     case v:                                                                                         \
-        launch_screen<v>(rb, nqb, st, d_XA, d_XB, d_nq, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); \
+        launch_screen<v>(nqb, st, d_XA, d_XB, d_nq, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr); \
         break;
             KNN_SCREEN_CASE(1) KNN_SCREEN_CASE(2) KNN_SCREEN_CASE(3) KNN_SCREEN_CASE(4)
             KNN_SCREEN_CASE(5) KNN_SCREEN_CASE(6) KNN_SCREEN_CASE(7)
-            default: launch_screen<8>(rb, nqb, st, d_XA, d_XB, d_nq, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr);
+            default: launch_screen<8>(nqb, st, d_XA, d_XB, d_nq, n, nkb, q0, L, cap, d_buf, d_cnt, d_thr);
 #undef KNN_SCREEN_CASE
         }
         (void)hipEventRecord(ev[3], st);
