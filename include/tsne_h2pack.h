@@ -280,6 +280,17 @@ int tsne_quality_score(const double *Y, int n, int dim, const int *hd_idx, int h
                        const int *P_col_idx, const double *P_values, const int *anchors,
                        int n_anchors, tsne_quality_t *out);
 
+/* [add] Persistent proxy-point cache (host only, no device needed).  The reference keeps
+ * its proxy file across runs (src/tsne_h2pack.c:1093-1146; TSNE_H2PACK_CACHE_DIR, else
+ * $XDG_CACHE_HOME/tsne_h2pack, else $HOME/.cache/tsne_h2pack; TSNE_H2PACK_NO_PROXY_CACHE=1
+ * turns it off); this library does the same with its own versioned binary format (magic
+ * "TSNEPRXY", version, table parameters, counts, points, FNV-1a checksum).  Loads or
+ * generates (and stores) the proxy table an H2 build with (low_dim, h2_tol) uses, and
+ * reports the cache file path ("" when off), a digest of the table and whether it was read
+ * from disk.  0 on success, -1 for invalid arguments. */
+int tsne_proxy_cache_info(int low_dim, double h2_tol, char *path, int path_len,
+                          unsigned long long *digest, int *from_disk);
+
 /* ------------------------------------------------------------------------------------
  * [add] Job server: continuous batching of independent jobs on one GPU.  No counterpart
  * in the reference (one job per process there).  Jobs are admitted in arrival order

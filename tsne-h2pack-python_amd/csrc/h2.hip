@@ -531,3 +531,20 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
 }
 
 }  // namespace tsne
+
+extern "C" int tsne_proxy_cache_info(int low_dim, double h2_tol, char *path, int path_len,
+                                     unsigned long long *digest, int *from_disk)
+{
+    if ((low_dim != 2 && low_dim != 3) || !(h2_tol > 0.0)) {
+        fprintf(stderr, "tsne_proxy_cache_info: low_dim must be 2 or 3 and h2_tol > 0\n");
+        return -1;
+    }
+    // the proxy tolerance of the H2 build for this h2_tol (H2 setup above)
+    const double ptol = std::min(1e-2, std::max(1e-6, 0.15 * h2_tol));
+    const int pmax = tsne::pmax_of(low_dim);
+    const tsne::ProxyTable &t = tsne::proxy_table(low_dim, ptol, 1.0, pmax);
+    if (path && path_len > 0) snprintf(path, (size_t)path_len, "%s", tsne::proxy_cache_file(low_dim, ptol, 1.0, pmax).c_str());
+    if (digest) *digest = tsne::proxy_table_digest(t);
+    if (from_disk) *from_disk = t.from_disk ? 1 : 0;
+    return 0;
+}

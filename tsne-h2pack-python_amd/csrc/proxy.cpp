@@ -14,12 +14,30 @@
 // cached.  The reference's proxy points come from H2Pack's generator (cited from
 // src/tsne_h2pack.c:979-1060 of xinxing02/tsne-h2pack-python, library absent); this
 // ladder plays the role of its box-size-quantised proxy file (src/tsne_h2pack.c:679-690).
+//
+// Persistent cache.  As the reference keeps its proxy file across runs
+// (src/tsne_h2pack.c:1093-1146: TSNE_H2PACK_CACHE_DIR, else $XDG_CACHE_HOME/tsne_h2pack,
+// else $HOME/.cache/tsne_h2pack; TSNE_H2PACK_NO_PROXY_CACHE=1 turns it off), a generated
+// table is written to that directory and read back by later processes.  The file format
+// is this library's own, little-endian:
+//   "TSNEPRXY" | u32 version (1) | u32 D | u32 pmax | u32 levels | u32 ladder steps per
+//   octave | u32 0 | f64 tolerance | f64 eta | f64 ladder base | f64 outer radius |
+//   i32 count[levels] | f64 points[levels][pmax][D] | u64 FNV-1a of all bytes before it.
+// A file whose header does not match the requested table exactly, or whose checksum
+// fails, is ignored and replaced (written to a temporary name, then renamed).
 #include "proxy.h"
+
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <map>
+#include <string>
 #include <mutex>
 #include <random>
 #include <thread>
@@ -159,7 +177,122 @@ std::vector<double> make_set(int D, double h, double tol, double eta, int pmax, 
 std::mutex g_mu;
 std::map<std::tuple<int, long long, long long, int>, ProxyTable> g_cache;
 
+constexpr char kMagic[8] = {'T', 'S', 'N', 'E', 'P', 'R', 'X', 'Y'};
+constexpr uint32_t kFormatVersion = 1;
+
+uint64_t fnv1a(const unsigned char *p, size_t n, uint64_t h = 0xcbf29ce484222325ULL)
+{
+    for (size_t i = 0; i < n; i++) {
+        h ^= p[i];
+        h *= 0x100000001b3ULL;
+    }
+    return h;
+}
+
+std::string cache_dir()
+{
+    const char *off = getenv("TSNE_H2PACK_NO_PROXY_CACHE");
+    if (off && atoi(off) != 0) return "";
+    const char *env = getenv("TSNE_H2PACK_CACHE_DIR"), *xdg = getenv("XDG_CACHE_HOME"),
+               *home = getenv("HOME");
+    std::string dir;
+    if (env && env[0]) {
+        dir = env;
+    } else if (xdg && xdg[0]) {
+        dir = std::string(xdg) + "/tsne_h2pack";
+    } else if (home && home[0]) {
+        mkdir((std::string(home) + "/.cache").c_str(), 0700);
+        dir = std::string(home) + "/.cache/tsne_h2pack";
+    }
+    if (!dir.empty()) mkdir(dir.c_str(), 0700);  // an existing directory is fine
+    return dir;
+}
+
+std::vector<unsigned char> serialise(const ProxyTable &t, double tol, double eta)
+{
+    std::vector<unsigned char> b;
+    auto put = [&](const void *p, size_t n) {
+        const unsigned char *c = (const unsigned char *)p;
+        b.insert(b.end(), c, c + n);
+    };
+    const uint32_t hdr[6] = {kFormatVersion, (uint32_t)t.D, (uint32_t)t.pmax, (uint32_t)t.levels,
+                             (uint32_t)kProxyQ, 0u};
+    const double par[4] = {tol, eta, kProxyBase, kProxyOuter};
+    put(kMagic, 8);
+    put(hdr, sizeof hdr);
+    put(par, sizeof par);
+    put(t.cnt.data(), t.cnt.size() * sizeof(int));
+    put(t.pts.data(), t.pts.size() * sizeof(double));
+    const uint64_t h = fnv1a(b.data(), b.size());
+    put(&h, sizeof h);
+    return b;
+}
+
+// Reads a table for exactly (D, pmax, tol, eta) from `path`; false if absent or invalid.
+bool load_table(const std::string &path, int D, int pmax, double tol, double eta, ProxyTable &t)
+{
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::vector<unsigned char> b;
+    unsigned char buf[65536];
+    size_t r;
+    while ((r = fread(buf, 1, sizeof buf, f)) > 0) b.insert(b.end(), buf, buf + r);
+    fclose(f);
+    const size_t head = 8 + 6 * 4 + 4 * 8;
+    const size_t body = (size_t)kProxyLevels * sizeof(int) + (size_t)kProxyLevels * pmax * D * sizeof(double);
+    if (b.size() != head + body + 8 || memcmp(b.data(), kMagic, 8) != 0) return false;
+    uint64_t h;
+    memcpy(&h, b.data() + head + body, 8);
+    if (h != fnv1a(b.data(), head + body)) return false;
+    uint32_t hdr[6];
+    double par[4];
+    memcpy(hdr, b.data() + 8, sizeof hdr);
+    memcpy(par, b.data() + 8 + sizeof hdr, sizeof par);
+    if (hdr[0] != kFormatVersion || hdr[1] != (uint32_t)D || hdr[2] != (uint32_t)pmax ||
+        hdr[3] != (uint32_t)kProxyLevels || hdr[4] != (uint32_t)kProxyQ || par[0] != tol || par[1] != eta ||
+        par[2] != kProxyBase || par[3] != kProxyOuter)
+        return false;
+    t.D = D;
+    t.pmax = pmax;
+    t.levels = kProxyLevels;
+    t.cnt.resize(kProxyLevels);
+    t.pts.resize((size_t)kProxyLevels * pmax * D);
+    memcpy(t.cnt.data(), b.data() + head, kProxyLevels * sizeof(int));
+    memcpy(t.pts.data(), b.data() + head + kProxyLevels * sizeof(int), t.pts.size() * sizeof(double));
+    for (int c : t.cnt)
+        if (c < 0 || c > pmax) return false;
+    return true;
+}
+
+void store_table(const std::string &path, const ProxyTable &t, double tol, double eta)
+{
+    const std::vector<unsigned char> b = serialise(t, tol, eta);
+    const std::string tmp = path + "." + std::to_string((long)getpid()) + ".tmp";
+    FILE *f = fopen(tmp.c_str(), "wb");
+    if (!f) return;
+    const bool ok = fwrite(b.data(), 1, b.size(), f) == b.size();
+    if (fclose(f) != 0 || !ok || rename(tmp.c_str(), path.c_str()) != 0) unlink(tmp.c_str());
+}
+
+std::string cache_path(int D, double tol, double eta, int pmax)
+{
+    const std::string dir = cache_dir();
+    if (dir.empty()) return "";
+    char name[160];
+    snprintf(name, sizeof name, "/proxy_amd_v%u_%dd_tol%.6e_eta%.3f_p%d.bin", kFormatVersion, D, tol, eta,
+             pmax);
+    return dir + name;
+}
+
 }  // namespace
+
+std::string proxy_cache_file(int D, double tol, double eta, int pmax) { return cache_path(D, tol, eta, pmax); }
+
+uint64_t proxy_table_digest(const ProxyTable &t)
+{
+    uint64_t h = fnv1a((const unsigned char *)t.cnt.data(), t.cnt.size() * sizeof(int));
+    return fnv1a((const unsigned char *)t.pts.data(), t.pts.size() * sizeof(double), h);
+}
 
 double proxy_half_side(int k) { return kProxyBase * std::exp2((double)k / kProxyQ); }
 
@@ -179,6 +312,11 @@ const ProxyTable &proxy_table(int D, double tol, double eta, int pmax)
     auto it = g_cache.find(key);
     if (it != g_cache.end()) return it->second;
     ProxyTable t;
+    const std::string path = cache_path(D, tol, eta, pmax);
+    if (!path.empty() && load_table(path, D, pmax, tol, eta, t)) {
+        t.from_disk = true;
+        return g_cache.emplace(key, std::move(t)).first->second;
+    }
     t.D = D;
     t.pmax = pmax;
     t.levels = kProxyLevels;
@@ -198,6 +336,7 @@ const ProxyTable &proxy_table(int D, double tol, double eta, int pmax)
         });
     }
     for (auto &th : pool) th.join();
+    if (!path.empty()) store_table(path, t, tol, eta);
     return g_cache.emplace(key, std::move(t)).first->second;
 }
 

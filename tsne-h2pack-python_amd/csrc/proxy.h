@@ -1,5 +1,7 @@
 // proxy.h -- proxy point sets of the H2 build (see proxy.cpp).
 #pragma once
+#include <cstdint>
+#include <string>
 #include <vector>
 
 namespace tsne {
@@ -18,12 +20,18 @@ struct ProxyTable {
     int D = 0, pmax = 0, levels = 0;
     std::vector<double> pts;  // [levels][pmax][D], relative to the node centre
     std::vector<int> cnt;     // proxies per level
+    bool from_disk = false;   // read from the persistent cache instead of generated
 };
 
 double proxy_half_side(int k);
 // Smallest ladder index whose half-side is >= h.
 int proxy_level_for(double h);
-// Cached table for (D, tolerance, eta, pmax); thread safe.
+// Cached table for (D, tolerance, eta, pmax); thread safe.  Read from the persistent
+// cache file when a valid one exists, else generated and written there.
 const ProxyTable &proxy_table(int D, double tol, double eta, int pmax);
+// Path of the persistent cache file of that table ("" when the cache is off).
+std::string proxy_cache_file(int D, double tol, double eta, int pmax);
+// FNV-1a of the table's counts and points.
+uint64_t proxy_table_digest(const ProxyTable &t);
 
 }  // namespace tsne
