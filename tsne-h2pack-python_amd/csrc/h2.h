@@ -15,6 +15,11 @@ struct H2;
 H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st);
 void h2_destroy(H2 *h);
 
+// A second stream (default priority) for work of the build that nothing before the
+// product reads (the NEAR / W pair sorts): it runs there beside the ID.  The build joins
+// it before returning.  The stream must outlive the H2 object.
+void h2_set_aux_stream(H2 *h, hipStream_t st_aux);
+
 // Rebuild from the float32 gather copy Yg (caller order).  Returns 0, or -1 on a
 // device error or a non-finite coordinate (reported in *nonfinite).
 int h2_build(H2 *h, const void *Yg, int *nonfinite);

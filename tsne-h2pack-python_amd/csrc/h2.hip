@@ -23,6 +23,9 @@ struct H2 {
     int n = 0, D = 2, leaf = 100;
     double tol = 1e-2, ptol = 3e-3, id_tol = 5e-3;
     hipStream_t st = nullptr;
+    // optional second stream: the NEAR / W pair sorts run there beside the ID
+    hipStream_t st_aux = nullptr;
+    hipEvent_t ev_tree = nullptr, ev_sorted = nullptr;
     H2Dev d{};
     // buffers not in H2Dev
     uint64_t *keys_alt = nullptr;
@@ -172,23 +175,23 @@ bool alloc_sized(H2 *h)
     return hipMallocAsync(&h->cub_tmp, h->cub_bytes, h->st) == hipSuccess;
 }
 
-int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end)
+int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end, hipStream_t s)
 {
     H2Dev &d = h->d;
-    if (hipMemsetAsync(beg, 0, sizeof(int) * d.node_cap, h->st) != hipSuccess ||
-        hipMemsetAsync(end, 0, sizeof(int) * d.node_cap, h->st) != hipSuccess)
+    if (hipMemsetAsync(beg, 0, sizeof(int) * d.node_cap, s) != hipSuccess ||
+        hipMemsetAsync(end, 0, sizeof(int) * d.node_cap, s) != hipSuccess)
         return -1;
     if (n <= 0) return 0;
     int bits = 32;
     while ((1LL << (bits - 32)) < d.node_cap) bits++;
     rocprim::double_buffer<uint64_t> db(buf, d.sort_tmp);
     size_t bytes = h->cub_bytes;
-    if (sort_keys64(h->cub_tmp, bytes, db, n, bits, h->st) != hipSuccess)
+    if (sort_keys64(h->cub_tmp, bytes, db, n, bits, s) != hipSuccess)
         return -1;
     if (db.current() != buf) {  // the sorted list now lives in the scratch buffer:
         std::swap(buf, d.sort_tmp);  // exchange them (both hold maxpair entries)
     }
-    return launch_pair_offsets(buf, n, beg, end, h->st);
+    return launch_pair_offsets(buf, n, beg, end, s);
 }
 
 // One attempt at a rebuild.  Returns 0 (built), 1 (capacity exceeded: grown, retry),
@@ -278,10 +281,19 @@ int build_once(H2 *h, const void *Yg)
     h->n_near = h->pin[CNT_NEAR];
     h->n_leaves = h->pin[CNT_LEAF];
     // ADM and X ranges were written per node by the list kernel; NEAR and W pairs are
-    // appended by several levels and are grouped by target here
-    if (sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end) ||
-        sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end))
+    // appended by several levels and are grouped by target here.  Only the leaf product
+    // reads those lists, so with a second stream the sorts run there, beside the ID.
+    hipStream_t ss = st;
+    if (h->st_aux) {
+        if (hipEventRecord(h->ev_tree, st) != hipSuccess ||
+            hipStreamWaitEvent(h->st_aux, h->ev_tree, 0) != hipSuccess)
+            return -1;
+        ss = h->st_aux;
+    }
+    if (sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end, ss) ||
+        sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end, ss))
         return -1;
+    if (h->st_aux && hipEventRecord(h->ev_sorted, h->st_aux) != hipSuccess) return -1;
     // ID + upward: every leaf at once (they need nothing from other nodes), then the
     // internal nodes level by level, finest first
     const int nlev = (int)h->lvl.size() - 1;
@@ -291,6 +303,7 @@ int build_once(H2 *h, const void *Yg)
         if (launch_id_up(D, d, nullptr, h->lvl[lv], h->lvl[lv + 1] - h->lvl[lv], h->id_tol, h->pq,
                          st))
             return -1;
+    if (h->st_aux && hipStreamWaitEvent(st, h->ev_sorted, 0) != hipSuccess) return -1;
     if (hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
         hipStreamSynchronize(st) != hipSuccess)
         return -1;
@@ -355,9 +368,25 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
     return h;
 }
 
+void h2_set_aux_stream(H2 *h, hipStream_t st_aux)
+{
+    if (!h || h->st_aux || !st_aux) return;
+    if (hipEventCreateWithFlags(&h->ev_tree, hipEventDisableTiming) == hipSuccess &&
+        hipEventCreateWithFlags(&h->ev_sorted, hipEventDisableTiming) == hipSuccess) {
+        h->st_aux = st_aux;
+        return;
+    }
+    if (h->ev_tree) (void)hipEventDestroy(h->ev_tree);
+    if (h->ev_sorted) (void)hipEventDestroy(h->ev_sorted);
+    h->ev_tree = h->ev_sorted = nullptr;
+}
+
 void h2_destroy(H2 *h)
 {
     if (!h) return;
+    if (h->st_aux) (void)hipStreamSynchronize(h->st_aux);
+    if (h->ev_tree) (void)hipEventDestroy(h->ev_tree);
+    if (h->ev_sorted) (void)hipEventDestroy(h->ev_sorted);
     if (h->st) (void)hipStreamSynchronize(h->st);
     free_sized(h);
     H2Dev &d = h->d;

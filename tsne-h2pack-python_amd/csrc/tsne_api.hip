@@ -211,6 +211,7 @@ int rebuild(tsne_workspace *ws)
         ws->h2 = h2_create(ws->n, ws->dim, ws->params.h2_leaf_size > 0 ? ws->params.h2_leaf_size : 100,
                            ws->params.h2_tol > 0 ? ws->params.h2_tol : 1e-6, ws->st);
         if (!ws->h2) return -1;
+        if (ws->st2) h2_set_aux_stream(ws->h2, ws->st2);
     }
     int bad = 0;
     if (h2_build(ws->h2, ws->d_Yg[ws->cur], &bad) != 0) {
