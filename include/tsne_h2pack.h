@@ -258,6 +258,15 @@ typedef struct {
 int tsne_knn_exact(const double *X, int n_samples, int n_features, int k, int *indices,
                    float *distances, tsne_knn_stats_t *stats);
 
+/* [add] Joint P from the points, entirely on the device: the exact k-NN graph of
+ * tsne_knn_exact (float32 distances, as the reference's exact provider returns them), the
+ * reference's neighbour indexing (columns sorted per row, distances to float64;
+ * tsne_h2pack/_backend.py:647-674), then tsne_joint_P_from_knn_distances.  The result is
+ * the joint P those three steps give on the host, bit for bit.  Returns a matrix from
+ * sparse_matrix_create (release with sparse_matrix_destroy) or NULL.  stats may be NULL. */
+sparse_matrix_t *tsne_joint_P_from_points(const double *X, int n_samples, int n_features, int k,
+                                          double perplexity, tsne_knn_stats_t *stats);
+
 /* [add] Embedding quality on a common footing, on the device: the reference's score()
  * (benchmarks/quality.py:56-81).  Y: n x dim float64 row-major embedding; hd_idx: n x hd_k
  * high-dimensional neighbours per point in increasing distance (the shared k-NN graph, self

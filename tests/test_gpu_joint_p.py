@@ -95,3 +95,18 @@ def test_joint_P_feeds_the_optimisation(backend):
         finally:
             backend.lib.tsne_destroy(ws)
     assert np.array_equal(out[0], out[1])
+
+
+@pytest.mark.parametrize("n,dim,k,perp", [(3000, 20, 30, 10.0), (12000, 50, 90, 30.0)])
+def test_joint_P_from_points_equals_host_pipeline(backend, n, dim, k, perp):
+    """tsne_joint_P_from_points (k-NN graph, neighbour indexing and joint P on the device)
+    equals the reference's flow on the host -- sklearn's exact graph (the ExactKNN call),
+    prepare_knn_csr, conditional P, scipy symmetrisation -- bit for bit."""
+    _say(f"joint P from points n={n} D={dim} k={k}")
+    X, _ = gaussian_mixture(n, dim=dim, n_clusters=5, seed=n)
+    ref = _check(backend, knn_graph(X, k), perp)
+    P, st = backend.joint_P_from_points(X, k, perp)
+    _say(f"  device P nnz {P.nnz}, k-NN uncertain rows {st.uncertain_rows}")
+    assert np.array_equal(ref.indptr, P.indptr)
+    assert np.array_equal(ref.indices, P.indices)
+    assert np.array_equal(ref.data.view(np.uint64), P.data.view(np.uint64))

@@ -40,6 +40,8 @@
 #include <vector>
 
 #include "common.h"
+#include "joint_p.h"
+#include "kernels.h"
 #include "knn.h"
 #include "../../include/tsne_h2pack.h"
 
@@ -882,7 +884,119 @@ done:
     return rc;
 }
 
+namespace {
+
+// Row i of a k-NN graph (k entries, distance order) in the reference's neighbour indexing:
+// columns increasing, distances as float64 (one wave per row; indices of a row are
+// distinct, so an entry's rank is the number of smaller indices).
+__global__ void __launch_bounds__(256)
+row_colsort_kernel(int n, int k, const int *__restrict__ idx, const float *__restrict__ dist,
+                   int *__restrict__ col, double *__restrict__ dd)
+{
+    __shared__ int s_i[4][kLMax];
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    const int row = blockIdx.x * 4 + w;
+    if (row >= n) return;  // whole wave
+    const size_t o = (size_t)row * k;
+    for (int e = l; e < k; e += 64) s_i[w][e] = idx[o + e];
+    wave_sync();
+    for (int e = l; e < k; e += 64) {
+        const int c = s_i[w][e];
+        int r = 0;
+        for (int f = 0; f < k; f++) r += s_i[w][f] < c;
+        col[o + r] = c;
+        dd[o + r] = (double)dist[o + e];
+    }
+}
+
+__global__ void __launch_bounds__(256) arange_kernel(int n, int k, int *__restrict__ ptr)
+{
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i <= n) ptr[i] = i * k;
+}
+
+}  // namespace
+
 }  // namespace tsne
+
+extern "C" sparse_matrix_t *tsne_joint_P_from_points(const double *X, int n_samples, int n_features, int k,
+                                                     double perplexity, tsne_knn_stats_t *stats)
+{
+    if (stats) *stats = tsne_knn_stats_t{};
+    if (!X || n_samples < 2 || n_features < 1 || k < 1 || k >= n_samples || tsne::knn_survivors(k) > 1024 ||
+        !(perplexity > 0.0) || (long long)n_samples * k > 0x7fffffffLL) {
+        fprintf(stderr, "tsne_joint_P_from_points: invalid arguments (need 1 <= k < n_samples, k <= 900, "
+                        "perplexity > 0)\n");
+        return nullptr;
+    }
+    const size_t nx = (size_t)n_samples * n_features;
+    for (size_t i = 0; i < nx; i++)
+        if (!std::isfinite(X[i])) {
+            fprintf(stderr, "tsne_joint_P_from_points: X holds a value that is not finite\n");
+            return nullptr;
+        }
+    hipStream_t st;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) {
+        fprintf(stderr, "tsne_joint_P_from_points: no HIP device available\n");
+        return nullptr;
+    }
+    const int n = n_samples, nnz = n_samples * k;
+    double *d_X = nullptr, *d_dd = nullptr, *d_p = nullptr;
+    int *d_idx = nullptr, *d_col = nullptr, *d_ptr = nullptr;
+    float *d_dist = nullptr;
+    tsne::KnnStats S{};
+    tsne::JointP J{};
+    sparse_matrix_t *P = nullptr;
+    bool ok = hipMallocAsync((void **)&d_X, nx * sizeof(double), st) == hipSuccess &&
+              hipMallocAsync((void **)&d_idx, (size_t)nnz * sizeof(int), st) == hipSuccess &&
+              hipMallocAsync((void **)&d_dist, (size_t)nnz * sizeof(float), st) == hipSuccess &&
+              hipMemcpyAsync(d_X, X, nx * sizeof(double), hipMemcpyHostToDevice, st) == hipSuccess &&
+              tsne::knn_exact_device(d_X, n, n_features, k, d_idx, d_dist, &S, st) == 0;
+    if (d_X) (void)hipFreeAsync(d_X, st);
+    ok = ok && hipMallocAsync((void **)&d_col, (size_t)nnz * sizeof(int), st) == hipSuccess &&
+         hipMallocAsync((void **)&d_dd, (size_t)nnz * sizeof(double), st) == hipSuccess &&
+         hipMallocAsync((void **)&d_ptr, ((size_t)n + 1) * sizeof(int), st) == hipSuccess &&
+         hipMallocAsync((void **)&d_p, (size_t)nnz * sizeof(double), st) == hipSuccess;
+    if (ok) {
+        hipLaunchKernelGGL(tsne::row_colsort_kernel, dim3((n + 3) / 4), dim3(256), 0, st, n, k, d_idx, d_dist,
+                           d_col, d_dd);
+        hipLaunchKernelGGL(tsne::arange_kernel, dim3((n + 1 + 255) / 256), dim3(256), 0, st, n, k, d_ptr);
+        ok = hipGetLastError() == hipSuccess &&
+             tsne::launch_perplexity(n, d_dd, d_ptr, perplexity, d_p, st) == 0 &&
+             tsne::joint_P_device(n, nnz, d_ptr, d_col, d_p, &J, st) == 0;
+    }
+    if (ok) {
+        P = sparse_matrix_create(n, n, J.nnz);
+        ok = P && hipMemcpyAsync(P->row_ptr, J.rp, ((size_t)n + 1) * sizeof(int), hipMemcpyDeviceToHost, st) ==
+                      hipSuccess &&
+             (J.nnz == 0 ||
+              (hipMemcpyAsync(P->col_idx, J.col, (size_t)J.nnz * sizeof(int), hipMemcpyDeviceToHost, st) ==
+                   hipSuccess &&
+               hipMemcpyAsync(P->values, J.val, (size_t)J.nnz * sizeof(double), hipMemcpyDeviceToHost, st) ==
+                   hipSuccess)) &&
+             hipStreamSynchronize(st) == hipSuccess;
+    }
+    for (void *q : {(void *)d_idx, (void *)d_dist, (void *)d_col, (void *)d_dd, (void *)d_ptr, (void *)d_p,
+                    (void *)J.rp, (void *)J.col, (void *)J.val})
+        if (q) (void)hipFreeAsync(q, st);
+    (void)hipStreamSynchronize(st);
+    (void)hipStreamDestroy(st);
+    if (!ok) {
+        fprintf(stderr, "tsne_joint_P_from_points: device failure\n");
+        if (P) sparse_matrix_destroy(P);
+        return nullptr;
+    }
+    if (stats) {
+        stats->survivors = S.survivors;
+        stats->uncertain_rows = S.uncertain_rows;
+        stats->overflow_rows = S.overflow_rows;
+        stats->eps = S.eps;
+        stats->device_ms = S.device_ms;
+        stats->screen_ms = S.screen_ms;
+        stats->screen_flops = S.screen_flops;
+    }
+    return P;
+}
 
 extern "C" int tsne_knn_exact(const double *X, int n_samples, int n_features, int k, int *indices,
                               float *distances, tsne_knn_stats_t *stats)

@@ -334,6 +334,25 @@ class TSNEBackend:
             raise RuntimeError("tsne_quality_score failed (see stderr)")
         return dict(kl=out.kl, knn_preservation=out.knn_preservation, Z=out.Z, span=out.span)
 
+    def joint_P_from_points(self, X, k, perplexity):
+        """tsne_joint_P_from_points: exact k-NN graph, neighbour indexing and joint P, all on
+        the device; equal bit for bit to joint_P(knn graph of ExactKNN).  Returns (scipy CSR,
+        KnnStats)."""
+        from scipy.sparse import csr_matrix
+        X = np.ascontiguousarray(X, dtype=np.float64)
+        n, d = X.shape
+        st = KnnStats()
+        ptr = self._lib.tsne_joint_P_from_points(X, n, d, int(k), ctypes.c_double(perplexity),
+                                                 ctypes.byref(st))
+        if not ptr:
+            raise RuntimeError("tsne_joint_P_from_points returned NULL")
+        m = ptr.contents
+        rp = np.ctypeslib.as_array(m.row_ptr, shape=(m.n_rows + 1,)).copy()
+        ci = np.ctypeslib.as_array(m.col_idx, shape=(max(m.nnz, 1),))[:m.nnz].copy()
+        va = np.ctypeslib.as_array(m.values, shape=(max(m.nnz, 1),))[:m.nnz].copy()
+        self._lib.sparse_matrix_destroy(ptr)
+        return csr_matrix((va, ci, rp), shape=(n, n)), st
+
     def joint_P(self, knn_distances_csr, perplexity, num_threads=1):
         """Neighbour indexing on the host, then the joint P on the device."""
         data, indices, indptr = self.prepare_knn_csr(knn_distances_csr)
@@ -482,6 +501,9 @@ def configure_signatures(lib):
         np.ctypeslib.ndpointer(dtype=np.float32, flags='C_CONTIGUOUS'),
         ctypes.POINTER(KnnStats)]
     lib.tsne_knn_exact.restype = ctypes.c_int
+    lib.tsne_joint_P_from_points.argtypes = [f64c, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_double, ctypes.POINTER(KnnStats)]
+    lib.tsne_joint_P_from_points.restype = c_sm
     i32p = ctypes.POINTER(ctypes.c_int)
     lib.tsne_quality_score.argtypes = [
         ctypes.POINTER(DTYPE), ctypes.c_int, ctypes.c_int, i32p, ctypes.c_int, ctypes.c_int,
