@@ -52,7 +52,8 @@ def test_knn_matches_sklearn_fixture(backend):
 
 
 @pytest.mark.parametrize("n,dim,k", [(777, 3, 10), (2049, 50, 90), (3000, 100, 150),
-                                     (1500, 300, 30), (6000, 50, 90), (300, 64, 299)])
+                                     (1500, 300, 30), (6000, 50, 90), (300, 64, 299),
+                                     (10, 1, 3), (65, 2, 64), (1100, 1030, 20)])
 def test_knn_matches_oracle(backend, n, dim, k):
     _say(f"knn: mixture n={n} D={dim} k={k}")
     X = _mixture(n, dim, 6, seed=n + dim)
