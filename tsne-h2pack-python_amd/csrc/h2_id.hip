@@ -19,6 +19,7 @@
 // against proxy points; src/tsne_h2pack.c:818, :908-919) and the upward sweep of the
 // fused product (src/tsne_h2_matmul.c:309-334).  One workgroup per node, one thread per
 // candidate column.
+#include <algorithm>
 #include <type_traits>
 
 #include "h2_internal.h"
@@ -91,9 +92,14 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     const RootInfo root = *h.root;
     const int kq = min(max(root.k0 - l, 0), kProxyLevels - 1);
     const int p = min(h.proxy_cnt[kq], PMAX);
-    double px[3] = {0.0, 0.0, 0.0};  // proxy t (PMAX <= NT), relative to the centre
-    if (t < p)
-        for (int d = 0; d < D; d++) px[d] = h.proxy_pts[((size_t)kq * PSTRIDE + t) * D + d];
+    // proxies t, t + NT, ... (relative to the centre) staged by this thread
+    constexpr int PPT = (PMAX + NT - 1) / NT;
+    double px[PPT][3] = {};
+#pragma unroll
+    for (int u = 0; u < PPT; u++)
+        if (t + u * NT < p)
+            for (int d = 0; d < D; d++)
+                px[u][d] = h.proxy_pts[((size_t)kq * PSTRIDE + t + u * NT) * D + d];
     int nb = 0, mt = 0, nc = 0;
     int rk[NCH];
     if (leafn) {
@@ -179,8 +185,10 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     // ---- proxies of this level, translated to the centre.  Everything below runs over
     // all PMAX proxy slots and all RMAX rank slots without branches: unused slots hold
     // zeros (scale 0, zero Q rows, identity R rows), which keeps the FP64 pipelines full.
-    if (t < PMAX)
-        for (int d = 0; d < D; d++) s_prox[t][d] = cd[d] + px[d];
+#pragma unroll
+    for (int u = 0; u < PPT; u++)
+        if (t + u * NT < PMAX)
+            for (int d = 0; d < D; d++) s_prox[t + u * NT][d] = cd[d] + px[u][d];
     for (int e = t; e < RMAX * RMAX; e += NT) s_R[e / RMAX][e % RMAX] = (e / RMAX == e % RMAX) ? 1.0 : 0.0;
     __syncthreads();
 
@@ -406,21 +414,35 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
 }
 
 // list != nullptr: the nodes list[0..count) (the leaves); otherwise the internal nodes
-// among [lb, lb + count).  One workgroup per node, in one size class.
+// among [lb, lb + count).  One workgroup per node at a time, in one size class; a grid
+// smaller than `count` walks the nodes with a stride, so a launch whose class holds few
+// of them costs few workgroups.
 template <int D, int NT, int PQ>
 __global__ void __launch_bounds__(NT)
-id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int m_lo, double tol)
+id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo, double tol)
 {
-    const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
-    if (!list && h.child0[node] < 0) return;  // leaves were done in their own launch
-    id_up_node<D, NT, PQ>(h, node, m_lo, tol);
+    for (int b = blockIdx.x; b < count; b += gridDim.x) {
+        const int node = list ? list[b] : lb + b;
+        __syncthreads();  // the previous node's LDS reads are complete
+        if (!list && h.child0[node] < 0) continue;  // leaves were done in their own launch
+        id_up_node<D, NT, PQ>(h, node, m_lo, tol);
+    }
 }
 
 // Nodes with at most 64 candidates run in single-wavefront workgroups (their barriers
-// are free), larger ones in 128 threads (2D); 3D uses 384 threads.  A level of at most
-// kOneClassLevel nodes (the top of the tree, where every launch is one node's latency)
-// runs all its nodes in one 128-thread launch instead of two.
+// are free), larger ones in 128 threads (2D).  A level of at most kOneClassLevel nodes
+// (the top of the tree, where every launch is one node's latency) runs all its nodes in
+// one launch of the largest class instead of one per class.
+// 3D (up to 8 x 48 candidates, 96 proxies): the 384-thread class needs ~106 KB of LDS, so
+// one workgroup per CU with six waves behind every barrier.  Leaves (tens of points
+// typically) run in classes of 64 and 128 threads (~60 KB, two per CU), on grids that
+// walk the leaf list with a stride so that a class's launch costs few workgroups when it
+// holds few leaves.  Internal levels keep one launch of the 384-thread class with one
+// workgroup per node: many of their nodes take the single-child shortcut, and walking
+// those one after another in a strided grid cost more than the smaller classes saved.
 constexpr int kOneClassLevel = 512;
+constexpr int kLeafGridSmall = 4096;  // workgroups of a 3D leaf launch, 64/128 threads
+constexpr int kLeafGridLarge = 1024;  // same, 384 threads (leaves of > 128 points)
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
                  int pq, hipStream_t st)
 {
@@ -429,18 +451,28 @@ int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, doub
         const bool one = count <= kOneClassLevel;
         if (pq <= 32) {
             if (!one)
-                hipLaunchKernelGGL((id_up_kernel<2, 64, 32>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
+                hipLaunchKernelGGL((id_up_kernel<2, 64, 32>), dim3(count), dim3(64), 0, st, h, list, lb,
+                                   count, 0, tol);
             hipLaunchKernelGGL((id_up_kernel<2, 128, 32>), dim3(count), dim3(128), 0, st, h, list, lb,
-                               one ? 0 : 64, tol);
+                               count, one ? 0 : 64, tol);
         } else {
             if (!one)
-                hipLaunchKernelGGL((id_up_kernel<2, 64, 48>), dim3(count), dim3(64), 0, st, h, list, lb, 0, tol);
+                hipLaunchKernelGGL((id_up_kernel<2, 64, 48>), dim3(count), dim3(64), 0, st, h, list, lb,
+                                   count, 0, tol);
             hipLaunchKernelGGL((id_up_kernel<2, 128, 48>), dim3(count), dim3(128), 0, st, h, list, lb,
-                               one ? 0 : 64, tol);
+                               count, one ? 0 : 64, tol);
         }
+    } else if (!list) {
+        hipLaunchKernelGGL((id_up_kernel<3, 384, 96>), dim3(count), dim3(384), 0, st, h, list, lb,
+                           count, 0, tol);
     } else {
-        hipLaunchKernelGGL((id_up_kernel<3, 384, 96>), dim3(count), dim3(384), 0, st, h, list, lb, 0,
-                           tol);
+        const int gs = std::min(count, kLeafGridSmall), gl = std::min(count, kLeafGridLarge);
+        hipLaunchKernelGGL((id_up_kernel<3, 64, 96>), dim3(gs), dim3(64), 0, st, h, list, lb, count,
+                           0, tol);
+        hipLaunchKernelGGL((id_up_kernel<3, 128, 96>), dim3(gs), dim3(128), 0, st, h, list, lb, count,
+                           64, tol);
+        hipLaunchKernelGGL((id_up_kernel<3, 384, 96>), dim3(gl), dim3(384), 0, st, h, list, lb, count,
+                           128, tol);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
