@@ -133,9 +133,10 @@ bool alloc_sized(H2 *h)
     d.w_cap = d.x_cap;
     d.nl_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 16 : 64));
     d.pend_cap = (int)std::min<double>(2e9, pf * nc * (D == 2 ? 8 : 32));
-    // U pool: n x RMAX floats for the leaves (placed by point), then the internal nodes'
-    // m x RMAX reservations (grown on overflow)
-    d.pool_cap = (long long)h->n * RMAX + (long long)(h->pool_f * (double)nc * (D == 2 ? 24 : 96) * RMAX);
+    // U pool: n x 2 RMAX floats for the leaves (placed by point), then the internal
+    // nodes' m x 2 RMAX reservations (grown on overflow).  A node's region holds its R
+    // entries in FP64 while its ID runs, then its U (m x r floats).
+    d.pool_cap = 2LL * h->n * RMAX + (long long)(h->pool_f * (double)nc * (D == 2 ? 24 : 32) * 2 * RMAX);
     const int maxpair = std::max({d.adm_cap, d.x_cap, d.w_cap, d.nl_cap});
     bool ok = dmalloc(&d.nb, nc, h->st) && dmalloc(&d.ne, nc, h->st) && dmalloc(&d.level, nc, h->st) &&
               dmalloc(&d.parent, nc, h->st) && dmalloc(&d.child0, nc, h->st) && dmalloc(&d.nchild, nc, h->st) &&

@@ -6,7 +6,11 @@
 // its centre, A = K(candidates, P) (m x p) is column-normalised and row-ID'd by
 // column-pivoted modified Gram-Schmidt on A^T in FP64:  A ~= U A[J, :], |J| = r, where
 // the pivoting stops once the largest residual column norm is below tol (relative to
-// the first pivot).  U (m x r, FP32) and the skeleton J are stored; then the upward
+// the first pivot).  The Gram-Schmidt dot products are the entries of R: each candidate
+// stores its column of R (FP64) in the node's region of the U pool as the pivoting runs,
+// and U = R11^{-1} R is formed from them, so R is never recomputed and only the current
+// row of Q is kept.  U (m x r, FP32, written over the R entries) and the skeleton J are
+// stored; then the upward
 // coefficients w_i = U^T X_i are formed, X_i holding one row [1, y - c_i] per leaf point
 // or, for an internal node, the children's coefficients moved to c_i
 // ([w0, wY + (c_child - c_i) w0]).
@@ -64,21 +68,18 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     constexpr int PSTRIDE = H2Cfg<D>::PMAX;  // row stride of the proxy table
     static_assert(PMAX % 4 == 0 && PMAX <= PSTRIDE, "proxy slots");
     constexpr int NW = NT / 64;
-    constexpr size_t QBYTES = sizeof(double) * RMAX * PMAX;
-    constexpr size_t UBYTES = sizeof(float) * NT * RMAX;
     __shared__ float4 s_xin[NT];
     __shared__ double s_prox[PMAX][D];
     __shared__ double s_part[NW][PMAX];
     __shared__ double s_scale[PMAX];
-    __shared__ __align__(16) unsigned char s_qu[QBYTES > UBYTES ? QBYTES : UBYTES];
+    __shared__ double s_Q[2][PMAX];  // Q row of the current step (two, alternating)
+    __shared__ float s_U[NT][RMAX];
     __shared__ double s_R[RMAX][RMAX];
     __shared__ int s_piv[RMAX];
     __shared__ double s_rv[NW];
     __shared__ int s_ri[NW];
     __shared__ double s_bv;
     __shared__ int s_bi, s_base;
-    double(*s_Q)[PMAX] = reinterpret_cast<double(*)[PMAX]>(s_qu);
-    float(*s_U)[RMAX] = reinterpret_cast<float(*)[RMAX]>(s_qu);
 
     constexpr int NCH = H2Cfg<D>::NCH;
     const int t = threadIdx.x;
@@ -120,12 +121,14 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     // of the leaves (the reservation is issued now, its latency hidden by the ID)
     // (region blockIdx mod kPoolParts of the internal pool, so that the reservations of
     // one launch spread over kPoolParts counters)
+    // Regions are m x 2 RMAX floats: the FP64 R entries (r x m) while the ID runs, then
+    // U (m x r floats) written over them.  Offsets are even (8-byte aligned doubles).
     const int part = blockIdx.x % kPoolParts;
-    const long long part_cap = (h.pool_cap - (long long)h.n * RMAX) / kPoolParts;
+    const long long part_cap = pool_part_cap(h, RMAX);
     if (t == 0)
-        s_base = leafn ? nb * RMAX
-                       : (int)((long long)h.n * RMAX + part * part_cap +
-                               atomicAdd(&h.cnt[CNT_POOL + part * kCntSlot], m * RMAX));
+        s_base = leafn ? nb * 2 * RMAX
+                       : (int)(2LL * h.n * RMAX + part * part_cap +
+                               atomicAdd(&h.cnt[CNT_POOL + part * kCntSlot], 2 * m * RMAX));
     const double cd[3] = {(double)c.x, (double)c.y, (double)c.z};
 
     // ---- candidates and the rows of X
@@ -162,8 +165,8 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             // single child: same skeleton, U = I, coefficients moved to this centre
             __syncthreads();
             const int base = s_base;
-            const bool fits = (long long)base + (long long)m * RMAX <=
-                              (long long)h.n * RMAX + (part + 1) * part_cap;
+            const bool fits = (long long)base + 2LL * m * RMAX <=
+                              2LL * h.n * RMAX + (part + 1) * part_cap;
             if (t == 0) {
                 if (!fits) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
                 h.rank[node] = m;
@@ -262,7 +265,12 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
         nrm = (t < m) ? (n0 + n1) + (n2 + n3) : -1.0;
     }
 
-    // ---- column-pivoted MGS
+    // ---- column-pivoted MGS.  R[k][t] goes to rpool[k * m + t] (the node's region; an
+    // internal node whose reservation overflowed the pool writes nothing, and the build
+    // is redone with a larger pool)
+    const int rbase = s_base;
+    const bool rfits = leafn || (long long)rbase + 2LL * m * RMAX <= 2LL * h.n * RMAX + (part + 1) * part_cap;
+    double *rpool = reinterpret_cast<double *>(h.upool + rbase);
     bool selected = false;
     double nrm0 = 0.0;
     int r = 0;
@@ -296,9 +304,11 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
         if (k >= rlim || !(bv > tol * tol * nrm0) || !(bv > 0.0)) break;
         if (t == bi) {
             selected = true;
-            const double inv = 1.0 / sqrt(bv);
+            const double rd = sqrt(bv);
+            s_R[k][k] = rd;  // R[k][k]
+            const double inv = 1.0 / rd;
 #pragma unroll
-            for (int q = 0; q < PMAX; q++) s_Q[k][q] = a[q] * inv;
+            for (int q = 0; q < PMAX; q++) s_Q[k & 1][q] = a[q] * inv;
             s_piv[k] = t;
         }
         __syncthreads();
@@ -306,19 +316,20 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
 #pragma unroll
             for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
-                d0 = fma(s_Q[k][q], a[q], d0);
-                d1 = fma(s_Q[k][q + 1], a[q + 1], d1);
-                d2 = fma(s_Q[k][q + 2], a[q + 2], d2);
-                d3 = fma(s_Q[k][q + 3], a[q + 3], d3);
+                d0 = fma(s_Q[k & 1][q], a[q], d0);
+                d1 = fma(s_Q[k & 1][q + 1], a[q + 1], d1);
+                d2 = fma(s_Q[k & 1][q + 2], a[q + 2], d2);
+                d3 = fma(s_Q[k & 1][q + 3], a[q + 3], d3);
             }
-            const double dot = (d0 + d1) + (d2 + d3);
+            const double dot = (d0 + d1) + (d2 + d3);  // R[k][t]
+            if (rfits) rpool[(size_t)k * m + t] = dot;
             double n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
 #pragma unroll
             for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
-                a[q] = fma(-dot, s_Q[k][q], a[q]);
-                a[q + 1] = fma(-dot, s_Q[k][q + 1], a[q + 1]);
-                a[q + 2] = fma(-dot, s_Q[k][q + 2], a[q + 2]);
-                a[q + 3] = fma(-dot, s_Q[k][q + 3], a[q + 3]);
+                a[q] = fma(-dot, s_Q[k & 1][q], a[q]);
+                a[q + 1] = fma(-dot, s_Q[k & 1][q + 1], a[q + 1]);
+                a[q + 2] = fma(-dot, s_Q[k & 1][q + 2], a[q + 2]);
+                a[q + 3] = fma(-dot, s_Q[k & 1][q + 3], a[q + 3]);
                 n0 = fma(a[q], a[q], n0);
                 n1 = fma(a[q + 1], a[q + 1], n1);
                 n2 = fma(a[q + 2], a[q + 2], n2);
@@ -337,29 +348,21 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             if (s_piv[k] == t) kk = k;
     auto tail = [&](auto rb) {
         constexpr int RB = decltype(rb)::value;
-        // unused rank slots below RB: zero Q rows (R rows there are identity)
-        for (int e = t; e < (RB - r) * PMAX; e += NT) s_Q[r + e / PMAX][e % PMAX] = 0.0;
-        __syncthreads();
-
-        // ---- interpolation coefficients x_t = R11^{-1} Q^T a_t
+        // this thread's column of R: rows 0..r-1 of a candidate, rows 0..kk of a pivot
         double rt[RB];
+        const int rows = (kk >= 0) ? kk : r;
 #pragma unroll
-        for (int k = 0; k < RB; k++) rt[k] = 0.0;
-        if (t < m) {
-            for (int q = 0; q < PMAX; q++) {
-                const double v = kern64(my, s_prox[q], D) * s_scale[q];
-#pragma unroll
-                for (int k = 0; k < RB; k++) rt[k] = fma(s_Q[k][q], v, rt[k]);
-            }
-        }
+        for (int k = 0; k < RB; k++)
+            rt[k] = (t < m && k < rows && rfits) ? rpool[(size_t)k * m + t] : 0.0;
         if (kk >= 0) {
 #pragma unroll
             for (int j = 0; j < RB; j++)
-                if (j <= kk) s_R[j][kk] = rt[j];
+                if (j < kk) s_R[j][kk] = rt[j];  // the diagonal was set at selection
         }
-        __syncthreads();  // s_R complete
+        __syncthreads();  // s_R complete; every R entry of the pool read
         if (t < RB) s_part[0][t] = 1.0 / s_R[t][t];  // s_part reused: inverse diagonal
-        __syncthreads();  // s_Q no longer read below (s_U aliases it)
+        __syncthreads();
+        // ---- interpolation coefficients x_t = R11^{-1} R[:, t]
         if (kk >= 0) {
 #pragma unroll
             for (int k = 0; k < RB; k++) rt[k] = (k == kk) ? 1.0 : 0.0;
@@ -385,8 +388,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     // ---- store U (pool), skeleton, and the upward coefficients
     if (t == 0) {
         const int base = s_base;
-        if (!leafn && (long long)base + (long long)m * RMAX > (long long)h.n * RMAX + (part + 1) * part_cap)
-            atomicOr(&h.flags[FLAG_POOL_OVF], 1);
+        if (!rfits) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
         h.rank[node] = r;
         h.uoff[node] = base;
     }
@@ -397,7 +399,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     }
     __syncthreads();
     const int base = s_base;
-    if (leafn || (long long)base + (long long)m * RMAX <= (long long)h.n * RMAX + (part + 1) * part_cap)
+    if (rfits)
         for (int e = t; e < m * r; e += NT) h.upool[(size_t)base + e] = s_U[e / r][e % r];
     if (t < r) {
         float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -414,17 +416,24 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
 }
 
 // list != nullptr: the nodes list[0..count) (the leaves); otherwise the internal nodes
-// among [lb, lb + count).  One workgroup per node at a time, in one size class; a grid
-// smaller than `count` walks the nodes with a stride, so a launch whose class holds few
-// of them costs few workgroups.
-template <int D, int NT, int PQ>
+// among [lb, lb + count).  One workgroup per node at a time, in one size class.  STRIDE:
+// a grid smaller than `count` walks the nodes with a stride, so a launch whose class
+// holds few of them costs few workgroups; otherwise one workgroup per node (the grid is
+// `count`, and the loop's state does not compete with the ID for registers).
+template <int D, int NT, int PQ, bool STRIDE>
 __global__ void __launch_bounds__(NT)
 id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo, double tol)
 {
-    for (int b = blockIdx.x; b < count; b += gridDim.x) {
-        const int node = list ? list[b] : lb + b;
-        __syncthreads();  // the previous node's LDS reads are complete
-        if (!list && h.child0[node] < 0) continue;  // leaves were done in their own launch
+    if constexpr (STRIDE) {
+        for (int b = blockIdx.x; b < count; b += gridDim.x) {
+            const int node = list ? list[b] : lb + b;
+            __syncthreads();  // the previous node's LDS reads are complete
+            if (!list && h.child0[node] < 0) continue;  // leaves were done in their own launch
+            id_up_node<D, NT, PQ>(h, node, m_lo, tol);
+        }
+    } else {
+        const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
+        if (!list && h.child0[node] < 0) return;
         id_up_node<D, NT, PQ>(h, node, m_lo, tol);
     }
 }
@@ -451,27 +460,27 @@ int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, doub
         const bool one = count <= kOneClassLevel;
         if (pq <= 32) {
             if (!one)
-                hipLaunchKernelGGL((id_up_kernel<2, 64, 32>), dim3(count), dim3(64), 0, st, h, list, lb,
+                hipLaunchKernelGGL((id_up_kernel<2, 64, 32, false>), dim3(count), dim3(64), 0, st, h, list, lb,
                                    count, 0, tol);
-            hipLaunchKernelGGL((id_up_kernel<2, 128, 32>), dim3(count), dim3(128), 0, st, h, list, lb,
+            hipLaunchKernelGGL((id_up_kernel<2, 128, 32, false>), dim3(count), dim3(128), 0, st, h, list, lb,
                                count, one ? 0 : 64, tol);
         } else {
             if (!one)
-                hipLaunchKernelGGL((id_up_kernel<2, 64, 48>), dim3(count), dim3(64), 0, st, h, list, lb,
+                hipLaunchKernelGGL((id_up_kernel<2, 64, 48, false>), dim3(count), dim3(64), 0, st, h, list, lb,
                                    count, 0, tol);
-            hipLaunchKernelGGL((id_up_kernel<2, 128, 48>), dim3(count), dim3(128), 0, st, h, list, lb,
+            hipLaunchKernelGGL((id_up_kernel<2, 128, 48, false>), dim3(count), dim3(128), 0, st, h, list, lb,
                                count, one ? 0 : 64, tol);
         }
     } else if (!list) {
-        hipLaunchKernelGGL((id_up_kernel<3, 384, 96>), dim3(count), dim3(384), 0, st, h, list, lb,
-                           count, 0, tol);
+        hipLaunchKernelGGL((id_up_kernel<3, 384, 96, false>), dim3(count), dim3(384), 0, st, h, list,
+                           lb, count, 0, tol);
     } else {
         const int gs = std::min(count, kLeafGridSmall), gl = std::min(count, kLeafGridLarge);
-        hipLaunchKernelGGL((id_up_kernel<3, 64, 96>), dim3(gs), dim3(64), 0, st, h, list, lb, count,
+        hipLaunchKernelGGL((id_up_kernel<3, 64, 96, true>), dim3(gs), dim3(64), 0, st, h, list, lb, count,
                            0, tol);
-        hipLaunchKernelGGL((id_up_kernel<3, 128, 96>), dim3(gs), dim3(128), 0, st, h, list, lb, count,
+        hipLaunchKernelGGL((id_up_kernel<3, 128, 96, true>), dim3(gs), dim3(128), 0, st, h, list, lb, count,
                            64, tol);
-        hipLaunchKernelGGL((id_up_kernel<3, 384, 96>), dim3(gl), dim3(384), 0, st, h, list, lb, count,
+        hipLaunchKernelGGL((id_up_kernel<3, 384, 96, true>), dim3(gl), dim3(384), 0, st, h, list, lb, count,
                            128, tol);
     }
     return hipGetLastError() == hipSuccess ? 0 : -1;

@@ -114,6 +114,13 @@ struct H2Dev {
     int *flags, *cnt;
 };
 
+// Floats per part of the internal nodes' U pool (after the leaves' 2 n RMAX), even so that
+// every region starts 8-byte aligned.
+__host__ __device__ inline long long pool_part_cap(const H2Dev &h, int rmax)
+{
+    return ((h.pool_cap - 2LL * h.n * rmax) / kPoolParts) & ~1LL;
+}
+
 // h2_tree.hip
 int launch_root_box(int D, const void *Yg, int n, const H2Dev &h, double *partial, int nblk,
                     hipStream_t st);

@@ -148,9 +148,12 @@ long long tsne_job_footprint(int n_point, int low_dim, long long nnz)
 {
     // P on the device (int32 columns, fp32 + fp64 values, 16-bit tile indices) and, until
     // the last renumbering, a second copy with its sort scratch and the tile build's;
-    // per-point state, gather copies, forces and the H2 structure; fixed buffers (proxy
+    // per-point state, gather copies and forces; the H2 structure (node arrays, pair
+    // lists and the U pool, whose regions hold each node's FP64 R entries during its ID:
+    // ~1.1 KB per point in 2D, ~2.4 KB in 3D at leaf size 100); fixed buffers (proxy
     // tables, sort scratch)
-    return 64LL * nnz + (long long)n_point * (32LL * low_dim + 600) + (48LL << 20);
+    const long long h2 = (low_dim == 2) ? 1100 : 2400;
+    return 64LL * nnz + (long long)n_point * (32LL * low_dim + h2) + (48LL << 20);
 }
 
 tsne_server_t *tsne_server_create(int slots, long long hbm_budget_bytes)
