@@ -48,10 +48,35 @@ __device__ __forceinline__ double kern64(const double *a, const double *b, int D
     return w * w;
 }
 
-__device__ __forceinline__ double wsum64(double v)
+// Reduce-scatter of column sums of squares over the wavefront: lane j returns the sum over
+// the 64 lanes of a[C0 + j]^2 (columns at or past NA count zero).  At each of six steps a
+// lane keeps half of its partial sums and adds the partner's copy of that half: 63
+// independent shuffles for 64 columns instead of a 6-step chain per column.
+template <int C0, int NA>
+__device__ __forceinline__ double colsq_scatter64(const double (&a)[NA], int lane)
 {
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-    return v;
+    double w1[32], w2[16], w3[8], w4[4], w5[2];
+    const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4, b1 = lane & 2,
+               b0 = lane & 1;
+#pragma unroll
+    for (int j = 0; j < 32; j++) {
+        const double lo = (C0 + j < NA) ? a[C0 + j] * a[C0 + j] : 0.0;
+        const double hi = (C0 + j + 32 < NA) ? a[C0 + j + 32] * a[C0 + j + 32] : 0.0;
+        w1[j] = (b5 ? hi : lo) + __shfl_xor(b5 ? lo : hi, 32, 64);
+    }
+#pragma unroll
+    for (int j = 0; j < 16; j++)
+        w2[j] = (b4 ? w1[j + 16] : w1[j]) + __shfl_xor(b4 ? w1[j] : w1[j + 16], 16, 64);
+#pragma unroll
+    for (int j = 0; j < 8; j++)
+        w3[j] = (b3 ? w2[j + 8] : w2[j]) + __shfl_xor(b3 ? w2[j] : w2[j + 8], 8, 64);
+#pragma unroll
+    for (int j = 0; j < 4; j++)
+        w4[j] = (b2 ? w3[j + 4] : w3[j]) + __shfl_xor(b2 ? w3[j] : w3[j + 4], 4, 64);
+#pragma unroll
+    for (int j = 0; j < 2; j++)
+        w5[j] = (b1 ? w4[j + 2] : w4[j]) + __shfl_xor(b1 ? w4[j] : w4[j + 2], 2, 64);
+    return (b0 ? w5[1] : w5[0]) + __shfl_xor(b0 ? w5[0] : w5[1], 1, 64);
 }
 
 }  // namespace
@@ -235,10 +260,13 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
         if (lane < PMAX) s_part[wid][lane] = w6;
         (void)b5;
     } else {
-#pragma unroll
-        for (int q = 0; q < PMAX; q++) {
-            const double v = wsum64(a[q] * a[q]);
-            if (lane == 0) s_part[wid][q] = v;
+        // 3D: columns 0-63, then 64-95, each reduce-scattered over the wavefront
+        static_assert(PMAX <= 128, "3D column count");
+        const double c0 = colsq_scatter64<0, PMAX>(a, lane);
+        if (lane < PMAX) s_part[wid][lane] = c0;
+        if constexpr (PMAX > 64) {
+            const double c1 = colsq_scatter64<64, PMAX>(a, lane);
+            if (lane + 64 < PMAX) s_part[wid][64 + lane] = c1;
         }
     }
     __syncthreads();
