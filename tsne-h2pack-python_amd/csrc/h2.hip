@@ -229,16 +229,18 @@ int build_once(H2 *h, const void *Yg)
     int l = 0, pend_in = 0, bound = 1;
     for (;;) {
         const int pend_out = pend_in ^ 1;
-        if (launch_split_count(D, d, l, bound, st)) return -1;
-        size_t bytes = h->cub_bytes;
-        if (hipcub::DeviceScan::ExclusiveSum(h->cub_tmp, bytes, d.childcnt, d.childoff, bound, st) !=
-            hipSuccess)
-            return -1;
-        if (launch_publish_level(d, l, pend_out, h->pin_dev, st)) return -1;
-        if (launch_split_write(D, d, l, bound, st)) return -1;
+        if (bound <= kSmallSplit) {
+            if (launch_split_small(D, d, l, pend_out, h->pin_dev, st)) return -1;
+        } else {
+            if (launch_split_count(D, d, l, bound, st)) return -1;
+            size_t bytes = h->cub_bytes;
+            if (hipcub::DeviceScan::ExclusiveSum(h->cub_tmp, bytes, d.childcnt, d.childoff, bound,
+                                                 st) != hipSuccess)
+                return -1;
+            if (launch_split_write(D, d, l, bound, pend_out, h->pin_dev, st)) return -1;
+        }
         const int nbound = (int)std::min<long long>(d.node_cap, (long long)NCH * bound);
-        if (launch_lists(D, d, l + 1, nbound, pend_out, st)) return -1;
-        if (launch_pending(D, d, pend_in, pend_out, st)) return -1;
+        if (launch_lists_pending(D, d, l + 1, nbound, pend_in, pend_out, st)) return -1;
         pend_in = pend_out;
         l++;
         bound = std::max(nbound, 1);

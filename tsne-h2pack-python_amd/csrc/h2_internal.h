@@ -129,14 +129,19 @@ int launch_gather_sorted(int D, const void *Yg, const H2Dev &h, hipStream_t st);
 int launch_init_root(const H2Dev &h, hipStream_t st);
 // Level kernels take the level's node range from h.lvl on the device; `bound` (an upper
 // bound on the level's node count, known to the host) sizes the grid.
+// A level of at most kSmallSplit nodes: split, scan, publication and children in one
+// single-workgroup launch.  Larger levels: split_count, a library scan, split_write (which
+// also publishes the level).  Publication writes lvl[l+2], zeroes the pending counter of
+// the next pass, and sets the pinned host words out[0] (children of level l), out[1]
+// (non-finite input) and out[2] (node overflow).
+constexpr int kSmallSplit = 256;
+int launch_split_small(int D, const H2Dev &h, int l, int pend_out, int *out, hipStream_t st);
 int launch_split_count(int D, const H2Dev &h, int l, int bound, hipStream_t st);
-int launch_split_write(int D, const H2Dev &h, int l, int bound, hipStream_t st);
-int launch_lists(int D, const H2Dev &h, int l, int bound, int pend_out, hipStream_t st);
-int launch_pending(int D, const H2Dev &h, int pend_in, int pend_out, hipStream_t st);
-// After the scan of level l's child counts: lvl[l+2], the zeroed pending counter of the
-// next pass, and (pinned host words) out[0] children of level l, out[1] non-finite
-// input, out[2] node overflow.
-int launch_publish_level(const H2Dev &h, int l, int pend_out, int *out, hipStream_t st);
+int launch_split_write(int D, const H2Dev &h, int l, int bound, int pend_out, int *out,
+                       hipStream_t st);
+// Interaction lists of level l and the pending pairs of pend_in, one launch.
+int launch_lists_pending(int D, const H2Dev &h, int l, int bound, int pend_in, int pend_out,
+                         hipStream_t st);
 int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, hipStream_t st);
 
 // h2_id.hip

@@ -276,16 +276,16 @@ int launch_init_root(const H2Dev &h, hipStream_t st)
 // NCH consecutive lanes per node: lane v >= 1 finds the first sorted point whose
 // level-(l+1) digit is >= v (the NCH - 1 binary searches run side by side instead of one
 // after another, which is what a node of the top levels, with up to n points, waits
-// for); lane 0 then collects the bounds and records the node.
+// for); lane 0 then collects the bounds and records the node.  Slot t of the level
+// (node t / NCH, lane t % NCH); every lane of the wavefront takes part (shuffles, a
+// ballot).  Slots of [level size, nbound) get a zero child count (the library scan runs
+// over nbound counts).
 template <int D>
-__global__ void __launch_bounds__(256)
-split_count_kernel(H2Dev h, int l, int nbound)
+__device__ __forceinline__ void split_count_slot(const H2Dev &h, int l, int lb, int le, int t,
+                                                 int nbound)
 {
     constexpr int B = H2Cfg<D>::B, NCH = H2Cfg<D>::NCH;
     static_assert((64 % NCH) == 0, "lanes per node");
-    if (h.flags[FLAG_NODE_OVF]) return;  // the build is redone with larger arrays
-    const int lb = h.lvl[l], le = h.lvl[l + 1];
-    const int t = blockIdx.x * 256 + threadIdx.x;
     const int i = lb + t / NCH, v = t % NCH;
     const bool valid = i < le;
     // the scan runs over `nbound` counts: slots past the level's end count no children
@@ -338,16 +338,37 @@ split_count_kernel(H2Dev h, int l, int nbound)
 
 template <int D>
 __global__ void __launch_bounds__(256)
-split_write_kernel(H2Dev h, int l)
+split_count_kernel(H2Dev h, int l, int nbound)
+{
+    if (h.flags[FLAG_NODE_OVF]) return;  // the build is redone with larger arrays
+    split_count_slot<D>(h, l, h.lvl[l], h.lvl[l + 1], blockIdx.x * 256 + threadIdx.x, nbound);
+}
+
+// After the scan of level l's child counts: the range of level l + 1 (T children after
+// the level's end le), the pending counter the next lists / pending pass appends to, and
+// the host's copy of T and of the flags.
+__device__ __forceinline__ void publish_level(const H2Dev &h, int l, int le, int T, int pend_out,
+                                              int *out)
+{
+    int ne = le + T, ovf = h.flags[FLAG_NODE_OVF];
+    if (ne > h.node_cap) {
+        ovf = 1;
+        ne = h.node_cap;
+        h.flags[FLAG_NODE_OVF] = 1;
+    }
+    h.lvl[l + 2] = ne;
+    h.cnt[CNT_PEND0 + pend_out * kCntSlot] = 0;
+    out[0] = T;
+    out[1] = h.flags[FLAG_NONFINITE];
+    out[2] = ovf;
+}
+
+// The children of node i of level l (lb its first node), from child slot `base` on.
+template <int D>
+__device__ __forceinline__ void split_write_node(const H2Dev &h, int l, int lb, int i, int base)
 {
     constexpr int NCH = H2Cfg<D>::NCH;
-    if (h.flags[FLAG_NODE_OVF]) return;
-    const int lb = h.lvl[l], le = h.lvl[l + 1];
-    const int i = lb + blockIdx.x * 256 + threadIdx.x;
-    if (i >= le) return;
-    if (h.childcnt[i - lb] == 0) return;
     const int *bd = h.bounds + (size_t)(i - lb) * (NCH + 1);
-    const int base = le + h.childoff[i - lb];
     const float4 c = h.cent[i];
     const float hc = 0.5f * c.w;
     int k = 0;
@@ -374,6 +395,72 @@ split_write_kernel(H2Dev h, int l)
     h.nchild[i] = k;
 }
 
+// Children of level l from the scanned counts; workgroup 0 also publishes the level.
+template <int D>
+__global__ void __launch_bounds__(256)
+split_write_kernel(H2Dev h, int l, int pend_out, int *out)
+{
+    const int lb = h.lvl[l], le = h.lvl[l + 1];
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const int cnt = le - lb;
+        const int T = (cnt > 0 && !h.flags[FLAG_NODE_OVF]) ? h.childoff[cnt - 1] + h.childcnt[cnt - 1] : 0;
+        publish_level(h, l, le, T, pend_out, out);
+    }
+    if (h.flags[FLAG_NODE_OVF]) return;  // children past node_cap are skipped either way
+    const int i = lb + blockIdx.x * 256 + threadIdx.x;
+    if (i >= le || h.childcnt[i - lb] == 0) return;
+    split_write_node<D>(h, l, lb, i, le + h.childoff[i - lb]);
+}
+
+// A level of at most kSmallSplit nodes (the top of the tree, and the deep levels inside
+// dense clusters) in one workgroup: counts, scan, publication and children in one launch
+// instead of four, the phases separated by barriers.
+template <int D>
+__global__ void __launch_bounds__(1024)
+split_small_kernel(H2Dev h, int l, int pend_out, int *out)
+{
+    constexpr int NCH = H2Cfg<D>::NCH;
+    __shared__ int s_off[kSmallSplit + 1];
+    __shared__ int s_wsum[16];
+    const int lb = h.lvl[l], le = h.lvl[l + 1];
+    const int cnt = le - lb;  // at most kSmallSplit (the host's bound on the level)
+    const bool ovf = h.flags[FLAG_NODE_OVF];
+    if (!ovf)
+        for (int base = 0; base < cnt * NCH; base += 1024)  // same trip count for every wave
+            split_count_slot<D>(h, l, lb, le, base + (int)threadIdx.x, 0);
+    __syncthreads();  // child counts and bounds of every node written
+    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const int c = (!ovf && t < cnt) ? h.childcnt[t] : 0;
+    int x = c;  // inclusive scan over the wavefront, then over the wavefront totals
+    for (int off = 1; off < 64; off <<= 1) {
+        const int y = __shfl_up(x, off, 64);
+        if (lane >= off) x += y;
+    }
+    if (lane == 63) s_wsum[w] = x;
+    __syncthreads();
+    int pre = 0;
+    for (int k = 0; k < w; k++) pre += s_wsum[k];
+    if (t < kSmallSplit) s_off[t] = pre + x - c;
+    if (t == 0) {
+        int T = 0;
+        for (int k = 0; k < 16; k++) T += s_wsum[k];
+        s_off[kSmallSplit] = T;
+    }
+    __syncthreads();
+    if (t == 0) publish_level(h, l, le, ovf ? 0 : s_off[kSmallSplit], pend_out, out);
+    if (ovf) return;
+    if (t < cnt && c > 0) split_write_node<D>(h, l, lb, lb + t, le + s_off[t]);
+}
+
+int launch_split_small(int D, const H2Dev &h, int l, int pend_out, int *out, hipStream_t st)
+{
+    if (D == 2)
+        hipLaunchKernelGGL(split_small_kernel<2>, dim3(1), dim3(1024), 0, st, h, l, pend_out, out);
+    else
+        hipLaunchKernelGGL(split_small_kernel<3>, dim3(1), dim3(1024), 0, st, h, l, pend_out, out);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_split_count(int D, const H2Dev &h, int l, int bound, hipStream_t st)
 {
     const int g = std::max(1, (bound * (D == 2 ? H2Cfg<2>::NCH : H2Cfg<3>::NCH) + 255) / 256);
@@ -384,13 +471,14 @@ int launch_split_count(int D, const H2Dev &h, int l, int bound, hipStream_t st)
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
-int launch_split_write(int D, const H2Dev &h, int l, int bound, hipStream_t st)
+int launch_split_write(int D, const H2Dev &h, int l, int bound, int pend_out, int *out,
+                       hipStream_t st)
 {
     const int g = std::max(1, (bound + 255) / 256);
     if (D == 2)
-        hipLaunchKernelGGL(split_write_kernel<2>, dim3(g), dim3(256), 0, st, h, l);
+        hipLaunchKernelGGL(split_write_kernel<2>, dim3(g), dim3(256), 0, st, h, l, pend_out, out);
     else
-        hipLaunchKernelGGL(split_write_kernel<3>, dim3(g), dim3(256), 0, st, h, l);
+        hipLaunchKernelGGL(split_write_kernel<3>, dim3(g), dim3(256), 0, st, h, l, pend_out, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -410,10 +498,8 @@ int launch_split_write(int D, const H2Dev &h, int l, int bound, hipStream_t st)
 // (Device-wide atomics on a shared counter are serialised across the GPU: one per node
 // and category made this kernel's time grow with the node count.)
 template <int D>
-__global__ void __launch_bounds__(256)
-lists_kernel(H2Dev h, int l, int pend_out)
+__device__ __forceinline__ void lists_block(const H2Dev &h, int l, int pend_out, int blk)
 {
-    if (h.flags[FLAG_NODE_OVF]) return;  // uniform over the grid
     const int lb = h.lvl[l], le = h.lvl[l + 1];
     constexpr int B = H2Cfg<D>::B, NEAR_CAP = H2Cfg<D>::NEAR_CAP, NCH = H2Cfg<D>::NCH;
     constexpr int RMAXR = (NEAR_CAP * NCH + 63) / 64;  // candidate rounds at most
@@ -421,7 +507,7 @@ lists_kernel(H2Dev h, int l, int pend_out)
     __shared__ unsigned long long s_m[4][RMAXR][4];  // ADM, X, NEAR pair, pending masks
     __shared__ int s_cnt[4][4], s_bas[4];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = lb + blockIdx.x * 4 + wv;
+    const int i = lb + blk * 4 + wv;
     const bool valid = i < le;  // uniform over the wavefront
     const unsigned long long lt = (1ull << lane) - 1ull;
     const int p = valid ? h.parent[i] : 0;
@@ -549,14 +635,12 @@ lists_kernel(H2Dev h, int l, int pend_out)
 
 // Pending pairs (leaf i, non-leaf c): each child d of c is a W source when i lies at
 // least one box of d away, a NEAR source when d is a leaf, otherwise pending again.
+// Pair k of the pending buffer pend_in (every lane of the wavefront takes part).
 template <int D>
-__global__ void __launch_bounds__(256)
-pending_kernel(H2Dev h, int pend_in, int pend_out)
+__device__ __forceinline__ void pending_pair(const H2Dev &h, int pend_in, int pend_out, int k)
 {
     constexpr int B = H2Cfg<D>::B;
-    if (h.flags[FLAG_NODE_OVF]) return;
     const int n_pend = min(h.cnt[CNT_PEND0 + pend_in * kCntSlot], h.pend_cap);
-    const int k = blockIdx.x * 256 + threadIdx.x;
     const bool valid = k < n_pend;
     int i = 0, c = 0, li = 0, nc = 0, c0 = 0;
     if (valid) {
@@ -590,49 +674,33 @@ pending_kernel(H2Dev h, int pend_in, int pend_out)
     }
 }
 
-int launch_lists(int D, const H2Dev &h, int l, int bound, int pend_out, hipStream_t st)
+// The lists of level l (the first nlists workgroups, four nodes each) and the pending
+// pairs of pend_in (the rest, one pair per thread) in one launch: the two read nothing
+// the other writes, and both only append to the shared pair buffers.
+template <int D>
+__global__ void __launch_bounds__(256)
+lists_pending_kernel(H2Dev h, int l, int pend_in, int pend_out, int nlists)
 {
-    const int g = std::max(1, (bound + 3) / 4);
-    if (D == 2)
-        hipLaunchKernelGGL(lists_kernel<2>, dim3(g), dim3(256), 0, st, h, l, pend_out);
+    if (h.flags[FLAG_NODE_OVF]) return;  // uniform over the grid
+    if ((int)blockIdx.x < nlists)
+        lists_block<D>(h, l, pend_out, blockIdx.x);
     else
-        hipLaunchKernelGGL(lists_kernel<3>, dim3(g), dim3(256), 0, st, h, l, pend_out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
+        pending_pair<D>(h, pend_in, pend_out, (blockIdx.x - nlists) * 256 + threadIdx.x);
 }
 
-// Grid over the whole pending buffer: the live count is on the device.
-int launch_pending(int D, const H2Dev &h, int pend_in, int pend_out, hipStream_t st)
+// Grid: the lists of the (at most `bound`) nodes of level l, then the whole pending buffer
+// (its live count is on the device).
+int launch_lists_pending(int D, const H2Dev &h, int l, int bound, int pend_in, int pend_out,
+                         hipStream_t st)
 {
-    const int g = std::max(1, (h.pend_cap + 255) / 256);
+    const int gl = std::max(1, (bound + 3) / 4);
+    const int g = gl + std::max(1, (h.pend_cap + 255) / 256);
     if (D == 2)
-        hipLaunchKernelGGL(pending_kernel<2>, dim3(g), dim3(256), 0, st, h, pend_in, pend_out);
+        hipLaunchKernelGGL(lists_pending_kernel<2>, dim3(g), dim3(256), 0, st, h, l, pend_in,
+                           pend_out, gl);
     else
-        hipLaunchKernelGGL(pending_kernel<3>, dim3(g), dim3(256), 0, st, h, pend_in, pend_out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-__global__ void publish_level_kernel(H2Dev h, int l, int pend_out, int *out)
-{
-    if (threadIdx.x != 0) return;
-    const int lb = h.lvl[l], le = h.lvl[l + 1];
-    const int cnt = le - lb;
-    const int T = (cnt > 0 && !h.flags[FLAG_NODE_OVF]) ? h.childoff[cnt - 1] + h.childcnt[cnt - 1] : 0;
-    int ne = le + T, ovf = h.flags[FLAG_NODE_OVF];
-    if (ne > h.node_cap) {
-        ovf = 1;
-        ne = h.node_cap;
-        h.flags[FLAG_NODE_OVF] = 1;
-    }
-    h.lvl[l + 2] = ne;
-    h.cnt[CNT_PEND0 + pend_out * kCntSlot] = 0;
-    out[0] = T;
-    out[1] = h.flags[FLAG_NONFINITE];
-    out[2] = ovf;
-}
-
-int launch_publish_level(const H2Dev &h, int l, int pend_out, int *out, hipStream_t st)
-{
-    hipLaunchKernelGGL(publish_level_kernel, dim3(1), dim3(64), 0, st, h, l, pend_out, out);
+        hipLaunchKernelGGL(lists_pending_kernel<3>, dim3(g), dim3(256), 0, st, h, l, pend_in,
+                           pend_out, gl);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
