@@ -1,6 +1,7 @@
-"""Mixed-precision H2 product (coupling on MFMA with BF16 operands and FP32 accumulation,
-FP32 fallback for nodes whose rounding-error estimate is too large) against the FP64
-exact gradient and against the plain FP32 product, through the C ABI.
+"""Mixed-precision H2 path (coupling on MFMA with BF16 operands and FP32 accumulation,
+FP32 fallback for nodes whose rounding-error estimate is too large; the ID's pivoted
+Gram-Schmidt in FP32 at ID tolerances >= 1e-3, as here) against the FP64 exact gradient
+and against the plain path, through the C ABI.
 
 Bar: the same h2_tol bound as the plain path (gradient error <= 1e-2 in 2D, 3e-2 in 3D,
 all rows, exaggeration 1 and 12), the mixed repulsion within h2_tol / 4 of the plain

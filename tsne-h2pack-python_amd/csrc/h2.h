@@ -19,6 +19,9 @@ void h2_destroy(H2 *h);
 // product reads (the NEAR / W pair sorts): it runs there beside the ID.  The build joins
 // it before returning.  The stream must outlive the H2 object.
 void h2_set_aux_stream(H2 *h, hipStream_t st_aux);
+// Mixed path: the ID's pivoted Gram-Schmidt runs in FP32 (when the ID tolerance, half of
+// h2_tol, is >= 1e-3); R entries, the solve and the coefficients stay FP64.
+void h2_set_id_mixed(H2 *h, int mixed);
 
 // Rebuild from the float32 gather copy Yg (caller order).  Returns 0, or -1 on a
 // device error or a non-finite coordinate (reported in *nonfinite).

@@ -22,6 +22,7 @@ namespace tsne {
 struct H2 {
     int n = 0, D = 2, leaf = 100;
     double tol = 1e-2, ptol = 3e-3, id_tol = 5e-3;
+    bool id_mixed = false;  // the mixed path's ID (FP32 Gram-Schmidt at id_tol >= kIdF32MinTol)
     hipStream_t st = nullptr;
     // optional second stream: the NEAR / W pair sorts run there beside the ID
     hipStream_t st_aux = nullptr;
@@ -52,6 +53,7 @@ struct H2 {
 namespace {
 
 constexpr int kLevelBatch = 4;   // tree levels built between two host checks
+constexpr double kIdF32MinTol = 1e-3;  // smallest ID tolerance of the mixed path's FP32 ID
 constexpr int kMaxLevels = 32;   // > H2Cfg<D>::B + 1 for both dimensions
 
 // Radix sorts of the Morton keys and pair lists.  From kOnesweepMin items on, always the
@@ -300,11 +302,12 @@ int build_once(H2 *h, const void *Yg)
     // ID + upward: every leaf at once (they need nothing from other nodes), then the
     // internal nodes level by level, finest first
     const int nlev = (int)h->lvl.size() - 1;
-    if (h->n_nodes > 1 && launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, h->pq, st))
+    const bool f32 = h->id_mixed && h->id_tol >= kIdF32MinTol;
+    if (h->n_nodes > 1 && launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, h->pq, f32, st))
         return -1;
     for (int lv = nlev - 1; lv >= 1; lv--)
         if (launch_id_up(D, d, nullptr, h->lvl[lv], h->lvl[lv + 1] - h->lvl[lv], h->id_tol, h->pq,
-                         st))
+                         f32, st))
             return -1;
     if (h->st_aux && hipStreamWaitEvent(st, h->ev_sorted, 0) != hipSuccess) return -1;
     if (hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
@@ -369,6 +372,11 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
         return nullptr;
     }
     return h;
+}
+
+void h2_set_id_mixed(H2 *h, int mixed)
+{
+    if (h) h->id_mixed = mixed != 0;
 }
 
 void h2_set_aux_stream(H2 *h, hipStream_t st_aux)

@@ -52,16 +52,16 @@ __device__ __forceinline__ double kern64(const double *a, const double *b, int D
 // the 64 lanes of a[C0 + j]^2 (columns at or past NA count zero).  At each of six steps a
 // lane keeps half of its partial sums and adds the partner's copy of that half: 63
 // independent shuffles for 64 columns instead of a 6-step chain per column.
-template <int C0, int NA>
-__device__ __forceinline__ double colsq_scatter64(const double (&a)[NA], int lane)
+template <int C0, int NA, typename T>
+__device__ __forceinline__ double colsq_scatter64(const T (&a)[NA], int lane)
 {
     double w1[32], w2[16], w3[8], w4[4], w5[2];
     const bool b5 = lane & 32, b4 = lane & 16, b3 = lane & 8, b2 = lane & 4, b1 = lane & 2,
                b0 = lane & 1;
 #pragma unroll
     for (int j = 0; j < 32; j++) {
-        const double lo = (C0 + j < NA) ? a[C0 + j] * a[C0 + j] : 0.0;
-        const double hi = (C0 + j + 32 < NA) ? a[C0 + j + 32] * a[C0 + j + 32] : 0.0;
+        const double lo = (C0 + j < NA) ? (double)a[C0 + j] * a[C0 + j] : 0.0;
+        const double hi = (C0 + j + 32 < NA) ? (double)a[C0 + j + 32] * a[C0 + j + 32] : 0.0;
         w1[j] = (b5 ? hi : lo) + __shfl_xor(b5 ? lo : hi, 32, 64);
     }
 #pragma unroll
@@ -85,8 +85,12 @@ __device__ __forceinline__ double colsq_scatter64(const double (&a)[NA], int lan
 // m_lo candidates, or (below the maximum size class) more than NT, are left to another
 // size class.
 // PQ proxy slots are processed (PQ <= the table's PMAX; at least the largest proxy count
-// of any level).
-template <int D, int NT, int PQ>
+// of any level).  T: arithmetic of the pivoted Gram-Schmidt (candidate columns, the Q row,
+// dot products and residual norms).  double everywhere by default; float for the mixed
+// path at ID tolerances >= 1e-3, where FP32 rounding (~6e-8) is far below the residual
+// ratios the pivoting compares (>= tol), and half the LDS bytes per broadcast Q read.  The
+// kernel values, column scales, R entries, triangular solve and U's coefficients stay FP64.
+template <int D, int NT, int PQ, typename T>
 __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m_lo, double tol)
 {
     constexpr int PMAX = PQ, RMAX = H2Cfg<D>::RMAX;
@@ -97,13 +101,13 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     __shared__ double s_prox[PMAX][D];
     __shared__ double s_part[NW][PMAX];
     __shared__ double s_scale[PMAX];
-    __shared__ double s_Q[2][PMAX];  // Q row of the current step (two, alternating)
+    __shared__ T s_Q[2][PMAX];  // Q row of the current step (two, alternating)
     __shared__ float s_U[NT][RMAX];
     __shared__ double s_R[RMAX][RMAX];
     __shared__ int s_piv[RMAX];
-    __shared__ double s_rv[NW];
+    __shared__ T s_rv[NW];
     __shared__ int s_ri[NW];
-    __shared__ double s_bv;
+    __shared__ T s_bv;
     __shared__ int s_bi, s_base;
 
     constexpr int NCH = H2Cfg<D>::NCH;
@@ -221,9 +225,9 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     __syncthreads();
 
     // ---- this thread's column of A^T, then column norms over the candidates
-    double a[PMAX];
+    T a[PMAX];
 #pragma unroll
-    for (int q = 0; q < PMAX; q++) a[q] = (t < m) ? kern64(my, s_prox[q], D) : 0.0;
+    for (int q = 0; q < PMAX; q++) a[q] = (t < m) ? (T)kern64(my, s_prox[q], D) : T(0);
     if constexpr (D == 2) {
         // reduce-scatter over the wavefront: at each step a lane keeps half of its
         // partial column sums and adds the partner's copy of that half, so after six
@@ -236,11 +240,11 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
 #pragma unroll
         for (int j = 0; j < 32; j++) {
             if constexpr (PMAX <= 32) {
-                const double v = (j < PMAX) ? a[j] * a[j] : 0.0;
+                const double v = (j < PMAX) ? (double)a[j] * a[j] : 0.0;
                 w1[j] = v + __shfl_xor(v, 32, 64);
             } else {
-                const double lo = a[j] * a[j];
-                const double hi = (j + 32 < PMAX) ? a[j + 32] * a[j + 32] : 0.0;
+                const double lo = (double)a[j] * a[j];
+                const double hi = (j + 32 < PMAX) ? (double)a[j + 32] * a[j + 32] : 0.0;
                 w1[j] = (b5 ? hi : lo) + __shfl_xor(b5 ? lo : hi, 32, 64);
             }
         }
@@ -276,21 +280,21 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
         s_scale[q] = (q < p && s2 > 0.0) ? 1.0 / sqrt(s2) : 0.0;
     }
     __syncthreads();
-    double nrm;
+    T nrm;
     {
-        double n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
+        T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
 #pragma unroll
         for (int q = 0; q < PMAX; q += 4) {
-            a[q] *= s_scale[q];
-            a[q + 1] *= s_scale[q + 1];
-            a[q + 2] *= s_scale[q + 2];
-            a[q + 3] *= s_scale[q + 3];
+            a[q] *= (T)s_scale[q];
+            a[q + 1] *= (T)s_scale[q + 1];
+            a[q + 2] *= (T)s_scale[q + 2];
+            a[q + 3] *= (T)s_scale[q + 3];
             n0 = fma(a[q], a[q], n0);
             n1 = fma(a[q + 1], a[q + 1], n1);
             n2 = fma(a[q + 2], a[q + 2], n2);
             n3 = fma(a[q + 3], a[q + 3], n3);
         }
-        nrm = (t < m) ? (n0 + n1) + (n2 + n3) : -1.0;
+        nrm = (t < m) ? (n0 + n1) + (n2 + n3) : T(-1);
     }
 
     // ---- column-pivoted MGS.  R[k][t] goes to rpool[k * m + t] (the node's region; an
@@ -300,14 +304,15 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     const bool rfits = leafn || (long long)rbase + 2LL * m * RMAX <= 2LL * h.n * RMAX + (part + 1) * part_cap;
     double *rpool = reinterpret_cast<double *>(h.upool + rbase);
     bool selected = false;
-    double nrm0 = 0.0;
+    T nrm0 = 0;
     int r = 0;
     const int rlim = min(RMAX, min(m, p));
+    const T tol2 = (T)(tol * tol);
     for (int k = 0; k < RMAX; k++) {
-        double v = (selected || t >= m) ? -1.0 : nrm;
+        T v = (selected || t >= m) ? T(-1) : nrm;
         int vi = t;
         for (int off = 32; off > 0; off >>= 1) {
-            const double ov = __shfl_xor(v, off, 64);
+            const T ov = __shfl_xor(v, off, 64);
             const int oi = __shfl_xor(vi, off, 64);
             if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
         }
@@ -315,7 +320,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             if (lane == 0) { s_rv[wid] = v; s_ri[wid] = vi; }
             __syncthreads();
             if (t == 0) {
-                double bv = s_rv[0];
+                T bv = s_rv[0];
                 int bi = s_ri[0];
                 for (int w = 1; w < NW; w++)
                     if (s_rv[w] > bv || (s_rv[w] == bv && s_ri[w] < bi)) { bv = s_rv[w]; bi = s_ri[w]; }
@@ -326,22 +331,22 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             v = s_bv;
             vi = s_bi;
         }
-        const double bv = v;
+        const T bv = v;
         const int bi = vi;
         if (k == 0) nrm0 = bv;
-        if (k >= rlim || !(bv > tol * tol * nrm0) || !(bv > 0.0)) break;
+        if (k >= rlim || !(bv > tol2 * nrm0) || !(bv > T(0))) break;
         if (t == bi) {
             selected = true;
-            const double rd = sqrt(bv);
+            const double rd = sqrt((double)bv);
             s_R[k][k] = rd;  // R[k][k]
-            const double inv = 1.0 / rd;
+            const T inv = (T)(1.0 / rd);
 #pragma unroll
             for (int q = 0; q < PMAX; q++) s_Q[k & 1][q] = a[q] * inv;
             s_piv[k] = t;
         }
         __syncthreads();
         if (!selected && t < m) {
-            double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+            T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
 #pragma unroll
             for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
                 d0 = fma(s_Q[k & 1][q], a[q], d0);
@@ -349,9 +354,9 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
                 d2 = fma(s_Q[k & 1][q + 2], a[q + 2], d2);
                 d3 = fma(s_Q[k & 1][q + 3], a[q + 3], d3);
             }
-            const double dot = (d0 + d1) + (d2 + d3);  // R[k][t]
-            if (rfits) rpool[(size_t)k * m + t] = dot;
-            double n0 = 0.0, n1 = 0.0, n2 = 0.0, n3 = 0.0;
+            const T dot = (d0 + d1) + (d2 + d3);  // R[k][t]
+            if (rfits) rpool[(size_t)k * m + t] = (double)dot;
+            T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
 #pragma unroll
             for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
                 a[q] = fma(-dot, s_Q[k & 1][q], a[q]);
@@ -448,7 +453,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
 // a grid smaller than `count` walks the nodes with a stride, so a launch whose class
 // holds few of them costs few workgroups; otherwise one workgroup per node (the grid is
 // `count`, and the loop's state does not compete with the ID for registers).
-template <int D, int NT, int PQ, bool STRIDE>
+template <int D, int NT, int PQ, bool STRIDE, typename T>
 __global__ void __launch_bounds__(NT)
 id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo, double tol)
 {
@@ -457,12 +462,12 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo,
             const int node = list ? list[b] : lb + b;
             __syncthreads();  // the previous node's LDS reads are complete
             if (!list && h.child0[node] < 0) continue;  // leaves were done in their own launch
-            id_up_node<D, NT, PQ>(h, node, m_lo, tol);
+            id_up_node<D, NT, PQ, T>(h, node, m_lo, tol);
         }
     } else {
         const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
         if (!list && h.child0[node] < 0) return;
-        id_up_node<D, NT, PQ>(h, node, m_lo, tol);
+        id_up_node<D, NT, PQ, T>(h, node, m_lo, tol);
     }
 }
 
@@ -480,37 +485,48 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo,
 constexpr int kOneClassLevel = 512;
 constexpr int kLeafGridSmall = 4096;  // workgroups of a 3D leaf launch, 64/128 threads
 constexpr int kLeafGridLarge = 1024;  // same, 384 threads (leaves of > 128 points)
-int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
-                 int pq, hipStream_t st)
+
+template <typename T>
+void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, double tol, int pq,
+                    hipStream_t st)
 {
-    if (count <= 0) return 0;
     if (D == 2) {
         const bool one = count <= kOneClassLevel;
         if (pq <= 32) {
             if (!one)
-                hipLaunchKernelGGL((id_up_kernel<2, 64, 32, false>), dim3(count), dim3(64), 0, st, h, list, lb,
-                                   count, 0, tol);
-            hipLaunchKernelGGL((id_up_kernel<2, 128, 32, false>), dim3(count), dim3(128), 0, st, h, list, lb,
-                               count, one ? 0 : 64, tol);
+                hipLaunchKernelGGL((id_up_kernel<2, 64, 32, false, T>), dim3(count), dim3(64), 0, st, h,
+                                   list, lb, count, 0, tol);
+            hipLaunchKernelGGL((id_up_kernel<2, 128, 32, false, T>), dim3(count), dim3(128), 0, st, h,
+                               list, lb, count, one ? 0 : 64, tol);
         } else {
             if (!one)
-                hipLaunchKernelGGL((id_up_kernel<2, 64, 48, false>), dim3(count), dim3(64), 0, st, h, list, lb,
-                                   count, 0, tol);
-            hipLaunchKernelGGL((id_up_kernel<2, 128, 48, false>), dim3(count), dim3(128), 0, st, h, list, lb,
-                               count, one ? 0 : 64, tol);
+                hipLaunchKernelGGL((id_up_kernel<2, 64, 48, false, T>), dim3(count), dim3(64), 0, st, h,
+                                   list, lb, count, 0, tol);
+            hipLaunchKernelGGL((id_up_kernel<2, 128, 48, false, T>), dim3(count), dim3(128), 0, st, h,
+                               list, lb, count, one ? 0 : 64, tol);
         }
     } else if (!list) {
-        hipLaunchKernelGGL((id_up_kernel<3, 384, 96, false>), dim3(count), dim3(384), 0, st, h, list,
+        hipLaunchKernelGGL((id_up_kernel<3, 384, 96, false, T>), dim3(count), dim3(384), 0, st, h, list,
                            lb, count, 0, tol);
     } else {
         const int gs = std::min(count, kLeafGridSmall), gl = std::min(count, kLeafGridLarge);
-        hipLaunchKernelGGL((id_up_kernel<3, 64, 96, true>), dim3(gs), dim3(64), 0, st, h, list, lb, count,
-                           0, tol);
-        hipLaunchKernelGGL((id_up_kernel<3, 128, 96, true>), dim3(gs), dim3(128), 0, st, h, list, lb, count,
-                           64, tol);
-        hipLaunchKernelGGL((id_up_kernel<3, 384, 96, true>), dim3(gl), dim3(384), 0, st, h, list, lb, count,
-                           128, tol);
+        hipLaunchKernelGGL((id_up_kernel<3, 64, 96, true, T>), dim3(gs), dim3(64), 0, st, h, list, lb,
+                           count, 0, tol);
+        hipLaunchKernelGGL((id_up_kernel<3, 128, 96, true, T>), dim3(gs), dim3(128), 0, st, h, list, lb,
+                           count, 64, tol);
+        hipLaunchKernelGGL((id_up_kernel<3, 384, 96, true, T>), dim3(gl), dim3(384), 0, st, h, list, lb,
+                           count, 128, tol);
     }
+}
+
+int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol, int pq,
+                 bool f32, hipStream_t st)
+{
+    if (count <= 0) return 0;
+    if (f32)
+        launch_id_up_t<float>(D, h, list, lb, count, tol, pq, st);
+    else
+        launch_id_up_t<double>(D, h, list, lb, count, tol, pq, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

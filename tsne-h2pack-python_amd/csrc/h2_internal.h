@@ -145,9 +145,10 @@ int launch_lists_pending(int D, const H2Dev &h, int l, int bound, int pend_in, i
 int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, hipStream_t st);
 
 // h2_id.hip
-// pq: proxy slots to process (>= the largest proxy count of the table)
+// pq: proxy slots to process (>= the largest proxy count of the table); f32: pivoted
+// Gram-Schmidt in FP32 (the mixed path at ID tolerances >= 1e-3), else FP64
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
-                 int pq, hipStream_t st);
+                 int pq, bool f32, hipStream_t st);
 
 // h2_product.hip
 // mixed: coupling on MFMA with BF16 operands and FP32 accumulation, nodes whose

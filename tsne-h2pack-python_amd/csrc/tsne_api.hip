@@ -214,6 +214,7 @@ int rebuild(tsne_workspace *ws)
         if (ws->st2) h2_set_aux_stream(ws->h2, ws->st2);
     }
     int bad = 0;
+    h2_set_id_mixed(ws->h2, ws->precision == TSNE_H2_MIXED);
     if (h2_build(ws->h2, ws->d_Yg[ws->cur], &bad) != 0) {
         if (bad) fprintf(stderr, "tsne_h2pack: the embedding contains NaN or Inf\n");
         ws->h2_ok = 0;
