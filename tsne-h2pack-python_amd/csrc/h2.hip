@@ -258,19 +258,18 @@ int build_once(H2 *h, const void *Yg)
         }
     }
     {
+        // level offsets, pair counters and flags of the finished tree: one wait
         std::vector<int> lv(l + 2);
         if (hipMemcpyAsync(lv.data(), d.lvl, sizeof(int) * lv.size(), hipMemcpyDeviceToHost, st) !=
                 hipSuccess ||
+            hipMemcpyAsync(h->pin, d.cnt, sizeof(int) * CNT_COUNT, hipMemcpyDeviceToHost, st) ||
+            hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
             hipStreamSynchronize(st) != hipSuccess)
             return -1;
         h->lvl.assign({0, 1});
         for (int k = 1; k + 1 < (int)lv.size() && lv[k + 1] > lv[k]; k++) h->lvl.push_back(lv[k + 1]);
     }
     h->n_nodes = h->lvl.back();
-    if (hipMemcpyAsync(h->pin, d.cnt, sizeof(int) * CNT_COUNT, hipMemcpyDeviceToHost, st) ||
-        hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
-        hipStreamSynchronize(st) != hipSuccess)
-        return -1;
     if (h->hflags[FLAG_NODE_OVF]) {
         h->node_f *= 2.0;
         return 1;

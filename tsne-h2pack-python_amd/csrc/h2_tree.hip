@@ -482,6 +482,8 @@ int launch_split_write(int D, const H2Dev &h, int l, int bound, int pend_out, in
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+constexpr int kPendBlocks = 256;  // workgroups of the pending-pair pass
+
 // ---------------------------------------------------------------------------------
 // Interaction lists of the nodes of level l (>= 1).  Candidates are the children of
 // the parent's near-list members (a leaf member is its own candidate).  A same-level
@@ -637,10 +639,10 @@ __device__ __forceinline__ void lists_block(const H2Dev &h, int l, int pend_out,
 // least one box of d away, a NEAR source when d is a leaf, otherwise pending again.
 // Pair k of the pending buffer pend_in (every lane of the wavefront takes part).
 template <int D>
-__device__ __forceinline__ void pending_pair(const H2Dev &h, int pend_in, int pend_out, int k)
+__device__ __forceinline__ void pending_pair(const H2Dev &h, int pend_in, int pend_out, int k,
+                                             int n_pend)
 {
     constexpr int B = H2Cfg<D>::B;
-    const int n_pend = min(h.cnt[CNT_PEND0 + pend_in * kCntSlot], h.pend_cap);
     const bool valid = k < n_pend;
     int i = 0, c = 0, li = 0, nc = 0, c0 = 0;
     if (valid) {
@@ -675,26 +677,31 @@ __device__ __forceinline__ void pending_pair(const H2Dev &h, int pend_in, int pe
 }
 
 // The lists of level l (the first nlists workgroups, four nodes each) and the pending
-// pairs of pend_in (the rest, one pair per thread) in one launch: the two read nothing
-// the other writes, and both only append to the shared pair buffers.
+// pairs of pend_in (the other kPendBlocks workgroups, one pair per thread, walking the live
+// pairs with a stride) in one launch: the two read nothing the other writes, and both only
+// append to the shared pair buffers.
 template <int D>
 __global__ void __launch_bounds__(256)
 lists_pending_kernel(H2Dev h, int l, int pend_in, int pend_out, int nlists)
 {
     if (h.flags[FLAG_NODE_OVF]) return;  // uniform over the grid
-    if ((int)blockIdx.x < nlists)
+    if ((int)blockIdx.x < nlists) {
         lists_block<D>(h, l, pend_out, blockIdx.x);
-    else
-        pending_pair<D>(h, pend_in, pend_out, (blockIdx.x - nlists) * 256 + threadIdx.x);
+    } else {
+        const int n_pend = min(h.cnt[CNT_PEND0 + pend_in * kCntSlot], h.pend_cap);
+        // the loop bound is uniform over the workgroup: every lane takes part in each round
+        for (int base = (blockIdx.x - nlists) * 256; base < n_pend; base += kPendBlocks * 256)
+            pending_pair<D>(h, pend_in, pend_out, base + threadIdx.x, n_pend);
+    }
 }
 
-// Grid: the lists of the (at most `bound`) nodes of level l, then the whole pending buffer
-// (its live count is on the device).
+// Grid: the lists of the (at most `bound`) nodes of level l, then kPendBlocks workgroups
+// for the pending pairs (their live count is on the device).
 int launch_lists_pending(int D, const H2Dev &h, int l, int bound, int pend_in, int pend_out,
                          hipStream_t st)
 {
     const int gl = std::max(1, (bound + 3) / 4);
-    const int g = gl + std::max(1, (h.pend_cap + 255) / 256);
+    const int g = gl + kPendBlocks;
     if (D == 2)
         hipLaunchKernelGGL(lists_pending_kernel<2>, dim3(g), dim3(256), 0, st, h, l, pend_in,
                            pend_out, gl);
