@@ -79,6 +79,32 @@ __device__ __forceinline__ double colsq_scatter64(const T (&a)[NA], int lane)
     return (b0 ? w5[1] : w5[0]) + __shfl_xor(b0 ? w5[0] : w5[1], 1, 64);
 }
 
+// Pivot key of a candidate: the float bits of its residual norm (monotonic for norms >= 0)
+// with the low 9 bits replaced by 511 - t, so that the unsigned maximum picks the largest
+// norm and, among norms equal to within 2^-14, the smallest t (t < 512).  0: out of play.
+__device__ __forceinline__ unsigned pivot_key(double nrm, int t, bool live)
+{
+    if (!live) return 0u;
+    const unsigned b = __float_as_uint((float)nrm);
+    return (b & ~0x1FFu) | (unsigned)(511 - t);
+}
+
+// Unsigned maximum over the wavefront, in every lane: DPP within each row of 16 lanes
+// (quad swaps, then rotations by 4 and 8), then the four row maxima through scalar
+// registers.  Every lane of the wavefront must be active.
+__device__ __forceinline__ unsigned wave_max_u32(unsigned k)
+{
+    k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0xB1, 0xF, 0xF, false));  // lanes ^1
+    k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x4E, 0xF, 0xF, false));  // lanes ^2
+    k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x124, 0xF, 0xF, false)); // row_ror:4
+    k = max(k, (unsigned)__builtin_amdgcn_update_dpp(0, (int)k, 0x128, 0xF, 0xF, false)); // row_ror:8
+    const unsigned r0 = (unsigned)__builtin_amdgcn_readlane((int)k, 0);
+    const unsigned r1 = (unsigned)__builtin_amdgcn_readlane((int)k, 16);
+    const unsigned r2 = (unsigned)__builtin_amdgcn_readlane((int)k, 32);
+    const unsigned r3 = (unsigned)__builtin_amdgcn_readlane((int)k, 48);
+    return max(max(r0, r1), max(r2, r3));
+}
+
 }  // namespace
 
 // ID + upward pass of one node, by one workgroup of NT threads; nodes with at most
@@ -105,10 +131,9 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     __shared__ float s_U[NT][RMAX];
     __shared__ double s_R[RMAX][RMAX];
     __shared__ int s_piv[RMAX];
-    __shared__ T s_rv[NW];
-    __shared__ int s_ri[NW];
+    __shared__ unsigned s_key[2][NW];  // wavefront pivot keys (two steps, alternating)
     __shared__ T s_bv;
-    __shared__ int s_bi, s_base;
+    __shared__ int s_base;
 
     constexpr int NCH = H2Cfg<D>::NCH;
     const int t = threadIdx.x;
@@ -309,42 +334,33 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     const int rlim = min(RMAX, min(m, p));
     const T tol2 = (T)(tol * tol);
     for (int k = 0; k < RMAX; k++) {
-        T v = (selected || t >= m) ? T(-1) : nrm;
-        int vi = t;
-        for (int off = 32; off > 0; off >>= 1) {
-            const T ov = __shfl_xor(v, off, 64);
-            const int oi = __shfl_xor(vi, off, 64);
-            if (ov > v || (ov == v && oi < vi)) { v = ov; vi = oi; }
-        }
+        // pivot: the candidate of largest residual norm (ties within 2^-14 to the smaller
+        // index), by one 32-bit key maximum over the workgroup
+        unsigned key = wave_max_u32(pivot_key((double)nrm, t, !selected && t < m));
         if (NW > 1) {
-            if (lane == 0) { s_rv[wid] = v; s_ri[wid] = vi; }
+            if (lane == 0) s_key[k & 1][wid] = key;
             __syncthreads();
-            if (t == 0) {
-                T bv = s_rv[0];
-                int bi = s_ri[0];
-                for (int w = 1; w < NW; w++)
-                    if (s_rv[w] > bv || (s_rv[w] == bv && s_ri[w] < bi)) { bv = s_rv[w]; bi = s_ri[w]; }
-                s_bv = bv;
-                s_bi = bi;
-            }
-            __syncthreads();
-            v = s_bv;
-            vi = s_bi;
+#pragma unroll
+            for (int w = 0; w < NW; w++) key = max(key, s_key[k & 1][w]);
         }
-        const T bv = v;
-        const int bi = vi;
-        if (k == 0) nrm0 = bv;
-        if (k >= rlim || !(bv > tol2 * nrm0) || !(bv > T(0))) break;
+        const int bi = key ? 511 - (int)(key & 0x1FFu) : -1;
+        // the pivot publishes its exact norm and, speculatively, its Q row (unused if the
+        // pivoting stops here)
         if (t == bi) {
-            selected = true;
-            const double rd = sqrt((double)bv);
-            s_R[k][k] = rd;  // R[k][k]
-            const T inv = (T)(1.0 / rd);
+            s_bv = nrm;
+            const T inv = (T)(1.0 / sqrt((double)nrm));
 #pragma unroll
             for (int q = 0; q < PMAX; q++) s_Q[k & 1][q] = a[q] * inv;
             s_piv[k] = t;
         }
         __syncthreads();
+        const T bv = key ? s_bv : T(0);
+        if (k == 0) nrm0 = bv;
+        if (k >= rlim || !(bv > tol2 * nrm0) || !(bv > T(0))) break;
+        if (t == bi) {
+            selected = true;
+            s_R[k][k] = sqrt((double)bv);  // R[k][k]
+        }
         if (!selected && t < m) {
             T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
 #pragma unroll
