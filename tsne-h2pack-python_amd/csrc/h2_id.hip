@@ -362,23 +362,35 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
             s_R[k][k] = sqrt((double)bv);  // R[k][k]
         }
         if (!selected && t < m) {
+            // the Q row is read from LDS once into registers where it fits beside the
+            // column (2D, and 3D in FP32), else once per pass
+            constexpr bool QREG = PMAX * sizeof(T) <= 384;
+            T qv[QREG ? PMAX : 1];
+            if constexpr (QREG) {
+#pragma unroll
+                for (int q = 0; q < PMAX; q++) qv[q] = s_Q[k & 1][q];
+            }
+            auto Qk = [&](int q) -> T {
+                if constexpr (QREG) return qv[q];
+                else return s_Q[k & 1][q];
+            };
             T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
 #pragma unroll
             for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
-                d0 = fma(s_Q[k & 1][q], a[q], d0);
-                d1 = fma(s_Q[k & 1][q + 1], a[q + 1], d1);
-                d2 = fma(s_Q[k & 1][q + 2], a[q + 2], d2);
-                d3 = fma(s_Q[k & 1][q + 3], a[q + 3], d3);
+                d0 = fma(Qk(q), a[q], d0);
+                d1 = fma(Qk(q + 1), a[q + 1], d1);
+                d2 = fma(Qk(q + 2), a[q + 2], d2);
+                d3 = fma(Qk(q + 3), a[q + 3], d3);
             }
             const T dot = (d0 + d1) + (d2 + d3);  // R[k][t]
             if (rfits) rpool[(size_t)k * m + t] = (double)dot;
             T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
 #pragma unroll
             for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
-                a[q] = fma(-dot, s_Q[k & 1][q], a[q]);
-                a[q + 1] = fma(-dot, s_Q[k & 1][q + 1], a[q + 1]);
-                a[q + 2] = fma(-dot, s_Q[k & 1][q + 2], a[q + 2]);
-                a[q + 3] = fma(-dot, s_Q[k & 1][q + 3], a[q + 3]);
+                a[q] = fma(-dot, Qk(q), a[q]);
+                a[q + 1] = fma(-dot, Qk(q + 1), a[q + 1]);
+                a[q + 2] = fma(-dot, Qk(q + 2), a[q + 2]);
+                a[q + 3] = fma(-dot, Qk(q + 3), a[q + 3]);
                 n0 = fma(a[q], a[q], n0);
                 n1 = fma(a[q + 1], a[q + 1], n1);
                 n2 = fma(a[q + 2], a[q + 2], n2);
