@@ -423,14 +423,34 @@ mean_finalize_kernel(int nblocks, int D, int n, const double *__restrict__ block
     const int t = threadIdx.x;
     double a[3] = {0.0, 0.0, 0.0};
     // thread t sums blocks t, t + 1024, ... in order (neighbouring threads read
-    // neighbouring records, so each pass is one coalesced sweep), then a fixed tree
-    for (int b = t; b < nblocks; b += 1024)
-        for (int d = 0; d < D; d++) a[d] += block_sums[(size_t)b * 3 + d];
+    // neighbouring records, so each pass is one coalesced sweep), then a fixed tree.  The
+    // records of eight passes are loaded before they are added (same order of additions),
+    // so the loads overlap instead of waiting one after another.
+    constexpr int U = 8;
+    for (int b0 = t; b0 < nblocks; b0 += 1024 * U) {
+        double v[U][3];
+#pragma unroll
+        for (int u = 0; u < U; u++) {
+            const int b = b0 + u * 1024;
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                v[u][d] = (b < nblocks && d < D) ? block_sums[(size_t)b * 3 + d] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; u++)
+            if (b0 + u * 1024 < nblocks)
+#pragma unroll
+                for (int d = 0; d < 3; d++)
+                    if (d < D) a[d] += v[u][d];
+    }
+#pragma unroll
     for (int d = 0; d < 3; d++) part[d][t] = a[d];
     __syncthreads();
     for (int off = 512; off > 0; off >>= 1) {
         if (t < off)
-            for (int d = 0; d < D; d++) part[d][t] += part[d][t + off];
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                if (d < D) part[d][t] += part[d][t + off];
         __syncthreads();
     }
     if (t < D) {
