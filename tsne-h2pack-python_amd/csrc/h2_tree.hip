@@ -424,6 +424,13 @@ split_small_kernel(H2Dev h, int l, int pend_out, int *out)
     __shared__ int s_wsum[16];
     const int lb = h.lvl[l], le = h.lvl[l + 1];
     const int cnt = le - lb;  // at most kSmallSplit (the host's bound on the level)
+    if (cnt > kSmallSplit) {  // cannot happen (the host launches this for such levels
+        if (threadIdx.x == 0) {  // only); fail the build rather than index past s_off
+            atomicOr(&h.flags[FLAG_NODE_OVF], 1);
+            publish_level(h, l, le, 0, pend_out, out);
+        }
+        return;
+    }
     const bool ovf = h.flags[FLAG_NODE_OVF];
     if (!ovf)
         for (int base = 0; base < cnt * NCH; base += 1024)  // same trip count for every wave
