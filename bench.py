@@ -617,8 +617,9 @@ def main():
     if a.tile_rows >= 0:
         os.environ["TSNE_ATTR_TILE_ROWS"] = str(a.tile_rows)
     if a.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the reference is single-process; see DESIGN.md on sharding)", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the reference is "
+              "single-process; jobs shard across GPUs by job, one server per device)",
+              file=sys.stderr)
         sys.exit(2)
     if a.pipeline:
         clusters = a.clusters if a.clusters > 0 else (100 if a.n >= 1_000_000 else 20)

@@ -1,6 +1,6 @@
 """H2 repulsive product on the GPU against the FP64 oracle (all rows), through the C ABI.
 
-Bar (north-star h2_tol): the gradient error ||g - g64|| / ||g64|| <= h2_tol (1e-2 in 2D)
+Bar (the h2_tol bound): the gradient error ||g - g64|| / ||g64|| <= h2_tol (1e-2 in 2D)
 at exaggeration 1 and 12, on embeddings of several scales (cluster spread from 0.05 to
 200 units, which puts the node boxes on both sides of the kernel's unit length).  The
 repulsive force alone must be within 2 * h2_tol.  Determinism: two builds of the same
