@@ -47,6 +47,8 @@ int launch_repulsive_direct_f32(int D, int n, const void *Yg, float *s, float *F
 int launch_z(int D, int n, const void *Yg, const float *s, const float *F, double *scal,
              double *partial, int nblocks, hipStream_t st);
 int launch_shift_snapshot(double *scal, hipStream_t st);
+int launch_f64_to_f32(const double *src, float *dst, size_t n, hipStream_t st);
+int launch_fill_f64(double *dst, double v, size_t n, hipStream_t st);
 int launch_kl(int D, int n, const int *row_ptr, const int *col_idx, const double *pval64,
               const double *Y64, const double *scal, double exag, double *row_kl,
               double *partial, int nblocks, double *dst, hipStream_t st);

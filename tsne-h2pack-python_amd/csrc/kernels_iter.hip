@@ -948,6 +948,37 @@ int launch_z(int D, int n, const void *Yg, const float *s, const float *F, doubl
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
+// P's FP32 values from its FP64 values (round to nearest, as a host cast), and a constant
+// fill: both on the device, so the workspace set-up uploads P once.
+__global__ void __launch_bounds__(256)
+f64_to_f32_kernel(const double *__restrict__ src, float *__restrict__ dst, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = (float)src[i];
+}
+
+__global__ void __launch_bounds__(256) fill_f64_kernel(double *__restrict__ dst, double v, size_t n)
+{
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256)
+        dst[i] = v;
+}
+
+int launch_f64_to_f32(const double *src, float *dst, size_t n, hipStream_t st)
+{
+    if (n == 0) return 0;
+    const int g = (int)std::min<size_t>(8192, (n + 255) / 256);
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(g), dim3(256), 0, st, src, dst, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_fill_f64(double *dst, double v, size_t n, hipStream_t st)
+{
+    if (n == 0) return 0;
+    const int g = (int)std::min<size_t>(8192, (n + 255) / 256);
+    hipLaunchKernelGGL(fill_f64_kernel, dim3(g), dim3(256), 0, st, dst, v, n);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
 int launch_shift_snapshot(double *scal, hipStream_t st)
 {
     hipLaunchKernelGGL(shift_snapshot_kernel, dim3(1), dim3(64), 0, st, scal);

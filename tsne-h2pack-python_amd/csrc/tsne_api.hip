@@ -603,19 +603,15 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
          dalloc(&ws->d_partial, ws->nblocks_red, ws->st) == 0 && dalloc(&ws->d_rowkl, n, ws->st) == 0 &&
          (!ws->overlap || dalloc(&ws->d_A, (size_t)n * D, ws->st) == 0);
     if (ok) {
-        std::vector<float> v32(nnz > 0 ? nnz : 1);
-        for (int e = 0; e < nnz; e++) v32[e] = (float)P->values[e];
-        std::vector<double> ones((size_t)n * D, 1.0);
+        // P uploaded once (FP64 values; the FP32 copy is made on the device), gains = 1
         ok = hipMemcpyAsync(ws->d_rowptr, P->row_ptr, ((size_t)n + 1) * sizeof(int),
                             hipMemcpyHostToDevice, ws->st) == hipSuccess &&
              hipMemcpyAsync(ws->d_col, P->col_idx, (size_t)nnz * sizeof(int),
                             hipMemcpyHostToDevice, ws->st) == hipSuccess &&
              hipMemcpyAsync(ws->d_val64, P->values, (size_t)nnz * sizeof(double),
                             hipMemcpyHostToDevice, ws->st) == hipSuccess &&
-             hipMemcpyAsync(ws->d_val32, v32.data(), (size_t)nnz * sizeof(float),
-                            hipMemcpyHostToDevice, ws->st) == hipSuccess &&
-             hipMemcpyAsync(ws->d_gains, ones.data(), ones.size() * sizeof(double),
-                            hipMemcpyHostToDevice, ws->st) == hipSuccess &&
+             launch_f64_to_f32(ws->d_val64, ws->d_val32, (size_t)nnz, ws->st) == 0 &&
+             launch_fill_f64(ws->d_gains, 1.0, (size_t)n * D, ws->st) == 0 &&
              hipMemsetAsync(ws->d_uY, 0, (size_t)n * D * sizeof(double), ws->st) == hipSuccess &&
              hipMemsetAsync(ws->d_F, 0, (size_t)n * D * sizeof(float), ws->st) == hipSuccess &&
              hipMemsetAsync(ws->d_Yg[1], 0, (size_t)n * kGatherBytes[D], ws->st) == hipSuccess &&
