@@ -511,7 +511,7 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo,
 // workgroup per node: many of their nodes take the single-child shortcut, and walking
 // those one after another in a strided grid cost more than the smaller classes saved.
 constexpr int kOneClassLevel = 512;
-constexpr int kLeafGridSmall = 4096;  // workgroups of a 3D leaf launch, 64/128 threads
+constexpr int kLeafGridSmall = 8192;  // workgroups of a 3D leaf launch, 64/128 threads
 constexpr int kLeafGridLarge = 1024;  // same, 384 threads (leaves of > 128 points)
 
 template <typename T>
