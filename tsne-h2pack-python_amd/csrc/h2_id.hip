@@ -507,11 +507,12 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo,
 // one workgroup per CU with six waves behind every barrier.  Leaves (tens of points
 // typically) run in classes of 64 and 128 threads (~60 KB, two per CU), on grids that
 // walk the leaf list with a stride so that a class's launch costs few workgroups when it
-// holds few leaves.  Internal levels of at most kOneClassLevel nodes run in one launch of the
+// holds few leaves.  Internal levels of at most kOneClassLevel3 nodes run in one launch of the
 // 384-thread class; larger ones in the 64, 128 and 384 classes with one workgroup per node
 // (walking single-child nodes one after another in a strided grid cost more than the
 // smaller classes saved).
 constexpr int kOneClassLevel = 512;
+constexpr int kOneClassLevel3 = 2048;  // 3D internal levels: one 384-thread launch up to this
 constexpr int kLeafGridSmall = 8192;  // workgroups of a 3D leaf launch, 64/128 threads
 constexpr int kLeafGridLarge = 1024;  // same, 384 threads (leaves of > 128 points)
 
@@ -534,7 +535,7 @@ void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, d
             hipLaunchKernelGGL((id_up_kernel<2, 128, 48, false, T>), dim3(count), dim3(128), 0, st, h,
                                list, lb, count, one ? 0 : 64, tol);
         }
-    } else if (!list && count <= kOneClassLevel) {
+    } else if (!list && count <= kOneClassLevel3) {
         hipLaunchKernelGGL((id_up_kernel<3, 384, 96, false, T>), dim3(count), dim3(384), 0, st, h, list,
                            lb, count, 0, tol);
     } else if (!list) {
