@@ -1,16 +1,19 @@
 """Benchmark of the t-SNE gradient path on one MI355X.
 
-Workload (default): N=1,000,000 points, D=50 float64 from a seeded mixture of 100
+Workload (default): N=1,000,000 points, D=50 float32 from a seeded mixture of 100
 Gaussians (Dirichlet(1) weights, per-cluster std U[0.5, 2], centres N(0, 5^2)); the exact
 k=90 graph from the library's own device search (tsne_knn_exact), joint P on the device,
-perplexity 30, 2D, h2_tol 1e-2, rebuild every 3 (10 in early exaggeration), leaf 100.
+perplexity 30, 2D, h2_tol 1e-2, rebuild every 3 (10 in early exaggeration), leaf 100,
+explicit seeded N(0, 1e-4^2) initial embedding.
 
 A step is one optimisation iteration of the library's loop on device-resident state.
---warmup W: an untimed job of W iterations first (its end state also gives the gradient
-error after warm-up).  Timed: a fresh job of exactly K iterations, its whole schedule,
-bracketed by device synchronisation.  The metric is iterations/s of the mixed-precision H2
-path; the plain FP32 path is timed the same way and reported beside it.  Prints ONE JSON
-line.  Other modes: --serving (jobs/s, p50/p99), --knn (exact k-NN graph seconds),
+--steps K --warmup W: one job of the full 1000-iteration schedule whose first 1000 - K
+iterations run untimed (tsne_run_range; at least W of them), then the last K iterations
+[1000 - K, 1000) are timed, bracketed by device synchronisation (an extra untimed
+W-iteration job first when 1000 - K < W).  The metric is iterations/s of the
+mixed-precision H2 path over that window; the plain FP32 path is timed the same way and
+reported beside it, each with a whole-schedule timing and a per-phase split.  Prints ONE
+JSON line.  Other modes: --serving (jobs/s, p50/p99), --knn (exact k-NN graph seconds),
 --pipeline (k-NN + joint P + optimisation + quality, seconds per stage).
 """
 import argparse
@@ -69,6 +72,9 @@ def make_inputs(n, dim_in, n_clusters, k, perplexity, seed=0):
 
 
 def make_workspace(be, csr, n, dim, precision, n_iter, leaf, tol, seed=0):
+    """Workspace of the benchmark job: the reference's TSNE defaults for this shape
+    (learning rate max(N/12, 200), rebuild every 3, 10 during early exaggeration) and an
+    explicit seeded N(0, 1e-4^2) initial embedding (numpy default_rng(seed))."""
     from scipy.sparse import csr_matrix
     p = be.create_default_params()
     p.verbose = 0
@@ -80,47 +86,68 @@ def make_workspace(be, csr, n, dim, precision, n_iter, leaf, tol, seed=0):
     p.learning_rate = max(n / 12.0, 200.0)
     p.random_seed = seed
     P = csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n))
-    ws = be.init_workspace(P, dim, p, init_Y=None)
+    Y0 = np.random.default_rng(seed).normal(scale=1e-4, size=(n, dim))
+    ws = be.init_workspace(P, dim, p, init_Y=Y0)
     be.lib.tsne_set_h2_precision(ws, 1 if precision == "mixed" else 0)
     return ws
 
 
-def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol):
-    """Warm-up: an untimed job of `warmup` iterations on the same P (the gradient error is
-    also measured at its end).  Timed: a fresh job of exactly `steps` iterations (its
-    whole optimisation schedule), bracketed by device synchronisation."""
+def rebuilds_in(lo, hi, freq=3, freq_early=10, exag_iter=250):
+    """Rebuild iterations of the schedule in [lo, hi) (the reference's cadence,
+    src/tsne_h2pack.c:1765-1776, with force reuse)."""
+    def reb(it):
+        eff = 1 if it < 20 else freq
+        if 20 <= it < exag_iter and freq_early > eff:
+            eff = freq_early
+        return it > 0 and it % eff == 0
+    return sum(reb(it) for it in range(lo, hi))
+
+
+def _timings(lib, ws):
+    a, r, pr, o = (ctypes.c_double() for _ in range(4))
+    nr = ctypes.c_int()
+    lib.tsne_run_timings(ws, ctypes.byref(a), ctypes.byref(r), ctypes.byref(pr),
+                         ctypes.byref(o), ctypes.byref(nr))
+    return dict(attr_ms=a.value, rebuild_ms=r.value, product_ms=pr.value, other_ms=o.value,
+                n_rebuild=nr.value)
+
+
+def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol, n_iter=1000,
+              full_schedule=True, phase_split=True):
+    """Steady-state window of one job: a job of n_iter iterations whose first
+    n_iter - K iterations run untimed (tsne_run_range), then the last K iterations
+    [n_iter - K, n_iter) are timed, bracketed by device synchronisation (K = steps).
+    When the untimed prefix is shorter than `warmup`, an untimed warm-up job of `warmup`
+    iterations runs first.  Also: the gradient error at the window's end; a fresh
+    n_iter-iteration job timed whole (full_schedule); a third job with per-phase HIP
+    events (synchronising every iteration, so kept out of both timed runs) for the phase
+    split of the whole schedule and of the window."""
     lib = be.lib
-    err_warm = None
-    if warmup > 0:
+    K = max(1, min(steps, n_iter))
+    lo = n_iter - K
+    if lo < warmup:
         ws = make_workspace(be, csr, n, dim, precision, warmup, leaf, tol)
         try:
             assert lib.tsne_run(ws) == 0
             lib.tsne_synchronize(ws)
-            err_warm = gradient_error(be, ws, n, dim)
         finally:
             lib.tsne_destroy(ws)
-    ws = make_workspace(be, csr, n, dim, precision, steps, leaf, tol)
+    ws = make_workspace(be, csr, n, dim, precision, n_iter, leaf, tol)
     try:
-        # mode 2: HIP events on the job's stream around every attractive+update launch,
-        # read after the run (no per-iteration synchronisation)
+        if lo > 0:
+            assert lib.tsne_run_range(ws, 0, lo) == 0
+        # mode 2: HIP events on the job's stream around every fused attractive+update
+        # launch of the window, read after the run (no per-iteration synchronisation)
         lib.tsne_enable_timing(ws, 2)
         lib.tsne_synchronize(ws)
         t0 = time.perf_counter()
-        rc = lib.tsne_run(ws)
+        rc = lib.tsne_run_range(ws, lo, n_iter)
         lib.tsne_synchronize(ws)
         dt = time.perf_counter() - t0
         assert rc == 0
         ka, kn = ctypes.c_double(), ctypes.c_int()
         lib.tsne_attr_kernel_time(ws, ctypes.byref(ka), ctypes.byref(kn))
         attr_kernel_us = 1e3 * ka.value / max(kn.value, 1)
-        ta, tn = ctypes.c_double(), ctypes.c_int()
-        lib.tsne_attr_kernel_time_tiled(ws, ctypes.byref(ta), ctypes.byref(tn))
-        tiled_us = 1e3 * ta.value / tn.value if tn.value else None
-        row_us = (1e3 * (ka.value - ta.value) / (kn.value - tn.value)
-                  if kn.value > tn.value else None)
-        tv = [ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(), ctypes.c_int()]
-        lib.tsne_attr_tile_stats(ws, *[ctypes.byref(v) for v in tv])
-        tiles = dict(zip(["rows", "ntiles", "umax", "ucount", "n_global"], [v.value for v in tv]))
         kl = float(lib.compute_kl_divergence(ws))
         st = None
         from tsne_h2pack._backend import H2Stats
@@ -130,23 +157,34 @@ def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol):
         err = gradient_error(be, ws, n, dim)
     finally:
         lib.tsne_destroy(ws)
-    # phase split from a second, instrumented run (per-phase HIP events synchronise
-    # every iteration, so they are kept out of the timed run)
-    ws = make_workspace(be, csr, n, dim, precision, steps, leaf, tol)
-    try:
-        lib.tsne_enable_timing(ws, 1)
-        assert lib.tsne_run(ws) == 0
-        a, r, pr, o = (ctypes.c_double() for _ in range(4))
-        nr = ctypes.c_int()
-        lib.tsne_run_timings(ws, ctypes.byref(a), ctypes.byref(r), ctypes.byref(pr),
-                             ctypes.byref(o), ctypes.byref(nr))
-    finally:
-        lib.tsne_destroy(ws)
-    return dict(seconds=dt, its=steps / dt, kl=kl, attr_kernel_us=attr_kernel_us,
-                attr_kernel_launches=kn.value, attr_tiled_us=tiled_us,
-                attr_tiled_launches=tn.value, attr_row_us=row_us, tiles=tiles, attr_ms=a.value, rebuild_ms=r.value,
-                product_ms=pr.value, other_ms=o.value, n_rebuild=nr.value, h2=st,
-                grad_err_final=err, grad_err_after_warmup=err_warm)
+    out = dict(window=[lo, n_iter], seconds=dt, its=K / dt, kl=kl,
+               attr_kernel_us=attr_kernel_us, attr_kernel_launches=kn.value,
+               window_rebuilds=rebuilds_in(lo, n_iter), h2=st, grad_err_final=err)
+    if full_schedule:
+        ws = make_workspace(be, csr, n, dim, precision, n_iter, leaf, tol)
+        try:
+            lib.tsne_synchronize(ws)
+            t0 = time.perf_counter()
+            assert lib.tsne_run(ws) == 0
+            lib.tsne_synchronize(ws)
+            tf = time.perf_counter() - t0
+            out["full_schedule"] = dict(iterations=n_iter, seconds=tf, its=n_iter / tf,
+                                        kl=float(lib.compute_kl_divergence(ws)))
+        finally:
+            lib.tsne_destroy(ws)
+    if phase_split:
+        ws = make_workspace(be, csr, n, dim, precision, n_iter, leaf, tol)
+        try:
+            lib.tsne_enable_timing(ws, 1)
+            assert lib.tsne_run_range(ws, 0, lo) == 0
+            t1 = _timings(lib, ws)
+            assert lib.tsne_run_range(ws, lo, n_iter) == 0
+            t2 = _timings(lib, ws)
+        finally:
+            lib.tsne_destroy(ws)
+        out["phases_full_schedule"] = t2
+        out["phases_window"] = {k: t2[k] - t1[k] for k in t2}
+    return out
 
 
 def gradient_error(be, ws, n, dim, n_rows=10000, seed=1):
@@ -180,18 +218,30 @@ def gradient_error(be, ws, n, dim, n_rows=10000, seed=1):
 
 
 def cpu_baseline(csr, n, dim, seconds_target=15.0):
-    """The oracle (FP64 restatement of the reference path with exact repulsion) timed on
-    the host cores on a bounded sample: the full attractive pass and the exact repulsive
-    force on a sample of rows, scaled to one iteration over all rows."""
+    """The oracle (FP64 C/OpenMP restatement of the reference's per-iteration path) on
+    the host cores, per phase, on a bounded sample of the benchmark job:
+      attractive  the full CSR-P pass over all N rows (timed whole);
+      update      the gains/momentum/position update and centring over all N rows
+                  (timed whole);
+      repulsion   the exact O(N^2) sum -- the reference's direct branch
+                  (src/tsne_h2pack.c:407-491), NOT its H2 approximation, which cannot be
+                  built here (its H2Pack submodule and BLAS/LAPACK are absent) -- on a row
+                  sample sized to about seconds_target, scaled to N rows.
+    value: iterations/s of that exact-repulsion path at N (it is the reference's
+    method='exact' path, not its H2 path)."""
     import oracle
     from scipy.sparse import csr_matrix
     P = csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n))
     rng = np.random.default_rng(0)
     Y = rng.normal(scale=30.0, size=(dim, n))
     t0 = time.perf_counter()
-    oracle.attractive(P, Y, 1.0)
+    g = oracle.attractive(P, Y, 1.0)
     t_attr = time.perf_counter() - t0
-    # exact repulsion on a row sample sized to about seconds_target of CPU work
+    uY = np.zeros_like(Y)
+    gains = np.ones_like(Y)
+    t0 = time.perf_counter()
+    oracle.update(Y, uY, gains, g, 0.8, max(n / 12.0, 200.0))
+    t_upd = time.perf_counter() - t0
     rows = np.sort(rng.choice(n, size=min(n, 64), replace=False)).astype(np.int32)
     t0 = time.perf_counter()
     oracle.repulsive_rows(Y, 1.0, rows)
@@ -200,13 +250,20 @@ def cpu_baseline(csr, n, dim, seconds_target=15.0):
     rows = np.sort(rng.choice(n, size=m, replace=False)).astype(np.int32)
     t0 = time.perf_counter()
     oracle.repulsive_rows(Y, 1.0, rows)
-    t_rep = (time.perf_counter() - t0) * n / rows.size
-    t_iter = t_attr + t_rep
+    t_rows = time.perf_counter() - t0
+    t_rep = t_rows * n / rows.size
+    t_iter = t_attr + t_upd + t_rep
     return {"value": 1.0 / t_iter, "unit": "iters/s", "cores": oracle.num_threads(),
             "kind": "port",
-            "sample": (f"oracle FP64 (exact O(N^2) repulsion; reference H2 not buildable): "
-                       f"full attractive pass {t_attr:.2f}s + exact repulsion on {rows.size} "
-                       f"sampled rows scaled to N={n}")}
+            "sample": (f"oracle FP64 port of the reference's exact-repulsion path "
+                       f"(method='exact', src/tsne_h2pack.c:407-491; its H2 path is not "
+                       f"buildable here): attractive pass over all N {t_attr:.3f}s, update "
+                       f"over all N {t_upd:.3f}s, exact repulsion on {rows.size} sampled rows "
+                       f"{t_rows:.1f}s scaled to N={n}"),
+            "phases_s_per_iteration": {"attractive": t_attr, "update": t_upd,
+                                       "repulsion_exact": t_rep},
+            "attractive_plus_update_iters_per_s": 1.0 / (t_attr + t_upd),
+            "reference_published_h2_its_12_threads_m5max": 36.0}
 
 
 def attr_algorithmic_bytes(n, dim, nnz):
@@ -218,53 +275,39 @@ def attr_algorithmic_bytes(n, dim, nnz):
     return 8 * nnz + 4 * (n + 1) + n * (2 * gath + 4 * dim + 3 * 2 * 8 * dim)
 
 
-def attr_tiled_algorithmic_bytes(n, dim, nnz, tiles):
-    """Bytes one launch of the tiled attractive+update kernel must move at least: per
-    nonzero its 16-bit tile-local index and fp32 value; per row the row pointer and the
-    same per-row state as the per-row kernel; per tile its offset, and per distinct
-    neighbour of a tile its column (int32) and one read of its gather record (the
-    staging into LDS)."""
-    gath = 8 if dim == 2 else 16
-    return (6 * nnz + 4 * (n + 1) + n * (2 * gath + 4 * dim + 3 * 2 * 8 * dim)
-            + 4 * (tiles["ntiles"] + 1) + tiles["ucount"] * (4 + gath))
-
-
 def attr_roofline(n, dim, nnz, head):
-    """Roofline of the dominant kernel: the tiled attractive kernel when the run used it
-    (every launch after the first renumbering), else the per-row kernel."""
-    tiled = head.get("attr_tiled_us") is not None
-    if tiled:
-        kernel_us = head["attr_tiled_us"]
-        by = attr_tiled_algorithmic_bytes(n, dim, nnz, head["tiles"])
-        name = "attr_tile_kernel"
-    else:
-        kernel_us = head["attr_kernel_us"]
-        by = attr_algorithmic_bytes(n, dim, nnz)
-        name = "attr_update_kernel"
+    """Roofline of the dominant kernel, the fused attractive + update pass
+    (attr_update_kernel), from its launches inside the timed window: algorithmic bytes
+    per launch / the average launch duration from HIP events on the job's stream.
+    traffic: HBM bytes per launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+    passes over the same window of the same workload (None when no pass matches)."""
+    kernel_us = head["attr_kernel_us"]
+    by = attr_algorithmic_bytes(n, dim, nnz)
     ach = by / (kernel_us * 1e-6) / 1e9
     roof = {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS, "traffic": None,
-            "kernel": name, "kernel_avg_us": kernel_us, "algorithmic_bytes": by,
-            "launches": head["attr_tiled_launches"] if tiled else head["attr_kernel_launches"]}
-    t = measured_traffic(n, dim, nnz, name)
+            "kernel": "attr_update_kernel", "kernel_avg_us": kernel_us,
+            "algorithmic_bytes": by, "launches": head["attr_kernel_launches"],
+            "window": head["window"]}
+    t = measured_traffic(n, dim, nnz, head["window"])
     if t is not None:
         roof["traffic"] = t["bytes"]
         roof["traffic_source"] = t["source"]
     return roof
 
 
-def measured_traffic(n, dim, nnz, kernel="attr_update_kernel"):
+def measured_traffic(n, dim, nnz, window, kernel="attr_update_kernel"):
     """HBM bytes per attractive launch from the committed rocprofv3 counter passes
-    (FETCH_SIZE and WRITE_SIZE, KiB per dispatch), each scaled by the ratio of a plain
-    copy's known bytes to what the same counter read for it; None when no pass matches
-    this workload."""
+    (FETCH_SIZE and WRITE_SIZE, KiB per dispatch, averaged over the fused launches of
+    the same timed window), each scaled by the ratio of a plain copy's known bytes to
+    what the same counter read for it; None when no pass matches this workload."""
     path = os.path.join(ROOT, "profiles", f"{kernel}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
         m = json.load(f)
     w = m.get("workload", {})
-    if (w.get("n"), w.get("dim"), w.get("nnz")) != (n, dim, nnz):
+    if (w.get("n"), w.get("dim"), w.get("nnz"), w.get("window")) != (n, dim, nnz, list(window)):
         return None
     cf = m["copy_bytes"] / (m["copy_fetch_kib"] * 1024.0)
     cw = m["copy_bytes"] / (m["copy_write_kib"] * 1024.0)
@@ -284,18 +327,23 @@ def copy_probe(mib=1024, reps=5):
     print(json.dumps({"copy_bytes": mib * (1 << 20), "reps": reps}), flush=True)
 
 
-def pmc_summary(paths):
-    """Per-kernel mean of every counter over its dispatches, from rocprofv3
-    counter_collection.csv files."""
+def pmc_summary(paths, last=0):
+    """Per-kernel mean of every counter over its dispatches (in dispatch order; with
+    last > 0 over the last `last` dispatches of each kernel only, e.g. the fused launches
+    of the timed window), from rocprofv3 counter_collection.csv files."""
     import csv
     import collections
     acc = collections.defaultdict(lambda: collections.defaultdict(list))
     for p in paths:
         with open(p) as f:
             for row in csv.DictReader(f):
-                acc[row["Kernel_Name"]][row["Counter_Name"]].append(float(row["Counter_Value"]))
+                acc[row["Kernel_Name"]][row["Counter_Name"]].append(
+                    (int(row["Dispatch_Id"]), float(row["Counter_Value"])))
     for k in sorted(acc):
         for c, v in sorted(acc[k].items()):
+            v = [x for _, x in sorted(v)]
+            if last > 0:
+                v = v[-last:]
             print(f"{c:12s} dispatches {len(v):6d} mean {np.mean(v):14.6g} KiB-or-count  {k[:90]}")
 
 
@@ -442,7 +490,8 @@ def mixture_points(n, dim_in, n_clusters, seed):
     w = rng.dirichlet(np.ones(n_clusters))
     std = rng.uniform(0.5, 2.0, size=n_clusters)
     labels = rng.choice(n_clusters, size=n, p=w)
-    return centers[labels] + rng.normal(size=(n, dim_in)) * std[labels, None]
+    X = centers[labels] + rng.normal(size=(n, dim_in)) * std[labels, None]
+    return X.astype(np.float32).astype(np.float64)  # float32 data, as the configs state
 
 
 def knn_bench(n, dim_in, k, clusters, reps, cpu_rows):
@@ -591,6 +640,10 @@ def main():
                     help="Gaussian clusters (default: 100 at N >= 1M, else 20)")
     ap.add_argument("--paths", default="mixed,plain")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-full-schedule", action="store_true",
+                    help="skip the whole-schedule timing job of each path")
+    ap.add_argument("--no-phase-split", action="store_true",
+                    help="skip the per-phase instrumented job of each path")
     ap.add_argument("--save-inputs", default="", help="write the prepared P here and exit")
     ap.add_argument("--load-inputs", default="", help="read a P written by --save-inputs")
     ap.add_argument("--serving", action="store_true",
@@ -655,31 +708,51 @@ def main():
         np.savez(a.save_inputs, row_ptr=csr[0], col_idx=csr[1], values=csr[2])
         return
     res = {}
+    n_iter = 1000
     for path in a.paths.split(","):
         leaf = a.leaf_mixed if (path == "mixed" and a.leaf_mixed) else a.leaf
         log(f"timing {path} (leaf {leaf}) warmup={a.warmup} steps={a.steps}")
-        res[path] = time_path(be, csr, n, dim, path, a.warmup, a.steps, leaf, a.tol)
+        res[path] = time_path(be, csr, n, dim, path, a.warmup, a.steps, leaf, a.tol,
+                              n_iter=n_iter, full_schedule=not a.no_full_schedule,
+                              phase_split=not a.no_phase_split)
         log(f"{path}: {res[path]['its']:.1f} it/s {json.dumps(res[path])}")
     head = res.get("mixed") or res[next(iter(res))]
     nnz = int(csr[1].size)
     roof = attr_roofline(n, dim, nnz, head)
     cpu = None if a.no_cpu_baseline else cpu_baseline(csr, n, dim)
+    lo, hi = head["window"]
     out = {
         "metric": "iters/s at N=1M 2D, mixed vs plain FP32 GPU @<=1e-2 grad err; jobs/s + p99",
         "value": head["its"], "unit": "iters/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": 1e3 / head["its"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32",
-        "data": "synthetic (seeded Gaussian mixture, exact k-NN on GPU)",
+        "data": "synthetic (seeded Gaussian mixture, float32 points, exact k-NN on GPU)",
         "config": {"workload": f"tsne-{dim}d N={n} D=50 clusters={clusters} k={a.k} "
                                f"perplexity={a.perplexity:g} h2_tol={a.tol} rebuild 3/10 "
-                               f"leaf {a.leaf}", "n": n, "dim": dim, "nnz_P": nnz,
-                   "paths": a.paths},
+                               f"leaf {a.leaf}; timed window = iterations [{lo}, {hi}) of "
+                               f"a {n_iter}-iteration run ({head['window_rebuilds']} H2 "
+                               f"rebuilds in the window), the {lo} iterations before it "
+                               f"untimed",
+                   "n": n, "dim": dim, "nnz_P": nnz, "paths": a.paths,
+                   "window": [lo, hi]},
+        "precision": {"plain": "near field, far field and attractive force FP32; ID, "
+                               "perplexity, KL, Z and the update state FP64",
+                      "mixed": "as plain, but the far-field coupling as BF16-in/FP32-acc "
+                               "MFMA (nodes whose error estimate exceeds h2_tol redone in "
+                               "FP32) and the ID's pivoted Gram-Schmidt in FP32 at ID "
+                               "tolerance >= 1e-3 (R entries, solve and interpolation "
+                               "coefficients FP64)",
+                      "reciprocal": "1-ulp hardware reciprocal in the FP32 kernels"},
         "paths": res,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
     if "plain" in res and "mixed" in res:
         out["mixed_over_plain"] = res["mixed"]["its"] / res["plain"]["its"]
+        if "full_schedule" in res["mixed"] and "full_schedule" in res["plain"]:
+            out["full_schedule_its"] = {k: res[k]["full_schedule"]["its"] for k in res}
+            out["mixed_over_plain_full_schedule"] = (res["mixed"]["full_schedule"]["its"] /
+                                                     res["plain"]["full_schedule"]["its"])
     print(json.dumps(out), flush=True)
 
 
@@ -687,6 +760,10 @@ if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "--copy-probe":
         copy_probe()
     elif len(sys.argv) > 1 and sys.argv[1] == "--pmc-summary":
-        pmc_summary(sys.argv[2:])
+        args = sys.argv[2:]
+        last = 0
+        if len(args) >= 2 and args[0] == "--last":
+            last, args = int(args[1]), args[2:]
+        pmc_summary(args, last)
     else:
         main()

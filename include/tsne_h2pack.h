@@ -82,10 +82,13 @@ tsne_workspace_t *tsne_init(const double *X, int n_point, int pt_dim, int low_di
                             const tsne_params_t *params);
 
 /* [ref] include/tsne_h2pack.h:196-203 / src/tsne_h2pack.c:1356-1547.  Takes ownership of
- * P (released with sparse_matrix_destroy by tsne_destroy) once validation passes;
+ * P (released with sparse_matrix_destroy by tsne_destroy) when it returns a workspace;
  * low_dim must be 2 or 3; P must be n_point x n_point.  Uploads P, initialises Y
  * (copy of init_Y or seeded Box-Muller), and for N >= 2000 builds the first H2
- * structure on the device.  NULL on failure. */
+ * structure on the device.  NULL on failure; on every NULL return P is left untouched
+ * and still belongs to the caller (the reference frees P when its initial H2 build
+ * fails, src/tsne_h2pack.c:1518-1523, and keeps it on earlier failures; a caller that
+ * frees P after NULL, as the reference's own Python layer should, is safe here). */
 tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int low_dim,
                                    const tsne_params_t *params, sparse_matrix_t *P);
 
@@ -181,6 +184,9 @@ typedef struct {
     double far_flops, near_flops;
     int mixed_fallback_nodes;
     int direct_fallbacks;  /* builds so far whose product used the direct sum instead */
+    int rank_capped_builds; /* builds so far in which some node's skeleton rank reached
+                               the cap (32 in 2D, 48 in 3D) before the ID tolerance; the
+                               first one also prints a warning on stderr */
 } tsne_h2_stats_t;
 int tsne_h2_stats(tsne_workspace_t *workspace, tsne_h2_stats_t *stats);
 
@@ -190,6 +196,14 @@ int tsne_h2_stats(tsne_workspace_t *workspace, tsne_h2_stats_t *stats);
 int tsne_run_timings(tsne_workspace_t *workspace, double *attr_update_ms,
                      double *rebuild_ms, double *product_ms, double *other_ms,
                      int *n_rebuild);
+
+/* State of the workspace's run so far: H2 rebuilds inside the loop since iteration 0
+ * (the initial build of tsne_init_with_P not counted), point renumberings, the Z of the
+ * last repulsive product (the one the KL of the last iteration uses) and the last KL
+ * evaluated (at the last iteration, and every 50th when verbose).  Any pointer may be
+ * NULL.  0 on success. */
+int tsne_run_state(tsne_workspace_t *workspace, int *n_rebuild, int *n_renumber, double *Z,
+                   double *kl);
 
 /* Run only iterations [it_begin, it_end) of the schedule of an n_iter-iteration run
  * (the schedule follows the absolute iteration number).  tsne_run equals
@@ -342,15 +356,18 @@ int tsne_server_submit(tsne_server_t *server, int n_point, int low_dim,
 int tsne_server_poll(tsne_server_t *server, int job_id, tsne_job_info_t *info);
 
 /* Block until the job ends; copy its embedding (low_dim x n_point, caller order) to Y_out
- * if not NULL, fill *info, and release the job.  0 if it succeeded, -1 otherwise. */
+ * if not NULL, fill *info, and release the job.  0 if it succeeded, -1 otherwise.  One
+ * waiter per job: while a wait on job_id is in progress (or after it returned), another
+ * wait on the same id returns -1 at once. */
 int tsne_server_wait(tsne_server_t *server, int job_id, double *Y_out, tsne_job_info_t *info);
 
 /* Queued and running job counts and the device bytes charged to running jobs. */
 int tsne_server_status(tsne_server_t *server, int *queued, int *running,
                        long long *hbm_in_use);
 
-/* Drops queued jobs, waits for running ones, frees everything.  Call it after every
- * tsne_server_wait of other threads has returned. */
+/* Drops queued jobs (they end as failed), waits for running ones, wakes every
+ * tsne_server_wait still blocked in the server and waits until each has returned, then
+ * frees everything.  No call may use the server once destroy has begun. */
 void tsne_server_destroy(tsne_server_t *server);
 
 #ifdef __cplusplus

@@ -51,8 +51,6 @@ def lib():
         L.oracle_update.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp,
                                     ctypes.c_double, ctypes.c_double]
         L.oracle_update.restype = None
-        L.oracle_init_random.argtypes = [_dp, ctypes.c_int, ctypes.c_int, ctypes.c_uint]
-        L.oracle_init_random.restype = None
         L.oracle_is_rebuild_iter.argtypes = [ctypes.c_int] * 5
         L.oracle_is_rebuild_iter.restype = ctypes.c_int
         L.oracle_run.argtypes = [ctypes.c_int, ctypes.c_int, _ip, _ip, _dp, _dp, _dp, _dp,
@@ -132,12 +130,6 @@ def update(Y, uY, gains, grad, momentum, lr):
     """In-place gains/momentum/position update plus centring (all (dim, n) float64)."""
     dim, n = Y.shape
     lib().oracle_update(n, dim, Y, uY, gains, grad, float(momentum), float(lr))
-
-
-def init_random(n, dim, seed):
-    Y = np.zeros((dim, n))
-    lib().oracle_init_random(Y, n, dim, seed)
-    return Y
 
 
 def is_rebuild_iter(it, freq=3, freq_early=10, early_exag_iter=250, reuse=True):

@@ -46,3 +46,10 @@ def joint_P_oracle(G, perplexity):
     P = Pc + Pc.T
     P.data /= P.sum()
     return P
+
+
+def init_normal(n, dim, seed, scale=1e-4):
+    """Explicit initial embedding (dim, n): numpy default_rng(seed) N(0, scale^2), the
+    distribution of the reference's random init (src/tsne_utils.c:1129-1148), passed to
+    the library as init_Y so that no result depends on glibc's rand() sequence."""
+    return np.random.default_rng(seed).normal(scale=scale, size=(dim, n))

@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import oracle
-from datasets import gaussian_mixture, joint_P_oracle, knn_graph
+from datasets import gaussian_mixture, init_normal, joint_P_oracle, knn_graph
 from tsne_h2pack._backend import H2Stats
 
 pytestmark = pytest.mark.gpu
@@ -119,7 +119,7 @@ def test_h2_run_kl_within_one_percent(backend, problem):
     _say("h2 full run N=6000")
     _, labels, P = problem
     n = P.shape[0]
-    Y0 = oracle.init_random(n, 2, 11)
+    Y0 = init_normal(n, 2, 11)
     p = backend.create_default_params()
     p.verbose = 0
     p.h2_tol = 1e-2
@@ -174,7 +174,7 @@ def test_renumbering_is_transparent(backend, problem):
     gives an O(1) difference)."""
     _say("renumbering transparency")
     _, _, P = problem
-    Y0 = oracle.init_random(P.shape[0], 2, 21)
+    Y0 = init_normal(P.shape[0], 2, 21)
     ref = _run(backend, P, Y0, 34, False)
     ren = _run(backend, P, Y0, 34, True)
     rel = np.abs(ref - ren).max() / np.abs(ref).max()
@@ -188,7 +188,7 @@ def test_renumbering_keeps_structure(backend, problem):
     tests/test_optimizations.py:98-111)."""
     from sklearn.neighbors import NearestNeighbors
     X, _, P = problem
-    Y0 = oracle.init_random(P.shape[0], 2, 22)
+    Y0 = init_normal(P.shape[0], 2, 22)
     Y = _run(backend, P, Y0, 1000, True).T
     k = 10
     hd = NearestNeighbors(n_neighbors=k).fit(X).kneighbors(return_distance=False)
@@ -357,7 +357,7 @@ def test_overlapped_attractive_pass_matches_fused(backend, problem):
     import os
     _say("overlap vs fused")
     _, _, P = problem
-    Y0 = oracle.init_random(P.shape[0], 2, 23)
+    Y0 = init_normal(P.shape[0], 2, 23)
     out = {}
     for ov in ("0", "1"):
         old = os.environ.get("TSNE_OVERLAP")

@@ -14,7 +14,7 @@ import numpy as np
 import pytest
 
 import oracle
-from datasets import gaussian_mixture, joint_P_oracle, knn_graph
+from datasets import gaussian_mixture, init_normal, joint_P_oracle, knn_graph
 from tsne_h2pack._backend import H2Stats
 
 pytestmark = pytest.mark.gpu
@@ -105,7 +105,7 @@ def test_mixed_full_run_kl_within_one_percent(backend, problem):
     the plain run from the same initial embedding."""
     _say("mixed full run")
     n = problem.shape[0]
-    Y0 = oracle.init_random(n, 2, 41)
+    Y0 = init_normal(n, 2, 41)
     kls = []
     for mixed in (0, 1):
         p = backend.create_default_params()

@@ -18,7 +18,7 @@ import numpy as np
 import pytest
 
 import oracle
-from datasets import gaussian_mixture, joint_P_oracle, knn_graph
+from datasets import gaussian_mixture, init_normal, joint_P_oracle, knn_graph
 
 pytestmark = pytest.mark.gpu
 
@@ -70,7 +70,7 @@ def _run(backend, P, dim, Y0, n_iter, tile_rows):
 
 
 def _compare(backend, P, dim, seed, tile_rows, n_iter=31):
-    Y0 = oracle.init_random(P.shape[0], dim, seed) * 1e2
+    Y0 = init_normal(P.shape[0], dim, seed) * 1e2
     ref, st0 = _run(backend, P, dim, Y0, n_iter, 0)
     til, st = _run(backend, P, dim, Y0, n_iter, tile_rows)
     rel = np.abs(ref - til).max() / np.abs(ref).max()
@@ -110,7 +110,7 @@ def test_tiled_full_run_kl_matches_row_kernel(backend, problem2d):
     """1000 iterations (six renumberings, each rebuilding the tiles): final KL of the tiled
     run within 0.5% of the per-row run."""
     _say("tiles full run")
-    Y0 = oracle.init_random(problem2d.shape[0], 2, 34)
+    Y0 = init_normal(problem2d.shape[0], 2, 34)
     kls = []
     for rows in (0, 256):
         old = os.environ.get("TSNE_ATTR_TILE_ROWS")

@@ -162,13 +162,6 @@ def test_update_centres():
     assert np.abs(Y.mean(axis=1)).max() < 1e-12
 
 
-def test_random_init_moments():
-    Y = oracle.init_random(20000, 2, 1234)
-    assert abs(Y.std() - 1e-4) < 3e-6 and abs(Y.mean()) < 3e-6
-    Y2 = oracle.init_random(20000, 2, 1234)
-    assert np.array_equal(Y, Y2)
-
-
 # ---------------------------------------------------------------- schedule (A6)
 
 def test_schedule_counts_292_rebuilds():
@@ -183,10 +176,10 @@ def test_schedule_exact_method_never_rebuilds_late():
 
 
 def test_run_small_decreases_kl():
-    from datasets import gaussian_mixture, joint_P_oracle, knn_graph
+    from datasets import gaussian_mixture, init_normal, joint_P_oracle, knn_graph
     X, _ = gaussian_mixture(400, dim=10, n_clusters=4, seed=1)
     P = joint_P_oracle(knn_graph(X, 30), 10.0)
-    Y0 = oracle.init_random(400, 2, 1)
+    Y0 = init_normal(400, 2, 1)
     _, kl_short = oracle.run(P, Y0, n_iter=300)
     _, kl_long = oracle.run(P, Y0, n_iter=600)
     assert kl_long < kl_short

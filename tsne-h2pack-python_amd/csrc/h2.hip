@@ -45,6 +45,7 @@ struct H2 {
     int pq = 0;  // proxy slots the ID processes: the table's largest count, rounded up
     int hflags[FLAG_COUNT] = {0};
     int fallbacks = 0;  // builds that left the product to the direct sum, and why (first)
+    int rank_capped = 0;  // builds in which some node's ID stopped at RMAX short of id_tol
     int last_mixed = 0; // the last product was the mixed-precision one
     // growth factors (grown on overflow)
     double node_f = 8.0, pair_f = 1.0, pool_f = 1.0;
@@ -422,6 +423,11 @@ int h2_build(H2 *h, const void *Yg, int *nonfinite)
         const int rc = build_once(h, Yg);
         if (rc == 0) {
             h->usable = !h->hflags[FLAG_BIG_LEAF];
+            if (h->usable && h->hflags[FLAG_RANK_CAP] && h->rank_capped++ == 0)
+                fprintf(stderr, "tsne_h2pack: warning: a skeleton rank reached its cap (%d) before "
+                                "the ID tolerance %.3g; the product may exceed h2_tol = %.3g "
+                                "(counted in tsne_h2_stats.rank_capped_builds)\n",
+                        rmax_of(h->D), h->id_tol, h->tol);
             if (h->usable) return 0;
             why = "a leaf at the deepest level holds more points than a leaf can";
             break;
@@ -513,6 +519,7 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     st->proxy_max = 0;
     st->mixed_fallback_nodes = 0;
     st->direct_fallbacks = h->fallbacks;
+    st->rank_capped_builds = h->rank_capped;
     if (!h->usable || h->n_nodes <= 0) return 0;
     // flop counts of one product from the built structure (host copies; diagnostics only)
     const int nn = h->n_nodes;

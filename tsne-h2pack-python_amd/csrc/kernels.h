@@ -18,8 +18,17 @@ enum ScalarSlot {
     SLOT_ZBAD = 8,     // non-positive / non-finite Z seen (replaced by 1)
     SLOT_KL = 9,       // last KL divergence
     SLOT_ZEXACT = 10,  // Z of the FP64 check path
+    SLOT_ZSKIP = 11,   // 1 while the last product's Z was non-positive / non-finite: the
+                       // updates then leave the repulsive force out, as the reference
+                       // leaves it out of the gradient (src/tsne_h2pack.c:604-611)
     SLOT_COUNT = 16
 };
+
+// Scale of the unscaled repulsive sum F in the gradient: 4 / Z, or 0 while SLOT_ZSKIP is set.
+__device__ __forceinline__ double rep_scale_of(const double *scal)
+{
+    return scal[SLOT_ZSKIP] != 0.0 ? 0.0 : 4.0 / scal[SLOT_Z];
+}
 
 int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
                        const void *Yg, const float *F, double *Y64, double *uY, double *gains,

@@ -121,7 +121,7 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const int row0 = lblk * rows_per_block;
     const int row1 = min(n, row0 + rows_per_block);
-    const double rep_scale = ATTR_ONLY ? 0.0 : 4.0 / scal[SLOT_Z];
+    const double rep_scale = ATTR_ONLY ? 0.0 : rep_scale_of(scal);
     const double m_d = (!ATTR_ONLY && sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
     double my_sum = 0.0;  // lane d of the group: sum of new coordinate d over its rows
 
@@ -269,7 +269,7 @@ attr_tile_kernel(int n, int ntiles, int tile_rows, const int *__restrict__ row_p
     const bool local = U <= ucap;  // uniform over the workgroup
     if (local)
         for (int u = threadIdx.x; u < U; u += NT) s_rec[u] = Yg[tcols[u0 + u]];
-    const double rep_scale = 4.0 / scal[SLOT_Z];
+    const double rep_scale = rep_scale_of(scal);
     const double m_d = (sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
     double my_sum = 0.0;
     __syncthreads();
@@ -379,7 +379,7 @@ update_kernel(int n, const float *__restrict__ A, const float *__restrict__ F,
     const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const int row0 = lblk * rows_per_block;
     const int row1 = min(n, row0 + rows_per_block);
-    const double rep_scale = 4.0 / scal[SLOT_Z];
+    const double rep_scale = rep_scale_of(scal);
     double m[3] = {0.0, 0.0, 0.0}, sum[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < D; d++) m[d] = scal[SLOT_MEAN + d];
     for (int i = row0 + threadIdx.x; i < row1; i += 256) {
@@ -556,8 +556,10 @@ z_finalize_kernel(int nblocks, const double *__restrict__ partial, double *__res
     if (t == 0) {
         double z = 0.0;
         for (int k = 0; k < 256; k++) z += part[k];
-        if (!(z > 0.0) || !isfinite(z)) { scal[SLOT_ZBAD] = 1.0; z = 1.0; }
+        const bool bad = !(z > 0.0) || !isfinite(z);
+        if (bad) { scal[SLOT_ZBAD] = 1.0; z = 1.0; }
         scal[SLOT_Z] = z;
+        scal[SLOT_ZSKIP] = bad ? 1.0 : 0.0;
     }
 }
 

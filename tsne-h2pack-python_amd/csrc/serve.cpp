@@ -40,6 +40,7 @@ struct Job {
     double t_submit = 0, t_start = 0, t_end = 0;
     double kl = 0;
     std::vector<double> Y;
+    bool claimed = false;  // a tsne_server_wait owns the job (later waiters get -1)
 };
 
 }  // namespace
@@ -50,8 +51,9 @@ struct tsne_server {
     long long in_use = 0;
     int running = 0;
     bool stop = false;
+    int waiters = 0;  // tsne_server_wait calls inside the server (destroy waits for them)
     std::mutex mu;
-    std::condition_variable cv_work, cv_done;
+    std::condition_variable cv_work, cv_done, cv_idle;
     std::deque<Job *> queue;
     std::map<int, Job *> jobs;
     int next_id = 0;
@@ -188,19 +190,20 @@ int tsne_server_submit(tsne_server_t *s, int n_point, int low_dim, const tsne_pa
     j->P = P;
     if (init_Y) j->init_Y.assign(init_Y, init_Y + (size_t)n_point * low_dim);
     j->footprint = tsne_job_footprint(n_point, low_dim, P->row_ptr ? P->row_ptr[n_point] : P->nnz);
+    int id;
     {
         std::lock_guard<std::mutex> lk(s->mu);
         if (s->stop) {
             delete j;
             return -1;
         }
-        j->id = s->next_id++;
+        id = j->id = s->next_id++;
         j->t_submit = s->now();
         s->jobs[j->id] = j;
         s->queue.push_back(j);
     }
     s->cv_work.notify_all();
-    return j->id;
+    return id;
 }
 
 int tsne_server_poll(tsne_server_t *s, int job_id, tsne_job_info_t *info)
@@ -218,14 +221,18 @@ int tsne_server_wait(tsne_server_t *s, int job_id, double *Y_out, tsne_job_info_
     if (!s) return -1;
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->jobs.find(job_id);
-    if (it == s->jobs.end()) return -1;
+    // one waiter per job: the first claims it and releases it; any other gets -1 at once
+    if (it == s->jobs.end() || it->second->claimed) return -1;
     Job *j = it->second;
+    j->claimed = true;
+    s->waiters++;
     s->cv_done.wait(lk, [&] { return j->status == TSNE_JOB_DONE || j->status == TSNE_JOB_FAILED; });
     fill_info(j, info);
     const int rc = j->status == TSNE_JOB_DONE ? 0 : -1;
     if (rc == 0 && Y_out) memcpy(Y_out, j->Y.data(), j->Y.size() * sizeof(double));
-    s->jobs.erase(it);
+    s->jobs.erase(j->id);
     delete j;
+    if (--s->waiters == 0) s->cv_idle.notify_all();
     return rc;
 }
 
@@ -254,7 +261,13 @@ void tsne_server_destroy(tsne_server_t *s)
     }
     s->cv_work.notify_all();
     for (auto &t : s->workers) t.join();
-    s->cv_done.notify_all();
+    {
+        // every job has ended (run or dropped): wake the waiters and let them leave
+        // before their jobs and the server are freed
+        std::unique_lock<std::mutex> lk(s->mu);
+        s->cv_done.notify_all();
+        s->cv_idle.wait(lk, [&] { return s->waiters == 0; });
+    }
     for (auto &kv : s->jobs) {
         if (kv.second->P) sparse_matrix_destroy(kv.second->P);
         delete kv.second;

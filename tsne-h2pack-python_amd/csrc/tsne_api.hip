@@ -190,19 +190,27 @@ int upload_embedding(tsne_workspace *ws, const double *Ydn)
     return 0;
 }
 
-// Repulsive sums s, F for the current gather copy; then the pending-mean snapshot and Z.
-int repulsive(tsne_workspace *ws, int allow_h2)
+// Repulsive sums s, F for the current gather copy (H2 product or direct sum).
+int repulsive_sums(tsne_workspace *ws, int allow_h2)
 {
     const void *Yg = ws->d_Yg[ws->cur];
-    int rc;
     if (allow_h2 && ws->h2 && ws->h2_ok)
-        rc = h2_product(ws->h2, Yg, ws->d_s, ws->d_F, ws->precision == TSNE_H2_MIXED);
-    else
-        rc = launch_repulsive_direct_f32(ws->dim, ws->n, Yg, ws->d_s, ws->d_F, ws->st);
-    if (rc) return rc;
+        return h2_product(ws->h2, Yg, ws->d_s, ws->d_F, ws->precision == TSNE_H2_MIXED);
+    return launch_repulsive_direct_f32(ws->dim, ws->n, Yg, ws->d_s, ws->d_F, ws->st);
+}
+
+// The pending-mean snapshot and Z of the last repulsive sums.
+int repulsive_z(tsne_workspace *ws)
+{
     if (launch_shift_snapshot(ws->d_scal, ws->st)) return -1;
-    return launch_z(ws->dim, ws->n, Yg, ws->d_s, ws->d_F, ws->d_scal, ws->d_partial,
-                    ws->nblocks_red, ws->st);
+    return launch_z(ws->dim, ws->n, ws->d_Yg[ws->cur], ws->d_s, ws->d_F, ws->d_scal,
+                    ws->d_partial, ws->nblocks_red, ws->st);
+}
+
+int repulsive(tsne_workspace *ws, int allow_h2)
+{
+    const int rc = repulsive_sums(ws, allow_h2);
+    return rc ? rc : repulsive_z(ws);
 }
 
 int rebuild(tsne_workspace *ws)
@@ -306,6 +314,9 @@ double read_scalar(tsne_workspace *ws, int slot)
         return NAN;
     return v;
 }
+
+// Columns of the per-iteration profile (seconds).
+enum { PROF_ATTR = 0, PROF_REBUILD, PROF_PRODUCT, PROF_Z, PROF_UPDATE, PROF_OTHER, kProfCols };
 
 struct EvPair {
     hipEvent_t a = nullptr, b = nullptr;
@@ -510,9 +521,10 @@ tsne_workspace_t *tsne_init_with_P(const double *X, int n_point, int pt_dim, int
 
 namespace tsne {
 
-// Body of tsne_init_with_P.  *taken tells whether P now belongs to the library (it does
-// once the workspace exists, also when the initial build then fails and the workspace
-// is destroyed with it).
+// Body of tsne_init_with_P.  *taken tells whether P now belongs to the library: it does
+// exactly when a workspace is returned.  On every NULL return P is left untouched and
+// still belongs to the caller (also when the initial H2 build fails after the workspace
+// existed: P is detached before that workspace is destroyed).
 tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_params_t *params,
                               sparse_matrix_t *P, bool *taken)
 {
@@ -652,15 +664,16 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         delete ws;
         return nullptr;
     }
-    ws->P = P;  // ownership taken once the workspace exists
-    *taken = true;
+    ws->P = P;
     if (!use_direct(ws)) {
         if (rebuild(ws) != 0) {
             fprintf(stderr, "Error: Failed to build initial H2 matrix\n");
+            ws->P = nullptr;  // P stays with the caller on a NULL return
             tsne_destroy(ws);
             return nullptr;
         }
     }
+    *taken = true;
     if (params->verbose)
         printf("Initialized t-SNE on the GPU (n=%d, low_dim=%d, nnz=%d, %s repulsion)\n", n, D, nnz,
                use_direct(ws) ? "direct" : "H2");
@@ -682,7 +695,9 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
     if (it_begin < 0) it_begin = 0;
     if (it_end > n_iter) it_end = n_iter;
     const bool timing = ws->timing == 1 || pp.enable_profiling;
-    hipEvent_t ev[5];
+    // ev: iteration start, after the rebuild, after the product, after Z, after the KL,
+    // after the wait for the overlapped attractive pass, end of the update
+    hipEvent_t ev[7];
     if (timing)
         for (auto &e : ev) (void)hipEventCreate(&e);
     // timing 2: a pair of events around every attractive launch, read after the last
@@ -704,7 +719,7 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         ws->t_attr_tiled = 0;
         ws->n_attr_tiled = 0;
         ws->n_rebuild = 0;
-        if (pp.enable_profiling) ws->prof.assign((size_t)n_iter * 4, 0.0);
+        if (pp.enable_profiling) ws->prof.assign((size_t)n_iter * kProfCols, 0.0);
     }
     if (pp.verbose && it_begin == 0)
         printf("\nStarting t-SNE optimization (%d iterations, learning rate %.1f)\n", n_iter, ws->lr);
@@ -754,22 +769,26 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
             ws->n_rebuild++;
         }
         if (timing) (void)hipEventRecord(ev[1], ws->st);
-        if (!reuse || rebuilt || !ws->rep_valid) {
-            if (repulsive(ws, !direct) != 0) return -1;
+        const bool fresh = !reuse || rebuilt || !ws->rep_valid;
+        if (fresh && repulsive_sums(ws, !direct) != 0) return -1;
+        if (timing) (void)hipEventRecord(ev[2], ws->st);
+        if (fresh) {
+            if (repulsive_z(ws) != 0) return -1;
             ws->rep_valid = reuse;
         }
-        if (timing) (void)hipEventRecord(ev[2], ws->st);
+        if (timing) (void)hipEventRecord(ev[3], ws->st);
         if ((pp.verbose && it % 50 == 0) || it == n_iter - 1) {
             if (launch_kl(D, ws->n, ws->d_rowptr, ws->d_col, ws->d_val64, ws->d_Y64, ws->d_scal,
                           exag, ws->d_rowkl, ws->d_partial, ws->nblocks_red, ws->d_scal + SLOT_KL,
                           ws->st) != 0)
                 return -1;
         }
-        if (timing) (void)hipEventRecord(ev[3], ws->st);
+        if (timing) (void)hipEventRecord(ev[4], ws->st);
         const int nxt = ws->cur ^ 1;
         hipEvent_t kb = kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin)];
         hipEvent_t ke = kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin) + 1];
         if (split) TSNE_HIP_CHECK(hipStreamWaitEvent(ws->st, ws->ev_a, 0));
+        if (timing) (void)hipEventRecord(ev[5], ws->st);
         const int arc =
             split ? launch_update(D, ws->n, ws->d_A, ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains,
                                   ws->d_Yg[nxt], ws->d_scal, exag, mom, lr, ws->d_block_sums,
@@ -790,22 +809,27 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         }
         ws->cur = nxt;
         if (timing) {
-            (void)hipEventRecord(ev[4], ws->st);
-            (void)hipEventSynchronize(ev[4]);
-            float a = 0, b = 0, c = 0, d = 0;
-            (void)hipEventElapsedTime(&a, ev[0], ev[1]);
-            (void)hipEventElapsedTime(&b, ev[1], ev[2]);
-            (void)hipEventElapsedTime(&c, ev[2], ev[3]);
-            (void)hipEventElapsedTime(&d, ev[3], ev[4]);
-            ws->t_rebuild += a;
-            ws->t_product += b;
-            ws->t_other += c;
-            ws->t_attr += d;
-            if (pp.enable_profiling && (size_t)it * 4 + 3 < ws->prof.size()) {
-                ws->prof[(size_t)it * 4 + 0] = d * 1e-3;
-                ws->prof[(size_t)it * 4 + 1] = a * 1e-3;
-                ws->prof[(size_t)it * 4 + 2] = b * 1e-3;
-                ws->prof[(size_t)it * 4 + 3] = c * 1e-3;
+            (void)hipEventRecord(ev[6], ws->st);
+            (void)hipEventSynchronize(ev[6]);
+            float t[6] = {0, 0, 0, 0, 0, 0};
+            for (int k = 0; k < 6; k++) (void)hipEventElapsedTime(&t[k], ev[k], ev[k + 1]);
+            // t: rebuild, product, Z, KL, attractive (the wait for the overlapped pass on
+            // split iterations, the fused attractive + update pass otherwise), update
+            // (split iterations only: the update kernel after the overlapped pass)
+            const float attr = split ? t[4] : t[4] + t[5];
+            const float upd = split ? t[5] : 0.f;
+            ws->t_rebuild += t[0];
+            ws->t_product += t[1] + t[2];
+            ws->t_other += t[3];
+            ws->t_attr += attr + upd;
+            if (pp.enable_profiling && (size_t)(it + 1) * kProfCols <= ws->prof.size()) {
+                double *r = &ws->prof[(size_t)it * kProfCols];
+                r[PROF_ATTR] = attr * 1e-3;
+                r[PROF_REBUILD] = t[0] * 1e-3;
+                r[PROF_PRODUCT] = t[1] * 1e-3;
+                r[PROF_Z] = t[2] * 1e-3;
+                r[PROF_UPDATE] = upd * 1e-3;
+                r[PROF_OTHER] = t[3] * 1e-3;
             }
         }
         if (pp.verbose && pp.print_freq > 0 && (it % pp.print_freq == 0 || it == n_iter - 1)) {
@@ -844,22 +868,37 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
     if (pp.enable_profiling && it_end == n_iter && !ws->profile_file.empty()) {
         FILE *f = fopen(ws->profile_file.c_str(), "w");
         if (f) {
-            // the reference's CSV schema, src/tsne_h2pack.c:1553-1628
+            // the reference's CSV schema and summary, src/tsne_h2pack.c:1553-1628.  The
+            // product is one fused pass, so h2_matvec_v is 0 as in the reference (:551);
+            // on iterations whose update is fused into the attractive pass, gradient is 0
+            // and p_matvec holds both
             fprintf(f, "iteration,p_matvec,h2_rebuild,h2_matvec_s,h2_matvec_v,z_computation,"
                        "gradient,other,total\n");
-            double tot[4] = {0, 0, 0, 0};
+            double tot[kProfCols] = {0, 0, 0, 0, 0, 0}, all = 0.0;
             for (int it = 0; it < n_iter; it++) {
-                const double *r = &ws->prof[(size_t)it * 4];
-                fprintf(f, "%d,%.6e,%.6e,%.6e,0,0,0,%.6e,%.6e\n", it, r[0], r[1], r[2], r[3],
-                        r[0] + r[1] + r[2] + r[3]);
-                for (int k = 0; k < 4; k++) tot[k] += r[k];
+                const double *r = &ws->prof[(size_t)it * kProfCols];
+                double t = 0.0;
+                for (int k = 0; k < kProfCols; k++) {
+                    tot[k] += r[k];
+                    t += r[k];
+                }
+                all += t;
+                fprintf(f, "%d,%.6f,%.6f,%.6f,%.6f,%.6f,%.6f,%.6f,%.6f\n", it, r[PROF_ATTR],
+                        r[PROF_REBUILD], r[PROF_PRODUCT], 0.0, r[PROF_Z], r[PROF_UPDATE],
+                        r[PROF_OTHER], t);
             }
-            fprintf(f, "# Total p_matvec time: %.6f\n", tot[0]);
-            fprintf(f, "# Total h2_rebuild time: %.6f\n", tot[1]);
-            fprintf(f, "# Total h2_matvec time: %.6f\n", tot[2]);
-            fprintf(f, "# Total other time: %.6f\n", tot[3]);
-            fprintf(f, "# Total k-NN time: %.6f\n", ws->knn_time);
-            fprintf(f, "# Total P construction time: %.6f\n", ws->p_time);
+            const double pct = all > 0.0 ? 100.0 / all : 0.0;
+            fprintf(f, "\n# Summary Statistics\n");
+            fprintf(f, "# k-NN time: %.3f seconds\n", ws->knn_time);
+            fprintf(f, "# P construction: %.3f seconds\n", ws->p_time);
+            fprintf(f, "# Total P-matvec: %.3f seconds (%.1f%%)\n", tot[PROF_ATTR], tot[PROF_ATTR] * pct);
+            fprintf(f, "# Total H²-rebuild: %.3f seconds (%.1f%%)\n", tot[PROF_REBUILD],
+                    tot[PROF_REBUILD] * pct);
+            fprintf(f, "# Total H²-matvec: %.3f seconds (%.1f%%)\n", tot[PROF_PRODUCT],
+                    tot[PROF_PRODUCT] * pct);
+            fprintf(f, "# Total gradient: %.3f seconds (%.1f%%)\n", tot[PROF_UPDATE],
+                    tot[PROF_UPDATE] * pct);
+            fprintf(f, "# Total optimization time: %.3f seconds\n", all);
             fclose(f);
         }
     }
@@ -982,7 +1021,9 @@ int tsne_compute_gradient(tsne_workspace_t *ws, double exag, int exact, double *
                            ws->st) != hipSuccess)
             return -1;
         Z = read_scalar(ws, SLOT_Z);
-        for (size_t k = 0; k < F.size(); k++) rep[k] = 4.0 / Z * (double)F[k];
+        // as the run: no repulsive force from a product whose Z was invalid
+        const double sc = read_scalar(ws, SLOT_ZSKIP) != 0.0 ? 0.0 : 4.0 / Z;
+        for (size_t k = 0; k < F.size(); k++) rep[k] = sc * (double)F[k];
         ws->rep_valid = 0;
     }
     // attractive part on the host in FP64 from the FP64 master coordinates, all in the
@@ -1064,6 +1105,16 @@ int tsne_run_timings(tsne_workspace_t *ws, double *attr, double *reb, double *pr
     if (prod) *prod = ws->t_product;
     if (other) *other = ws->t_other;
     if (n_rebuild) *n_rebuild = ws->n_rebuild;
+    return 0;
+}
+
+int tsne_run_state(tsne_workspace_t *ws, int *n_rebuild, int *n_renumber, double *Z, double *kl)
+{
+    if (!ws) return -1;
+    if (n_rebuild) *n_rebuild = ws->n_rebuild;
+    if (n_renumber) *n_renumber = ws->n_renumber;
+    if (Z) *Z = read_scalar(ws, SLOT_Z);
+    if (kl) *kl = read_scalar(ws, SLOT_KL);
     return 0;
 }
 

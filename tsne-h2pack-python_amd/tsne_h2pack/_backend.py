@@ -85,6 +85,7 @@ class H2Stats(ctypes.Structure):
         ('proxy_max', ctypes.c_int), ('far_flops', ctypes.c_double),
         ('near_flops', ctypes.c_double), ('mixed_fallback_nodes', ctypes.c_int),
         ('direct_fallbacks', ctypes.c_int),
+        ('rank_capped_builds', ctypes.c_int),
     ]
 
 
@@ -477,6 +478,8 @@ def configure_signatures(lib):
     lib.tsne_h2_stats.restype = ctypes.c_int
     lib.tsne_run_timings.argtypes = [c_ws, dp, dp, dp, dp, ip]
     lib.tsne_run_timings.restype = ctypes.c_int
+    lib.tsne_run_state.argtypes = [c_ws, ip, ip, dp, dp]
+    lib.tsne_run_state.restype = ctypes.c_int
     lib.tsne_run_range.argtypes = [c_ws, ctypes.c_int, ctypes.c_int]
     lib.tsne_run_range.restype = ctypes.c_int
     lib.tsne_synchronize.argtypes = [c_ws]
