@@ -170,6 +170,19 @@ int tsne_compute_gradient(tsne_workspace_t *workspace, double exag, int exact,
 int tsne_exact_repulsive_rows(tsne_workspace_t *workspace, const int *rows, int n_rows,
                               double *out, double *Z_out);
 
+/* [add] No counterpart in the reference.  For callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * tsne_exact_repulsive_rows with the same kernels and the same result, bit for bit, but
+ * no allocation of any kind.  Its partial results (the FP64 sums of all points, the
+ * selected rows, the caller-to-current row map) live in one static 48 MiB device array
+ * per GPU, so N * (12 + 8 * low_dim) + n_rows * (16 + 8 * low_dim) bytes (each part
+ * rounded up to 256) must fit in 48 MiB -- e.g. N up to about 1.6 million in 2D with
+ * 10,000 rows; larger requests fail with a message.  The inputs (rows, a host array)
+ * and the output (out, n_rows x low_dim, host) are the caller's.  All calls on one GPU
+ * share that array: calls that overlap on one GPU race.  0 on success, -1 otherwise. */
+int tsne_exact_repulsive_rows_noalloc(tsne_workspace_t *workspace, const int *rows,
+                                      int n_rows, double *out, double *Z_out);
+
 /* Statistics of the last H2 build: levels, nodes, leaves, admissible and near pairs,
  * maximum and mean skeleton rank, proxy points of the finest level, flop counts of one
  * product; of the last mixed-precision product, the nodes whose BF16 coupling was redone

@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <ctime>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -22,6 +23,7 @@ using namespace tsne;
 
 struct tsne_workspace {
     int n = 0, pt_dim = 0, dim = 0;
+    int device = 0;             // the HIP device the workspace's memory and streams are on
     tsne_params_t params{};
     std::string profile_file;
     double lr = 200.0;
@@ -324,6 +326,62 @@ struct EvPair {
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------
+// Scratch of tsne_exact_repulsive_rows_noalloc: one file-scope device array per GPU
+// (this code object's copy on each device), so that the entry point allocates nothing.
+// Host code reaches its memory only through the address hipGetSymbolAddress returns
+// for the device, fetched once per device and kept in g_scratch_ptr; the array's name is
+// used for nothing else.  Every call on a device shares that device's array: calls that
+// overlap on one GPU race.
+// ---------------------------------------------------------------------------------
+constexpr size_t kNoAllocScratchBytes = 48u << 20;
+constexpr int kNoAllocMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+
+void *g_scratch_ptr[kNoAllocMaxDevices] = {nullptr};
+std::mutex g_scratch_mu;
+
+// The device address of `scratch` on `device` (which must be current), or nullptr.
+void *scratch_ptr(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "tsne_h2pack: hipGetSymbolAddress(scratch) failed on device %d: %s\n",
+                    device, hipGetErrorString(e));
+            return nullptr;
+        }
+        g_scratch_ptr[device] = p;
+    }
+    return g_scratch_ptr[device];
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+// inv[order[k]] = k: current index of every caller point.
+__global__ void __launch_bounds__(256) invert_order_kernel(int n, const int *__restrict__ order,
+                                                           int *__restrict__ inv)
+{
+    const int k = blockIdx.x * 256 + threadIdx.x;
+    if (k < n) inv[order[k]] = k;
+}
+
+// Caller rows to current rows.
+__global__ void __launch_bounds__(256) map_rows_kernel(int n_rows, const int *__restrict__ rows,
+                                                       const int *__restrict__ inv,
+                                                       int *__restrict__ out)
+{
+    const int r = blockIdx.x * 256 + threadIdx.x;
+    if (r < n_rows) out[r] = inv[rows[r]];
+}
+
+}  // namespace
+
 extern "C" {
 
 tsne_params_t tsne_create_default_params(void)
@@ -593,7 +651,8 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         const char *e = getenv("TSNE_OVERLAP");
         ws->overlap = e ? (e[0] != '0') : (n_point >= (1 << 18));
     }
-    bool ok = hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
+    bool ok = hipGetDevice(&ws->device) == hipSuccess &&
+              hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
     if (ok && ws->overlap)
         ok = hipStreamCreateWithFlags(&ws->st2, hipStreamNonBlocking) == hipSuccess &&
              hipEventCreateWithFlags(&ws->ev_y, hipEventDisableTiming) == hipSuccess &&
@@ -1088,6 +1147,73 @@ int tsne_exact_repulsive_rows(tsne_workspace_t *ws, const int *rows, int n_rows,
     for (size_t k = 0; k < F.size(); k++) out[k] = 4.0 / Z * F[k];
     if (Z_out) *Z_out = Z;
     return 0;
+}
+
+int tsne_exact_repulsive_rows_noalloc(tsne_workspace_t *ws, const int *rows, int n_rows,
+                                      double *out, double *Z_out)
+{
+    if (!ws || !rows || n_rows <= 0 || !out) return -1;
+    const int n = ws->n, D = ws->dim;
+    for (int r = 0; r < n_rows; r++)
+        if (rows[r] < 0 || rows[r] >= n) {
+            fprintf(stderr, "tsne_exact_repulsive_rows_noalloc: row %d out of range\n", rows[r]);
+            return -1;
+        }
+    // scratch layout: inverse order, caller rows, current rows, s and F of all points,
+    // s and F of the selected rows; each part 256-byte aligned
+    const size_t b_inv = align256((size_t)n * sizeof(int));
+    const size_t b_rows = align256((size_t)n_rows * sizeof(int));
+    const size_t b_s = align256((size_t)n * sizeof(double));
+    const size_t b_F = align256((size_t)n * D * sizeof(double));
+    const size_t b_s2 = align256((size_t)n_rows * sizeof(double));
+    const size_t b_F2 = align256((size_t)n_rows * D * sizeof(double));
+    const size_t need = b_inv + 2 * b_rows + b_s + b_F + b_s2 + b_F2;
+    if (need > kNoAllocScratchBytes) {
+        fprintf(stderr, "tsne_exact_repulsive_rows_noalloc: n=%d, %d rows need %zu bytes of "
+                        "scratch, above its %zu-byte maximum (use tsne_exact_repulsive_rows)\n",
+                n, n_rows, need, kNoAllocScratchBytes);
+        return -1;
+    }
+    int cur_dev = 0;
+    if (hipGetDevice(&cur_dev) != hipSuccess) return -1;
+    if (cur_dev != ws->device && hipSetDevice(ws->device) != hipSuccess) return -1;
+    int rc = 0;
+    char *base = (char *)scratch_ptr(ws->device);
+    if (!base) rc = -1;
+    if (rc == 0) {
+        int *d_inv = (int *)base;
+        int *d_rows_in = (int *)(base + b_inv);
+        int *d_rows = (int *)(base + b_inv + b_rows);
+        double *d_s = (double *)(base + b_inv + 2 * b_rows);
+        double *d_F = (double *)((char *)d_s + b_s);
+        double *d_s2 = (double *)((char *)d_F + b_F);
+        double *d_F2 = (double *)((char *)d_s2 + b_s2);
+        hipStream_t st = ws->st;
+        rc = hipMemcpyAsync(d_rows_in, rows, (size_t)n_rows * sizeof(int), hipMemcpyHostToDevice,
+                            st) != hipSuccess;
+        if (rc == 0) {
+            hipLaunchKernelGGL(invert_order_kernel, dim3((n + 255) / 256), dim3(256), 0, st, n,
+                               ws->d_order, d_inv);
+            hipLaunchKernelGGL(map_rows_kernel, dim3((n_rows + 255) / 256), dim3(256), 0, st, n_rows,
+                               d_rows_in, d_inv, d_rows);
+            rc = hipGetLastError() != hipSuccess;
+        }
+        // the same kernels and order of operations as tsne_exact_repulsive_rows
+        rc = rc || launch_repulsive_direct_f64(D, n, ws->d_Y64, nullptr, n, d_s, d_F, st);
+        rc = rc || launch_z_f64(D, n, ws->d_Y64, d_s, d_F, ws->d_scal, ws->d_partial,
+                                ws->nblocks_red, ws->d_scal + SLOT_ZEXACT, st);
+        rc = rc || launch_repulsive_direct_f64(D, n, ws->d_Y64, d_rows, n_rows, d_s2, d_F2, st);
+        rc = rc || hipMemcpyAsync(out, d_F2, (size_t)n_rows * D * sizeof(double),
+                                  hipMemcpyDeviceToHost, st) != hipSuccess;
+        rc = rc || hipStreamSynchronize(st) != hipSuccess;
+        if (rc == 0) {
+            const double Z = read_scalar(ws, SLOT_ZEXACT);
+            for (size_t k = 0; k < (size_t)n_rows * D; k++) out[k] = 4.0 / Z * out[k];
+            if (Z_out) *Z_out = Z;
+        }
+    }
+    if (cur_dev != ws->device) (void)hipSetDevice(cur_dev);
+    return rc ? -1 : 0;
 }
 
 int tsne_h2_stats(tsne_workspace_t *ws, tsne_h2_stats_t *stats)

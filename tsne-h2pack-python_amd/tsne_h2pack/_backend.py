@@ -474,6 +474,8 @@ def configure_signatures(lib):
         c_ws, np.ctypeslib.ndpointer(dtype=np.int32, flags='C_CONTIGUOUS'), ctypes.c_int,
         f64c, dp]
     lib.tsne_exact_repulsive_rows.restype = ctypes.c_int
+    lib.tsne_exact_repulsive_rows_noalloc.argtypes = lib.tsne_exact_repulsive_rows.argtypes
+    lib.tsne_exact_repulsive_rows_noalloc.restype = ctypes.c_int
     lib.tsne_h2_stats.argtypes = [c_ws, ctypes.POINTER(H2Stats)]
     lib.tsne_h2_stats.restype = ctypes.c_int
     lib.tsne_run_timings.argtypes = [c_ws, dp, dp, dp, dp, ip]
