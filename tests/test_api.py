@@ -117,3 +117,57 @@ def test_precomputed_validation():
         t.fit_transform(csr_matrix(np.ones((3, 4))))
     with pytest.raises(ValueError, match="more than perplexity"):
         t.fit_transform(csr_matrix(np.eye(4, k=1) + np.eye(4, k=-1)))
+
+
+@pytest.mark.parametrize("kw,pattern", [
+    # the fragments the reference's own tests match (tests/test_validation.py,
+    # tests/test_tsne.py:104-185)
+    (dict(metric='manhattan'), "metric=.*not supported"),
+    (dict(init='spectral'), "init=.*not supported"),
+    (dict(init=np.zeros(10)), "init"),
+    (dict(init=np.zeros((50, 3))), "n_components"),
+    (dict(method='invalid'), "method=.*not supported"),
+    (dict(n_components=-10), "n_components must be >= 1"),
+    (dict(perplexity=-30.0), "perplexity must be > 0"),
+    (dict(learning_rate=0), "learning_rate must be > 0"),
+    (dict(learning_rate=-200.0), "learning_rate must be > 0"),
+    (dict(max_iter=-100), "max_iter must be >= 1"),
+])
+def test_reference_message_fragments(kw, pattern):
+    with pytest.raises(ValueError, match=pattern):
+        TSNE(**kw)
+
+
+def test_reference_data_and_init_fragments():
+    t = TSNE(perplexity=5)
+    with pytest.raises(ValueError, match="n_samples=0 should be >= 2"):
+        t._validate_data(np.zeros((0, 3)))
+    with pytest.raises(ValueError, match="perplexity=.* must be less than"):
+        t._validate_data(np.zeros((5, 3)))
+    with pytest.raises(ValueError, match="contains NaN or inf"):
+        t._validate_data(np.full((10, 3), np.nan))
+    X = np.random.default_rng(0).normal(size=(50, 2))
+    with pytest.raises(ValueError, match="n_components"):
+        TSNE(init='pca', n_components=3, perplexity=5)._initial_embedding(X, 50)
+    with pytest.raises(ValueError, match="shape"):
+        TSNE(init=np.zeros((40, 2)))._initial_embedding(np.zeros((50, 5)), 50)
+    with pytest.warns(UserWarning, match="init='random'"):
+        TSNE(metric='precomputed')._initial_embedding(None, 50)
+    with pytest.raises(NotImplementedError, match="fit\\(\\) is not implemented"):
+        TSNE().fit(X)
+
+
+def test_precomputed_short_rows_fragment():
+    from scipy.sparse import csr_matrix
+    with pytest.raises(ValueError, match="neighbors"):
+        TSNE(metric='precomputed', perplexity=2).fit_transform(
+            csr_matrix(np.eye(4, k=1) + np.eye(4, k=-1)))
+
+
+def test_profiling_file_reaches_the_c_struct():
+    t = TSNE(enable_profiling=True, profiling_output_file="/tmp/x.csv", random_state=7,
+             n_iter_without_progress=123, verbose=2)
+    p = t._map_params_to_c_struct(n_samples=5000)
+    assert p.profiling_output_file == b"/tmp/x.csv" and p.enable_profiling == 1
+    assert p.random_seed == 7 and p.convergence_patience == 123 and p.verbose == 1
+    assert p.learning_rate == pytest.approx(5000 / 12.0) and p.print_freq == 50
