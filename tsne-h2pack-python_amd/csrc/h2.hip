@@ -16,6 +16,7 @@
 #include "h2.h"
 #include "h2_internal.h"
 #include "proxy.h"
+#include "radix.h"
 
 namespace tsne {
 
@@ -70,6 +71,16 @@ using SortCfg = rocprim::radix_sort_config<
                                         10, rocprim::block_radix_rank_algorithm::match>,
     0>;
 constexpr int kOnesweepMin = 1 << 18;
+// From this many items on, the library's own radix sort (radix.hip: no device-wide
+// atomics) sorts the Morton keys and pair lists; below it the library default, which
+// needs fewer launches.  TSNE_SORT=rocprim keeps the library sorts at every size.
+constexpr int kOwnSortMin = 1 << 15;
+
+bool own_sort(int n)
+{
+    const char *e = getenv("TSNE_SORT");  // read per sort: a test switches it at run time
+    return !(e && strcmp(e, "rocprim") == 0) && n >= kOwnSortMin;
+}
 
 template <typename V>
 hipError_t sort_pairs64(void *tmp, size_t &bytes, rocprim::double_buffer<uint64_t> &k,
@@ -180,7 +191,7 @@ bool alloc_sized(H2 *h)
     (void)rocprim::radix_sort_keys<SortCfg>(nullptr, b3, pb, maxpair, 0, 64, h->st);
     (void)rocprim::radix_sort_keys(nullptr, b3a, pb, maxpair, 0, 64, h->st);
     b3 = std::max(b3, b3a);
-    h->cub_bytes = std::max({b1, b2, b3});
+    h->cub_bytes = std::max({b1, b2, b3, radix_scratch_bytes(std::max(h->n, maxpair))});
     return hipMallocAsync(&h->cub_tmp, h->cub_bytes, h->st) == hipSuccess;
 }
 
@@ -193,6 +204,13 @@ int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end, hipStream_
     if (n <= 0) return 0;
     int bits = 32;
     while ((1LL << (bits - 32)) < d.node_cap) bits++;
+    if (own_sort(n)) {
+        // on return buf holds the sorted list (buf and sort_tmp exchanged as needed;
+        // both hold maxpair entries)
+        if (radix_sort_keys_u64(buf, d.sort_tmp, n, 0, bits, h->cub_tmp, h->cub_bytes, s) != 0)
+            return -1;
+        return launch_pair_offsets(buf, n, beg, end, s);
+    }
     rocprim::double_buffer<uint64_t> db(buf, d.sort_tmp);
     size_t bytes = h->cub_bytes;
     if (sort_keys64(h->cub_tmp, bytes, db, n, bits, s) != hipSuccess)
@@ -215,11 +233,15 @@ int build_once(H2 *h, const void *Yg)
         return -1;
     if (launch_root_box(D, Yg, h->n, d, h->part, h->nblk, st)) return -1;
     if (launch_morton(D, Yg, d, st)) return -1;
-    {
+    const int endbit = D * (D == 2 ? H2Cfg<2>::B : H2Cfg<3>::B);
+    if (own_sort(h->n)) {
+        if (radix_sort_pairs_u64(d.keys, h->keys_alt, d.perm, h->perm_alt, h->n, 0, endbit,
+                                 h->cub_tmp, h->cub_bytes, st) != 0)
+            return -1;
+    } else {
         rocprim::double_buffer<uint64_t> kb(d.keys, h->keys_alt);
         rocprim::double_buffer<int> vb(d.perm, h->perm_alt);
         size_t bytes = h->cub_bytes;
-        const int endbit = D * (D == 2 ? H2Cfg<2>::B : H2Cfg<3>::B);
         if (sort_pairs64(h->cub_tmp, bytes, kb, vb, h->n, endbit, st) != hipSuccess)
             return -1;
         if (kb.current() != d.keys) std::swap(d.keys, h->keys_alt);
@@ -291,8 +313,10 @@ int build_once(H2 *h, const void *Yg)
     h->n_near = h->pin[CNT_NEAR];
     h->n_leaves = h->pin[CNT_LEAF];
     // ADM and X ranges were written per node by the list kernel; NEAR and W pairs are
-    // appended by several levels and are grouped by target here.  Only the leaf product
-    // reads those lists, so with a second stream the sorts run there, beside the ID.
+    // appended by several levels and are grouped by target below.  Only the leaf product
+    // reads those lists, so with a second stream the sorts run there, beside the ID.  The
+    // ID launches are queued first: queueing the sorts' launches takes the host a few
+    // hundred microseconds, which the main stream would otherwise spend idle.
     hipStream_t ss = st;
     if (h->st_aux) {
         if (hipEventRecord(h->ev_tree, st) != hipSuccess ||
@@ -300,10 +324,6 @@ int build_once(H2 *h, const void *Yg)
             return -1;
         ss = h->st_aux;
     }
-    if (sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end, ss) ||
-        sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end, ss))
-        return -1;
-    if (h->st_aux && hipEventRecord(h->ev_sorted, h->st_aux) != hipSuccess) return -1;
     // ID + upward: every leaf at once (they need nothing from other nodes), then the
     // internal nodes level by level, finest first
     const int nlev = (int)h->lvl.size() - 1;
@@ -314,6 +334,10 @@ int build_once(H2 *h, const void *Yg)
         if (launch_id_up(D, d, nullptr, h->lvl[lv], h->lvl[lv + 1] - h->lvl[lv], h->id_tol, h->pq,
                          f32, st))
             return -1;
+    if (sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end, ss) ||
+        sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end, ss))
+        return -1;
+    if (h->st_aux && hipEventRecord(h->ev_sorted, h->st_aux) != hipSuccess) return -1;
     if (h->st_aux && hipStreamWaitEvent(st, h->ev_sorted, 0) != hipSuccess) return -1;
     if (hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
         hipStreamSynchronize(st) != hipSuccess)
