@@ -108,7 +108,7 @@ struct H2Dev {
     float *upool;
     long long pool_cap;
     float4 *wc, *zc;   // [node][RMAX] upward / downward coefficients
-    int *nflag;        // [node] the mixed coupling of this node needs the FP32 pass
+    int *nflag;        // [node] the last mixed coupling redid this node in FP32
     const double *proxy_pts;  // [levels][PMAX][D]
     const int *proxy_cnt;
     int *flags, *cnt;
@@ -151,8 +151,9 @@ int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, doub
                  int pq, bool f32, hipStream_t st);
 
 // h2_product.hip
-// mixed: coupling on MFMA with BF16 operands and FP32 accumulation, nodes whose
-// rounding-error estimate exceeds tau x |z| are flagged and recomputed by the FP32 pass.
+// mixed: coupling on MFMA with BF16 operands and FP32 accumulation; a node whose
+// rounding-error estimate exceeds tau times an entry's scale is flagged (nflag) and
+// recomputed in FP32 by the same wavefront.
 int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, float tau, hipStream_t st);
 int launch_down(int D, const H2Dev &h, int lb, int le, int mixed, hipStream_t st);
 int launch_leaf_eval(int D, const H2Dev &h, int n_leaves, float *s, float *F, hipStream_t st);

@@ -101,24 +101,15 @@ __device__ __forceinline__ void stage_source(const H2Dev &h, float4 ci, int m0, 
     }
 }
 
-// FP32 coupling: one wavefront per target node (all non-root nodes); lanes are (group g,
-// row k): row k of the node's skeleton, the groups split each staged round of 64
-// sources, which every lane reads from LDS.  only_flagged: just the nodes the mixed
-// kernel flagged.
+// FP32 coupling of target node i by one wavefront; lanes are (group g, row k): row k of
+// the node's skeleton, the groups split each staged round of 64 sources, which every lane
+// reads from LDS (the wavefront's own staging arrays).
 template <int D>
-__global__ void __launch_bounds__(256)
-couple_kernel(H2Dev h, int n_nodes, int only_flagged)
+__device__ __forceinline__ void couple_node_f32(const H2Dev &h, int i, int r, int lane,
+                                                float4 *s_x, float4 *s_w, int *s_off, int *s_j)
 {
     constexpr int RMAX = H2Cfg<D>::RMAX;
     constexpr int CH = 64;
-    __shared__ float4 s_x[4][CH], s_w[4][CH];
-    __shared__ int s_off[4][65], s_j[4][64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int i = 1 + blockIdx.x * 4 + wv;
-    if (i >= n_nodes) return;  // per wavefront: only wavefront-level LDS traffic below
-    if (only_flagged && h.nflag[i] == 0) return;
-    const int r = h.rank[i];
-    if (r <= 0) return;
     const int RS = rank_stride(r), G = 64 / RS;
     const int k = lane % RS, g = lane / RS;
     const float4 ci = h.cent[i];
@@ -127,18 +118,18 @@ couple_kernel(H2Dev h, int n_nodes, int only_flagged)
     const int a0 = h.adm_beg[i], a1 = h.adm_end[i], x0 = h.x_beg[i], x1 = h.x_end[i];
     const int na = a1 - a0, nm = na + (x1 - x0);
     for (int m0 = 0; m0 < nm; m0 += 64) {
-        const int T = member_round<D>(h, m0, na, nm, a0, x0, lane, s_off[wv], s_j[wv]);
+        const int T = member_round<D>(h, m0, na, nm, a0, x0, lane, s_off, s_j);
         const int nmem = min(64, nm - m0);
         for (int c0 = 0; c0 < T; c0 += CH) {
             float4 xs, ws;
-            stage_source<D>(h, ci, m0, na, nmem, c0 + lane, T, s_off[wv], s_j[wv], xs, ws);
-            s_x[wv][lane] = xs;
-            s_w[wv][lane] = ws;
+            stage_source<D>(h, ci, m0, na, nmem, c0 + lane, T, s_off, s_j, xs, ws);
+            s_x[lane] = xs;
+            s_w[lane] = ws;
             __builtin_amdgcn_wave_barrier();
             const int cn = min(CH, T - c0);
             for (int sl = g; sl < cn; sl += G) {
-                const float4 q = s_x[wv][sl];
-                const float4 w = s_w[wv][sl];
+                const float4 q = s_x[sl];
+                const float4 w = s_w[sl];
                 const float K = kern32(xk.x - q.x, xk.y - q.y, D == 3 ? xk.z - q.z : 0.f);
                 acc.x = fmaf(K, w.x, acc.x);
                 acc.y = fmaf(K, w.y, acc.y);
@@ -151,6 +142,21 @@ couple_kernel(H2Dev h, int n_nodes, int only_flagged)
     }
     acc = group_sum(acc, RS);
     if (g == 0 && k < r) h.zc[(size_t)i * RMAX + k] = acc;
+}
+
+// FP32 coupling: one wavefront per target node (all non-root nodes).
+template <int D>
+__global__ void __launch_bounds__(256)
+couple_kernel(H2Dev h, int n_nodes)
+{
+    __shared__ float4 s_x[4][64], s_w[4][64];
+    __shared__ int s_off[4][65], s_j[4][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int i = 1 + blockIdx.x * 4 + wv;
+    if (i >= n_nodes) return;  // per wavefront: only wavefront-level LDS traffic below
+    const int r = h.rank[i];
+    if (r <= 0) return;
+    couple_node_f32<D>(h, i, r, lane, s_x[wv], s_w[wv], s_off[wv], s_j[wv]);
 }
 
 // ---------------------------------------------------------------------------------
@@ -170,8 +176,10 @@ couple_kernel(H2Dev h, int n_nodes, int only_flagged)
 // Error control: rounding K to BF16 perturbs each term by at most 2^-9 relative with
 // signs that vary over the sources, so the error of an entry of z is estimated as
 // 2^-9 sqrt(sum (K W')^2) (a third MFMA accumulates K^2 W'^2).  A node where the estimate
-// exceeds tau |z| for any entry (strong cancellation between its sources) is flagged
-// and left to the FP32 pass (couple_kernel with only_flagged).
+// exceeds tau times the entry's scale for any entry (strong cancellation between its
+// sources) is flagged and redone at once by the same wavefront in FP32 (couple_node_f32);
+// the scale is |z| for the s column and max(|z|, L |z_0|) for a force column (L: the
+// node's half-side).
 // ---------------------------------------------------------------------------------
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
@@ -200,11 +208,15 @@ couple_mfma_kernel(H2Dev h, int n_nodes, float tau)
     __shared__ __align__(8) short s_wh[4][4][CH], s_wl[4][4][CH], s_w2[4][4][CH];
     __shared__ int s_off[4][65], s_j[4][64];
     __shared__ float s_z[4][16][16];
+    __shared__ float4 s_wf[4][CH];  // W' in FP32 for a node redone by the FP32 pass
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int i = 1 + blockIdx.x * 4 + wv;
     if (i >= n_nodes) return;  // per wavefront: only wavefront-level LDS traffic below
     const int r = h.rank[i];
-    if (r <= 0) return;
+    if (r <= 0) {
+        if (lane == 0) h.nflag[i] = 0;
+        return;
+    }
     const int G = r <= 4 ? 1 : r <= 8 ? 2 : r <= 16 ? 4 : r <= 32 ? 8 : 16;
     const int P = 16 / G;
     const int b = lane >> 2, u = lane & 3, g = b % G, ph = b / G;
@@ -263,14 +275,25 @@ couple_mfma_kernel(H2Dev h, int n_nodes, float tau)
             acc[q] += __shfl_xor(acc[q], o, 64);
             acc2[q] += __shfl_xor(acc2[q], o, 64);
         }
-    // lane (block b, column u) now holds z[4g + q][u], q = 0..3
+    // lane (block b, column u) now holds z[4g + q][u], q = 0..3.  The scale an entry's
+    // error is held to: |z| for the s column (u = 0); for a force column, max(|z|, L |z_0|)
+    // with L the node's half-side, since the leaf expansion adds it next to
+    // (y_t - c_i) z_0 with |y_t - c_i| up to L (a force column that cancels to near zero
+    // is small against that term, not a reason to redo the node in FP32)
+    const float L = ci.w;
     bool bad = false;
 #pragma unroll
-    for (int q = 0; q < 4; q++)
-        if (4 * g + q < r && 0.001953125f * sqrtf(fmaxf(acc2[q], 0.f)) > tau * fabsf(acc[q]))
+    for (int q = 0; q < 4; q++) {
+        const float z0 = __shfl(acc[q], lane & ~3, 64);
+        const float scale = (u == 0) ? fabsf(acc[q]) : fmaxf(fabsf(acc[q]), L * fabsf(z0));
+        if (4 * g + q < r && 0.001953125f * sqrtf(fmaxf(acc2[q], 0.f)) > tau * scale)
             bad = true;
-    if (__ballot(bad) != 0) {
-        if (lane == 0) h.nflag[i] = 1;
+    }
+    const bool redo = __ballot(bad) != 0;
+    if (lane == 0) h.nflag[i] = redo ? 1 : 0;  // every node's flag: no clearing pass
+    if (redo) {
+        __builtin_amdgcn_wave_barrier();
+        couple_node_f32<D>(h, i, r, lane, s_x[wv], s_wf[wv], s_off[wv], s_j[wv]);
         return;
     }
     if (ph == 0)
@@ -479,17 +502,15 @@ int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, float tau, hipS
     const int g = (n_nodes - 1 + 3) / 4;
     if (g <= 0) return 0;
     if (mixed) {
-        if (hipMemsetAsync(h.nflag, 0, sizeof(int) * (size_t)n_nodes, st) != hipSuccess)
-            return -1;
         if (D == 2)
             hipLaunchKernelGGL(couple_mfma_kernel<2>, dim3(g), dim3(256), 0, st, h, n_nodes, tau);
         else
             hipLaunchKernelGGL(couple_mfma_kernel<3>, dim3(g), dim3(256), 0, st, h, n_nodes, tau);
+    } else if (D == 2) {
+        hipLaunchKernelGGL(couple_kernel<2>, dim3(g), dim3(256), 0, st, h, n_nodes);
+    } else {
+        hipLaunchKernelGGL(couple_kernel<3>, dim3(g), dim3(256), 0, st, h, n_nodes);
     }
-    if (D == 2)
-        hipLaunchKernelGGL(couple_kernel<2>, dim3(g), dim3(256), 0, st, h, n_nodes, mixed);
-    else
-        hipLaunchKernelGGL(couple_kernel<3>, dim3(g), dim3(256), 0, st, h, n_nodes, mixed);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
