@@ -155,11 +155,13 @@ def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol, n_iter=1000,
         if lib.tsne_h2_stats(ws, ctypes.byref(s)) == 0:
             st = {f: getattr(s, f) for f, _ in H2Stats._fields_}
         err = gradient_error(be, ws, n, dim)
+        probe = attr_pattern_probe(lib, ws)
     finally:
         lib.tsne_destroy(ws)
     out = dict(window=[lo, n_iter], seconds=dt, its=K / dt, kl=kl,
                attr_kernel_us=attr_kernel_us, attr_kernel_launches=kn.value,
-               window_rebuilds=rebuilds_in(lo, n_iter), h2=st, grad_err_final=err)
+               window_rebuilds=rebuilds_in(lo, n_iter), h2=st, grad_err_final=err,
+               attr_pattern_us=probe)
     if full_schedule:
         ws = make_workspace(be, csr, n, dim, precision, n_iter, leaf, tol)
         try:
@@ -180,11 +182,45 @@ def time_path(be, csr, n, dim, precision, warmup, steps, leaf, tol, n_iter=1000,
             t1 = _timings(lib, ws)
             assert lib.tsne_run_range(ws, lo, n_iter) == 0
             t2 = _timings(lib, ws)
+            from tsne_h2pack._backend import H2Stats
+            s = H2Stats()
+            idr = None
+            if lib.tsne_h2_stats(ws, ctypes.byref(s)) == 0 and s.id_ms > 0:
+                idr = id_roofline(s.id_flops, s.id_ms, precision)
         finally:
             lib.tsne_destroy(ws)
         out["phases_full_schedule"] = t2
         out["phases_window"] = {k: t2[k] - t1[k] for k in t2}
+        out["id_roofline"] = idr
     return out
+
+
+def attr_pattern_probe(lib, ws, reps=20):
+    """Average launch time (us) of the attractive pass's memory pattern over the job's
+    final P and embedding: P streamed alone, P with the neighbour gathers, and the
+    attractive sums without the update (tsne_attr_probe modes 0-2).  The fused kernel's
+    time against the second one is its distance from its own access pattern's ceiling."""
+    out = {}
+    for mode, name in ((0, "p_stream"), (1, "p_and_gathers"), (2, "attractive_sums")):
+        ms = ctypes.c_double()
+        if lib.tsne_attr_probe(ws, mode, reps, ctypes.byref(ms)) != 0:
+            return None
+        out[name] = 1e3 * ms.value
+    return out
+
+
+def id_roofline(flops, ms, precision):
+    """The last rebuild's interpolative decompositions (proxy-point pivoted Gram-Schmidt
+    and interpolation solve, fused with the upward pass) against the FP64 vector peak:
+    flops = sum over nodes of (3d+3) m p + 4 m p r (kernel evaluation of m points against
+    p proxies, then the pivoted Gram-Schmidt to rank r), from tsne_h2_stats; time from HIP
+    events around the ID launches of that build.  The mixed path runs the Gram-Schmidt in
+    FP32 (peak 157.3 TFLOP/s)."""
+    peak = 157.3 if precision == "mixed" else 78.6
+    ach = flops / (ms * 1e-3) / 1e12
+    return {"bound": "fp64" if precision != "mixed" else "fp32", "achieved": ach,
+            "peak": peak, "unit": "TFLOP/s", "frac": ach / peak, "flops": flops,
+            "id_ms": ms}
 
 
 def gradient_error(be, ws, n, dim, n_rows=10000, seed=1):
@@ -385,6 +421,19 @@ def _job_init(sp):
     return np.random.default_rng(sp["seed"]).normal(scale=1e-2, size=(sp["n"], sp["dim"]))
 
 
+def _job_params(p, sp):
+    """The TSNE front end's settings for a job of this dimension (h2_tol 1e-2 in 2D, 3e-2
+    in 3D, rebuild every 3, 10 during early exaggeration, lr max(N/12, 200)), on a
+    tsne_params_t from the library defaults."""
+    p.verbose = 0
+    p.learning_rate = max(sp["n"] / 12.0, 200.0)
+    p.h2_tol = 1e-2 if sp["dim"] == 2 else 3e-2
+    p.h2_rebuild_freq = 3
+    p.h2_rebuild_freq_early = 10
+    p.h2_leaf_size = 100
+    return p
+
+
 def serving_bench(n_jobs, slots_list, seq_sample, load=0.7):
     """Serving workload: capacity (all jobs queued at once), then Poisson arrivals at
     `load` x capacity with per-job latency (arrival -> result); baseline: one job at a
@@ -408,9 +457,7 @@ def serving_bench(n_jobs, slots_list, seq_sample, load=0.7):
     t_seq, kl_seq = 0.0, {}
     for j in sample:
         sp = specs[j]
-        p = be.create_default_params()
-        p.verbose = 0
-        p.learning_rate = max(sp["n"] / 12.0, 200.0)
+        p = _job_params(be.create_default_params(), sp)
         t0 = time.perf_counter()
         ws = be.init_workspace(Ps[j], sp["dim"], p, init_Y=inits[j])
         assert lib.tsne_run(ws) == 0
@@ -434,7 +481,8 @@ def serving_bench(n_jobs, slots_list, seq_sample, load=0.7):
                     dt = arrivals[j] - (time.perf_counter() - t0)
                     if dt > 0:
                         time.sleep(dt)
-                ids.append(srv.submit(Ps[j], sp["dim"], srv.default_params(sp["n"]),
+                ids.append(srv.submit(Ps[j], sp["dim"],
+                                      _job_params(srv.default_params(sp["n"]), sp),
                                       init_Y=inits[j]))
             infos = []
             for k, (i, sp) in enumerate(zip(ids, specs)):

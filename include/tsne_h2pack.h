@@ -200,6 +200,11 @@ typedef struct {
     int rank_capped_builds; /* builds so far in which some node's skeleton rank reached
                                the cap (32 in 2D, 48 in 3D) before the ID tolerance; the
                                first one also prints a warning on stderr */
+    double id_flops;        /* FP64 work of the last build's IDs: sum over nodes of
+                               (3 low_dim + 3) m p + 4 m p r (m candidates, p proxies,
+                               r rank; single-child nodes excluded) */
+    double id_ms;           /* duration of the last build's ID launches (ms), measured
+                               with HIP events while timing mode 1 is on; else 0 */
 } tsne_h2_stats_t;
 int tsne_h2_stats(tsne_workspace_t *workspace, tsne_h2_stats_t *stats);
 
@@ -209,6 +214,25 @@ int tsne_h2_stats(tsne_workspace_t *workspace, tsne_h2_stats_t *stats);
 int tsne_run_timings(tsne_workspace_t *workspace, double *attr_update_ms,
                      double *rebuild_ms, double *product_ms, double *other_ms,
                      int *n_rebuild);
+
+/* Graph mode for later runs: iterations that reuse the repulsive force (no rebuild, no KL,
+ * no timing events) are replayed from HIP graphs of their two kernels (attractive force +
+ * update, centring), captured on first use per gather-buffer parity and schedule phase;
+ * results are bit-identical to the eager launches.  Default off, or TSNE_GRAPH=1 at
+ * tsne_init_with_P; the job server turns it on for its jobs (TSNE_GRAPH=0 keeps it off).
+ * 0 on success. */
+int tsne_set_graph_mode(tsne_workspace_t *workspace, int on);
+
+/* Iterations replayed from graphs so far (-1 for a NULL workspace). */
+int tsne_graph_launches(tsne_workspace_t *workspace);
+
+/* [add] Diagnostics, no counterpart in the reference: average time (ms, HIP events on the
+ * workspace stream, over `reps` launches after one warm-up) of the attractive pass's memory
+ * pattern over the workspace's current P and embedding.  mode 0: P streamed only; 1: P and
+ * the neighbour gathers (sum of p*y_j); 2: the full attractive sums without the update.
+ * The fused attractive/update kernel's time against modes 0 and 1 says how close it runs
+ * to its own access pattern's ceiling.  Leaves the embedding unchanged. */
+int tsne_attr_probe(tsne_workspace_t *workspace, int mode, int reps, double *avg_ms);
 
 /* State of the workspace's run so far: H2 rebuilds inside the loop since iteration 0
  * (the initial build of tsne_init_with_P not counted), point renumberings, the Z of the

@@ -22,6 +22,9 @@ void h2_set_aux_stream(H2 *h, hipStream_t st_aux);
 // Mixed path: the ID's pivoted Gram-Schmidt runs in FP32 (when the ID tolerance, half of
 // h2_tol, is >= 1e-3); R entries, the solve and the coefficients stay FP64.
 void h2_set_id_mixed(H2 *h, int mixed);
+// HIP events around each build's ID launches (their time of the last build in
+// tsne_h2_stats.id_ms).
+void h2_set_timing(H2 *h, int on);
 
 // Rebuild from the float32 gather copy Yg (caller order).  Returns 0, or -1 on a
 // device error or a non-finite coordinate (reported in *nonfinite).

@@ -47,6 +47,10 @@ struct H2 {
     int hflags[FLAG_COUNT] = {0};
     int fallbacks = 0;  // builds that left the product to the direct sum, and why (first)
     int rank_capped = 0;  // builds in which some node's ID stopped at RMAX short of id_tol
+    // timing of the ID launches of each build (HIP events on the build stream)
+    bool timing = false;
+    hipEvent_t ev_id0 = nullptr, ev_id1 = nullptr;
+    double id_ms = 0.0;   // the last build's ID time
     int last_mixed = 0; // the last product was the mixed-precision one
     // growth factors (grown on overflow)
     double node_f = 8.0, pair_f = 1.0, pool_f = 1.0;
@@ -328,12 +332,15 @@ int build_once(H2 *h, const void *Yg)
     // internal nodes level by level, finest first
     const int nlev = (int)h->lvl.size() - 1;
     const bool f32 = h->id_mixed && h->id_tol >= kIdF32MinTol;
+    const bool tid = h->timing && h->ev_id0 && h->ev_id1;
+    if (tid && hipEventRecord(h->ev_id0, st) != hipSuccess) return -1;
     if (h->n_nodes > 1 && launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, h->pq, f32, st))
         return -1;
     for (int lv = nlev - 1; lv >= 1; lv--)
         if (launch_id_up(D, d, nullptr, h->lvl[lv], h->lvl[lv + 1] - h->lvl[lv], h->id_tol, h->pq,
                          f32, st))
             return -1;
+    if (tid && hipEventRecord(h->ev_id1, st) != hipSuccess) return -1;
     if (sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end, ss) ||
         sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end, ss))
         return -1;
@@ -345,6 +352,10 @@ int build_once(H2 *h, const void *Yg)
     if (h->hflags[FLAG_POOL_OVF]) {
         h->pool_f *= 2.0;
         return 1;
+    }
+    if (tid) {
+        float ms = 0.f;
+        if (hipEventElapsedTime(&ms, h->ev_id0, h->ev_id1) == hipSuccess) h->id_ms = ms;
     }
     return 0;
 }
@@ -403,6 +414,19 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
     return h;
 }
 
+void h2_set_timing(H2 *h, int on)
+{
+    if (!h) return;
+    if (on && !h->ev_id0) {
+        if (hipEventCreate(&h->ev_id0) != hipSuccess || hipEventCreate(&h->ev_id1) != hipSuccess) {
+            if (h->ev_id0) (void)hipEventDestroy(h->ev_id0);
+            h->ev_id0 = h->ev_id1 = nullptr;
+            return;
+        }
+    }
+    h->timing = on != 0;
+}
+
 void h2_set_id_mixed(H2 *h, int mixed)
 {
     if (h) h->id_mixed = mixed != 0;
@@ -428,6 +452,8 @@ void h2_destroy(H2 *h)
     if (h->ev_tree) (void)hipEventDestroy(h->ev_tree);
     if (h->ev_sorted) (void)hipEventDestroy(h->ev_sorted);
     if (h->st) (void)hipStreamSynchronize(h->st);
+    if (h->ev_id0) (void)hipEventDestroy(h->ev_id0);
+    if (h->ev_id1) (void)hipEventDestroy(h->ev_id1);
     free_sized(h);
     H2Dev &d = h->d;
     dfree(d.keys, h->st); dfree(h->keys_alt, h->st); dfree(d.perm, h->st); dfree(h->perm_alt, h->st); dfree(d.Ys, h->st);
@@ -547,7 +573,7 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     if (!h->usable || h->n_nodes <= 0) return 0;
     // flop counts of one product from the built structure (host copies; diagnostics only)
     const int nn = h->n_nodes;
-    std::vector<int> nb(nn), ne(nn), rk(nn), c0(nn), par(nn), fl(nn);
+    std::vector<int> nb(nn), ne(nn), rk(nn), c0(nn), par(nn), fl(nn), nch(nn), lev(nn);
     RootInfo root{};
     std::vector<uint64_t> adm(h->n_adm), xl(h->n_x), wl(h->n_w), nl(h->n_near);
     bool ok = hipMemcpyAsync(nb.data(), h->d.nb, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
@@ -556,6 +582,8 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
               hipMemcpyAsync(c0.data(), h->d.child0, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
               hipMemcpyAsync(par.data(), h->d.parent, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
               hipMemcpyAsync(fl.data(), h->d.nflag, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+              hipMemcpyAsync(nch.data(), h->d.nchild, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
+              hipMemcpyAsync(lev.data(), h->d.level, nn * sizeof(int), hipMemcpyDeviceToHost, h->st) == hipSuccess &&
               hipMemcpyAsync(&root, h->d.root, sizeof(RootInfo), hipMemcpyDeviceToHost, h->st) == hipSuccess;
     auto cp = [&](std::vector<uint64_t> &v, const uint64_t *src) {
         return v.empty() || hipMemcpyAsync(v.data(), src, v.size() * sizeof(uint64_t),
@@ -602,6 +630,30 @@ int h2_stats(const H2 *h, tsne_h2_stats_t *st)
     st->near_point_pairs = (long long)near_pairs;
     st->near_flops = near_pairs * per_entry;
     st->far_flops = far_entries * per_entry + basis * 2 * (D + 1);
+    // FP64 work of the rebuild's IDs: per node with m candidates (its points, at most the
+    // candidate cap, or its children's skeleton points), p proxies of its level and rank
+    // r, the kernel matrix (3D+3) m p and the pivoted Gram-Schmidt 4 m p r; a single-child
+    // node keeps its child's skeleton and costs nothing
+    {
+        const ProxyTable &pt = proxy_table(h->D, h->ptol, 1.0, pmax_of(h->D));
+        const int mcap = h->D == 2 ? H2Cfg<2>::MCAP : H2Cfg<3>::MCAP;
+        double idf = 0.0;
+        for (int i = 1; i < nn; i++) {
+            double m;
+            if (c0[i] < 0) {
+                m = std::min(ne[i] - nb[i], mcap);
+            } else {
+                if (nch[i] == 1) continue;
+                m = 0;
+                for (int q = 0; q < nch[i]; q++) m += rk[c0[i] + q];
+            }
+            const int kq = std::min(std::max(root.k0 - lev[i], 0), kProxyLevels - 1);
+            const double p = kq < (int)pt.cnt.size() ? std::min(pt.cnt[kq], h->pq) : h->pq;
+            idf += (3 * D + 3) * m * p + 4.0 * m * p * rk[i];
+        }
+        st->id_flops = idf;
+    }
+    st->id_ms = h->id_ms;
     return 0;
 }
 

@@ -51,6 +51,10 @@ int launch_attr_only(int D, int n, const int *row_ptr, const int *col_idx, const
 int launch_update(int D, int n, const float *A, const float *F, double *Y64, double *uY,
                   double *gains, void *Yg_next, const double *scal, double exag, double mom,
                   double lr, double *block_sums, int nblocks, hipStream_t st);
+// Diagnostics: the attractive pass's memory pattern; mode 0 P only, 1 P and the
+// neighbour gathers, 2 the full attractive sums (launch_attr_only).  A: float [n][D].
+int launch_attr_probe(int D, int n, int mode, const int *row_ptr, const int *col_idx,
+                      const float *pval, const void *Yg, float *A, int nblocks, hipStream_t st);
 int launch_repulsive_direct_f32(int D, int n, const void *Yg, float *s, float *F,
                                 hipStream_t st);
 int launch_z(int D, int n, const void *Yg, const float *s, const float *F, double *scal,

@@ -231,6 +231,98 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
 }
 
 // ---------------------------------------------------------------------------------
+// Probe of the attractive pass's memory pattern (diagnostics, tsne_attr_probe): the same
+// row mapping and chunked non-temporal P reads as attr_update_kernel, with the per-nonzero
+// arithmetic reduced to a sum.  GATHER = false: only P streams (sum of p); GATHER = true:
+// P plus the neighbour gathers (sum of p y_j).  One float per row and dimension is written
+// to A.  The fused kernel's time against these two measures how close it is to the
+// ceiling of its own access pattern, apart from the HBM peak.
+// ---------------------------------------------------------------------------------
+template <int D, bool GATHER>
+__global__ void __launch_bounds__(256)
+attr_probe_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
+                  const float *__restrict__ pval, const typename Vec<D>::T *__restrict__ Yg,
+                  int rows_per_block, float *__restrict__ A)
+{
+    using V = typename Vec<D>::T;
+    constexpr int LANES = 16, BATCH = 3;
+    const int sub = threadIdx.x & (LANES - 1);
+    const int grp = threadIdx.x / LANES;
+    const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    const int row0 = lblk * rows_per_block;
+    const int row1 = min(n, row0 + rows_per_block);
+    for (int i = row0 + grp; i < row1; i += (256 / LANES)) {
+        const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
+        float ax = 0.f, ay = 0.f, az = 0.f;
+        const int c1 = (e1 + 3) >> 2;
+        for (int cb = (e0 >> 2) + sub; cb < c1; cb += BATCH * LANES) {
+            int jj[BATCH][4];
+            float pp[BATCH][4];
+#pragma unroll
+            for (int t = 0; t < BATCH; t++) {
+                const int c = cb + t * LANES;
+                v4i cc = {i, i, i, i};
+                v4f pv = {0.f, 0.f, 0.f, 0.f};
+                if (c < c1) {
+                    cc = __builtin_nontemporal_load((const v4i *)col_idx + c);
+                    pv = __builtin_nontemporal_load((const v4f *)pval + c);
+                }
+                const int e = 4 * c;
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    const bool ok = e + u >= e0 && e + u < e1;
+                    jj[t][u] = ok ? cc[u] : i;
+                    pp[t][u] = ok ? pv[u] : 0.f;
+                }
+            }
+#pragma unroll
+            for (int t = 0; t < BATCH; t++)
+#pragma unroll
+                for (int u = 0; u < 4; u++) {
+                    if constexpr (GATHER) {
+                        const V y = Yg[jj[t][u]];
+                        ax = fmaf(pp[t][u], y.x, ax);
+                        ay = fmaf(pp[t][u], y.y, ay);
+                        if constexpr (D == 3) az = fmaf(pp[t][u], y.z, az);
+                    } else {
+                        ax += pp[t][u];
+                        ay += (float)jj[t][u];
+                    }
+                }
+        }
+        for (int off = LANES / 2; off > 0; off >>= 1) {
+            ax += __shfl_xor(ax, off, 64);
+            ay += __shfl_xor(ay, off, 64);
+            if constexpr (D == 3) az += __shfl_xor(az, off, 64);
+        }
+        if (sub < D) A[(size_t)i * D + sub] = (sub == 0) ? ax : (sub == 1) ? ay : az;
+    }
+}
+
+int launch_attr_probe(int D, int n, int mode, const int *row_ptr, const int *col_idx,
+                      const float *pval, const void *Yg, float *A, int nblocks, hipStream_t st)
+{
+    const int rpb = (n + nblocks - 1) / nblocks;
+    if (mode == 2) return launch_attr_only(D, n, row_ptr, col_idx, pval, Yg, A, nblocks, st);
+    if (D == 2) {
+        if (mode == 1)
+            hipLaunchKernelGGL((attr_probe_kernel<2, true>), dim3(nblocks), dim3(256), 0, st, n,
+                               row_ptr, col_idx, pval, (const float2 *)Yg, rpb, A);
+        else
+            hipLaunchKernelGGL((attr_probe_kernel<2, false>), dim3(nblocks), dim3(256), 0, st, n,
+                               row_ptr, col_idx, pval, (const float2 *)Yg, rpb, A);
+    } else {
+        if (mode == 1)
+            hipLaunchKernelGGL((attr_probe_kernel<3, true>), dim3(nblocks), dim3(256), 0, st, n,
+                               row_ptr, col_idx, pval, (const float4 *)Yg, rpb, A);
+        else
+            hipLaunchKernelGGL((attr_probe_kernel<3, false>), dim3(nblocks), dim3(256), 0, st, n,
+                               row_ptr, col_idx, pval, (const float4 *)Yg, rpb, A);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+// ---------------------------------------------------------------------------------
 // Attractive force + fused update over row tiles (points numbered along the tree).
 //
 // One workgroup per tile of `tile_rows` consecutive rows.  The gather records of the

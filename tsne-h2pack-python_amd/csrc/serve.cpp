@@ -16,6 +16,7 @@
 #include <chrono>
 #include <condition_variable>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -96,6 +97,12 @@ int run_job(Job *j)
     tsne_workspace_t *ws = tsne::init_with_P(j->n, 0, j->dim, &p, j->P, &taken);
     if (!taken) sparse_matrix_destroy(j->P);
     j->P = nullptr;
+    if (ws) {
+        // force-reusing iterations from captured graphs: fewer host launches per job,
+        // which the slots' host threads share (TSNE_GRAPH=0 keeps the eager launches)
+        const char *e = getenv("TSNE_GRAPH");
+        tsne_set_graph_mode(ws, !(e && e[0] == '0'));
+    }
     bool ok = ws != nullptr && tsne_run(ws) == 0;
     if (ok) {
         j->Y.resize((size_t)j->n * j->dim);

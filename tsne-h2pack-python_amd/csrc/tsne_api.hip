@@ -79,6 +79,18 @@ struct tsne_workspace {
     int tile_rows = 0;          // 0: tiles off
     int tile_lds = 0;           // LDS bytes per tile workgroup for the staged records
     int tile_nt = 256;          // threads per tile workgroup
+    // graph mode: iterations that reuse the repulsive force (attractive + update + centring,
+    // two kernels) are replayed from HIP graphs captured on first use, one per (gather
+    // buffer parity, exaggeration, momentum, learning rate); renumbering (which swaps P's
+    // buffers) and tile builds drop them
+    int graph_mode = 0;
+    struct GraphSlot {
+        int cur;
+        double exag, mom, lr;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphSlot> graphs;
+    int n_graph_launches = 0;
 };
 
 namespace {
@@ -147,8 +159,15 @@ void free_renumber_buffers(tsne_workspace *ws)
     if (ws->d_tmp) { (void)hipFreeAsync(ws->d_tmp, ws->st); ws->d_tmp = nullptr; }
 }
 
+void drop_graphs(tsne_workspace *ws)
+{
+    for (auto &g : ws->graphs) (void)hipGraphExecDestroy(g.exec);
+    ws->graphs.clear();
+}
+
 void free_device(tsne_workspace *ws)
 {
+    drop_graphs(ws);
     free_renumber_buffers(ws);
     tiles_free(ws->tiles, ws->st);
     dfree(ws->d_order, ws->st);
@@ -225,6 +244,7 @@ int rebuild(tsne_workspace *ws)
     }
     int bad = 0;
     h2_set_id_mixed(ws->h2, ws->precision == TSNE_H2_MIXED);
+    h2_set_timing(ws->h2, ws->timing == 1);
     if (h2_build(ws->h2, ws->d_Yg[ws->cur], &bad) != 0) {
         if (bad) fprintf(stderr, "tsne_h2pack: the embedding contains NaN or Inf\n");
         ws->h2_ok = 0;
@@ -267,6 +287,7 @@ int renumber(tsne_workspace *ws)
     cur.col = ws->d_col;
     cur.v32 = ws->d_val32;
     cur.v64 = ws->d_val64;
+    drop_graphs(ws);  // they hold P's current buffers
     if (renumber_csr(cur, ord, ws->P2, ws->rs, st) != 0) return -1;
     std::swap(ws->d_rowptr, ws->P2.rp);
     std::swap(ws->d_col, ws->P2.col);
@@ -315,6 +336,37 @@ double read_scalar(tsne_workspace *ws, int slot)
         hipStreamSynchronize(ws->st) != hipSuccess)
         return NAN;
     return v;
+}
+
+// One force-reusing iteration (attractive + update + centring) from a captured graph.
+int graph_step(tsne_workspace *ws, double exag, double mom, double lr)
+{
+    const int nxt = ws->cur ^ 1;
+    tsne_workspace::GraphSlot *slot = nullptr;
+    for (auto &g : ws->graphs)
+        if (g.cur == ws->cur && g.exag == exag && g.mom == mom && g.lr == lr) slot = &g;
+    if (!slot) {
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        if (hipStreamBeginCapture(ws->st, hipStreamCaptureModeThreadLocal) != hipSuccess)
+            return -1;
+        const int lrc = launch_attr_update(ws->dim, ws->n, ws->d_rowptr, ws->d_col, ws->d_val32,
+                                           ws->d_Yg[ws->cur], ws->d_F, ws->d_Y64, ws->d_uY,
+                                           ws->d_gains, ws->d_Yg[nxt], ws->d_scal, exag, mom, lr,
+                                           ws->d_block_sums, ws->nblocks_upd, ws->st);
+        const hipError_t ec = hipStreamEndCapture(ws->st, &graph);
+        if (lrc != 0 || ec != hipSuccess || !graph ||
+            hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            fprintf(stderr, "tsne_h2pack: graph capture of an iteration failed\n");
+            return -1;
+        }
+        (void)hipGraphDestroy(graph);
+        if (ws->graphs.size() >= 8) drop_graphs(ws);  // the schedule uses a handful
+        ws->graphs.push_back({ws->cur, exag, mom, lr, exec});
+        slot = &ws->graphs.back();
+    }
+    return hipGraphLaunch(slot->exec, ws->st) == hipSuccess ? 0 : -1;
 }
 
 // Columns of the per-iteration profile (seconds).
@@ -645,6 +697,10 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
     }
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
     {
+        const char *e = getenv("TSNE_GRAPH");  // 1 / 0: graph mode on / off (default off)
+        ws->graph_mode = e ? (e[0] == '1') : 0;
+    }
+    {
         // the attractive pass of rebuild iterations on a second stream: from 2^18 points
         // on (below, the two extra launches and the event waits cost more than the
         // overlap saves); TSNE_OVERLAP=0 / 1 turns it off / on for any size
@@ -848,6 +904,18 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
         hipEvent_t ke = kev.empty() ? nullptr : kev[2 * (size_t)(it - it_begin) + 1];
         if (split) TSNE_HIP_CHECK(hipStreamWaitEvent(ws->st, ws->ev_a, 0));
         if (timing) (void)hipEventRecord(ev[5], ws->st);
+        // graph replay of an iteration that reuses the force (no per-launch events, no
+        // KL, the per-row kernel)
+        const bool kl_now = (pp.verbose && it % 50 == 0) || it == n_iter - 1;
+        if (ws->graph_mode && !timing && kev.empty() && !split && !fresh && !kl_now &&
+            ws->tiles.ntiles == 0) {
+            if (graph_step(ws, exag, mom, lr) != 0) return -1;
+            ws->cur ^= 1;
+            ws->n_graph_launches++;
+            if (pp.verbose && pp.print_freq > 0 && (it % pp.print_freq == 0 || it == n_iter - 1))
+                printf("  Iteration %4d/%d: Z=%.3e\n", it + 1, n_iter, read_scalar(ws, SLOT_Z));
+            continue;
+        }
         const int arc =
             split ? launch_update(D, ws->n, ws->d_A, ws->d_F, ws->d_Y64, ws->d_uY, ws->d_gains,
                                   ws->d_Yg[nxt], ws->d_scal, exag, mom, lr, ws->d_block_sums,
@@ -1232,6 +1300,43 @@ int tsne_run_timings(tsne_workspace_t *ws, double *attr, double *reb, double *pr
     if (other) *other = ws->t_other;
     if (n_rebuild) *n_rebuild = ws->n_rebuild;
     return 0;
+}
+
+int tsne_set_graph_mode(tsne_workspace_t *ws, int on)
+{
+    if (!ws) return -1;
+    ws->graph_mode = on ? 1 : 0;
+    if (!on) drop_graphs(ws);
+    return 0;
+}
+
+int tsne_graph_launches(tsne_workspace_t *ws)
+{
+    return ws ? ws->n_graph_launches : -1;
+}
+
+int tsne_attr_probe(tsne_workspace_t *ws, int mode, int reps, double *avg_ms)
+{
+    if (!ws || mode < 0 || mode > 2 || reps < 1 || !avg_ms) return -1;
+    if (!ws->d_A && dalloc(&ws->d_A, (size_t)ws->n * ws->dim, ws->st) != 0) return -1;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    TSNE_HIP_CHECK(hipEventCreate(&e0));
+    TSNE_HIP_CHECK(hipEventCreate(&e1));
+    int rc = launch_attr_probe(ws->dim, ws->n, mode, ws->d_rowptr, ws->d_col, ws->d_val32,
+                               ws->d_Yg[ws->cur], ws->d_A, ws->nblocks_upd, ws->st);  // warm-up
+    (void)hipEventRecord(e0, ws->st);
+    for (int r = 0; r < reps && rc == 0; r++)
+        rc = launch_attr_probe(ws->dim, ws->n, mode, ws->d_rowptr, ws->d_col, ws->d_val32,
+                               ws->d_Yg[ws->cur], ws->d_A, ws->nblocks_upd, ws->st);
+    (void)hipEventRecord(e1, ws->st);
+    float ms = 0.f;
+    if (rc == 0 && hipEventSynchronize(e1) == hipSuccess && hipEventElapsedTime(&ms, e0, e1) == hipSuccess)
+        *avg_ms = ms / reps;
+    else
+        rc = -1;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
 }
 
 int tsne_run_state(tsne_workspace_t *ws, int *n_rebuild, int *n_renumber, double *Z, double *kl)

@@ -86,6 +86,7 @@ class H2Stats(ctypes.Structure):
         ('near_flops', ctypes.c_double), ('mixed_fallback_nodes', ctypes.c_int),
         ('direct_fallbacks', ctypes.c_int),
         ('rank_capped_builds', ctypes.c_int),
+        ('id_flops', ctypes.c_double), ('id_ms', ctypes.c_double),
     ]
 
 
@@ -480,6 +481,12 @@ def configure_signatures(lib):
     lib.tsne_h2_stats.restype = ctypes.c_int
     lib.tsne_run_timings.argtypes = [c_ws, dp, dp, dp, dp, ip]
     lib.tsne_run_timings.restype = ctypes.c_int
+    lib.tsne_set_graph_mode.argtypes = [c_ws, ctypes.c_int]
+    lib.tsne_set_graph_mode.restype = ctypes.c_int
+    lib.tsne_graph_launches.argtypes = [c_ws]
+    lib.tsne_graph_launches.restype = ctypes.c_int
+    lib.tsne_attr_probe.argtypes = [c_ws, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+    lib.tsne_attr_probe.restype = ctypes.c_int
     lib.tsne_run_state.argtypes = [c_ws, ip, ip, dp, dp]
     lib.tsne_run_state.restype = ctypes.c_int
     lib.tsne_run_range.argtypes = [c_ws, ctypes.c_int, ctypes.c_int]
