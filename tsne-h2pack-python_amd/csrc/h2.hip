@@ -2,7 +2,6 @@
 // sequence (root box, Morton sort, level-by-level tree and interaction lists, sorted
 // pair lists, per-level ID + upward pass) and the product (coupling, downward sweep,
 // leaves).  See h2_internal.h for the layout.
-#include <hipcub/hipcub.hpp>
 #include <rocprim/device/device_radix_sort.hpp>
 
 #include <algorithm>
@@ -28,6 +27,7 @@ struct H2 {
     // optional second stream: the NEAR / W pair sorts run there beside the ID
     hipStream_t st_aux = nullptr;
     hipEvent_t ev_tree = nullptr, ev_sorted = nullptr;
+    hipEvent_t ev_lvl[2] = {nullptr, nullptr};  // ends of the last two queued level batches
     H2Dev d{};
     // buffers not in H2Dev
     uint64_t *keys_alt = nullptr;
@@ -58,9 +58,10 @@ struct H2 {
 
 namespace {
 
-constexpr int kLevelBatch = 4;   // tree levels built between two host checks
+constexpr int kLevelBatch = 4;   // tree levels queued between two host checks
+constexpr int kPinRec = 4;       // pinned words per published level
 constexpr double kIdF32MinTol = 1e-3;  // smallest ID tolerance of the mixed path's FP32 ID
-constexpr int kMaxLevels = 32;   // > H2Cfg<D>::B + 1 for both dimensions
+constexpr int kMaxLevels = 32;   // >= H2Cfg<D>::B + 1 for both dimensions
 
 // Radix sorts of the Morton keys and pair lists.  From kOnesweepMin items on, always the
 // onesweep algorithm: the library default sorts up to 2^20 items with a block sort plus
@@ -165,7 +166,7 @@ bool alloc_sized(H2 *h)
               dmalloc(&d.parent, nc, h->st) && dmalloc(&d.child0, nc, h->st) && dmalloc(&d.nchild, nc, h->st) &&
               dmalloc(&d.cent, nc, h->st) && dmalloc(&d.icoord, nc * 3, h->st) &&
               dmalloc(&d.bounds, nc * (NCH + 1), h->st) && dmalloc(&d.childcnt, nc, h->st) &&
-              dmalloc(&d.childoff, nc, h->st) && dmalloc(&d.near, nc * nearcap_of(D), h->st) &&
+              dmalloc(&d.childoff, std::max(nc, (size_t)kSplitBlocks), h->st) && dmalloc(&d.near, nc * nearcap_of(D), h->st) &&
               dmalloc(&d.near_cnt, nc, h->st) && dmalloc(&d.leaves, nc, h->st) &&
               // the four lists and the sort scratch trade buffers after every sort (the
               // sorted keys stay where the sort left them), so all five have the same size
@@ -181,8 +182,8 @@ bool alloc_sized(H2 *h)
               dmalloc(&d.wc, nc * RMAX, h->st) && dmalloc(&d.zc, nc * RMAX, h->st) &&
               dmalloc(&d.nflag, nc, h->st);
     if (!ok) return false;
-    // temporary storage for the library sorts / scans (largest of the three uses)
-    size_t b1 = 0, b2 = 0, b3 = 0;
+    // temporary storage for the sorts (largest of the uses)
+    size_t b1 = 0, b3 = 0;
     rocprim::double_buffer<uint64_t> kb(d.keys, h->keys_alt);
     rocprim::double_buffer<int> vb(d.perm, h->perm_alt);
     // temporary storage: the larger of the two algorithms at the sizes in use
@@ -190,12 +191,11 @@ bool alloc_sized(H2 *h)
     (void)rocprim::radix_sort_pairs<SortCfg>(nullptr, b1, kb, vb, h->n, 0, 64, h->st);
     (void)rocprim::radix_sort_pairs(nullptr, b1a, kb, vb, h->n, 0, 64, h->st);
     b1 = std::max(b1, b1a);
-    (void)hipcub::DeviceScan::ExclusiveSum(nullptr, b2, d.childcnt, d.childoff, (int)nc, h->st);
     rocprim::double_buffer<uint64_t> pb(d.adm, d.sort_tmp);
     (void)rocprim::radix_sort_keys<SortCfg>(nullptr, b3, pb, maxpair, 0, 64, h->st);
     (void)rocprim::radix_sort_keys(nullptr, b3a, pb, maxpair, 0, 64, h->st);
     b3 = std::max(b3, b3a);
-    h->cub_bytes = std::max({b1, b2, b3, radix_scratch_bytes(std::max(h->n, maxpair))});
+    h->cub_bytes = std::max({b1, b3, radix_scratch_bytes(std::max(h->n, maxpair))});
     return hipMallocAsync(&h->cub_tmp, h->cub_bytes, h->st) == hipSuccess;
 }
 
@@ -253,41 +253,43 @@ int build_once(H2 *h, const void *Yg)
     }
     if (launch_gather_sorted(D, Yg, d, st)) return -1;
     if (launch_init_root(d, st)) return -1;
-    // Levels are built kLevelBatch at a time without waiting for the host: each level's
-    // kernels take its node range from d.lvl (written by the previous level's
-    // publish_level) and are sized by an upper bound on its node count (NCH times the
-    // previous bound; exact after each check).  Levels past the last one are empty and
-    // cost only their launches.
+    // Levels are queued in batches of kLevelBatch; the host never waits for the level it
+    // has just queued.  After queuing batch k it waits for batch k - 1 (recorded in
+    // ev_lvl) and reads that batch's last published level from pinned memory (one
+    // record of kPinRec words per level, so later levels cannot overwrite it): whether
+    // the tree has ended, and the exact node count that bounds the next grids.  Levels
+    // past the last one are empty and cost only their launches.
     const int NCH = (D == 2) ? H2Cfg<2>::NCH : H2Cfg<3>::NCH;
     const int maxlev = (D == 2) ? H2Cfg<2>::B + 1 : H2Cfg<3>::B + 1;  // level B holds leaves only
-    int l = 0, pend_in = 0, bound = 1;
-    for (;;) {
-        const int pend_out = pend_in ^ 1;
-        if (bound <= kSmallSplit) {
-            if (launch_split_small(D, d, l, pend_out, h->pin_dev, st)) return -1;
-        } else {
-            if (launch_split_count(D, d, l, bound, st)) return -1;
-            size_t bytes = h->cub_bytes;
-            if (hipcub::DeviceScan::ExclusiveSum(h->cub_tmp, bytes, d.childcnt, d.childoff, bound,
-                                                 st) != hipSuccess)
+    int *plev = h->pin + CNT_COUNT, *plev_dev = h->pin_dev + CNT_COUNT;
+    for (int k = 0; k < maxlev * kPinRec; k++) plev[k] = -1;
+    int l = 0, pend = 0;            // the level pass of level l appends to pend[pend]
+    int known_l = 0, known_n = 1;   // level known_l has exactly known_n nodes
+    for (int batch = 0; l < maxlev; batch++) {
+        for (int k = 0; k < kLevelBatch && l < maxlev; k++, l++) {
+            long long bound = known_n;
+            for (int q = known_l; q < l && bound < d.node_cap; q++) bound *= NCH;
+            bound = std::min<long long>(bound, d.node_cap);
+            int nsplit = 1;
+            if (launch_level_pass(D, d, l, (int)bound, pend ^ 1, pend, &nsplit, st) ||
+                launch_level_write(D, d, l, nsplit, pend ^ 1, plev_dev + kPinRec * l, st))
                 return -1;
-            if (launch_split_write(D, d, l, bound, pend_out, h->pin_dev, st)) return -1;
+            pend ^= 1;
         }
-        const int nbound = (int)std::min<long long>(d.node_cap, (long long)NCH * bound);
-        if (launch_lists_pending(D, d, l + 1, nbound, pend_in, pend_out, st)) return -1;
-        pend_in = pend_out;
-        l++;
-        bound = std::max(nbound, 1);
-        if (l % kLevelBatch == 0 || l >= maxlev) {
+        if (hipEventRecord(h->ev_lvl[batch & 1], st) != hipSuccess) return -1;
+        if (batch == 0) continue;
+        if (hipEventSynchronize(h->ev_lvl[(batch - 1) & 1]) != hipSuccess) return -1;
+        const int lc = std::min(l, kLevelBatch * batch) - 1;  // last level of batch - 1
+        const int *rec = plev + kPinRec * lc;
+        if (rec[1] || rec[2]) {  // non-finite input / node arrays full: drain, then report
             if (hipStreamSynchronize(st) != hipSuccess) return -1;
-            if (h->pin[1]) return -2;  // non-finite coordinate
-            if (h->pin[2]) {           // node arrays full
-                h->node_f *= 2.0;
-                return 1;
-            }
-            if (h->pin[0] == 0 || l >= maxlev) break;  // level l is empty
-            bound = h->pin[0];                         // level l's exact node count
+            if (rec[1]) return -2;
+            h->node_f *= 2.0;
+            return 1;
         }
+        known_l = lc + 1;
+        known_n = rec[0];
+        if (known_n == 0) break;  // level lc + 1 is empty: the tree has ended
     }
     {
         // level offsets, pair counters and flags of the finished tree: one wait
@@ -302,6 +304,7 @@ int build_once(H2 *h, const void *Yg)
         for (int k = 1; k + 1 < (int)lv.size() && lv[k + 1] > lv[k]; k++) h->lvl.push_back(lv[k + 1]);
     }
     h->n_nodes = h->lvl.back();
+    if (h->hflags[FLAG_NONFINITE]) return -2;
     if (h->hflags[FLAG_NODE_OVF]) {
         h->node_f *= 2.0;
         return 1;
@@ -388,9 +391,11 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
               dmalloc(&d.root, 1, h->st) && dmalloc(&d.flags, FLAG_COUNT, h->st) && dmalloc(&d.cnt, CNT_COUNT, h->st) &&
               dmalloc(&d.lvl, kMaxLevels + 4, h->st) &&
               dmalloc(&h->part, (size_t)h->nblk * 7, h->st) &&
-              hipHostMalloc((void **)&h->pin, sizeof(int) * CNT_COUNT,
+              hipHostMalloc((void **)&h->pin, sizeof(int) * (CNT_COUNT + kPinRec * kMaxLevels),
                             hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
-              hipHostGetDevicePointer((void **)&h->pin_dev, h->pin, 0) == hipSuccess;
+              hipHostGetDevicePointer((void **)&h->pin_dev, h->pin, 0) == hipSuccess &&
+              hipEventCreateWithFlags(&h->ev_lvl[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&h->ev_lvl[1], hipEventDisableTiming) == hipSuccess;
     if (ok) {
         const ProxyTable &pt = proxy_table(dim, h->ptol, 1.0, pmax_of(dim));
         int cmax = 0;
@@ -419,7 +424,9 @@ void h2_set_timing(H2 *h, int on)
     if (!h) return;
     if (on && !h->ev_id0) {
         if (hipEventCreate(&h->ev_id0) != hipSuccess || hipEventCreate(&h->ev_id1) != hipSuccess) {
-            if (h->ev_id0) (void)hipEventDestroy(h->ev_id0);
+            for (hipEvent_t e : h->ev_lvl)
+        if (e) (void)hipEventDestroy(e);
+    if (h->ev_id0) (void)hipEventDestroy(h->ev_id0);
             h->ev_id0 = h->ev_id1 = nullptr;
             return;
         }

@@ -93,7 +93,7 @@ struct H2Dev {
     float4 *cent;      // centre (xyz) of each node's box, w = half-side
     int *icoord;       // [node][3] integer box coordinates at the node's level
     int *bounds;       // split scratch [node][NCH+1]
-    int *childcnt, *childoff;
+    int *childcnt, *childoff;  // per-node child counts; per-chunk totals of a level
     int *near, *near_cnt;  // [node][NEAR_CAP]
     int *leaves;       // leaf node ids (CNT_LEAF of them)
     // pair buffers (target << 32 | source)
@@ -127,21 +127,22 @@ int launch_root_box(int D, const void *Yg, int n, const H2Dev &h, double *partia
 int launch_morton(int D, const void *Yg, const H2Dev &h, hipStream_t st);
 int launch_gather_sorted(int D, const void *Yg, const H2Dev &h, hipStream_t st);
 int launch_init_root(const H2Dev &h, hipStream_t st);
-// Level kernels take the level's node range from h.lvl on the device; `bound` (an upper
-// bound on the level's node count, known to the host) sizes the grid.
-// A level of at most kSmallSplit nodes: split, scan, publication and children in one
-// single-workgroup launch.  Larger levels: split_count, a library scan, split_write (which
-// also publishes the level).  Publication writes lvl[l+2], zeroes the pending counter of
-// the next pass, and sets the pinned host words out[0] (children of level l), out[1]
+// Tree levels, two launches each, with no host round trip in between: the kernels take
+// the level's node range from h.lvl on the device, and `bound` (an upper bound on the
+// level's node count known to the host) only sizes the grid; the loops inside cover any
+// count.  The level pass splits the level's nodes (child counts and bounds, per-chunk
+// totals in childoff[0..nsplit)), builds their interaction lists and descends the pending
+// pairs of pend_in; the level write scans the chunk totals, writes the children and
+// publishes the level: lvl[l+2], the reset of the pending counter `pend_next` the next
+// level pass appends to, and the pinned words out[0] (children of level l), out[1]
 // (non-finite input) and out[2] (node overflow).
-constexpr int kSmallSplit = 256;
-int launch_split_small(int D, const H2Dev &h, int l, int pend_out, int *out, hipStream_t st);
-int launch_split_count(int D, const H2Dev &h, int l, int bound, hipStream_t st);
-int launch_split_write(int D, const H2Dev &h, int l, int bound, int pend_out, int *out,
+constexpr int kSplitBlocks = 256;  // split workgroups per level at most (childoff entries)
+constexpr int kListBlocks = 1024;  // list workgroups per level at most
+constexpr int kPendBlocks = 256;   // pending-pair workgroups per level
+int launch_level_pass(int D, const H2Dev &h, int l, int bound, int pend_in, int pend_out,
+                      int *nsplit, hipStream_t st);
+int launch_level_write(int D, const H2Dev &h, int l, int nsplit, int pend_next, int *out,
                        hipStream_t st);
-// Interaction lists of level l and the pending pairs of pend_in, one launch.
-int launch_lists_pending(int D, const H2Dev &h, int l, int bound, int pend_in, int pend_out,
-                         hipStream_t st);
 int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, hipStream_t st);
 
 // h2_id.hip

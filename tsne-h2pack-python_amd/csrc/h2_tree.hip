@@ -281,8 +281,8 @@ int launch_init_root(const H2Dev &h, hipStream_t st)
 // ballot).  Slots of [level size, nbound) get a zero child count (the library scan runs
 // over nbound counts).
 template <int D>
-__device__ __forceinline__ void split_count_slot(const H2Dev &h, int l, int lb, int le, int t,
-                                                 int nbound)
+__device__ __forceinline__ int split_count_slot(const H2Dev &h, int l, int lb, int le, int t,
+                                                int nbound)
 {
     constexpr int B = H2Cfg<D>::B, NCH = H2Cfg<D>::NCH;
     static_assert((64 % NCH) == 0, "lanes per node");
@@ -318,14 +318,14 @@ __device__ __forceinline__ void split_count_slot(const H2Dev &h, int l, int lb, 
     int lbase = 0;
     if ((threadIdx.x & 63) == 0 && ml) lbase = atomicAdd(&h.cnt[CNT_LEAF], __popcll(ml));
     lbase = __shfl(lbase, 0, 64);
-    if (!valid || v != 0) return;
+    if (!valid || v != 0) return 0;
     if (leaf) {
         h.childcnt[i - lb] = 0;
         h.child0[i] = -1;
         h.nchild[i] = 0;
         h.leaves[lbase + __popcll(ml & ((1ull << (threadIdx.x & 63)) - 1ull))] = i;
         if (e - b > H2Cfg<D>::LEAF_CAP) atomicOr(&h.flags[FLAG_BIG_LEAF], 1);
-        return;
+        return 0;
     }
     int *bdp = h.bounds + (size_t)(i - lb) * (NCH + 1);
     int c = 0;
@@ -334,14 +334,7 @@ __device__ __forceinline__ void split_count_slot(const H2Dev &h, int l, int lb, 
 #pragma unroll
     for (int q = 0; q < NCH; q++) c += bd[q + 1] > bd[q];
     h.childcnt[i - lb] = c;
-}
-
-template <int D>
-__global__ void __launch_bounds__(256)
-split_count_kernel(H2Dev h, int l, int nbound)
-{
-    if (h.flags[FLAG_NODE_OVF]) return;  // the build is redone with larger arrays
-    split_count_slot<D>(h, l, h.lvl[l], h.lvl[l + 1], blockIdx.x * 256 + threadIdx.x, nbound);
+    return c;
 }
 
 // After the scan of level l's child counts: the range of level l + 1 (T children after
@@ -395,101 +388,6 @@ __device__ __forceinline__ void split_write_node(const H2Dev &h, int l, int lb, 
     h.nchild[i] = k;
 }
 
-// Children of level l from the scanned counts; workgroup 0 also publishes the level.
-template <int D>
-__global__ void __launch_bounds__(256)
-split_write_kernel(H2Dev h, int l, int pend_out, int *out)
-{
-    const int lb = h.lvl[l], le = h.lvl[l + 1];
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        const int cnt = le - lb;
-        const int T = (cnt > 0 && !h.flags[FLAG_NODE_OVF]) ? h.childoff[cnt - 1] + h.childcnt[cnt - 1] : 0;
-        publish_level(h, l, le, T, pend_out, out);
-    }
-    if (h.flags[FLAG_NODE_OVF]) return;  // children past node_cap are skipped either way
-    const int i = lb + blockIdx.x * 256 + threadIdx.x;
-    if (i >= le || h.childcnt[i - lb] == 0) return;
-    split_write_node<D>(h, l, lb, i, le + h.childoff[i - lb]);
-}
-
-// A level of at most kSmallSplit nodes (the top of the tree, and the deep levels inside
-// dense clusters) in one workgroup: counts, scan, publication and children in one launch
-// instead of four, the phases separated by barriers.
-template <int D>
-__global__ void __launch_bounds__(1024)
-split_small_kernel(H2Dev h, int l, int pend_out, int *out)
-{
-    constexpr int NCH = H2Cfg<D>::NCH;
-    __shared__ int s_off[kSmallSplit + 1];
-    __shared__ int s_wsum[16];
-    const int lb = h.lvl[l], le = h.lvl[l + 1];
-    const int cnt = le - lb;  // at most kSmallSplit (the host's bound on the level)
-    if (cnt > kSmallSplit) {  // cannot happen (the host launches this for such levels
-        if (threadIdx.x == 0) {  // only); fail the build rather than index past s_off
-            atomicOr(&h.flags[FLAG_NODE_OVF], 1);
-            publish_level(h, l, le, 0, pend_out, out);
-        }
-        return;
-    }
-    const bool ovf = h.flags[FLAG_NODE_OVF];
-    if (!ovf)
-        for (int base = 0; base < cnt * NCH; base += 1024)  // same trip count for every wave
-            split_count_slot<D>(h, l, lb, le, base + (int)threadIdx.x, 0);
-    __syncthreads();  // child counts and bounds of every node written
-    const int t = threadIdx.x, lane = t & 63, w = t >> 6;
-    const int c = (!ovf && t < cnt) ? h.childcnt[t] : 0;
-    int x = c;  // inclusive scan over the wavefront, then over the wavefront totals
-    for (int off = 1; off < 64; off <<= 1) {
-        const int y = __shfl_up(x, off, 64);
-        if (lane >= off) x += y;
-    }
-    if (lane == 63) s_wsum[w] = x;
-    __syncthreads();
-    int pre = 0;
-    for (int k = 0; k < w; k++) pre += s_wsum[k];
-    if (t < kSmallSplit) s_off[t] = pre + x - c;
-    if (t == 0) {
-        int T = 0;
-        for (int k = 0; k < 16; k++) T += s_wsum[k];
-        s_off[kSmallSplit] = T;
-    }
-    __syncthreads();
-    if (t == 0) publish_level(h, l, le, ovf ? 0 : s_off[kSmallSplit], pend_out, out);
-    if (ovf) return;
-    if (t < cnt && c > 0) split_write_node<D>(h, l, lb, lb + t, le + s_off[t]);
-}
-
-int launch_split_small(int D, const H2Dev &h, int l, int pend_out, int *out, hipStream_t st)
-{
-    if (D == 2)
-        hipLaunchKernelGGL(split_small_kernel<2>, dim3(1), dim3(1024), 0, st, h, l, pend_out, out);
-    else
-        hipLaunchKernelGGL(split_small_kernel<3>, dim3(1), dim3(1024), 0, st, h, l, pend_out, out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_split_count(int D, const H2Dev &h, int l, int bound, hipStream_t st)
-{
-    const int g = std::max(1, (bound * (D == 2 ? H2Cfg<2>::NCH : H2Cfg<3>::NCH) + 255) / 256);
-    if (D == 2)
-        hipLaunchKernelGGL(split_count_kernel<2>, dim3(g), dim3(256), 0, st, h, l, bound);
-    else
-        hipLaunchKernelGGL(split_count_kernel<3>, dim3(g), dim3(256), 0, st, h, l, bound);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-int launch_split_write(int D, const H2Dev &h, int l, int bound, int pend_out, int *out,
-                       hipStream_t st)
-{
-    const int g = std::max(1, (bound + 255) / 256);
-    if (D == 2)
-        hipLaunchKernelGGL(split_write_kernel<2>, dim3(g), dim3(256), 0, st, h, l, pend_out, out);
-    else
-        hipLaunchKernelGGL(split_write_kernel<3>, dim3(g), dim3(256), 0, st, h, l, pend_out, out);
-    return hipGetLastError() == hipSuccess ? 0 : -1;
-}
-
-constexpr int kPendBlocks = 256;  // workgroups of the pending-pair pass
 
 // ---------------------------------------------------------------------------------
 // Interaction lists of the nodes of level l (>= 1).  Candidates are the children of
@@ -683,38 +581,135 @@ __device__ __forceinline__ void pending_pair(const H2Dev &h, int pend_in, int pe
     }
 }
 
-// The lists of level l (the first nlists workgroups, four nodes each) and the pending
-// pairs of pend_in (the other kPendBlocks workgroups, one pair per thread, walking the live
-// pairs with a stride) in one launch: the two read nothing the other writes, and both only
-// append to the shared pair buffers.
+// Workgroup-wide sum over 256 threads (every thread gets it); s_red: 4 ints of LDS.
+__device__ __forceinline__ int block_sum256(int x, int *s_red)
+{
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    __syncthreads();  // s_red free (a previous call may still be reading it)
+    if ((threadIdx.x & 63) == 0) s_red[threadIdx.x >> 6] = x;
+    __syncthreads();
+    return s_red[0] + s_red[1] + s_red[2] + s_red[3];
+}
+
+// The nodes of level l split into `nsplit` contiguous chunks, one per split workgroup of
+// the level pass and of the level write: chunk b is [b C, min(cnt, (b + 1) C)).
+__device__ __forceinline__ void split_chunk(int cnt, int nsplit, int b, int &c0, int &c1)
+{
+    const int C = (cnt + nsplit - 1) / nsplit;
+    c0 = min(cnt, b * C);
+    c1 = min(cnt, c0 + C);
+}
+
+// Level pass of level l, one launch.  Workgroups [0, nsplit): the child counts and child
+// bounds of a chunk of the level's nodes, and the chunk's total (childoff[b]).
+// Workgroups [nsplit, nsplit + nlists): the interaction lists of the level's nodes, four
+// per workgroup round (l >= 1; the root's lists come from init_root).  The last
+// kPendBlocks workgroups: the pending pairs of pend_in.  The three parts read nothing
+// the others write: the lists read the children of the parents' near-list members and
+// the pending pairs the children of coarser nodes, all made by earlier levels' writes.
 template <int D>
 __global__ void __launch_bounds__(256)
-lists_pending_kernel(H2Dev h, int l, int pend_in, int pend_out, int nlists)
+level_pass_kernel(H2Dev h, int l, int nsplit, int nlists, int pend_in, int pend_out)
 {
-    if (h.flags[FLAG_NODE_OVF]) return;  // uniform over the grid
-    if ((int)blockIdx.x < nlists) {
-        lists_block<D>(h, l, pend_out, blockIdx.x);
+    constexpr int NCH = H2Cfg<D>::NCH;
+    __shared__ int s_red[4];
+    if (h.flags[FLAG_NODE_OVF]) return;  // uniform over the grid; the build is redone
+    const int lb = h.lvl[l], le = h.lvl[l + 1];
+    const int cnt = le - lb;
+    const int b = blockIdx.x;
+    if (b < nsplit) {
+        int c0, c1;
+        split_chunk(cnt, nsplit, b, c0, c1);
+        int mine = 0;
+        // slots NCH per node, 256 per round; the trip count is uniform over the workgroup
+        for (int t = c0 * NCH; t < c1 * NCH; t += 256)
+            mine += split_count_slot<D>(h, l, lb, lb + c1, t + (int)threadIdx.x, 0);
+        const int tot = block_sum256(mine, s_red);
+        if (threadIdx.x == 0) h.childoff[b] = tot;
+    } else if (b < nsplit + nlists) {
+        if (l == 0) return;
+        for (int blk = b - nsplit; blk * 4 < cnt; blk += nlists) {
+            lists_block<D>(h, l, pend_out, blk);
+            __syncthreads();  // the next round reuses the workgroup's LDS
+        }
     } else {
         const int n_pend = min(h.cnt[CNT_PEND0 + pend_in * kCntSlot], h.pend_cap);
-        // the loop bound is uniform over the workgroup: every lane takes part in each round
-        for (int base = (blockIdx.x - nlists) * 256; base < n_pend; base += kPendBlocks * 256)
+        for (int base = (b - nsplit - nlists) * 256; base < n_pend; base += kPendBlocks * 256)
             pending_pair<D>(h, pend_in, pend_out, base + threadIdx.x, n_pend);
     }
 }
 
-// Grid: the lists of the (at most `bound`) nodes of level l, then kPendBlocks workgroups
-// for the pending pairs (their live count is on the device).
-int launch_lists_pending(int D, const H2Dev &h, int l, int bound, int pend_in, int pend_out,
-                         hipStream_t st)
+// Level write of level l: workgroup b takes its chunk's first child slot from the chunk
+// totals before it (childoff[0..b)), scans its nodes' child counts and writes their
+// children; workgroup 0 publishes the level (level l + 1's end, the reset of the pending
+// counter the next level pass appends to, and the pinned words out[0..2]).
+template <int D>
+__global__ void __launch_bounds__(256)
+level_write_kernel(H2Dev h, int l, int nsplit, int pend_next, int *out)
 {
-    const int gl = std::max(1, (bound + 3) / 4);
-    const int g = gl + kPendBlocks;
+    __shared__ int s_red[4], s_wsum[4];
+    const int lb = h.lvl[l], le = h.lvl[l + 1];
+    const int cnt = le - lb;
+    const int b = blockIdx.x, t = threadIdx.x, lane = t & 63, w = t >> 6;
+    const bool ovf = h.flags[FLAG_NODE_OVF];
+    // chunk totals: the prefix of this chunk and the level's total
+    int pre = 0, tot = 0;
+    for (int k = t; k < nsplit; k += 256) {
+        const int x = ovf ? 0 : h.childoff[k];
+        tot += x;
+        if (k < b) pre += x;
+    }
+    pre = block_sum256(pre, s_red);
+    tot = block_sum256(tot, s_red);
+    if (b == 0 && t == 0) publish_level(h, l, le, ovf ? 0 : tot, pend_next, out);
+    if (ovf) return;
+    int c0, c1;
+    split_chunk(cnt, nsplit, b, c0, c1);
+    int run = le + pre;  // first child slot of the round
+    for (int r0 = c0; r0 < c1; r0 += 256) {  // uniform trip count
+        const int k = r0 + t;
+        const int c = (k < c1) ? h.childcnt[k] : 0;
+        int x = c;  // inclusive scan over the wavefront, then over the four wavefronts
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off, 64);
+            if (lane >= off) x += y;
+        }
+        __syncthreads();
+        if (lane == 63) s_wsum[w] = x;
+        __syncthreads();
+        int wp = 0;
+        for (int q = 0; q < w; q++) wp += s_wsum[q];
+        if (c > 0) split_write_node<D>(h, l, lb, lb + k, run + wp + x - c);
+        run += s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+    }
+}
+
+int launch_level_pass(int D, const H2Dev &h, int l, int bound, int pend_in, int pend_out,
+                      int *nsplit_out, hipStream_t st)
+{
+    const int NCH = (D == 2) ? H2Cfg<2>::NCH : H2Cfg<3>::NCH;
+    const int nsplit = std::max(1, std::min(kSplitBlocks, (int)(((long long)bound * NCH + 255) / 256)));
+    const int nlists = (l == 0) ? 0 : std::max(1, std::min(kListBlocks, (bound + 3) / 4));
+    const int g = nsplit + nlists + ((l == 0) ? 0 : kPendBlocks);
     if (D == 2)
-        hipLaunchKernelGGL(lists_pending_kernel<2>, dim3(g), dim3(256), 0, st, h, l, pend_in,
-                           pend_out, gl);
+        hipLaunchKernelGGL(level_pass_kernel<2>, dim3(g), dim3(256), 0, st, h, l, nsplit, nlists,
+                           pend_in, pend_out);
     else
-        hipLaunchKernelGGL(lists_pending_kernel<3>, dim3(g), dim3(256), 0, st, h, l, pend_in,
-                           pend_out, gl);
+        hipLaunchKernelGGL(level_pass_kernel<3>, dim3(g), dim3(256), 0, st, h, l, nsplit, nlists,
+                           pend_in, pend_out);
+    *nsplit_out = nsplit;
+    return hipGetLastError() == hipSuccess ? 0 : -1;
+}
+
+int launch_level_write(int D, const H2Dev &h, int l, int nsplit, int pend_next, int *out,
+                       hipStream_t st)
+{
+    if (D == 2)
+        hipLaunchKernelGGL(level_write_kernel<2>, dim3(nsplit), dim3(256), 0, st, h, l, nsplit,
+                           pend_next, out);
+    else
+        hipLaunchKernelGGL(level_write_kernel<3>, dim3(nsplit), dim3(256), 0, st, h, l, nsplit,
+                           pend_next, out);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
