@@ -26,6 +26,12 @@ void h2_set_id_mixed(H2 *h, int mixed);
 // tsne_h2_stats.id_ms).
 void h2_set_timing(H2 *h, int on);
 
+// One-shot hook for the next h2_build: called (on the build's host thread) once the
+// build's front -- root box, Morton keys, sort, sorted gather copy -- is queued on the
+// build stream, so that work the caller queues on another stream from the hook overlaps
+// the tree, ID and sorts rather than the front; a build that fails before its front
+// still calls it before returning.  A non-zero return fails the build.
+void h2_set_front_hook(H2 *h, int (*fn)(void *), void *arg);
 // Rebuild from the float32 gather copy Yg (caller order).  Returns 0, or -1 on a
 // device error or a non-finite coordinate (reported in *nonfinite).
 int h2_build(H2 *h, const void *Yg, int *nonfinite);

@@ -19,6 +19,10 @@
 
 namespace tsne {
 
+// timeline marks: build start, front queued, tree queued, ID starts (after the host's
+// readback), ID done, sorts done (second stream), build done
+enum { TL_START = 0, TL_FRONT, TL_TREE, TL_ID0, TL_ID1, TL_SORTED, TL_END, kTlEvents };
+
 struct H2 {
     int n = 0, D = 2, leaf = 100;
     double tol = 1e-2, ptol = 3e-3, id_tol = 5e-3;
@@ -28,6 +32,14 @@ struct H2 {
     hipStream_t st_aux = nullptr;
     hipEvent_t ev_tree = nullptr, ev_sorted = nullptr;
     hipEvent_t ev_lvl[2] = {nullptr, nullptr};  // ends of the last two queued level batches
+    // diagnostics (TSNE_H2_TIMELINE=1): HIP events at the build's phase boundaries, summed
+    // over builds and printed by h2_destroy
+    bool tl_on = false;
+    hipEvent_t tl_ev[kTlEvents] = {};
+    double tl_ms[kTlEvents] = {};
+    int tl_builds = 0;
+    int (*front_hook)(void *) = nullptr;  // one-shot: queued work after the next build's front
+    void *front_arg = nullptr;
     H2Dev d{};
     // buffers not in H2Dev
     uint64_t *keys_alt = nullptr;
@@ -225,6 +237,16 @@ int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end, hipStream_
     return launch_pair_offsets(buf, n, beg, end, s);
 }
 
+// The caller's one-shot hook, queued once the build's front (root box, keys, sort, sorted
+// gather copy) is on the stream.
+int run_front_hook(H2 *h)
+{
+    int (*fn)(void *) = h->front_hook;
+    if (!fn) return 0;
+    h->front_hook = nullptr;
+    return fn(h->front_arg);
+}
+
 // One attempt at a rebuild.  Returns 0 (built), 1 (capacity exceeded: grown, retry),
 // -1 (error), -2 (non-finite input).
 int build_once(H2 *h, const void *Yg)
@@ -235,6 +257,7 @@ int build_once(H2 *h, const void *Yg)
     if (hipMemsetAsync(d.cnt, 0, sizeof(int) * CNT_COUNT, st) != hipSuccess ||
         hipMemsetAsync(d.flags, 0, sizeof(int) * FLAG_COUNT, st) != hipSuccess)
         return -1;
+    if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_START], st);
     if (launch_root_box(D, Yg, h->n, d, h->part, h->nblk, st)) return -1;
     if (launch_morton(D, Yg, d, st)) return -1;
     const int endbit = D * (D == 2 ? H2Cfg<2>::B : H2Cfg<3>::B);
@@ -252,6 +275,8 @@ int build_once(H2 *h, const void *Yg)
         if (vb.current() != d.perm) std::swap(d.perm, h->perm_alt);
     }
     if (launch_gather_sorted(D, Yg, d, st)) return -1;
+    if (run_front_hook(h)) return -1;
+    if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_FRONT], st);
     if (launch_init_root(d, st)) return -1;
     // Levels are queued in batches of kLevelBatch; the host never waits for the level it
     // has just queued.  After queuing batch k it waits for batch k - 1 (recorded in
@@ -291,6 +316,7 @@ int build_once(H2 *h, const void *Yg)
         known_n = rec[0];
         if (known_n == 0) break;  // level lc + 1 is empty: the tree has ended
     }
+    if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_TREE], st);
     {
         // level offsets, pair counters and flags of the finished tree: one wait
         std::vector<int> lv(l + 2);
@@ -336,6 +362,7 @@ int build_once(H2 *h, const void *Yg)
     const int nlev = (int)h->lvl.size() - 1;
     const bool f32 = h->id_mixed && h->id_tol >= kIdF32MinTol;
     const bool tid = h->timing && h->ev_id0 && h->ev_id1;
+    if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_ID0], st);
     if (tid && hipEventRecord(h->ev_id0, st) != hipSuccess) return -1;
     if (h->n_nodes > 1 && launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, h->pq, f32, st))
         return -1;
@@ -344,10 +371,12 @@ int build_once(H2 *h, const void *Yg)
                          f32, st))
             return -1;
     if (tid && hipEventRecord(h->ev_id1, st) != hipSuccess) return -1;
+    if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_ID1], st);
     if (sort_pairs_list(h, d.wl, h->n_w, d.w_beg, d.w_end, ss) ||
         sort_pairs_list(h, d.nl, h->n_near, d.nl_beg, d.nl_end, ss))
         return -1;
     if (h->st_aux && hipEventRecord(h->ev_sorted, h->st_aux) != hipSuccess) return -1;
+    if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_SORTED], ss);
     if (h->st_aux && hipStreamWaitEvent(st, h->ev_sorted, 0) != hipSuccess) return -1;
     if (hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
         hipStreamSynchronize(st) != hipSuccess)
@@ -355,6 +384,21 @@ int build_once(H2 *h, const void *Yg)
     if (h->hflags[FLAG_POOL_OVF]) {
         h->pool_f *= 2.0;
         return 1;
+    }
+    if (h->tl_on) {
+        // the end mark goes in after the host's last wait: the gap to the product's first
+        // launch shows in the caller's timing, not here
+        (void)hipEventRecord(h->tl_ev[TL_END], st);
+        (void)hipEventSynchronize(h->tl_ev[TL_END]);
+        for (int k = 1; k < kTlEvents; k++) {
+            float ms = 0.f;
+            const int from = (k == TL_SORTED) ? TL_ID0 : (k == TL_END ? TL_SORTED : k - 1);
+            if (hipEventElapsedTime(&ms, h->tl_ev[from], h->tl_ev[k]) == hipSuccess) h->tl_ms[k] += ms;
+        }
+        float all = 0.f;
+        if (hipEventElapsedTime(&all, h->tl_ev[TL_START], h->tl_ev[TL_END]) == hipSuccess)
+            h->tl_ms[0] += all;
+        h->tl_builds++;
     }
     if (tid) {
         float ms = 0.f;
@@ -396,6 +440,10 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
               hipHostGetDevicePointer((void **)&h->pin_dev, h->pin, 0) == hipSuccess &&
               hipEventCreateWithFlags(&h->ev_lvl[0], hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&h->ev_lvl[1], hipEventDisableTiming) == hipSuccess;
+    if (ok && getenv("TSNE_H2_TIMELINE") && getenv("TSNE_H2_TIMELINE")[0] == '1') {
+        h->tl_on = true;
+        for (hipEvent_t &e : h->tl_ev) ok = ok && hipEventCreate(&e) == hipSuccess;
+    }
     if (ok) {
         const ProxyTable &pt = proxy_table(dim, h->ptol, 1.0, pmax_of(dim));
         int cmax = 0;
@@ -424,9 +472,7 @@ void h2_set_timing(H2 *h, int on)
     if (!h) return;
     if (on && !h->ev_id0) {
         if (hipEventCreate(&h->ev_id0) != hipSuccess || hipEventCreate(&h->ev_id1) != hipSuccess) {
-            for (hipEvent_t e : h->ev_lvl)
-        if (e) (void)hipEventDestroy(e);
-    if (h->ev_id0) (void)hipEventDestroy(h->ev_id0);
+            if (h->ev_id0) (void)hipEventDestroy(h->ev_id0);
             h->ev_id0 = h->ev_id1 = nullptr;
             return;
         }
@@ -459,6 +505,18 @@ void h2_destroy(H2 *h)
     if (h->ev_tree) (void)hipEventDestroy(h->ev_tree);
     if (h->ev_sorted) (void)hipEventDestroy(h->ev_sorted);
     if (h->st) (void)hipStreamSynchronize(h->st);
+    if (h->tl_on && h->tl_builds > 0) {
+        const double b = h->tl_builds;
+        fprintf(stderr, "[h2 timeline] %d builds, ms per build: total %.3f | front %.3f tree %.3f "
+                        "readback+host %.3f ID %.3f | ID start to sorts done %.3f, sorts done to end %.3f\n",
+                h->tl_builds, h->tl_ms[0] / b, h->tl_ms[TL_FRONT] / b, h->tl_ms[TL_TREE] / b,
+                h->tl_ms[TL_ID0] / b, h->tl_ms[TL_ID1] / b, h->tl_ms[TL_SORTED] / b,
+                h->tl_ms[TL_END] / b);
+    }
+    for (hipEvent_t e : h->tl_ev)
+        if (e) (void)hipEventDestroy(e);
+    for (hipEvent_t e : h->ev_lvl)
+        if (e) (void)hipEventDestroy(e);
     if (h->ev_id0) (void)hipEventDestroy(h->ev_id0);
     if (h->ev_id1) (void)hipEventDestroy(h->ev_id1);
     free_sized(h);
@@ -471,7 +529,22 @@ void h2_destroy(H2 *h)
     delete h;
 }
 
+static int build_all(H2 *h, const void *Yg, int *nonfinite);
+
+void h2_set_front_hook(H2 *h, int (*fn)(void *), void *arg)
+{
+    h->front_hook = fn;
+    h->front_arg = arg;
+}
+
 int h2_build(H2 *h, const void *Yg, int *nonfinite)
+{
+    const int rc = build_all(h, Yg, nonfinite);
+    const int hr = run_front_hook(h);  // a build that stopped before its front
+    return rc != 0 ? rc : (hr != 0 ? -1 : 0);
+}
+
+static int build_all(H2 *h, const void *Yg, int *nonfinite)
 {
     *nonfinite = 0;
     h->usable = 0;

@@ -77,6 +77,7 @@ struct tsne_workspace {
     // the points are still in the caller's order and the per-row kernel runs)
     AttrTiles tiles;
     int tile_rows = 0;          // 0: tiles off
+    int attr_after_front = 0;   // rebuild iterations: attractive sums queued after the build's front
     int tile_lds = 0;           // LDS bytes per tile workgroup for the staged records
     int tile_nt = 256;          // threads per tile workgroup
     // graph mode: iterations that reuse the repulsive force (attractive + update + centring,
@@ -232,6 +233,20 @@ int repulsive(tsne_workspace *ws, int allow_h2)
 {
     const int rc = repulsive_sums(ws, allow_h2);
     return rc ? rc : repulsive_z(ws);
+}
+
+// The attractive sums of a rebuild iteration on the second stream (into d_A), after
+// what the main stream has queued so far.
+int queue_split_attr(void *arg)
+{
+    tsne_workspace *ws = static_cast<tsne_workspace *>(arg);
+    TSNE_HIP_CHECK(hipEventRecord(ws->ev_y, ws->st));
+    TSNE_HIP_CHECK(hipStreamWaitEvent(ws->st2, ws->ev_y, 0));
+    if (launch_attr_only(ws->dim, ws->n, ws->d_rowptr, ws->d_col, ws->d_val32, ws->d_Yg[ws->cur],
+                         ws->d_A, ws->nblocks_upd, ws->st2) != 0)
+        return -1;
+    TSNE_HIP_CHECK(hipEventRecord(ws->ev_a, ws->st2));
+    return 0;
 }
 
 int rebuild(tsne_workspace *ws)
@@ -701,6 +716,10 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         ws->graph_mode = e ? (e[0] == '1') : 0;
     }
     {
+        const char *e = getenv("TSNE_ATTR_AFTER_FRONT");
+        ws->attr_after_front = e ? (e[0] == '1') : 0;
+    }
+    {
         // the attractive pass of rebuild iterations on a second stream: from 2^18 points
         // on (below, the two extra launches and the event waits cost more than the
         // overlap saves); TSNE_OVERLAP=0 / 1 turns it off / on for any size
@@ -869,12 +888,10 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
             }
             if (ws->st2 && ws->tiles.ntiles == 0) {
                 split = true;
-                TSNE_HIP_CHECK(hipEventRecord(ws->ev_y, ws->st));
-                TSNE_HIP_CHECK(hipStreamWaitEvent(ws->st2, ws->ev_y, 0));
-                if (launch_attr_only(D, ws->n, ws->d_rowptr, ws->d_col, ws->d_val32,
-                                     ws->d_Yg[ws->cur], ws->d_A, ws->nblocks_upd, ws->st2) != 0)
+                if (ws->attr_after_front && ws->h2)
+                    h2_set_front_hook(ws->h2, queue_split_attr, ws);
+                else if (queue_split_attr(ws) != 0)
                     return -1;
-                TSNE_HIP_CHECK(hipEventRecord(ws->ev_a, ws->st2));
             }
             if (rebuild(ws) != 0) {
                 fprintf(stderr, "ERROR: Failed to rebuild H2 matrix at iteration %d\n", it);
