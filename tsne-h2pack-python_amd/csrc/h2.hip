@@ -214,9 +214,7 @@ bool alloc_sized(H2 *h)
 int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end, hipStream_t s)
 {
     H2Dev &d = h->d;
-    if (hipMemsetAsync(beg, 0, sizeof(int) * d.node_cap, s) != hipSuccess ||
-        hipMemsetAsync(end, 0, sizeof(int) * d.node_cap, s) != hipSuccess)
-        return -1;
+    // beg / end are zero for every node of the tree (set where the node is made)
     if (n <= 0) return 0;
     int bits = 32;
     while ((1LL << (bits - 32)) < d.node_cap) bits++;
@@ -254,9 +252,7 @@ int build_once(H2 *h, const void *Yg)
     H2Dev &d = h->d;
     hipStream_t st = h->st;
     const int D = h->D;
-    if (hipMemsetAsync(d.cnt, 0, sizeof(int) * CNT_COUNT, st) != hipSuccess ||
-        hipMemsetAsync(d.flags, 0, sizeof(int) * FLAG_COUNT, st) != hipSuccess)
-        return -1;
+    // counters and flags are cleared by the root-box kernel
     if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_START], st);
     if (launch_root_box(D, Yg, h->n, d, h->part, h->nblk, st)) return -1;
     if (launch_morton(D, Yg, d, st)) return -1;

@@ -129,6 +129,10 @@ bbox_final_kernel(int nblk, const double *__restrict__ part, H2Dev h)
     const int t = threadIdx.x;
     double lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
     int bad = 0;
+    // the build's counters and flags start at zero (nothing of this build has touched
+    // them yet; the non-finite flag below is set after the barrier)
+    for (int k = t; k < CNT_COUNT; k += 256) h.cnt[k] = 0;
+    for (int k = t; k < FLAG_COUNT; k += 256) h.flags[k] = 0;
     for (int b = t; b < nblk; b += 256) {
         for (int d = 0; d < D; d++) {
             lo[d] = fmin(lo[d], part[b * 7 + d]);
@@ -253,6 +257,7 @@ __global__ void init_root_kernel(H2Dev h, int B)
     h.near[0] = 0;
     h.near_cnt[0] = 1;
     h.adm_beg[0] = h.adm_end[0] = h.x_beg[0] = h.x_end[0] = 0;
+    h.w_beg[0] = h.w_end[0] = h.nl_beg[0] = h.nl_end[0] = 0;
     h.lvl[0] = 0;
     h.lvl[1] = 1;
     if (h.n <= h.leaf || B == 0) {
@@ -375,6 +380,8 @@ __device__ __forceinline__ void split_write_node(const H2Dev &h, int l, int lb, 
         h.parent[ch] = i;
         h.child0[ch] = -1;
         h.nchild[ch] = 0;
+        // empty W / NEAR ranges until the sorted lists' offsets are written
+        h.w_beg[ch] = h.w_end[ch] = h.nl_beg[ch] = h.nl_end[ch] = 0;
         float4 cc;
         cc.x = c.x + ((v & 1) ? hc : -hc);
         cc.y = c.y + ((v & 2) ? hc : -hc);

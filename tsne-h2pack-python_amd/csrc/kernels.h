@@ -51,6 +51,9 @@ int launch_attr_only(int D, int n, const int *row_ptr, const int *col_idx, const
 int launch_update(int D, int n, const float *A, const float *F, double *Y64, double *uY,
                   double *gains, void *Yg_next, const double *scal, double exag, double mom,
                   double lr, double *block_sums, int nblocks, hipStream_t st);
+// Workgroups of the first stage of the centring-mean reduction: block-sum buffers hold
+// kMeanParts records (3 doubles each) beyond the blocks of the attractive launch.
+constexpr int kMeanParts = 256;
 // Diagnostics: the attractive pass's memory pattern; mode 0 P only, 1 P and the
 // neighbour gathers, 2 the full attractive sums (launch_attr_only).  A: float [n][D].
 int launch_attr_probe(int D, int n, int mode, const int *row_ptr, const int *col_idx,

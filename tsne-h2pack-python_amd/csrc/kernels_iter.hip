@@ -505,51 +505,67 @@ update_kernel(int n, const float *__restrict__ A, const float *__restrict__ F,
     if (threadIdx.x < D) block_sums[(size_t)lblk * 3 + threadIdx.x] = red[threadIdx.x][0];
 }
 
-// Sum of the per-block sums in block order -> new pending mean (SLOT_MEAN..+D-1).
-// Also folds a finiteness flag (SLOT_BAD) for the host.
-__global__ void __launch_bounds__(1024)
-mean_finalize_kernel(int nblocks, int D, int n, const double *__restrict__ block_sums,
-                     double *__restrict__ scal)
+// Sum of the per-block sums -> new pending mean (SLOT_MEAN..+D-1), in two launches with a
+// fixed order of additions: kMeanParts workgroups each add a contiguous run of the block
+// records (thread t the records t, t + 256, ... of the run, then a fixed tree) into
+// partials stored after the records; one workgroup adds the partials (fixed tree) and
+// folds a finiteness flag (SLOT_BAD) for the host.  The block-sum buffer holds
+// kMeanParts records beyond the blocks of any launch.
+static_assert(kMeanParts == 256, "one partial per thread of the final workgroup");
+
+__device__ __forceinline__ void tree_sum3_256(double (&a)[3], double (*part)[256], int D)
 {
-    __shared__ double part[3][1024];
     const int t = threadIdx.x;
-    double a[3] = {0.0, 0.0, 0.0};
-    // thread t sums blocks t, t + 1024, ... in order (neighbouring threads read
-    // neighbouring records, so each pass is one coalesced sweep), then a fixed tree.  The
-    // records of eight passes are loaded before they are added (same order of additions),
-    // so the loads overlap instead of waiting one after another.
-    constexpr int U = 8;
-    for (int b0 = t; b0 < nblocks; b0 += 1024 * U) {
-        double v[U][3];
-#pragma unroll
-        for (int u = 0; u < U; u++) {
-            const int b = b0 + u * 1024;
-#pragma unroll
-            for (int d = 0; d < 3; d++)
-                v[u][d] = (b < nblocks && d < D) ? block_sums[(size_t)b * 3 + d] : 0.0;
-        }
-#pragma unroll
-        for (int u = 0; u < U; u++)
-            if (b0 + u * 1024 < nblocks)
-#pragma unroll
-                for (int d = 0; d < 3; d++)
-                    if (d < D) a[d] += v[u][d];
-    }
 #pragma unroll
     for (int d = 0; d < 3; d++) part[d][t] = a[d];
     __syncthreads();
-    for (int off = 512; off > 0; off >>= 1) {
+    for (int off = 128; off > 0; off >>= 1) {
         if (t < off)
 #pragma unroll
             for (int d = 0; d < 3; d++)
                 if (d < D) part[d][t] += part[d][t + off];
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(256)
+mean_partial_kernel(int nblocks, int D, double *__restrict__ block_sums)
+{
+    __shared__ double part[3][256];
+    const int per = (nblocks + kMeanParts - 1) / kMeanParts;
+    const int b0 = blockIdx.x * per, b1 = min(nblocks, b0 + per);
+    double a[3] = {0.0, 0.0, 0.0};
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += 256)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            if (d < D) a[d] += block_sums[(size_t)b * 3 + d];
+    tree_sum3_256(a, part, D);
+    if (threadIdx.x < 3) block_sums[((size_t)nblocks + blockIdx.x) * 3 + threadIdx.x] = part[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(256)
+mean_finalize_kernel(int nblocks, int D, int n, const double *__restrict__ block_sums,
+                     double *__restrict__ scal)
+{
+    __shared__ double part[3][256];
+    const int t = threadIdx.x;
+    double a[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        if (d < D) a[d] = block_sums[((size_t)nblocks + t) * 3 + d];  // kMeanParts == 256
+    tree_sum3_256(a, part, D);
     if (t < D) {
         const double m = part[t][0] / (double)n;
         scal[SLOT_MEAN + t] = m;
         if (!isfinite(m)) scal[SLOT_BAD] = 1.0;
     }
+}
+
+int launch_mean(int D, int n, int nblocks, double *block_sums, double *scal, hipStream_t st)
+{
+    hipLaunchKernelGGL(mean_partial_kernel, dim3(kMeanParts), dim3(256), 0, st, nblocks, D, block_sums);
+    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(256), 0, st, nblocks, D, n, block_sums, scal);
+    return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
 // ---------------------------------------------------------------------------------
@@ -950,8 +966,7 @@ int launch_attr_tile(int D, int n, const AttrTiles &tl, int nt, const int *row_p
     }
 #undef TSNE_TILE_ARGS
     if (ev_end) (void)hipEventRecord(ev_end, st);
-    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, grid, D, n, block_sums,
-                       (double *)scal);
+    if (launch_mean(D, n, grid, block_sums, (double *)scal, st) != 0) return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -980,8 +995,7 @@ int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, con
     }
 #undef TSNE_ATTR_ARGS
     if (ev_end) (void)hipEventRecord(ev_end, st);
-    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, nblocks, D, n,
-                       block_sums, (double *)scal);
+    if (launch_mean(D, n, nblocks, block_sums, (double *)scal, st) != 0) return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
@@ -1011,8 +1025,7 @@ int launch_update(int D, int n, const float *A, const float *F, double *Y64, dou
     else
         hipLaunchKernelGGL(update_kernel<3>, dim3(nblocks), dim3(256), 0, st, n, A, F, Y64, uY,
                            gains, (float4 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
-    hipLaunchKernelGGL(mean_finalize_kernel, dim3(1), dim3(1024), 0, st, nblocks, D, n,
-                       block_sums, (double *)scal);
+    if (launch_mean(D, n, nblocks, block_sums, (double *)scal, st) != 0) return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

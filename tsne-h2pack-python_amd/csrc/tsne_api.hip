@@ -728,10 +728,11 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
     }
     bool ok = hipGetDevice(&ws->device) == hipSuccess &&
               hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
-    if (ok && ws->overlap)
-        ok = hipStreamCreateWithFlags(&ws->st2, hipStreamNonBlocking) == hipSuccess &&
-             hipEventCreateWithFlags(&ws->ev_y, hipEventDisableTiming) == hipSuccess &&
+    if (ok && ws->overlap) {
+        ok = hipStreamCreateWithFlags(&ws->st2, hipStreamNonBlocking) == hipSuccess;
+        ok = ok && hipEventCreateWithFlags(&ws->ev_y, hipEventDisableTiming) == hipSuccess &&
              hipEventCreateWithFlags(&ws->ev_a, hipEventDisableTiming) == hipSuccess;
+    }
     if (!ok) {
         fprintf(stderr, "tsne_init_with_P: no HIP device available\n");
         delete ws;
@@ -745,7 +746,7 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
          dalloc((char **)&ws->d_Yg[1], (size_t)n * kGatherBytes[D], ws->st) == 0 &&
          dalloc(&ws->d_s, n, ws->st) == 0 && dalloc(&ws->d_F, (size_t)n * D, ws->st) == 0 &&
          dalloc(&ws->d_scal, SLOT_COUNT, ws->st) == 0 &&
-         dalloc(&ws->d_block_sums, (size_t)ws->nblocks_upd * 3, ws->st) == 0 &&
+         dalloc(&ws->d_block_sums, ((size_t)ws->nblocks_upd + kMeanParts) * 3, ws->st) == 0 &&
          dalloc(&ws->d_partial, ws->nblocks_red, ws->st) == 0 && dalloc(&ws->d_rowkl, n, ws->st) == 0 &&
          (!ws->overlap || dalloc(&ws->d_A, (size_t)n * D, ws->st) == 0);
     if (ok) {
