@@ -12,6 +12,9 @@
   bar of the other H2 tests); 200 of those rows of the FP64 exact
   repulsion are checked against the CPU oracle at the same Z (1e-9); then both run to
   iteration 1000 and their final KLs agree within 1%.
+* 200K, 2D, the whole schedule: the H2 runs (mixed and plain) end within 1% of the final
+  KL of the same run with the exact repulsion (direct O(N^2) sum) in place of every H2
+  product, from the same start.
 * 500K, 3D: the mixed path on the same check as the plain one (test_gpu_h2.py): 20,000
   sampled rows within 3e-2 of FP64 after 250 iterations.
 * Serving: a seeded sample of the 256-job serving mix (8 jobs with N from 2,000 to
@@ -133,6 +136,62 @@ def test_config_1m_2d_mixed_and_plain(backend):
         assert r["cpu_rel"] <= 1e-9
         assert r["n_rebuild"] == 292 and r["n_renumber"] == 6
     assert abs(res["mixed"]["kl"] - res["plain"]["kl"]) / res["plain"]["kl"] < 0.01
+
+
+def _full_run_kl(backend, P, precision, mode, Y0):
+    """mode 'h2': the H2 path; 'exact': method='exact' (direct sum every iteration);
+    'exact_reuse': the direct sum on the H2 path's rebuild cadence and force reuse."""
+    import os
+    n, dim = P.shape[0], 2
+    p = backend.create_default_params()
+    p.verbose = 0
+    p.n_iter = 1000
+    p.h2_tol = 1e-2
+    p.h2_rebuild_freq = 999999 if mode == "exact" else 3  # > 1e5: method='exact'
+    p.h2_rebuild_freq_early = 10
+    p.learning_rate = n / 12.0
+    if mode == "exact_reuse":
+        os.environ["TSNE_DIRECT_REUSE"] = "1"
+    try:
+        ws = backend.init_workspace(P, dim, p, init_Y=Y0)
+    finally:
+        os.environ.pop("TSNE_DIRECT_REUSE", None)
+    try:
+        assert backend.lib.tsne_set_h2_precision(ws, 1 if precision == "mixed" else 0) == 0
+        assert backend.lib.tsne_run(ws) == 0
+        kl = backend.lib.compute_kl_divergence(ws)
+        nr = ctypes.c_int()
+        assert backend.lib.tsne_run_state(ws, ctypes.byref(nr), None, None, None) == 0
+    finally:
+        backend.lib.tsne_destroy(ws)
+    return kl, nr.value
+
+
+def test_config_200k_final_kl_against_exact_repulsion(backend):
+    """The whole schedule at N = 200,000 (2D, the headline's data model, k = 90,
+    perplexity 30, h2_tol 1e-2).  Anchor: the same run with the exact repulsion (direct
+    O(N^2) sum, FP32 products) in place of every H2 product, on the same rebuild cadence
+    and force reuse, from the same initial embedding; the H2 runs, mixed and plain, end
+    within 1% of its final KL.  method='exact' (a fresh direct sum every iteration, no
+    reuse, as the reference's exact path) is reported beside it: it differs from both by
+    the reuse of the repulsive force between rebuilds as well."""
+    n = 200_000
+    X, _ = gaussian_mixture(n, dim=50, n_clusters=40, seed=5, spread=5.0, dirichlet=True,
+                            std_range=(0.5, 2.0), dtype=np.float32)
+    P, _ = backend.joint_P_from_points(X.astype(np.float64), 90, 30.0)
+    del X
+    Y0 = np.random.default_rng(2).normal(scale=1e-4, size=(n, 2))
+    _say("200K 2D: exact repulsion on the H2 schedule")
+    kl_ref, nr_ref = _full_run_kl(backend, P, "plain", "exact_reuse", Y0)
+    kl_exact, _ = _full_run_kl(backend, P, "plain", "exact", Y0)
+    _say(f"  exact on the H2 schedule: final KL {kl_ref:.5f}; method='exact': {kl_exact:.5f}")
+    assert nr_ref == 0  # no H2 structure was built
+    for precision in ("plain", "mixed"):
+        kl, nr = _full_run_kl(backend, P, precision, "h2", Y0)
+        _say(f"  H2 {precision}: final KL {kl:.5f} ({(kl / kl_ref - 1) * 100:+.3f}% vs exact on "
+             f"the schedule, {(kl / kl_exact - 1) * 100:+.3f}% vs method='exact'), rebuilds {nr}")
+        assert nr == 292
+        assert abs(kl - kl_ref) / kl_ref < 0.01
 
 
 def test_config_500k_3d_mixed_sampled_rows(backend):

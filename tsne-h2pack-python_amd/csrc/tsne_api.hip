@@ -78,6 +78,10 @@ struct tsne_workspace {
     AttrTiles tiles;
     int tile_rows = 0;          // 0: tiles off
     int attr_after_front = 0;   // rebuild iterations: attractive sums queued after the build's front
+    // check mode (TSNE_DIRECT_REUSE=1): the direct O(N^2) sum in place of the H2 product, on
+    // the H2 path's rebuild cadence and force reuse -- the exact counterpart of an H2 run,
+    // differing from it only by the H2 approximation
+    int direct_reuse = 0;
     int tile_lds = 0;           // LDS bytes per tile workgroup for the staged records
     int tile_nt = 256;          // threads per tile workgroup
     // graph mode: iterations that reuse the repulsive force (attractive + update + centring,
@@ -121,8 +125,9 @@ template <typename T> void dfree(T *&p, hipStream_t st)
 bool use_direct(const tsne_workspace *ws)
 {
     // N < 2000 and method='exact' (rebuild frequency > 1e5) take the direct sum, as the
-    // reference does (src/tsne_h2pack.c:404-405, :1378).
-    return ws->n < 2000 || ws->params.h2_rebuild_freq > 100000;
+    // reference does (src/tsne_h2pack.c:404-405, :1378); so does the exact-with-reuse
+    // check mode (TSNE_DIRECT_REUSE=1)
+    return ws->n < 2000 || ws->params.h2_rebuild_freq > 100000 || ws->direct_reuse;
 }
 
 int rebuild_iter(const tsne_workspace *ws, int it, int reuse)
@@ -720,6 +725,10 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         ws->attr_after_front = e ? (e[0] == '1') : 0;
     }
     {
+        const char *e = getenv("TSNE_DIRECT_REUSE");
+        ws->direct_reuse = e ? (e[0] == '1') : 0;
+    }
+    {
         // the attractive pass of rebuild iterations on a second stream: from 2^18 points
         // on (below, the two extra launches and the event waits cost more than the
         // overlap saves); TSNE_OVERLAP=0 / 1 turns it off / on for any size
@@ -825,7 +834,8 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
     const tsne_params_t &pp = ws->params;
     const int n_iter = pp.n_iter;
     const int direct = use_direct(ws);
-    const int reuse = !direct;  // src/tsne_h2pack.c:1699 (rep_reuse when N >= 2000)
+    // src/tsne_h2pack.c:1699 (rep_reuse when N >= 2000)
+    const int reuse = !direct || (ws->direct_reuse && ws->n >= 2000);
     const int D = ws->dim;
     if (it_begin < 0) it_begin = 0;
     if (it_end > n_iter) it_end = n_iter;
@@ -900,6 +910,8 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
             }
             rebuilt = 1;
             ws->n_rebuild++;
+        } else if (direct && reuse && rebuild_iter(ws, it, reuse)) {
+            rebuilt = 1;  // check mode: a fresh direct sum where the H2 path rebuilds
         }
         if (timing) (void)hipEventRecord(ev[1], ws->st);
         const bool fresh = !reuse || rebuilt || !ws->rep_valid;
