@@ -71,7 +71,6 @@ struct H2 {
 namespace {
 
 constexpr int kLevelBatch = 4;   // tree levels queued between two host checks
-constexpr int kPinRec = 4;       // pinned words per published level
 constexpr double kIdF32MinTol = 1e-3;  // smallest ID tolerance of the mixed path's FP32 ID
 constexpr int kMaxLevels = 32;   // >= H2Cfg<D>::B + 1 for both dimensions
 
@@ -286,6 +285,7 @@ int build_once(H2 *h, const void *Yg)
     for (int k = 0; k < maxlev * kPinRec; k++) plev[k] = -1;
     int l = 0, pend = 0;            // the level pass of level l appends to pend[pend]
     int known_l = 0, known_n = 1;   // level known_l has exactly known_n nodes
+    hipEvent_t last_ev = nullptr;   // end of the last queued batch
     for (int batch = 0; l < maxlev; batch++) {
         for (int k = 0; k < kLevelBatch && l < maxlev; k++, l++) {
             long long bound = known_n;
@@ -298,6 +298,7 @@ int build_once(H2 *h, const void *Yg)
             pend ^= 1;
         }
         if (hipEventRecord(h->ev_lvl[batch & 1], st) != hipSuccess) return -1;
+        last_ev = h->ev_lvl[batch & 1];
         if (batch == 0) continue;
         if (hipEventSynchronize(h->ev_lvl[(batch - 1) & 1]) != hipSuccess) return -1;
         const int lc = std::min(l, kLevelBatch * batch) - 1;  // last level of batch - 1
@@ -314,14 +315,26 @@ int build_once(H2 *h, const void *Yg)
     }
     if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_TREE], st);
     {
-        // level offsets, pair counters and flags of the finished tree: one wait
+        // level offsets, pair counters and flags of the finished tree: one wait for the last
+        // batch, then the per-level records (the last queued level's record holds the final
+        // counts and flags: no level after the tree's end adds anything)
+        if (!last_ev || hipEventSynchronize(last_ev) != hipSuccess) return -1;
         std::vector<int> lv(l + 2);
-        if (hipMemcpyAsync(lv.data(), d.lvl, sizeof(int) * lv.size(), hipMemcpyDeviceToHost, st) !=
-                hipSuccess ||
-            hipMemcpyAsync(h->pin, d.cnt, sizeof(int) * CNT_COUNT, hipMemcpyDeviceToHost, st) ||
-            hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
-            hipStreamSynchronize(st) != hipSuccess)
-            return -1;
+        lv[0] = 0;
+        lv[1] = 1;
+        for (int k = 0; k < l; k++) lv[k + 2] = plev[kPinRec * k + 9];
+        const int *fin = plev + kPinRec * (l - 1);
+        for (int &f : h->hflags) f = 0;
+        h->hflags[FLAG_NONFINITE] = fin[1];
+        h->hflags[FLAG_NODE_OVF] = fin[2];
+        h->hflags[FLAG_NEAR_OVF] = fin[3] & 1;
+        h->hflags[FLAG_PAIR_OVF] = (fin[3] >> 1) & 1;
+        h->hflags[FLAG_BIG_LEAF] = (fin[3] >> 2) & 1;
+        h->pin[CNT_ADM] = fin[4];
+        h->pin[CNT_X] = fin[5];
+        h->pin[CNT_W] = fin[6];
+        h->pin[CNT_NEAR] = fin[7];
+        h->pin[CNT_LEAF] = fin[8];
         h->lvl.assign({0, 1});
         for (int k = 1; k + 1 < (int)lv.size() && lv[k + 1] > lv[k]; k++) h->lvl.push_back(lv[k + 1]);
     }

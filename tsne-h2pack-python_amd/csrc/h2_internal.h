@@ -135,7 +135,10 @@ int launch_init_root(const H2Dev &h, hipStream_t st);
 // pairs of pend_in; the level write scans the chunk totals, writes the children and
 // publishes the level: lvl[l+2], the reset of the pending counter `pend_next` the next
 // level pass appends to, and the pinned words out[0] (children of level l), out[1]
-// (non-finite input) and out[2] (node overflow).
+// (non-finite input), out[2] (node overflow), out[3] (near-list / pair-buffer overflow and
+// oversized-leaf bits), out[4..8] (ADM, X, W, NEAR pair and leaf counts) and out[9]
+// (lvl[l+2]).  kPinRec words per level.
+constexpr int kPinRec = 16;
 constexpr int kSplitBlocks = 256;  // split workgroups per level at most (childoff entries)
 constexpr int kListBlocks = 1024;  // list workgroups per level at most
 constexpr int kPendBlocks = 256;   // pending-pair workgroups per level

@@ -359,6 +359,16 @@ __device__ __forceinline__ void publish_level(const H2Dev &h, int l, int le, int
     out[0] = T;
     out[1] = h.flags[FLAG_NONFINITE];
     out[2] = ovf;
+    // the build's state after this level: once no level follows, the finished tree's
+    // flags, pair and leaf counts and level end, without a copy from device memory
+    out[3] = (h.flags[FLAG_NEAR_OVF] ? 1 : 0) | (h.flags[FLAG_PAIR_OVF] ? 2 : 0) |
+             (h.flags[FLAG_BIG_LEAF] ? 4 : 0);
+    out[4] = h.cnt[CNT_ADM];
+    out[5] = h.cnt[CNT_X];
+    out[6] = h.cnt[CNT_W];
+    out[7] = h.cnt[CNT_NEAR];
+    out[8] = h.cnt[CNT_LEAF];
+    out[9] = ne;
 }
 
 // The children of node i of level l (lb its first node), from child slot `base` on.
