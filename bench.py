@@ -434,10 +434,11 @@ def _job_params(p, sp):
     return p
 
 
-def serving_bench(n_jobs, slots_list, seq_sample, load=0.7):
+def serving_bench(n_jobs, slots_list, seq_sample, load=0.7, precision="plain"):
     """Serving workload: capacity (all jobs queued at once), then Poisson arrivals at
     `load` x capacity with per-job latency (arrival -> result); baseline: one job at a
-    time through tsne_init_with_P + tsne_run (sequential plain runs) on a sample."""
+    time through tsne_init_with_P + tsne_run (sequential plain runs) on a sample.
+    precision: the H2 precision of the served jobs (the baseline stays plain)."""
     import tempfile
     import shutil
     from tsne_h2pack._backend import get_backend
@@ -473,7 +474,7 @@ def serving_bench(n_jobs, slots_list, seq_sample, load=0.7):
     slots = slots_list[0]
 
     def run(arrivals):
-        with TSNEServer(slots=slots) as srv:
+        with TSNEServer(slots=slots, h2_precision=precision) as srv:
             ids = []
             t0 = time.perf_counter()
             for j, sp in enumerate(specs):
@@ -518,7 +519,9 @@ def serving_bench(n_jobs, slots_list, seq_sample, load=0.7):
         "metric": "jobs/s + p99 (serving: 256 t-SNE jobs, N log-uniform 2e3..2e5, 2D:3D 3:1)",
         "value": cap, "unit": "jobs/s", "n_gpus": 1, "higher_is_better": True,
         "dtype": "f32", "data": "synthetic (seeded Gaussian mixtures, exact k-NN on GPU)",
-        "config": {"workload": f"serving {n_jobs} jobs, 1000 iterations each, slots {slots}",
+        "config": {"workload": f"serving {n_jobs} jobs, 1000 iterations each, slots {slots}, "
+                               f"{precision} H2 products (sequential baseline plain)",
+                   "precision": precision,
                    "jobs": n_jobs, "slots": slots, "n_total": int(sum(s["n"] for s in specs)),
                    "hw_queues": os.environ.get("GPU_MAX_HW_QUEUES", "default")},
         "capacity_jobs_per_s": cap, "capacity_by_slots": caps,
@@ -527,7 +530,7 @@ def serving_bench(n_jobs, slots_list, seq_sample, load=0.7):
         "poisson": {"offered_jobs_per_s": rate, "achieved_jobs_per_s": rate_p,
                     "p50_s": float(np.percentile(lat, 50)),
                     "p99_s": float(np.percentile(lat, 99)), "max_s": float(lat.max())},
-        "kl_max_rel_diff_mix_vs_alone": kl_dev,
+        "kl_max_rel_diff_mix_vs_alone": kl_dev,  # 0 for plain; mixed vs plain otherwise
     }
 
 
@@ -702,6 +705,8 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for this process (serving; 0 leaves the default)")
     ap.add_argument("--seq-sample", type=int, default=32)
+    ap.add_argument("--serving-precision", default="plain", choices=("plain", "mixed"),
+                    help="H2 precision of the served jobs (the sequential baseline is plain)")
     ap.add_argument("--blocking-sync", action="store_true",
                     help="serving: host threads block (instead of spinning) while they wait "
                          "for the GPU (hipDeviceScheduleBlockingSync, set before HIP starts)")
@@ -740,7 +745,8 @@ def main():
         if a.hw_queues:
             os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(a.hw_queues, 1), 16))
         slots = [int(x) for x in str(a.slots).split(",")]
-        print(json.dumps(serving_bench(a.jobs, slots, a.seq_sample)), flush=True)
+        print(json.dumps(serving_bench(a.jobs, slots, a.seq_sample,
+                                       precision=a.serving_precision)), flush=True)
         return
     from tsne_h2pack._backend import get_backend
     be = get_backend()

@@ -389,6 +389,11 @@ tsne_server_t *tsne_server_create(int slots, long long hbm_budget_bytes);
 int tsne_server_submit(tsne_server_t *server, int n_point, int low_dim,
                        const tsne_params_t *params, sparse_matrix_t *P, const double *init_Y);
 
+/* H2 product precision (0 plain, 1 mixed, as tsne_set_h2_precision) of the jobs submitted
+ * after this call; until it is called, each job keeps the workspace default
+ * (TSNE_H2_PRECISION).  0 on success. */
+int tsne_server_set_h2_precision(tsne_server_t *server, int mode);
+
 /* Status of a job (TSNE_JOB_*), -2 for an unknown id; fills *info if not NULL. */
 int tsne_server_poll(tsne_server_t *server, int job_id, tsne_job_info_t *info);
 

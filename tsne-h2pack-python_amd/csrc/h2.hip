@@ -10,6 +10,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <new>
+#include <mutex>
 #include <vector>
 
 #include "h2.h"
@@ -234,6 +235,53 @@ int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end, hipStream_
     return launch_pair_offsets(buf, n, beg, end, s);
 }
 
+// Pinned host blocks of the H2 objects (mapped, coherent; the per-level records and host
+// scratch), kept on a process-wide free list per device: freeing pinned memory can
+// synchronise the whole device, which would stall every other job on it each time one
+// job ends.
+constexpr size_t kPinBytes = sizeof(int) * (CNT_COUNT + kPinRec * kMaxLevels);
+struct PinBlock {
+    int device;
+    int *host, *dev;
+};
+std::mutex g_pin_mu;
+std::vector<PinBlock> g_pin_free;
+
+bool pin_take(int **host, int **dev)
+{
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return false;
+    {
+        std::lock_guard<std::mutex> lk(g_pin_mu);
+        for (size_t k = 0; k < g_pin_free.size(); k++)
+            if (g_pin_free[k].device == device) {
+                *host = g_pin_free[k].host;
+                *dev = g_pin_free[k].dev;
+                g_pin_free.erase(g_pin_free.begin() + (long)k);
+                return true;
+            }
+    }
+    if (hipHostMalloc((void **)host, kPinBytes, hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess) {
+        *host = nullptr;
+        return false;
+    }
+    if (hipHostGetDevicePointer((void **)dev, *host, 0) != hipSuccess) {
+        (void)hipHostFree(*host);
+        *host = nullptr;
+        return false;
+    }
+    return true;
+}
+
+// Back to the free list; the caller has synchronised every stream that wrote the block.
+void pin_give(int *host, int *dev)
+{
+    int device = 0;
+    if (hipGetDevice(&device) != hipSuccess) return;  // kept out of the list (leaked)
+    std::lock_guard<std::mutex> lk(g_pin_mu);
+    g_pin_free.push_back({device, host, dev});
+}
+
 // The caller's one-shot hook, queued once the build's front (root box, keys, sort, sorted
 // gather copy) is on the stream.
 int run_front_hook(H2 *h)
@@ -444,9 +492,7 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
               dmalloc(&d.root, 1, h->st) && dmalloc(&d.flags, FLAG_COUNT, h->st) && dmalloc(&d.cnt, CNT_COUNT, h->st) &&
               dmalloc(&d.lvl, kMaxLevels + 4, h->st) &&
               dmalloc(&h->part, (size_t)h->nblk * 7, h->st) &&
-              hipHostMalloc((void **)&h->pin, sizeof(int) * (CNT_COUNT + kPinRec * kMaxLevels),
-                            hipHostMallocMapped | hipHostMallocCoherent) == hipSuccess &&
-              hipHostGetDevicePointer((void **)&h->pin_dev, h->pin, 0) == hipSuccess &&
+              pin_take(&h->pin, &h->pin_dev) &&
               hipEventCreateWithFlags(&h->ev_lvl[0], hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&h->ev_lvl[1], hipEventDisableTiming) == hipSuccess;
     if (ok && getenv("TSNE_H2_TIMELINE") && getenv("TSNE_H2_TIMELINE")[0] == '1') {
@@ -534,7 +580,7 @@ void h2_destroy(H2 *h)
     dfree(d.root, h->st); dfree(d.flags, h->st); dfree(d.cnt, h->st); dfree(h->part, h->st);
     dfree(d.lvl, h->st);
     dfree(h->d_proxy, h->st); dfree(h->d_proxy_cnt, h->st);
-    if (h->pin) (void)hipHostFree(h->pin);
+    if (h->pin) pin_give(h->pin, h->pin_dev);
     delete h;
 }
 

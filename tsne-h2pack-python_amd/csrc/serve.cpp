@@ -33,6 +33,7 @@ using Clock = std::chrono::steady_clock;
 struct Job {
     int id = 0;
     int n = 0, dim = 0;
+    int precision = -1;  // H2 product precision for the job (-1: the workspace's default)
     tsne_params_t params{};
     sparse_matrix_t *P = nullptr;
     std::vector<double> init_Y;
@@ -53,6 +54,7 @@ struct tsne_server {
     int running = 0;
     bool stop = false;
     int waiters = 0;  // tsne_server_wait calls inside the server (destroy waits for them)
+    int precision = -1;  // for jobs submitted from now on (tsne_server_set_h2_precision)
     std::mutex mu;
     std::condition_variable cv_work, cv_done, cv_idle;
     std::deque<Job *> queue;
@@ -102,6 +104,7 @@ int run_job(Job *j)
         // which the slots' host threads share (TSNE_GRAPH=0 keeps the eager launches)
         const char *e = getenv("TSNE_GRAPH");
         tsne_set_graph_mode(ws, !(e && e[0] == '0'));
+        if (j->precision >= 0) tsne_set_h2_precision(ws, j->precision);
     }
     bool ok = ws != nullptr && tsne_run(ws) == 0;
     if (ok) {
@@ -177,6 +180,20 @@ tsne_server_t *tsne_server_create(int slots, long long hbm_budget_bytes)
     tsne_server *s = new tsne_server();
     s->slots = slots;
     s->budget = hbm_budget_bytes > 0 ? hbm_budget_bytes : (long long)(0.9 * (double)free_b);
+    {
+        // jobs allocate their device arrays from the device's default stream-ordered pool;
+        // by default it returns freed memory to the system at every synchronisation, so
+        // each new job would allocate afresh.  Keep up to the budget cached instead.
+        int dev = 0;
+        hipMemPool_t pool = nullptr;
+        uint64_t keep = (uint64_t)s->budget;
+        if (hipGetDevice(&dev) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+            uint64_t cur = 0;
+            if (hipMemPoolGetAttribute(pool, hipMemPoolAttrReleaseThreshold, &cur) == hipSuccess &&
+                cur < keep)
+                (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep);
+        }
+    }
     s->t0 = Clock::now();
     for (int k = 0; k < slots; k++) s->workers.emplace_back(worker, s);
     return s;
@@ -205,12 +222,21 @@ int tsne_server_submit(tsne_server_t *s, int n_point, int low_dim, const tsne_pa
             return -1;
         }
         id = j->id = s->next_id++;
+        j->precision = s->precision;
         j->t_submit = s->now();
         s->jobs[j->id] = j;
         s->queue.push_back(j);
     }
     s->cv_work.notify_all();
     return id;
+}
+
+int tsne_server_set_h2_precision(tsne_server_t *s, int mode)
+{
+    if (!s || mode < 0 || mode > 1) return -1;
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->precision = mode;
+    return 0;
 }
 
 int tsne_server_poll(tsne_server_t *s, int job_id, tsne_job_info_t *info)

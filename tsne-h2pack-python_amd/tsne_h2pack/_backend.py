@@ -532,6 +532,8 @@ def configure_signatures(lib):
     lib.tsne_server_create.restype = c_srv
     lib.tsne_server_submit.argtypes = [c_srv, ctypes.c_int, ctypes.c_int, c_pp, c_sm, f64n]
     lib.tsne_server_submit.restype = ctypes.c_int
+    lib.tsne_server_set_h2_precision.argtypes = [c_srv, ctypes.c_int]
+    lib.tsne_server_set_h2_precision.restype = ctypes.c_int
     lib.tsne_server_poll.argtypes = [c_srv, ctypes.c_int, ctypes.POINTER(JobInfo)]
     lib.tsne_server_poll.restype = ctypes.c_int
     lib.tsne_server_wait.argtypes = [c_srv, ctypes.c_int, f64n, ctypes.POINTER(JobInfo)]

@@ -25,15 +25,22 @@ class TSNEServer:
     """Job server over the library's continuous-batching scheduler.
 
     slots: jobs that may run at once; hbm_budget_bytes: device memory that running
-    jobs may hold together (0: 90% of the free memory at creation)."""
+    jobs may hold together (0: 90% of the free memory at creation); h2_precision:
+    'plain' or 'mixed' for every job, or None for the library default
+    (TSNE_H2_PRECISION)."""
 
-    def __init__(self, slots: int = 8, hbm_budget_bytes: int = 0):
+    def __init__(self, slots: int = 8, hbm_budget_bytes: int = 0,
+                 h2_precision: Optional[str] = None):
+        if h2_precision not in (None, 'plain', 'mixed'):
+            raise ValueError("h2_precision must be 'plain', 'mixed' or None")
         self._be = get_backend()
         self._lib = self._be.lib
         self._srv = self._lib.tsne_server_create(int(slots), int(hbm_budget_bytes))
         if not self._srv:
             raise RuntimeError("tsne_server_create failed (no HIP device?)")
         self.slots = slots
+        if h2_precision is not None:
+            self._lib.tsne_server_set_h2_precision(self._srv, 1 if h2_precision == 'mixed' else 0)
 
     def default_params(self, n_point: int, **overrides):
         """Reference defaults (learning rate max(N/12, 200)), verbose off, with overrides
