@@ -322,9 +322,9 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
         nrm = (t < m) ? (n0 + n1) + (n2 + n3) : T(-1);
     }
 
-    // ---- column-pivoted MGS.  R[k][t] goes to rpool[k * m + t] (the node's region; an
-    // internal node whose reservation overflowed the pool writes nothing, and the build
-    // is redone with a larger pool)
+    // ---- column-pivoted MGS.  R[k][t] stays in thread t's registers (2D, <= 32 proxy
+    // slots) or goes to rpool[k * m + t] (the node's region; an internal node whose reservation
+    // overflowed the pool writes nothing, and the build is redone with a larger pool)
     const int rbase = s_base;
     const bool rfits = leafn || (long long)rbase + 2LL * m * RMAX <= 2LL * h.n * RMAX + (part + 1) * part_cap;
     double *rpool = reinterpret_cast<double *>(h.upool + rbase);
@@ -333,7 +333,16 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     int r = 0;
     const int rlim = min(RMAX, min(m, p));
     const T tol2 = (T)(tol * tol);
-    for (int k = 0; k < RMAX; k++) {
+    // 2D with up to 32 proxy slots: each candidate keeps its column of R in registers (the
+    // step loop is unrolled, so every index is static); otherwise (3D, 48 proxy slots),
+    // where the columns already fill the registers, it stores them in the pool region and
+    // reads them back for the solve
+    constexpr bool RREG = (D == 2 && PMAX <= 32);
+    double rreg[RREG ? RMAX : 1];
+#pragma unroll
+    for (int k = 0; k < (RREG ? RMAX : 1); k++) rreg[k] = 0.0;
+    // one pivoting step; false once the pivoting stops
+    auto gs_step = [&](int k, double &rk) __attribute__((always_inline)) -> bool {
         // pivot: the candidate of largest residual norm (ties within 2^-14 to the smaller
         // index), by one 32-bit key maximum over the workgroup
         unsigned key = wave_max_u32(pivot_key((double)nrm, t, !selected && t < m));
@@ -356,7 +365,7 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
         __syncthreads();
         const T bv = key ? s_bv : T(0);
         if (k == 0) nrm0 = bv;
-        if (k >= rlim || !(bv > tol2 * nrm0) || !(bv > T(0))) break;
+        if (k >= rlim || !(bv > tol2 * nrm0) || !(bv > T(0))) return false;
         if (t == bi) {
             selected = true;
             s_R[k][k] = sqrt((double)bv);  // R[k][k]
@@ -383,7 +392,8 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
                 d3 = fma(Qk(q + 3), a[q + 3], d3);
             }
             const T dot = (d0 + d1) + (d2 + d3);  // R[k][t]
-            if (rfits) rpool[(size_t)k * m + t] = (double)dot;
+            if constexpr (RREG) rk = (double)dot;
+            else if (rfits) rpool[(size_t)k * m + t] = (double)dot;
             T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
 #pragma unroll
             for (int q = 0; q < PMAX; q += 4) if (D == 2 || q < p) {  // 3D: skip empty slots (register pressure)
@@ -400,6 +410,16 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
         }
         r = k + 1;
         if (k == RMAX - 1 && t == 0) atomicOr(&h.flags[FLAG_RANK_CAP], 1);
+        return true;
+    };
+    if constexpr (RREG) {
+#pragma unroll
+        for (int k = 0; k < RMAX; k++)
+            if (!gs_step(k, rreg[k])) break;
+    } else {
+        double unused = 0.0;
+        for (int k = 0; k < RMAX; k++)
+            if (!gs_step(k, unused)) break;
     }
     // The rest runs over RB rank slots, RB the smallest bucket (RMAX/4, RMAX/2, RMAX)
     // that holds r: a uniform branch per workgroup, each bucket fully unrolled.
@@ -414,7 +434,8 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
         const int rows = (kk >= 0) ? kk : r;
 #pragma unroll
         for (int k = 0; k < RB; k++)
-            rt[k] = (t < m && k < rows && rfits) ? rpool[(size_t)k * m + t] : 0.0;
+            if constexpr (RREG) rt[k] = (t < m && k < rows) ? rreg[k] : 0.0;
+            else rt[k] = (t < m && k < rows && rfits) ? rpool[(size_t)k * m + t] : 0.0;
         if (kk >= 0) {
 #pragma unroll
             for (int j = 0; j < RB; j++)
