@@ -71,7 +71,18 @@ struct H2 {
 
 namespace {
 
-constexpr int kLevelBatch = 4;   // tree levels queued between two host checks
+constexpr int kLevelBatch = 4;   // tree levels queued between two host checks (default)
+
+// Levels per batch: kLevelBatch, or TSNE_LEVEL_BATCH (1..8) when set; read once.
+int level_batch()
+{
+    static const int b = [] {
+        const char *e = getenv("TSNE_LEVEL_BATCH");
+        const int v = e ? atoi(e) : kLevelBatch;
+        return (v >= 1 && v <= 8) ? v : kLevelBatch;
+    }();
+    return b;
+}
 constexpr double kIdF32MinTol = 1e-3;  // smallest ID tolerance of the mixed path's FP32 ID
 constexpr int kMaxLevels = 32;   // >= H2Cfg<D>::B + 1 for both dimensions
 
@@ -332,10 +343,11 @@ int build_once(H2 *h, const void *Yg)
     int *plev = h->pin + CNT_COUNT, *plev_dev = h->pin_dev + CNT_COUNT;
     for (int k = 0; k < maxlev * kPinRec; k++) plev[k] = -1;
     int l = 0, pend = 0;            // the level pass of level l appends to pend[pend]
+    const int lb = level_batch();
     int known_l = 0, known_n = 1;   // level known_l has exactly known_n nodes
     hipEvent_t last_ev = nullptr;   // end of the last queued batch
     for (int batch = 0; l < maxlev; batch++) {
-        for (int k = 0; k < kLevelBatch && l < maxlev; k++, l++) {
+        for (int k = 0; k < lb && l < maxlev; k++, l++) {
             long long bound = known_n;
             for (int q = known_l; q < l && bound < d.node_cap; q++) bound *= NCH;
             bound = std::min<long long>(bound, d.node_cap);
@@ -349,7 +361,7 @@ int build_once(H2 *h, const void *Yg)
         last_ev = h->ev_lvl[batch & 1];
         if (batch == 0) continue;
         if (hipEventSynchronize(h->ev_lvl[(batch - 1) & 1]) != hipSuccess) return -1;
-        const int lc = std::min(l, kLevelBatch * batch) - 1;  // last level of batch - 1
+        const int lc = std::min(l, lb * batch) - 1;  // last level of batch - 1
         const int *rec = plev + kPinRec * lc;
         if (rec[1] || rec[2]) {  // non-finite input / node arrays full: drain, then report
             if (hipStreamSynchronize(st) != hipSuccess) return -1;
