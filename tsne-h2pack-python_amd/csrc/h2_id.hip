@@ -24,6 +24,7 @@
 // fused product (src/tsne_h2_matmul.c:309-334).  One workgroup per node, one thread per
 // candidate column.
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "h2_internal.h"
@@ -534,6 +535,17 @@ id_up_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo,
 // smaller classes saved).
 constexpr int kOneClassLevel = 512;
 constexpr int kOneClassLevel3 = 2048;  // 3D internal levels: one 384-thread launch up to this
+
+// kOneClassLevel3, or TSNE_ID_ONECLASS3 when set (read once)
+int one_class_level3()
+{
+    static const int v = [] {
+        const char *e = getenv("TSNE_ID_ONECLASS3");
+        const int x = e ? atoi(e) : kOneClassLevel3;
+        return x >= 0 ? x : kOneClassLevel3;
+    }();
+    return v;
+}
 constexpr int kLeafGridSmall = 8192;  // workgroups of a 3D leaf launch, 64/128 threads
 constexpr int kLeafGridLarge = 1024;  // same, 384 threads (leaves of > 128 points)
 
@@ -556,7 +568,7 @@ void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, d
             hipLaunchKernelGGL((id_up_kernel<2, 128, 48, false, T>), dim3(count), dim3(128), 0, st, h,
                                list, lb, count, one ? 0 : 64, tol);
         }
-    } else if (!list && count <= kOneClassLevel3) {
+    } else if (!list && count <= one_class_level3()) {
         hipLaunchKernelGGL((id_up_kernel<3, 384, 96, false, T>), dim3(count), dim3(384), 0, st, h, list,
                            lb, count, 0, tol);
     } else if (!list) {
