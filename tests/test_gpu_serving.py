@@ -7,6 +7,8 @@
 * Admission by device footprint: with a budget below one job's footprint, jobs run one
   at a time, in arrival order.
 * Invalid submissions are rejected without taking P.
+* A server set to the mixed H2 precision runs each job as that job alone with
+  tsne_set_h2_precision(ws, 1) (same tolerance), and differently from the plain run.
 """
 import ctypes
 import sys
@@ -44,10 +46,12 @@ def _params(backend, n, n_iter):
     return p
 
 
-def _alone(backend, P, dim, Y0, n_iter):
+def _alone(backend, P, dim, Y0, n_iter, mixed=False):
     n = P.shape[0]
     ws = backend.init_workspace(P, dim, _params(backend, n, n_iter), init_Y=Y0)
     try:
+        if mixed:
+            assert backend.lib.tsne_set_h2_precision(ws, 1) == 0
         assert backend.lib.tsne_run(ws) == 0
         Y = np.zeros((dim, n))
         backend.lib.tsne_get_embedding(ws, Y)
@@ -107,3 +111,26 @@ def test_invalid_submission_rejected(backend, jobs):
         assert rc == -1
         backend.lib.sparse_matrix_destroy(P_c)   # not taken
         assert srv.status()["queued"] == 0
+
+
+def test_mixed_precision_server_equals_mixed_alone(backend, jobs):
+    from tsne_h2pack.serving import TSNEServer
+    n_iter = 300
+    sel = [j for j in jobs if j[0].shape[0] >= 2000][:4]  # H2 jobs (N >= 2000)
+    mixed = [_alone(backend, P, dim, Y0, n_iter, mixed=True) for P, dim, Y0 in sel]
+    plain = [_alone(backend, P, dim, Y0, n_iter) for P, dim, Y0 in sel]
+    with TSNEServer(slots=4, h2_precision="mixed") as srv:
+        ids = [srv.submit(P, dim, srv.default_params(P.shape[0], n_iter=n_iter), init_Y=Y0)
+               for P, dim, Y0 in sel]
+        got = [srv.wait(j, P.shape[0], dim) for j, (P, dim, _) in zip(ids, sel)]
+    differs = 0
+    for (Ya, kla), (Yp, _), (Ym, info) in zip(mixed, plain, got):
+        rel = np.abs(Ya - Ym).max() / np.abs(Ya).max()
+        _say(f"  mixed n={info['n_point']} d={info['low_dim']} rel diff {rel:.2e} "
+             f"KL {kla:.6f}/{info['kl']:.6f}")
+        assert rel <= 1e-9
+        assert abs(kla - info["kl"]) <= 1e-9 * abs(kla)
+        differs += int(not np.array_equal(Ya, Yp))
+    assert differs > 0, "mixed jobs ran as plain"
+    with pytest.raises(ValueError):
+        TSNEServer(slots=1, h2_precision="half")
