@@ -498,6 +498,409 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     }
 }
 
+// A DPP move of a 64-bit value (two 32-bit halves); CTRL a DPP control within a row.
+template <int CTRL>
+__device__ __forceinline__ double dpp_f64(double v)
+{
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(b & 0xffffffffLL), CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int CTRL>
+__device__ __forceinline__ float dpp_f32(float v)
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
+// Sum over the eight lanes 8i..8i+7: lanes ^1 and ^2 (quad permutations), then the other
+// quad through the half-row mirror (lane j <-> 7 - j).  Every lane of the eight gets the
+// same bits (each step adds the same two operands in either order).  All eight lanes
+// must be active.
+template <typename T>
+__device__ __forceinline__ T sum8(T v)
+{
+    if constexpr (std::is_same<T, double>::value) {
+        v += dpp_f64<0xB1>(v);
+        v += dpp_f64<0x4E>(v);
+        v += dpp_f64<0x141>(v);
+    } else {
+        v += dpp_f32<0xB1>(v);
+        v += dpp_f32<0x4E>(v);
+        v += dpp_f32<0x141>(v);
+    }
+    return v;
+}
+
+// 3D ID + upward pass of one node by one workgroup of NT threads, the candidate columns
+// split into tiles: thread t holds the CT x QT block of A for candidates g CT .. g CT +
+// CT - 1 (g = t / 8) and proxy slots q8 QT .. q8 QT + QT - 1 (q8 = t % 8, QT = PMAX / 8).
+// A pivot step's dot products and residual norms are then sums over the eight lanes of
+// a group (DPP), each thread reads QT entries of the broadcast Q row instead of PMAX and
+// holds CT QT values of A instead of PMAX (no register spills at PMAX = 96).  The
+// candidates' set-up, the triangular solve and the stores run one candidate per thread.
+// Same arithmetic as id_up_node (FP64 kernel values, column scales, R entries and
+// solve; T for the Gram-Schmidt); sums are taken in another order.  Nodes with
+// m_lo < m <= MC = CT NT / 8 candidates (MC = MCAP takes any larger ones, capped).
+template <int NT, int CT, typename T>
+__device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node, int m_lo, double tol)
+{
+    constexpr int D = 3, PMAX = H2Cfg<3>::PMAX, RMAX = H2Cfg<3>::RMAX, NCH = H2Cfg<3>::NCH;
+    constexpr int QG = 8, QT = PMAX / QG, NGRP = NT / QG, MC = NGRP * CT, NW = NT / 64;
+    static_assert(QT * QG == PMAX && QT % 4 == 0 && MC <= NT && NT >= PMAX && NT % 64 == 0,
+                  "tile shape");
+    static_assert(MC <= 511, "pivot keys carry 9-bit candidate indices");
+    __shared__ float4 s_xin[MC];
+    __shared__ double s_cy[MC][D];
+    __shared__ double s_prox[PMAX][D];
+    __shared__ double s_part[NW][PMAX];
+    __shared__ double s_scale[PMAX];
+    __shared__ T s_Q[2][PMAX];
+    __shared__ float s_U[MC][RMAX];
+    __shared__ double s_R[RMAX][RMAX];
+    __shared__ int s_piv[RMAX];
+    __shared__ unsigned s_key[2][NW];
+    __shared__ T s_bv;
+    __shared__ int s_base;
+
+    const int t = threadIdx.x;
+    const int lane = t & 63, wid = t >> 6;
+    const int g = t / QG, q8 = t % QG;
+    const int ch0 = h.child0[node];
+    const bool leafn = ch0 < 0;
+    const int l = h.level[node];
+    const float4 c = h.cent[node];
+    const RootInfo root = *h.root;
+    const int kq = min(max(root.k0 - l, 0), kProxyLevels - 1);
+    const int p = min(h.proxy_cnt[kq], PMAX);
+    double px[D] = {0.0, 0.0, 0.0};
+    if (t < p)
+        for (int d = 0; d < D; d++) px[d] = h.proxy_pts[((size_t)kq * PMAX + t) * D + d];
+    int nb = 0, mt = 0, nc = 0;
+    int rk[NCH];
+    if (leafn) {
+        nb = h.nb[node];
+        mt = h.ne[node] - nb;
+    } else {
+        nc = h.nchild[node];
+#pragma unroll
+        for (int q = 0; q < NCH; q++) rk[q] = (q < nc) ? h.rank[ch0 + q] : 0;
+#pragma unroll
+        for (int q = 0; q < NCH; q++) mt += rk[q];
+    }
+    if (mt <= m_lo || (mt > MC && MC < H2Cfg<3>::MCAP)) return;
+    const int m = min(mt, MC);
+    // U region of the node (as id_up_node): R entries (FP64) during the ID, then U
+    const int part = blockIdx.x % kPoolParts;
+    const long long part_cap = pool_part_cap(h, RMAX);
+    if (t == 0)
+        s_base = leafn ? nb * 2 * RMAX
+                       : (int)(2LL * h.n * RMAX + part * part_cap +
+                               atomicAdd(&h.cnt[CNT_POOL + part * kCntSlot], 2 * m * RMAX));
+    const double cd[3] = {(double)c.x, (double)c.y, (double)c.z};
+
+    // ---- candidate t: coordinates and its row of X
+    double my[3] = {0.0, 0.0, 0.0};
+    float4 xin = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (leafn) {
+        if (t < m) {
+            const float4 y = ((const float4 *)h.Ys)[nb + t];
+            my[0] = y.x;
+            my[1] = y.y;
+            my[2] = y.z;
+            xin = make_float4(1.f, y.x - c.x, y.y - c.y, y.z - c.z);
+        }
+    } else {
+        int cj = -1, k = 0, acc = 0;
+#pragma unroll
+        for (int q = 0; q < NCH; q++) {
+            if (t >= acc && t < acc + rk[q]) {
+                cj = ch0 + q;
+                k = t - acc;
+            }
+            acc += rk[q];
+        }
+        if (cj >= 0) {
+            for (int d = 0; d < D; d++) my[d] = h.skel[((size_t)cj * RMAX + k) * D + d];
+            const float4 w = h.wc[(size_t)cj * RMAX + k];
+            const float4 cc = h.cent[cj];
+            xin = make_float4(w.x, w.y + (cc.x - c.x) * w.x, w.z + (cc.y - c.y) * w.x,
+                              w.w + (cc.z - c.z) * w.x);
+        }
+        if (nc == 1) {
+            // single child: same skeleton, U = I, coefficients moved to this centre
+            __syncthreads();
+            const int base = s_base;
+            const bool fits = (long long)base + 2LL * m * RMAX <=
+                              2LL * h.n * RMAX + (part + 1) * part_cap;
+            if (t == 0) {
+                if (!fits) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
+                h.rank[node] = m;
+                h.uoff[node] = base;
+            }
+            if (t < m) {
+                if (fits)
+                    for (int q = 0; q < m; q++) h.upool[(size_t)base + (size_t)t * m + q] = (q == t) ? 1.f : 0.f;
+                for (int d = 0; d < D; d++) h.skel[((size_t)node * RMAX + t) * D + d] = my[d];
+                h.skelf[(size_t)node * RMAX + t] =
+                    make_float4((float)my[0], (float)my[1], (float)my[2], 0.f);
+                h.wc[(size_t)node * RMAX + t] = xin;
+            }
+            return;
+        }
+    }
+    if (t < MC) {
+        s_xin[t] = xin;
+        for (int d = 0; d < D; d++) s_cy[t][d] = my[d];
+    }
+    if (t < PMAX)
+        for (int d = 0; d < D; d++) s_prox[t][d] = cd[d] + px[d];
+    for (int e = t; e < RMAX * RMAX; e += NT) s_R[e / RMAX][e % RMAX] = (e / RMAX == e % RMAX) ? 1.0 : 0.0;
+    __syncthreads();
+
+    // ---- this thread's tile of A, then column norms over the candidates
+    T a[CT][QT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+        const int ci = g * CT + ct;
+#pragma unroll
+        for (int j = 0; j < QT; j++)
+            a[ct][j] = (ci < m) ? (T)kern64(s_cy[ci], s_prox[q8 * QT + j], D) : T(0);
+    }
+    {
+        // a column's sum of squares: over the thread's CT candidates, then over the eight
+        // groups of the wavefront (lanes ^8, ^16, ^32), then over the wavefronts
+        double cs[QT];
+#pragma unroll
+        for (int j = 0; j < QT; j++) {
+            double s = 0.0;
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) s = fma((double)a[ct][j], (double)a[ct][j], s);
+            cs[j] = s;
+        }
+#pragma unroll
+        for (int j = 0; j < QT; j++) {
+            cs[j] += __shfl_xor(cs[j], 8, 64);
+            cs[j] += __shfl_xor(cs[j], 16, 64);
+            cs[j] += __shfl_xor(cs[j], 32, 64);
+        }
+        if (lane < QG)
+#pragma unroll
+            for (int j = 0; j < QT; j++) s_part[wid][q8 * QT + j] = cs[j];
+    }
+    __syncthreads();
+    for (int q = t; q < PMAX; q += NT) {
+        double s2 = 0.0;
+        for (int w = 0; w < NW; w++) s2 += s_part[w][q];
+        s_scale[q] = (q < p && s2 > 0.0) ? 1.0 / sqrt(s2) : 0.0;
+    }
+    __syncthreads();
+    T nrm[CT];
+    unsigned live = 0;  // bit ct: candidate g CT + ct exists and is not yet a pivot
+    {
+        T sc[QT];
+#pragma unroll
+        for (int j = 0; j < QT; j++) sc[j] = (T)s_scale[q8 * QT + j];
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+            T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+#pragma unroll
+            for (int j = 0; j < QT; j += 4) {
+                a[ct][j] *= sc[j];
+                a[ct][j + 1] *= sc[j + 1];
+                a[ct][j + 2] *= sc[j + 2];
+                a[ct][j + 3] *= sc[j + 3];
+                n0 = fma(a[ct][j], a[ct][j], n0);
+                n1 = fma(a[ct][j + 1], a[ct][j + 1], n1);
+                n2 = fma(a[ct][j + 2], a[ct][j + 2], n2);
+                n3 = fma(a[ct][j + 3], a[ct][j + 3], n3);
+            }
+            nrm[ct] = sum8((n0 + n1) + (n2 + n3));
+            if (g * CT + ct < m) live |= 1u << ct;
+        }
+    }
+
+    // ---- column-pivoted MGS; R[k][i] of candidate i goes to rpool[k * m + i] (written by
+    // lane 0 of the candidate's group)
+    const int rbase = s_base;
+    const bool rfits = leafn || (long long)rbase + 2LL * m * RMAX <= 2LL * h.n * RMAX + (part + 1) * part_cap;
+    double *rpool = reinterpret_cast<double *>(h.upool + rbase);
+    T nrm0 = 0;
+    int r = 0;
+    const int rlim = min(RMAX, min(m, p));
+    const T tol2 = (T)(tol * tol);
+    for (int k = 0; k < RMAX; k++) {
+        unsigned key = 0;
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++)
+            key = max(key, pivot_key((double)nrm[ct], g * CT + ct, (live >> ct) & 1u));
+        key = wave_max_u32(key);
+        if (NW > 1) {
+            if (lane == 0) s_key[k & 1][wid] = key;
+            __syncthreads();
+#pragma unroll
+            for (int w = 0; w < NW; w++) key = max(key, s_key[k & 1][w]);
+        }
+        const int bi = key ? 511 - (int)(key & 0x1FFu) : -1;
+        const int gb = bi >= 0 ? bi / CT : -1;
+        const int cb = bi - gb * CT;
+        // the pivot's group publishes its exact norm and, speculatively, its Q row
+        if (g == gb) {
+            T nb_ = nrm[0];
+#pragma unroll
+            for (int ct = 1; ct < CT; ct++)
+                if (ct == cb) nb_ = nrm[ct];
+            const T inv = (T)(1.0 / sqrt((double)nb_));
+#pragma unroll
+            for (int j = 0; j < QT; j++) {
+                T v = a[0][j];
+#pragma unroll
+                for (int ct = 1; ct < CT; ct++)
+                    if (ct == cb) v = a[ct][j];
+                s_Q[k & 1][q8 * QT + j] = v * inv;
+            }
+            if (q8 == 0) {
+                s_bv = nb_;
+                s_piv[k] = bi;
+            }
+        }
+        __syncthreads();
+        const T bv = key ? s_bv : T(0);
+        if (k == 0) nrm0 = bv;
+        if (k >= rlim || !(bv > tol2 * nrm0) || !(bv > T(0))) break;
+        if (g == gb) {
+            live &= ~(1u << cb);
+            if (q8 == 0) s_R[k][k] = sqrt((double)bv);
+        }
+        if (live) {  // the same in the group's eight lanes
+            T qv[QT];
+#pragma unroll
+            for (int j = 0; j < QT; j++) qv[j] = s_Q[k & 1][q8 * QT + j];
+            T dot[CT];
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                T d0 = 0, d1 = 0, d2 = 0, d3 = 0;
+#pragma unroll
+                for (int j = 0; j < QT; j += 4) {
+                    d0 = fma(qv[j], a[ct][j], d0);
+                    d1 = fma(qv[j + 1], a[ct][j + 1], d1);
+                    d2 = fma(qv[j + 2], a[ct][j + 2], d2);
+                    d3 = fma(qv[j + 3], a[ct][j + 3], d3);
+                }
+                dot[ct] = sum8((d0 + d1) + (d2 + d3));  // R[k][g CT + ct]
+            }
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                if (!((live >> ct) & 1u)) continue;
+                if (q8 == 0 && rfits) rpool[(size_t)k * m + g * CT + ct] = (double)dot[ct];
+                T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+#pragma unroll
+                for (int j = 0; j < QT; j += 4) {
+                    a[ct][j] = fma(-dot[ct], qv[j], a[ct][j]);
+                    a[ct][j + 1] = fma(-dot[ct], qv[j + 1], a[ct][j + 1]);
+                    a[ct][j + 2] = fma(-dot[ct], qv[j + 2], a[ct][j + 2]);
+                    a[ct][j + 3] = fma(-dot[ct], qv[j + 3], a[ct][j + 3]);
+                    n0 = fma(a[ct][j], a[ct][j], n0);
+                    n1 = fma(a[ct][j + 1], a[ct][j + 1], n1);
+                    n2 = fma(a[ct][j + 2], a[ct][j + 2], n2);
+                    n3 = fma(a[ct][j + 3], a[ct][j + 3], n3);
+                }
+                nrm[ct] = sum8((n0 + n1) + (n2 + n3));
+            }
+        }
+        r = k + 1;
+        if (k == RMAX - 1 && t == 0) atomicOr(&h.flags[FLAG_RANK_CAP], 1);
+    }
+    __syncthreads();  // every R entry of the pool written
+
+    // ---- one candidate per thread: its column of R, the solve, U
+    int kk = -1;
+    for (int k = 0; k < r; k++)
+        if (s_piv[k] == t) kk = k;
+    auto tail = [&](auto rb) {
+        constexpr int RB = decltype(rb)::value;
+        double rt[RB];
+        const int rows = (kk >= 0) ? kk : r;
+#pragma unroll
+        for (int k = 0; k < RB; k++)
+            rt[k] = (t < m && k < rows && rfits) ? rpool[(size_t)k * m + t] : 0.0;
+        if (kk >= 0) {
+#pragma unroll
+            for (int j = 0; j < RB; j++)
+                if (j < kk) s_R[j][kk] = rt[j];
+        }
+        __syncthreads();
+        if (t < RB) s_part[0][t] = 1.0 / s_R[t][t];
+        __syncthreads();
+        if (kk >= 0) {
+#pragma unroll
+            for (int k = 0; k < RB; k++) rt[k] = (k == kk) ? 1.0 : 0.0;
+        } else if (t < m) {
+#pragma unroll
+            for (int k = RB - 1; k >= 0; k--) {
+                double v = rt[k];
+#pragma unroll
+                for (int j = k + 1; j < RB; j++) v = fma(-s_R[k][j], rt[j], v);
+                rt[k] = v * s_part[0][k];
+            }
+        }
+        if (t < MC)
+#pragma unroll
+            for (int k = 0; k < RB; k++) s_U[t][k] = (float)rt[k];
+    };
+    if (r <= RMAX / 4)
+        tail(std::integral_constant<int, RMAX / 4>{});
+    else if (r <= RMAX / 2)
+        tail(std::integral_constant<int, RMAX / 2>{});
+    else
+        tail(std::integral_constant<int, RMAX>{});
+
+    // ---- store U (pool), skeleton, and the upward coefficients
+    if (t == 0) {
+        if (!rfits) atomicOr(&h.flags[FLAG_POOL_OVF], 1);
+        h.rank[node] = r;
+        h.uoff[node] = s_base;
+    }
+    if (kk >= 0) {
+        for (int d = 0; d < D; d++) h.skel[((size_t)node * RMAX + kk) * D + d] = my[d];
+        h.skelf[(size_t)node * RMAX + kk] = make_float4((float)my[0], (float)my[1], (float)my[2], 0.f);
+    }
+    __syncthreads();
+    const int base = s_base;
+    if (rfits)
+        for (int e = t; e < m * r; e += NT) h.upool[(size_t)base + e] = s_U[e / r][e % r];
+    if (t < r) {
+        float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (int u = 0; u < m; u++) {
+            const float uk = s_U[u][t];
+            const float4 x = s_xin[u];
+            w.x = fmaf(uk, x.x, w.x);
+            w.y = fmaf(uk, x.y, w.y);
+            w.z = fmaf(uk, x.z, w.z);
+            w.w = fmaf(uk, x.w, w.w);
+        }
+        h.wc[(size_t)node * RMAX + t] = w;
+    }
+}
+
+template <int NT, int CT, bool STRIDE, typename T>
+__global__ void __launch_bounds__(NT)
+id_tile3_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo, double tol)
+{
+    if constexpr (STRIDE) {
+        for (int b = blockIdx.x; b < count; b += gridDim.x) {
+            const int node = list ? list[b] : lb + b;
+            __syncthreads();  // the previous node's LDS reads are complete
+            if (!list && h.child0[node] < 0) continue;
+            id_up_node_tile3<NT, CT, T>(h, node, m_lo, tol);
+        }
+    } else {
+        const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
+        if (!list && h.child0[node] < 0) return;
+        id_up_node_tile3<NT, CT, T>(h, node, m_lo, tol);
+    }
+}
+
 // list != nullptr: the nodes list[0..count) (the leaves); otherwise the internal nodes
 // among [lb, lb + count).  One workgroup per node at a time, in one size class.  STRIDE:
 // a grid smaller than `count` walks the nodes with a stride, so a launch whose class
@@ -546,6 +949,16 @@ int one_class_level3()
     }();
     return v;
 }
+// 3D IDs in tiles (id_tile3_kernel), unless TSNE_ID_TILE3=0 (the one-thread-per-candidate
+// kernel) (read once)
+bool tile3()
+{
+    static const bool v = [] {
+        const char *e = getenv("TSNE_ID_TILE3");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
 constexpr int kLeafGridSmall = 8192;  // workgroups of a 3D leaf launch, 64/128 threads
 constexpr int kLeafGridLarge = 1024;  // same, 384 threads (leaves of > 128 points)
 
@@ -567,6 +980,27 @@ void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, d
                                    list, lb, count, 0, tol);
             hipLaunchKernelGGL((id_up_kernel<2, 128, 48, false, T>), dim3(count), dim3(128), 0, st, h,
                                list, lb, count, one ? 0 : 64, tol);
+        }
+    } else if (tile3()) {
+        // 3D in tiles: classes of 64 (128 threads), 128 (256) and 384 candidates (768)
+        if (!list && count <= one_class_level3()) {
+            hipLaunchKernelGGL((id_tile3_kernel<768, 4, false, T>), dim3(count), dim3(768), 0, st, h,
+                               list, lb, count, 0, tol);
+        } else if (!list) {
+            hipLaunchKernelGGL((id_tile3_kernel<128, 4, false, T>), dim3(count), dim3(128), 0, st, h,
+                               list, lb, count, 0, tol);
+            hipLaunchKernelGGL((id_tile3_kernel<256, 4, false, T>), dim3(count), dim3(256), 0, st, h,
+                               list, lb, count, 64, tol);
+            hipLaunchKernelGGL((id_tile3_kernel<768, 4, false, T>), dim3(count), dim3(768), 0, st, h,
+                               list, lb, count, 128, tol);
+        } else {
+            const int gs = std::min(count, kLeafGridSmall), gl = std::min(count, kLeafGridLarge);
+            hipLaunchKernelGGL((id_tile3_kernel<128, 4, true, T>), dim3(gs), dim3(128), 0, st, h,
+                               list, lb, count, 0, tol);
+            hipLaunchKernelGGL((id_tile3_kernel<256, 4, true, T>), dim3(gs), dim3(256), 0, st, h,
+                               list, lb, count, 64, tol);
+            hipLaunchKernelGGL((id_tile3_kernel<768, 4, true, T>), dim3(gl), dim3(768), 0, st, h,
+                               list, lb, count, 128, tol);
         }
     } else if (!list && count <= one_class_level3()) {
         hipLaunchKernelGGL((id_up_kernel<3, 384, 96, false, T>), dim3(count), dim3(384), 0, st, h, list,
