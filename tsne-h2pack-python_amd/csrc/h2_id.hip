@@ -513,42 +513,44 @@ __device__ __forceinline__ float dpp_f32(float v)
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
 }
 
-// Sum over the eight lanes 8i..8i+7: lanes ^1 and ^2 (quad permutations), then the other
-// quad through the half-row mirror (lane j <-> 7 - j).  Every lane of the eight gets the
-// same bits (each step adds the same two operands in either order).  All eight lanes
-// must be active.
-template <typename T>
-__device__ __forceinline__ T sum8(T v)
+// Sum over the QG (4 or 8) lanes QG i .. QG i + QG - 1: lanes ^1 and ^2 (quad
+// permutations), then for eight lanes the other quad through the half-row mirror (lane
+// j <-> 7 - j).  Every lane of the group gets the same bits (each step adds the same two
+// operands in either order).  All lanes of the group must be active.
+template <int QG, typename T>
+__device__ __forceinline__ T sum_group(T v)
 {
+    static_assert(QG == 4 || QG == 8, "group width");
     if constexpr (std::is_same<T, double>::value) {
         v += dpp_f64<0xB1>(v);
         v += dpp_f64<0x4E>(v);
-        v += dpp_f64<0x141>(v);
+        if constexpr (QG == 8) v += dpp_f64<0x141>(v);
     } else {
         v += dpp_f32<0xB1>(v);
         v += dpp_f32<0x4E>(v);
-        v += dpp_f32<0x141>(v);
+        if constexpr (QG == 8) v += dpp_f32<0x141>(v);
     }
     return v;
 }
 
-// 3D ID + upward pass of one node by one workgroup of NT threads, the candidate columns
+// ID + upward pass of one node by one workgroup of NT threads, the candidate columns
 // split into tiles: thread t holds the CT x QT block of A for candidates g CT .. g CT +
-// CT - 1 (g = t / 8) and proxy slots q8 QT .. q8 QT + QT - 1 (q8 = t % 8, QT = PMAX / 8).
-// A pivot step's dot products and residual norms are then sums over the eight lanes of
-// a group (DPP), each thread reads QT entries of the broadcast Q row instead of PMAX and
-// holds CT QT values of A instead of PMAX (no register spills at PMAX = 96).  The
-// candidates' set-up, the triangular solve and the stores run one candidate per thread.
-// Same arithmetic as id_up_node (FP64 kernel values, column scales, R entries and
-// solve; T for the Gram-Schmidt); sums are taken in another order.  Nodes with
-// m_lo < m <= MC = CT NT / 8 candidates (MC = MCAP takes any larger ones, capped).
-template <int NT, int CT, typename T>
-__device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node, int m_lo, double tol)
+// CT - 1 (g = t / QG) and proxy slots qg QT .. qg QT + QT - 1 (qg = t % QG, QT = PQ / QG).
+// A pivot step's dot products and residual norms are then sums over the QG lanes of a
+// group (DPP), each thread reads QT entries of the broadcast Q row instead of PQ and
+// holds CT QT values of A instead of PQ (in 3D, QG = 8 and PQ = 96: no register spills).
+// The candidates' set-up, the triangular solve and the stores run one candidate per
+// thread.  Same arithmetic as id_up_node (FP64 kernel values, column scales, R entries and
+// solve; T for the Gram-Schmidt); sums are taken in another order.  PQ proxy slots are
+// processed (at least the largest proxy count of any level).  Nodes with m_lo < m <=
+// MC = CT NT / QG candidates (MC = MCAP takes any larger ones, capped).
+template <int D, int NT, int CT, int QG, int PQ, typename T>
+__device__ __forceinline__ void id_up_node_tile(const H2Dev &h, const int node, int m_lo, double tol)
 {
-    constexpr int D = 3, PMAX = H2Cfg<3>::PMAX, RMAX = H2Cfg<3>::RMAX, NCH = H2Cfg<3>::NCH;
-    constexpr int QG = 8, QT = PMAX / QG, NGRP = NT / QG, MC = NGRP * CT, NW = NT / 64;
-    static_assert(QT * QG == PMAX && QT % 4 == 0 && MC <= NT && NT >= PMAX && NT % 64 == 0,
-                  "tile shape");
+    constexpr int PMAX = PQ, PSTRIDE = H2Cfg<D>::PMAX, RMAX = H2Cfg<D>::RMAX, NCH = H2Cfg<D>::NCH;
+    constexpr int QT = PMAX / QG, NGRP = NT / QG, MC = NGRP * CT, NW = NT / 64;
+    static_assert(QT * QG == PMAX && QT % 4 == 0 && MC <= NT && NT >= PMAX && NT % 64 == 0 &&
+                  PMAX <= PSTRIDE, "tile shape");
     static_assert(MC <= 511, "pivot keys carry 9-bit candidate indices");
     __shared__ float4 s_xin[MC];
     __shared__ double s_cy[MC][D];
@@ -565,7 +567,7 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
 
     const int t = threadIdx.x;
     const int lane = t & 63, wid = t >> 6;
-    const int g = t / QG, q8 = t % QG;
+    const int g = t / QG, qg = t % QG;
     const int ch0 = h.child0[node];
     const bool leafn = ch0 < 0;
     const int l = h.level[node];
@@ -573,9 +575,9 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
     const RootInfo root = *h.root;
     const int kq = min(max(root.k0 - l, 0), kProxyLevels - 1);
     const int p = min(h.proxy_cnt[kq], PMAX);
-    double px[D] = {0.0, 0.0, 0.0};
+    double px[3] = {0.0, 0.0, 0.0};
     if (t < p)
-        for (int d = 0; d < D; d++) px[d] = h.proxy_pts[((size_t)kq * PMAX + t) * D + d];
+        for (int d = 0; d < D; d++) px[d] = h.proxy_pts[((size_t)kq * PSTRIDE + t) * D + d];
     int nb = 0, mt = 0, nc = 0;
     int rk[NCH];
     if (leafn) {
@@ -588,7 +590,7 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
 #pragma unroll
         for (int q = 0; q < NCH; q++) mt += rk[q];
     }
-    if (mt <= m_lo || (mt > MC && MC < H2Cfg<3>::MCAP)) return;
+    if (mt <= m_lo || (mt > MC && MC < H2Cfg<D>::MCAP)) return;
     const int m = min(mt, MC);
     // U region of the node (as id_up_node): R entries (FP64) during the ID, then U
     const int part = blockIdx.x % kPoolParts;
@@ -604,11 +606,14 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
     float4 xin = make_float4(0.f, 0.f, 0.f, 0.f);
     if (leafn) {
         if (t < m) {
-            const float4 y = ((const float4 *)h.Ys)[nb + t];
+            const VT<D> y = ((const VT<D> *)h.Ys)[nb + t];
             my[0] = y.x;
             my[1] = y.y;
-            my[2] = y.z;
-            xin = make_float4(1.f, y.x - c.x, y.y - c.y, y.z - c.z);
+            xin = make_float4(1.f, y.x - c.x, y.y - c.y, 0.f);
+            if constexpr (D == 3) {
+                my[2] = y.z;
+                xin.w = y.z - c.z;
+            }
         }
     } else {
         int cj = -1, k = 0, acc = 0;
@@ -665,11 +670,11 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
         const int ci = g * CT + ct;
 #pragma unroll
         for (int j = 0; j < QT; j++)
-            a[ct][j] = (ci < m) ? (T)kern64(s_cy[ci], s_prox[q8 * QT + j], D) : T(0);
+            a[ct][j] = (ci < m) ? (T)kern64(s_cy[ci], s_prox[qg * QT + j], D) : T(0);
     }
     {
-        // a column's sum of squares: over the thread's CT candidates, then over the eight
-        // groups of the wavefront (lanes ^8, ^16, ^32), then over the wavefronts
+        // a column's sum of squares: over the thread's CT candidates, then over the groups
+        // of the wavefront (lanes ^QG, ^2QG, ..., ^32), then over the wavefronts
         double cs[QT];
 #pragma unroll
         for (int j = 0; j < QT; j++) {
@@ -679,14 +684,12 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
             cs[j] = s;
         }
 #pragma unroll
-        for (int j = 0; j < QT; j++) {
-            cs[j] += __shfl_xor(cs[j], 8, 64);
-            cs[j] += __shfl_xor(cs[j], 16, 64);
-            cs[j] += __shfl_xor(cs[j], 32, 64);
-        }
+        for (int j = 0; j < QT; j++)
+#pragma unroll
+            for (int o = QG; o < 64; o <<= 1) cs[j] += __shfl_xor(cs[j], o, 64);
         if (lane < QG)
 #pragma unroll
-            for (int j = 0; j < QT; j++) s_part[wid][q8 * QT + j] = cs[j];
+            for (int j = 0; j < QT; j++) s_part[wid][qg * QT + j] = cs[j];
     }
     __syncthreads();
     for (int q = t; q < PMAX; q += NT) {
@@ -700,7 +703,7 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
     {
         T sc[QT];
 #pragma unroll
-        for (int j = 0; j < QT; j++) sc[j] = (T)s_scale[q8 * QT + j];
+        for (int j = 0; j < QT; j++) sc[j] = (T)s_scale[qg * QT + j];
 #pragma unroll
         for (int ct = 0; ct < CT; ct++) {
             T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
@@ -715,7 +718,7 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
                 n2 = fma(a[ct][j + 2], a[ct][j + 2], n2);
                 n3 = fma(a[ct][j + 3], a[ct][j + 3], n3);
             }
-            nrm[ct] = sum8((n0 + n1) + (n2 + n3));
+            nrm[ct] = sum_group<QG>((n0 + n1) + (n2 + n3));
             if (g * CT + ct < m) live |= 1u << ct;
         }
     }
@@ -757,9 +760,9 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
 #pragma unroll
                 for (int ct = 1; ct < CT; ct++)
                     if (ct == cb) v = a[ct][j];
-                s_Q[k & 1][q8 * QT + j] = v * inv;
+                s_Q[k & 1][qg * QT + j] = v * inv;
             }
-            if (q8 == 0) {
+            if (qg == 0) {
                 s_bv = nb_;
                 s_piv[k] = bi;
             }
@@ -770,12 +773,12 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
         if (k >= rlim || !(bv > tol2 * nrm0) || !(bv > T(0))) break;
         if (g == gb) {
             live &= ~(1u << cb);
-            if (q8 == 0) s_R[k][k] = sqrt((double)bv);
+            if (qg == 0) s_R[k][k] = sqrt((double)bv);
         }
         if (live) {  // the same in the group's eight lanes
             T qv[QT];
 #pragma unroll
-            for (int j = 0; j < QT; j++) qv[j] = s_Q[k & 1][q8 * QT + j];
+            for (int j = 0; j < QT; j++) qv[j] = s_Q[k & 1][qg * QT + j];
             T dot[CT];
 #pragma unroll
             for (int ct = 0; ct < CT; ct++) {
@@ -787,12 +790,12 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
                     d2 = fma(qv[j + 2], a[ct][j + 2], d2);
                     d3 = fma(qv[j + 3], a[ct][j + 3], d3);
                 }
-                dot[ct] = sum8((d0 + d1) + (d2 + d3));  // R[k][g CT + ct]
+                dot[ct] = sum_group<QG>((d0 + d1) + (d2 + d3));  // R[k][g CT + ct]
             }
 #pragma unroll
             for (int ct = 0; ct < CT; ct++) {
                 if (!((live >> ct) & 1u)) continue;
-                if (q8 == 0 && rfits) rpool[(size_t)k * m + g * CT + ct] = (double)dot[ct];
+                if (qg == 0 && rfits) rpool[(size_t)k * m + g * CT + ct] = (double)dot[ct];
                 T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
 #pragma unroll
                 for (int j = 0; j < QT; j += 4) {
@@ -805,7 +808,7 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
                     n2 = fma(a[ct][j + 2], a[ct][j + 2], n2);
                     n3 = fma(a[ct][j + 3], a[ct][j + 3], n3);
                 }
-                nrm[ct] = sum8((n0 + n1) + (n2 + n3));
+                nrm[ct] = sum_group<QG>((n0 + n1) + (n2 + n3));
             }
         }
         r = k + 1;
@@ -883,21 +886,21 @@ __device__ __forceinline__ void id_up_node_tile3(const H2Dev &h, const int node,
     }
 }
 
-template <int NT, int CT, bool STRIDE, typename T>
+template <int D, int NT, int CT, int QG, int PQ, bool STRIDE, typename T>
 __global__ void __launch_bounds__(NT)
-id_tile3_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo, double tol)
+id_tile_kernel(H2Dev h, const int *__restrict__ list, int lb, int count, int m_lo, double tol)
 {
     if constexpr (STRIDE) {
         for (int b = blockIdx.x; b < count; b += gridDim.x) {
             const int node = list ? list[b] : lb + b;
             __syncthreads();  // the previous node's LDS reads are complete
             if (!list && h.child0[node] < 0) continue;
-            id_up_node_tile3<NT, CT, T>(h, node, m_lo, tol);
+            id_up_node_tile<D, NT, CT, QG, PQ, T>(h, node, m_lo, tol);
         }
     } else {
         const int node = list ? list[blockIdx.x] : lb + blockIdx.x;
         if (!list && h.child0[node] < 0) return;
-        id_up_node_tile3<NT, CT, T>(h, node, m_lo, tol);
+        id_up_node_tile<D, NT, CT, QG, PQ, T>(h, node, m_lo, tol);
     }
 }
 
@@ -949,12 +952,20 @@ int one_class_level3()
     }();
     return v;
 }
-// 3D IDs in tiles (id_tile3_kernel), unless TSNE_ID_TILE3=0 (the one-thread-per-candidate
-// kernel) (read once)
+// IDs in tiles (id_tile_kernel), unless TSNE_ID_TILE3=0 (3D) / TSNE_ID_TILE2=0 (2D) select
+// the one-thread-per-candidate kernel (read once)
 bool tile3()
 {
     static const bool v = [] {
         const char *e = getenv("TSNE_ID_TILE3");
+        return !(e && e[0] == '0');
+    }();
+    return v;
+}
+bool tile2()
+{
+    static const bool v = [] {
+        const char *e = getenv("TSNE_ID_TILE2");
         return !(e && e[0] == '0');
     }();
     return v;
@@ -966,7 +977,24 @@ template <typename T>
 void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, double tol, int pq,
                     hipStream_t st)
 {
-    if (D == 2) {
+    if (D == 2 && tile2()) {
+        // 2D in tiles of 2 candidates x PQ/4 proxy slots: classes of 64 (128 threads) and
+        // 128 candidates (256 threads)
+        const bool one = count <= kOneClassLevel;
+        if (pq <= 32) {
+            if (!one)
+                hipLaunchKernelGGL((id_tile_kernel<2, 128, 2, 4, 32, false, T>), dim3(count), dim3(128), 0,
+                                   st, h, list, lb, count, 0, tol);
+            hipLaunchKernelGGL((id_tile_kernel<2, 256, 2, 4, 32, false, T>), dim3(count), dim3(256), 0, st,
+                               h, list, lb, count, one ? 0 : 64, tol);
+        } else {
+            if (!one)
+                hipLaunchKernelGGL((id_tile_kernel<2, 128, 2, 4, 48, false, T>), dim3(count), dim3(128), 0,
+                                   st, h, list, lb, count, 0, tol);
+            hipLaunchKernelGGL((id_tile_kernel<2, 256, 2, 4, 48, false, T>), dim3(count), dim3(256), 0, st,
+                               h, list, lb, count, one ? 0 : 64, tol);
+        }
+    } else if (D == 2) {
         const bool one = count <= kOneClassLevel;
         if (pq <= 32) {
             if (!one)
@@ -984,22 +1012,22 @@ void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, d
     } else if (tile3()) {
         // 3D in tiles: classes of 64 (128 threads), 128 (256) and 384 candidates (768)
         if (!list && count <= one_class_level3()) {
-            hipLaunchKernelGGL((id_tile3_kernel<768, 4, false, T>), dim3(count), dim3(768), 0, st, h,
+            hipLaunchKernelGGL((id_tile_kernel<3, 768, 4, 8, 96, false, T>), dim3(count), dim3(768), 0, st, h,
                                list, lb, count, 0, tol);
         } else if (!list) {
-            hipLaunchKernelGGL((id_tile3_kernel<128, 4, false, T>), dim3(count), dim3(128), 0, st, h,
+            hipLaunchKernelGGL((id_tile_kernel<3, 128, 4, 8, 96, false, T>), dim3(count), dim3(128), 0, st, h,
                                list, lb, count, 0, tol);
-            hipLaunchKernelGGL((id_tile3_kernel<256, 4, false, T>), dim3(count), dim3(256), 0, st, h,
+            hipLaunchKernelGGL((id_tile_kernel<3, 256, 4, 8, 96, false, T>), dim3(count), dim3(256), 0, st, h,
                                list, lb, count, 64, tol);
-            hipLaunchKernelGGL((id_tile3_kernel<768, 4, false, T>), dim3(count), dim3(768), 0, st, h,
+            hipLaunchKernelGGL((id_tile_kernel<3, 768, 4, 8, 96, false, T>), dim3(count), dim3(768), 0, st, h,
                                list, lb, count, 128, tol);
         } else {
             const int gs = std::min(count, kLeafGridSmall), gl = std::min(count, kLeafGridLarge);
-            hipLaunchKernelGGL((id_tile3_kernel<128, 4, true, T>), dim3(gs), dim3(128), 0, st, h,
+            hipLaunchKernelGGL((id_tile_kernel<3, 128, 4, 8, 96, true, T>), dim3(gs), dim3(128), 0, st, h,
                                list, lb, count, 0, tol);
-            hipLaunchKernelGGL((id_tile3_kernel<256, 4, true, T>), dim3(gs), dim3(256), 0, st, h,
+            hipLaunchKernelGGL((id_tile_kernel<3, 256, 4, 8, 96, true, T>), dim3(gs), dim3(256), 0, st, h,
                                list, lb, count, 64, tol);
-            hipLaunchKernelGGL((id_tile3_kernel<768, 4, true, T>), dim3(gl), dim3(768), 0, st, h,
+            hipLaunchKernelGGL((id_tile_kernel<3, 768, 4, 8, 96, true, T>), dim3(gl), dim3(768), 0, st, h,
                                list, lb, count, 128, tol);
         }
     } else if (!list && count <= one_class_level3()) {
