@@ -498,6 +498,15 @@ __device__ __forceinline__ void id_up_node(const H2Dev &h, const int node, int m
     }
 }
 
+// FP64 residual norms of the tiled 3D ID: downdated per step (true) or recomputed from the
+// updated column (TSNE_ID_NO_DOWNDATE at compile time).  2D recomputes them (the step is
+// short there, and the downdated form costs the 64-candidate class registers).
+#ifdef TSNE_ID_NO_DOWNDATE
+constexpr bool kDowndate = false;
+#else
+constexpr bool kDowndate = true;
+#endif
+
 // A DPP move of a 64-bit value (two 32-bit halves); CTRL a DPP control within a row.
 template <int CTRL>
 __device__ __forceinline__ double dpp_f64(double v)
@@ -796,19 +805,29 @@ __device__ __forceinline__ void id_up_node_tile(const H2Dev &h, const int node, 
             for (int ct = 0; ct < CT; ct++) {
                 if (!((live >> ct) & 1u)) continue;
                 if (qg == 0 && rfits) rpool[(size_t)k * m + g * CT + ct] = (double)dot[ct];
-                T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+                if constexpr (kDowndate && D == 3 && std::is_same<T, double>::value) {
+                    // FP64: the residual norm downdated, |a - dot q|^2 = |a|^2 - dot^2 (q
+                    // of unit norm), as LAPACK's pivoted QR does.  The pivoting stops at a
+                    // residual ratio of tol^2 >= 1e-6, where the downdate's rounding
+                    // (~1e-16 of the first norm per step) is 1e-9 of the norms compared.
 #pragma unroll
-                for (int j = 0; j < QT; j += 4) {
-                    a[ct][j] = fma(-dot[ct], qv[j], a[ct][j]);
-                    a[ct][j + 1] = fma(-dot[ct], qv[j + 1], a[ct][j + 1]);
-                    a[ct][j + 2] = fma(-dot[ct], qv[j + 2], a[ct][j + 2]);
-                    a[ct][j + 3] = fma(-dot[ct], qv[j + 3], a[ct][j + 3]);
-                    n0 = fma(a[ct][j], a[ct][j], n0);
-                    n1 = fma(a[ct][j + 1], a[ct][j + 1], n1);
-                    n2 = fma(a[ct][j + 2], a[ct][j + 2], n2);
-                    n3 = fma(a[ct][j + 3], a[ct][j + 3], n3);
+                    for (int j = 0; j < QT; j++) a[ct][j] = fma(-dot[ct], qv[j], a[ct][j]);
+                    nrm[ct] = fma(-dot[ct], dot[ct], nrm[ct]);
+                } else {
+                    T n0 = 0, n1 = 0, n2 = 0, n3 = 0;
+#pragma unroll
+                    for (int j = 0; j < QT; j += 4) {
+                        a[ct][j] = fma(-dot[ct], qv[j], a[ct][j]);
+                        a[ct][j + 1] = fma(-dot[ct], qv[j + 1], a[ct][j + 1]);
+                        a[ct][j + 2] = fma(-dot[ct], qv[j + 2], a[ct][j + 2]);
+                        a[ct][j + 3] = fma(-dot[ct], qv[j + 3], a[ct][j + 3]);
+                        n0 = fma(a[ct][j], a[ct][j], n0);
+                        n1 = fma(a[ct][j + 1], a[ct][j + 1], n1);
+                        n2 = fma(a[ct][j + 2], a[ct][j + 2], n2);
+                        n3 = fma(a[ct][j + 3], a[ct][j + 3], n3);
+                    }
+                    nrm[ct] = sum_group<QG>((n0 + n1) + (n2 + n3));
                 }
-                nrm[ct] = sum_group<QG>((n0 + n1) + (n2 + n3));
             }
         }
         r = k + 1;
