@@ -434,11 +434,13 @@ def _job_params(p, sp):
     return p
 
 
-def serving_bench(n_jobs, slots_list, seq_sample, load=0.7, precision="plain"):
+def serving_bench(n_jobs, slots_list, seq_sample, load=0.7, precision="plain", autotune=True):
     """Serving workload: capacity (all jobs queued at once), then Poisson arrivals at
     `load` x capacity with per-job latency (arrival -> result); baseline: one job at a
     time through tsne_init_with_P + tsne_run (sequential plain runs) on a sample.
-    precision: the H2 precision of the served jobs (the baseline stays plain)."""
+    precision: the H2 precision of the served jobs (the baseline stays plain).
+    autotune: first tune the launch shapes of every size bucket of the mix (untimed; the
+    table then serves the baseline and the served jobs alike)."""
     import tempfile
     import shutil
     from tsne_h2pack._backend import get_backend
@@ -453,6 +455,15 @@ def serving_bench(n_jobs, slots_list, seq_sample, load=0.7, precision="plain"):
         shutil.rmtree(tmp, ignore_errors=True)
     be = get_backend()
     lib = be.lib
+    tuned = {}
+    if autotune:
+        from tsne_h2pack.serving import autotune_sizes
+        t0 = time.perf_counter()
+        dims = sorted({sp["dim"] for sp in specs})
+        tuned = {f"{d}d_b{b}": v for (d, b), v in
+                 autotune_sizes([sp["n"] for sp in specs], dims=dims).items()}
+        log(f"serving: autotuned {len(tuned)} (dimension, size bucket) launch shapes in "
+            f"{time.perf_counter() - t0:.1f}s: {tuned}")
     # sequential baseline on every (n_jobs // seq_sample)-th job
     sample = sorted(set(np.linspace(0, n_jobs - 1, min(seq_sample, n_jobs)).astype(int).tolist()))
     t_seq, kl_seq = 0.0, {}
@@ -531,6 +542,7 @@ def serving_bench(n_jobs, slots_list, seq_sample, load=0.7, precision="plain"):
                     "p50_s": float(np.percentile(lat, 50)),
                     "p99_s": float(np.percentile(lat, 99)), "max_s": float(lat.max())},
         "kl_max_rel_diff_mix_vs_alone": kl_dev,  # 0 for plain; mixed vs plain otherwise
+        "launch_shapes": tuned or "built-in defaults (no autotune)",
     }
 
 
@@ -705,6 +717,8 @@ def main():
     ap.add_argument("--hw-queues", type=int, default=0,
                     help="GPU_MAX_HW_QUEUES for this process (serving; 0 leaves the default)")
     ap.add_argument("--seq-sample", type=int, default=32)
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="serving: keep the built-in launch shapes (no tsne_autotune first)")
     ap.add_argument("--serving-precision", default="plain", choices=("plain", "mixed"),
                     help="H2 precision of the served jobs (the sequential baseline is plain)")
     ap.add_argument("--blocking-sync", action="store_true",
@@ -746,7 +760,8 @@ def main():
             os.environ["GPU_MAX_HW_QUEUES"] = str(min(max(a.hw_queues, 1), 16))
         slots = [int(x) for x in str(a.slots).split(",")]
         print(json.dumps(serving_bench(a.jobs, slots, a.seq_sample,
-                                       precision=a.serving_precision)), flush=True)
+                                       precision=a.serving_precision,
+                                       autotune=not a.no_autotune)), flush=True)
         return
     from tsne_h2pack._backend import get_backend
     be = get_backend()

@@ -226,6 +226,54 @@ int tsne_set_graph_mode(tsne_workspace_t *workspace, int on);
 /* Iterations replayed from graphs so far (-1 for a NULL workspace). */
 int tsne_graph_launches(tsne_workspace_t *workspace);
 
+/* [add] Launch shapes of a job, and their autotuner.  No counterpart in the reference
+ * (a CPU library has no launch shapes).  None of these choices changes a result: a job
+ * gives the same embedding, bit for bit, under every one of them.
+ *   overlap          1: on rebuild iterations the attractive pass runs on a second stream
+ *                    beside the rebuild (default: from 2^18 points on)
+ *   level_batch      tree levels queued between two host checks (1..8, default 4)
+ *   graph            1: force-reusing iterations replayed from HIP graphs (as
+ *                    tsne_set_graph_mode; default 0)
+ *   one_class_level  internal tree levels of at most this many nodes run their IDs in one
+ *                    launch of the widest workgroup shape, larger ones split by size
+ *                    class (-1: the default, 512 in 2D, 2048 in 3D)
+ *   ms               the tuning run's time (table entries; ignored by tsne_set_tuning) */
+typedef struct {
+    int overlap;
+    int level_batch;
+    int graph;
+    int one_class_level;
+    double ms;
+} tsne_tuning_t;
+
+/* Sets a workspace's launch shapes for its later iterations and builds (fields outside
+ * their range keep the current choice).  tsne_init_with_P applies the device's tuning
+ * table entry for the job's (low_dim, round(log2 N)) when one exists; the job server runs
+ * its jobs with the entry's level_batch and one_class_level, graph replay on and overlap
+ * off (the table is tuned one job at a time; jobs served together already fill the GPU's
+ * queues); environment variables TSNE_OVERLAP / TSNE_GRAPH, when set, take precedence.
+ * 0 on success. */
+int tsne_set_tuning(tsne_workspace_t *workspace, const tsne_tuning_t *tuning);
+
+/* The tuning table entry for a job of n_point points in low_dim dimensions: 1 and the
+ * entry if the device's table holds one for its size bucket, 0 and the built-in defaults
+ * otherwise, -1 for invalid arguments. */
+int tsne_tuning_lookup(int low_dim, int n_point, tsne_tuning_t *out);
+
+/* Autotuner: times n_iter iterations (the front end's schedule: h2_tol 1e-2 in 2D, 3e-2 in
+ * 3D, rebuild every 3, 10 during early exaggeration, leaf 100) of a synthetic job of
+ * n_point points (a seeded 10-cluster mixture in 16 dimensions, its exact k = 90 graph and
+ * joint P at perplexity 30) under each launch shape, one choice at a time from the
+ * defaults, two runs each; checks that every run gives the first run's embedding bit for
+ * bit; stores the fastest shapes in the device's table (persisted in the cache directory,
+ * as tsne_proxy_cache_info describes, file tune_amd_v1_<arch>_<CUs>cu.txt) and returns
+ * them in *best.  n_point >= 2000 (the H2 path), n_iter >= 50.  0 on success. */
+int tsne_autotune(int low_dim, int n_point, int n_iter, tsne_tuning_t *best);
+
+/* Path of the device's tuning table file; -1 (and "") when the cache is off or no device
+ * is present. */
+int tsne_tuning_path(char *path, int path_len);
+
 /* [add] Diagnostics, no counterpart in the reference: average time (ms, HIP events on the
  * workspace stream, over `reps` launches after one warm-up) of the attractive pass's memory
  * pattern over the workspace's current P and embedding.  mode 0: P streamed only; 1: P and

@@ -127,3 +127,21 @@ def test_numpy_sum_replica_is_bit_exact():
         a = rng.random(n) * rng.choice([1e-9, 1.0, 1e7], n)
         got = lib.tsne_numpy_sum(a.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
         assert got == np.sum(a), n
+
+
+def test_tuning_entry_points_without_device():
+    """Launch-shape tuning: invalid sizes are rejected; without a device there is no table
+    (no device kind to key it), so a lookup gives the built-in shapes and the autotuner
+    refuses sizes below the H2 path."""
+    be = B.get_backend()
+    t = B.Tuning()
+    assert be.lib.tsne_tuning_lookup(4, 1000, ctypes.byref(t)) == -1
+    assert be.lib.tsne_tuning_lookup(2, 0, ctypes.byref(t)) == -1
+    assert be.lib.tsne_autotune(2, 1000, 400, ctypes.byref(t)) == -1
+    assert be.lib.tsne_autotune(2, 5000, 10, ctypes.byref(t)) == -1
+    if not os.path.exists("/dev/kfd"):
+        shapes, found = be.tuning_lookup(2, 300_000)
+        assert not found
+        assert shapes["overlap"] == 1 and shapes["level_batch"] == 4 and shapes["graph"] == 0
+        assert be.tuning_lookup(3, 5000)[0]["overlap"] == 0
+        assert be.tuning_path() is None

@@ -25,6 +25,10 @@ void h2_set_id_mixed(H2 *h, int mixed);
 // HIP events around each build's ID launches (their time of the last build in
 // tsne_h2_stats.id_ms).
 void h2_set_timing(H2 *h, int on);
+// Launch shapes of later builds (none changes a result): tree levels queued between two
+// host checks (1..8; <= 0 keeps the default) and the largest internal level whose IDs run
+// in one launch of the widest size class (< 0 keeps the default).
+void h2_set_tuning(H2 *h, int level_batch, int one_class_level);
 
 // One-shot hook for the next h2_build: called (on the build's host thread) once the
 // build's front -- root box, Morton keys, sort, sorted gather copy -- is queued on the

@@ -67,6 +67,8 @@ struct H2 {
     int last_mixed = 0; // the last product was the mixed-precision one
     // growth factors (grown on overflow)
     double node_f = 8.0, pair_f = 1.0, pool_f = 1.0;
+    // launch shapes (h2_set_tuning): 0 / -1 keep the defaults
+    int tune_level_batch = 0, tune_one_class = -1;
 };
 
 namespace {
@@ -343,7 +345,7 @@ int build_once(H2 *h, const void *Yg)
     int *plev = h->pin + CNT_COUNT, *plev_dev = h->pin_dev + CNT_COUNT;
     for (int k = 0; k < maxlev * kPinRec; k++) plev[k] = -1;
     int l = 0, pend = 0;            // the level pass of level l appends to pend[pend]
-    const int lb = level_batch();
+    const int lb = h->tune_level_batch > 0 ? h->tune_level_batch : level_batch();
     int known_l = 0, known_n = 1;   // level known_l has exactly known_n nodes
     hipEvent_t last_ev = nullptr;   // end of the last queued batch
     for (int batch = 0; l < maxlev; batch++) {
@@ -433,11 +435,12 @@ int build_once(H2 *h, const void *Yg)
     const bool tid = h->timing && h->ev_id0 && h->ev_id1;
     if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_ID0], st);
     if (tid && hipEventRecord(h->ev_id0, st) != hipSuccess) return -1;
-    if (h->n_nodes > 1 && launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, h->pq, f32, st))
+    if (h->n_nodes > 1 &&
+        launch_id_up(D, d, d.leaves, 0, h->n_leaves, h->id_tol, h->pq, f32, h->tune_one_class, st))
         return -1;
     for (int lv = nlev - 1; lv >= 1; lv--)
         if (launch_id_up(D, d, nullptr, h->lvl[lv], h->lvl[lv + 1] - h->lvl[lv], h->id_tol, h->pq,
-                         f32, st))
+                         f32, h->tune_one_class, st))
             return -1;
     if (tid && hipEventRecord(h->ev_id1, st) != hipSuccess) return -1;
     if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_ID1], st);
@@ -532,6 +535,13 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
         return nullptr;
     }
     return h;
+}
+
+void h2_set_tuning(H2 *h, int level_batch, int one_class_level)
+{
+    if (!h) return;
+    h->tune_level_batch = (level_batch >= 1 && level_batch <= 8) ? level_batch : 0;
+    h->tune_one_class = one_class_level >= 0 ? one_class_level : -1;
 }
 
 void h2_set_timing(H2 *h, int on)

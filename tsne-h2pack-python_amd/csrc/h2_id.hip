@@ -994,12 +994,14 @@ constexpr int kLeafGridLarge = 1024;  // same, 384 threads (leaves of > 128 poin
 
 template <typename T>
 void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, double tol, int pq,
-                    hipStream_t st)
+                    int one_class, hipStream_t st)
 {
+    const int oc2 = one_class >= 0 ? one_class : kOneClassLevel;
+    const int oc3 = one_class >= 0 ? one_class : one_class_level3();
     if (D == 2 && tile2()) {
         // 2D in tiles of 2 candidates x PQ/4 proxy slots: classes of 64 (128 threads) and
         // 128 candidates (256 threads)
-        const bool one = count <= kOneClassLevel;
+        const bool one = count <= oc2;
         if (pq <= 32) {
             if (!one)
                 hipLaunchKernelGGL((id_tile_kernel<2, 128, 2, 4, 32, false, T>), dim3(count), dim3(128), 0,
@@ -1014,7 +1016,7 @@ void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, d
                                h, list, lb, count, one ? 0 : 64, tol);
         }
     } else if (D == 2) {
-        const bool one = count <= kOneClassLevel;
+        const bool one = count <= oc2;
         if (pq <= 32) {
             if (!one)
                 hipLaunchKernelGGL((id_up_kernel<2, 64, 32, false, T>), dim3(count), dim3(64), 0, st, h,
@@ -1030,7 +1032,7 @@ void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, d
         }
     } else if (tile3()) {
         // 3D in tiles: classes of 64 (128 threads), 128 (256) and 384 candidates (768)
-        if (!list && count <= one_class_level3()) {
+        if (!list && count <= oc3) {
             hipLaunchKernelGGL((id_tile_kernel<3, 768, 4, 8, 96, false, T>), dim3(count), dim3(768), 0, st, h,
                                list, lb, count, 0, tol);
         } else if (!list) {
@@ -1049,7 +1051,7 @@ void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, d
             hipLaunchKernelGGL((id_tile_kernel<3, 768, 4, 8, 96, true, T>), dim3(gl), dim3(768), 0, st, h,
                                list, lb, count, 128, tol);
         }
-    } else if (!list && count <= one_class_level3()) {
+    } else if (!list && count <= oc3) {
         hipLaunchKernelGGL((id_up_kernel<3, 384, 96, false, T>), dim3(count), dim3(384), 0, st, h, list,
                            lb, count, 0, tol);
     } else if (!list) {
@@ -1074,13 +1076,13 @@ void launch_id_up_t(int D, const H2Dev &h, const int *list, int lb, int count, d
 }
 
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol, int pq,
-                 bool f32, hipStream_t st)
+                 bool f32, int one_class, hipStream_t st)
 {
     if (count <= 0) return 0;
     if (f32)
-        launch_id_up_t<float>(D, h, list, lb, count, tol, pq, st);
+        launch_id_up_t<float>(D, h, list, lb, count, tol, pq, one_class, st);
     else
-        launch_id_up_t<double>(D, h, list, lb, count, tol, pq, st);
+        launch_id_up_t<double>(D, h, list, lb, count, tol, pq, one_class, st);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 

@@ -150,9 +150,12 @@ int launch_pair_offsets(const uint64_t *pairs, int n_pairs, int *beg, int *end, 
 
 // h2_id.hip
 // pq: proxy slots to process (>= the largest proxy count of the table); f32: pivoted
-// Gram-Schmidt in FP32 (the mixed path at ID tolerances >= 1e-3), else FP64
+// Gram-Schmidt in FP32 (the mixed path at ID tolerances >= 1e-3), else FP64; one_class:
+// internal levels of at most this many nodes run in one launch of the widest size class
+// (< 0: the default of the dimension).  With the tiled kernels the size class that runs a
+// node does not change its result.
 int launch_id_up(int D, const H2Dev &h, const int *list, int lb, int count, double tol,
-                 int pq, bool f32, hipStream_t st);
+                 int pq, bool f32, int one_class, hipStream_t st);
 
 // h2_product.hip
 // mixed: coupling on MFMA with BF16 operands and FP32 accumulation; a node whose

@@ -348,6 +348,79 @@ down_kernel(H2Dev h, int lb, int le)
     }
 }
 
+// Mixed-precision downward sweep: the same update as down_kernel as a batched small GEMM
+// on MFMA, one wavefront per child c: (r_c x r_p) rows of U_p times z_p (r_p x 4).  As in
+// couple_mfma_kernel, v_mfma_f32_4x4x4bf16_1k runs 16 blocks: block b = lane / 4 takes
+// rows 4g..4g+3 of the child (g = b mod G) against 4 parent ranks of phase b / G; lane u
+// of a block supplies row 4g+u of U_p and column u of z_p.  Both operands are split into
+// two BF16 terms (hi + lo, ~16 bits each); U_h z_h + U_h z_l + U_l z_h accumulate in FP32
+// (the dropped U_l z_l term is below 2^-16 of the product).
+template <int D>
+__global__ void __launch_bounds__(256)
+down_mfma_kernel(H2Dev h, int lb, int le)
+{
+    constexpr int RMAX = H2Cfg<D>::RMAX;
+    static_assert(RMAX <= 64, "one staged round of parent ranks");
+    __shared__ __align__(8) short s_zh[4][4][64], s_zl[4][4][64];
+    __shared__ float s_z[4][16][16];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int c = lb + blockIdx.x * 4 + wv;
+    if (c >= le) return;  // per wavefront: only wavefront-level LDS traffic below
+    const int p = h.parent[c];
+    const int rp = h.rank[p], rc = h.rank[c];
+    if (rc <= 0) return;
+    int row0 = 0;
+    for (int s = h.child0[p]; s < c; s++) row0 += h.rank[s];
+    // z_p staged as BF16 hi / lo columns (zero past r_p)
+    {
+        const float4 z = (lane < rp) ? h.zc[(size_t)p * RMAX + lane] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const v4f zv = {z.x, z.y, z.z, z.w};
+        const v4s zh = bf16x4(zv), zl = bf16x4(zv - f32x4(zh));
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            s_zh[wv][q][lane] = zh[q];
+            s_zl[wv][q][lane] = zl[q];
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int G = rc <= 4 ? 1 : rc <= 8 ? 2 : rc <= 16 ? 4 : rc <= 32 ? 8 : 16;
+    const int P = 16 / G;
+    const int b = lane >> 2, u = lane & 3, g = b % G, ph = b / G;
+    const int tr = 4 * g + u;  // this lane's row of the child
+    const float *urow = h.upool + h.uoff[p] + (size_t)(row0 + min(tr, rc - 1)) * rp;
+    v4f acc = {0.f, 0.f, 0.f, 0.f};
+    for (int k0 = 4 * ph; k0 < rp; k0 += 4 * P) {
+        v4f a;
+#pragma unroll
+        for (int k = 0; k < 4; k++) a[k] = (tr < rc && k0 + k < rp) ? urow[k0 + k] : 0.f;
+        const v4s Ah = bf16x4(a), Al = bf16x4(a - f32x4(Ah));
+        const v4s Bh = *reinterpret_cast<const v4s *>(&s_zh[wv][u][k0]);
+        const v4s Bl = *reinterpret_cast<const v4s *>(&s_zl[wv][u][k0]);
+        acc = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Ah, Bh, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Ah, Bl, acc, 0, 0, 0);
+        acc = __builtin_amdgcn_mfma_f32_4x4x4bf16_1k(Al, Bh, acc, 0, 0, 0);
+    }
+    // blocks of one row group in other phases are 4G lanes apart; phases past r_p
+    // contributed nothing (their loop ran no step), so the sum is over every phase
+    for (int o = 4 * G; o < 64; o <<= 1)
+#pragma unroll
+        for (int q = 0; q < 4; q++) acc[q] += __shfl_xor(acc[q], o, 64);
+    if (ph == 0)
+#pragma unroll
+        for (int q = 0; q < 4; q++) s_z[wv][b][q * 4 + u] = acc[q];
+    __builtin_amdgcn_wave_barrier();
+    if (lane < rc) {
+        const float *zr = &s_z[wv][lane >> 2][(lane & 3) * 4];
+        const float4 cc = h.cent[c], cp = h.cent[p];
+        float4 z = h.zc[(size_t)c * RMAX + lane];
+        z.x += zr[0];
+        z.y += zr[1] + (cc.x - cp.x) * zr[0];
+        z.z += zr[2] + (cc.y - cp.y) * zr[0];
+        z.w += zr[3] + (cc.z - cp.z) * zr[0];
+        h.zc[(size_t)c * RMAX + lane] = z;
+    }
+}
+
 // Leaves: one wavefront per leaf, each lane one or two target points.  Far part from
 // the leaf's coefficients, then the NEAR leaves (sources staged in LDS) and the W
 // skeletons.  The leaf is in its own near list, so every target met itself once with
@@ -516,10 +589,13 @@ int launch_couple(int D, const H2Dev &h, int n_nodes, int mixed, float tau, hipS
 
 int launch_down(int D, const H2Dev &h, int lb, int le, int mixed, hipStream_t st)
 {
-    (void)mixed;
     const int g = (le - lb + 3) / 4;
     if (g <= 0) return 0;
-    if (D == 2)
+    if (mixed && D == 2)
+        hipLaunchKernelGGL(down_mfma_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le);
+    else if (mixed)
+        hipLaunchKernelGGL(down_mfma_kernel<3>, dim3(g), dim3(256), 0, st, h, lb, le);
+    else if (D == 2)
         hipLaunchKernelGGL(down_kernel<2>, dim3(g), dim3(256), 0, st, h, lb, le);
     else
         hipLaunchKernelGGL(down_kernel<3>, dim3(g), dim3(256), 0, st, h, lb, le);

@@ -458,21 +458,24 @@ attr_tile_kernel(int n, int ntiles, int tile_rows, const int *__restrict__ row_p
 
 // Update from a separately computed attractive sum A (the ATTR_ONLY kernel): the same
 // gains/momentum rule, record and centring sums as the fused kernel, one thread per row,
-// blocks over the same row ranges as the fused kernel.
+// blocks over the same row ranges as the fused kernel.  The block's centring sum is added
+// in the fused kernel's order (groups rows g, g + G, ... each summed in turn, then the
+// groups in order, G = 256 / its lanes per row), so the split pass gives the fused pass's
+// bits.
 template <int D>
 __global__ void __launch_bounds__(256)
 update_kernel(int n, const float *__restrict__ A, const float *__restrict__ F,
               double *__restrict__ Y64, double *__restrict__ uY, double *__restrict__ gains,
               typename Vec<D>::T *__restrict__ Yg_next, const double *__restrict__ scal,
               double exag4, double mom, double lr, int rows_per_block,
-              double *__restrict__ block_sums)
+              double *__restrict__ block_sums, int groups)
 {
     __shared__ double red[3][256];
     const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
     const int row0 = lblk * rows_per_block;
     const int row1 = min(n, row0 + rows_per_block);
     const double rep_scale = rep_scale_of(scal);
-    double m[3] = {0.0, 0.0, 0.0}, sum[3] = {0.0, 0.0, 0.0};
+    double m[3] = {0.0, 0.0, 0.0};
     for (int d = 0; d < D; d++) m[d] = scal[SLOT_MEAN + d];
     for (int i = row0 + threadIdx.x; i < row1; i += 256) {
         double yn[3] = {0.0, 0.0, 0.0};
@@ -487,7 +490,6 @@ update_kernel(int n, const float *__restrict__ A, const float *__restrict__ F,
             Y64[kd] = yn[d];
             uY[kd] = u;
             gains[kd] = gn;
-            sum[d] += yn[d];
         }
         typename Vec<D>::T rec;
         rec.x = (float)yn[0];
@@ -495,14 +497,19 @@ update_kernel(int n, const float *__restrict__ A, const float *__restrict__ F,
         if constexpr (D == 3) { rec.z = (float)yn[2]; rec.w = 0.f; }
         Yg_next[i] = rec;
     }
-    for (int d = 0; d < 3; d++) red[d][threadIdx.x] = sum[d];
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off)
-            for (int d = 0; d < D; d++) red[d][threadIdx.x] += red[d][threadIdx.x + off];
-        __syncthreads();
+    __syncthreads();  // the block's new coordinates are written (read back below)
+    if (threadIdx.x < groups) {
+        double s[3] = {0.0, 0.0, 0.0};
+        for (int i = row0 + threadIdx.x; i < row1; i += groups)
+            for (int d = 0; d < D; d++) s[d] += Y64[(size_t)i * D + d];
+        for (int d = 0; d < 3; d++) red[d][threadIdx.x] = s[d];
     }
-    if (threadIdx.x < D) block_sums[(size_t)lblk * 3 + threadIdx.x] = red[threadIdx.x][0];
+    __syncthreads();
+    if (threadIdx.x < D) {
+        double t = 0.0;
+        for (int w = 0; w < groups; w++) t += red[threadIdx.x][w];
+        block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
+    }
 }
 
 // Sum of the per-block sums -> new pending mean (SLOT_MEAN..+D-1), in two launches with a
@@ -1019,12 +1026,17 @@ int launch_update(int D, int n, const float *A, const float *F, double *Y64, dou
                   double lr, double *block_sums, int nblocks, hipStream_t st)
 {
     const int rpb = (n + nblocks - 1) / nblocks;
+    // row groups of the fused kernel (its centring-sum order): 16 lanes per row in the
+    // chunked form, else the configured row width
+    const int groups = 256 / (attr_vec() ? 16 : attr_lanes());
     if (D == 2)
         hipLaunchKernelGGL(update_kernel<2>, dim3(nblocks), dim3(256), 0, st, n, A, F, Y64, uY,
-                           gains, (float2 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
+                           gains, (float2 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums,
+                           groups);
     else
         hipLaunchKernelGGL(update_kernel<3>, dim3(nblocks), dim3(256), 0, st, n, A, F, Y64, uY,
-                           gains, (float4 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums);
+                           gains, (float4 *)Yg_next, scal, 4.0 * exag, mom, lr, rpb, block_sums,
+                           groups);
     if (launch_mean(D, n, nblocks, block_sums, (double *)scal, st) != 0) return -1;
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }

@@ -286,6 +286,8 @@ std::string cache_path(int D, double tol, double eta, int pmax)
 
 }  // namespace
 
+std::string cache_dir_path() { return cache_dir(); }
+
 std::string proxy_cache_file(int D, double tol, double eta, int pmax) { return cache_path(D, tol, eta, pmax); }
 
 uint64_t proxy_table_digest(const ProxyTable &t)

@@ -33,5 +33,7 @@ const ProxyTable &proxy_table(int D, double tol, double eta, int pmax);
 std::string proxy_cache_file(int D, double tol, double eta, int pmax);
 // FNV-1a of the table's counts and points.
 uint64_t proxy_table_digest(const ProxyTable &t);
+// The persistent cache directory (created if missing; "" when the cache is off).
+std::string cache_dir_path();
 
 }  // namespace tsne

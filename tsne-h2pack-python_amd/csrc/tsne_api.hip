@@ -17,6 +17,7 @@
 #include "kernels.h"
 #include "renumber.h"
 #include "tiles.h"
+#include "tune.h"
 #include "../../include/tsne_h2pack.h"
 
 using namespace tsne;
@@ -35,6 +36,9 @@ struct tsne_workspace {
     hipEvent_t ev_y = nullptr, ev_a = nullptr;
     float *d_A = nullptr;       // attractive sums of a split iteration, n x D
     int overlap = 1;
+    // launch shapes of the H2 builds (tsne_set_tuning / the tuning table): tree levels per
+    // host check (0: default) and the one-launch ID level limit (-1: default)
+    int tune_level_batch = 0, tune_one_class = -1;
     int nnz = 0;
     int *d_rowptr = nullptr, *d_col = nullptr;
     float *d_val32 = nullptr;
@@ -261,6 +265,7 @@ int rebuild(tsne_workspace *ws)
                            ws->params.h2_tol > 0 ? ws->params.h2_tol : 1e-6, ws->st);
         if (!ws->h2) return -1;
         if (ws->st2) h2_set_aux_stream(ws->h2, ws->st2);
+        h2_set_tuning(ws->h2, ws->tune_level_batch, ws->tune_one_class);
     }
     int bad = 0;
     h2_set_id_mixed(ws->h2, ws->precision == TSNE_H2_MIXED);
@@ -656,7 +661,7 @@ namespace tsne {
 // still belongs to the caller (also when the initial H2 build fails after the workspace
 // existed: P is detached before that workspace is destroyed).
 tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_params_t *params,
-                              sparse_matrix_t *P, bool *taken)
+                              sparse_matrix_t *P, bool *taken, const tsne_tuning_t *shapes)
 {
     *taken = false;
     if (low_dim != 2 && low_dim != 3) {
@@ -716,9 +721,21 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         ws->tile_nt = (t && atoi(t) == 1024) ? 1024 : 256;
     }
     ws->nblocks_red = std::max(1, std::min(1024, (n + 1023) / 1024));
+    // launch shapes: the device's tuning table entry for this job size, else the built-in
+    // defaults (none changes a result); TSNE_GRAPH / TSNE_OVERLAP, when set, win
+    tsne_tuning_t tune = tune_defaults(low_dim, n_point);
+    (void)tune_lookup(low_dim, n_point, &tune);
+    if (shapes) {
+        if (shapes->overlap == 0 || shapes->overlap == 1) tune.overlap = shapes->overlap;
+        if (shapes->graph == 0 || shapes->graph == 1) tune.graph = shapes->graph;
+        if (shapes->level_batch >= 1 && shapes->level_batch <= 8) tune.level_batch = shapes->level_batch;
+        if (shapes->one_class_level >= 0) tune.one_class_level = shapes->one_class_level;
+    }
+    ws->tune_level_batch = tune.level_batch;
+    ws->tune_one_class = tune.one_class_level;
     {
-        const char *e = getenv("TSNE_GRAPH");  // 1 / 0: graph mode on / off (default off)
-        ws->graph_mode = e ? (e[0] == '1') : 0;
+        const char *e = getenv("TSNE_GRAPH");  // 1 / 0: graph mode on / off
+        ws->graph_mode = e ? (e[0] == '1') : tune.graph;
     }
     {
         const char *e = getenv("TSNE_ATTR_AFTER_FRONT");
@@ -729,11 +746,10 @@ tsne_workspace_t *init_with_P(int n_point, int pt_dim, int low_dim, const tsne_p
         ws->direct_reuse = e ? (e[0] == '1') : 0;
     }
     {
-        // the attractive pass of rebuild iterations on a second stream: from 2^18 points
-        // on (below, the two extra launches and the event waits cost more than the
-        // overlap saves); TSNE_OVERLAP=0 / 1 turns it off / on for any size
+        // the attractive pass of rebuild iterations on a second stream (the tuning's
+        // choice; TSNE_OVERLAP=0 / 1 turns it off / on for any size)
         const char *e = getenv("TSNE_OVERLAP");
-        ws->overlap = e ? (e[0] != '0') : (n_point >= (1 << 18));
+        ws->overlap = e ? (e[0] != '0') : tune.overlap;
     }
     bool ok = hipGetDevice(&ws->device) == hipSuccess &&
               hipStreamCreateWithFlags(&ws->st, hipStreamNonBlocking) == hipSuccess;
@@ -897,7 +913,7 @@ int tsne_run_range(tsne_workspace_t *ws, int it_begin, int it_end)
                 if (renumber(ws) < 0) return -1;
                 if (ws->next_reorder >= nro) free_renumber_buffers(ws);  // last one
             }
-            if (ws->st2 && ws->tiles.ntiles == 0) {
+            if (ws->overlap && ws->st2 && ws->tiles.ntiles == 0) {
                 split = true;
                 if (ws->attr_after_front && ws->h2)
                     h2_set_front_hook(ws->h2, queue_split_attr, ws);
@@ -1337,6 +1353,32 @@ int tsne_set_graph_mode(tsne_workspace_t *ws, int on)
     if (!ws) return -1;
     ws->graph_mode = on ? 1 : 0;
     if (!on) drop_graphs(ws);
+    return 0;
+}
+
+int tsne_set_tuning(tsne_workspace_t *ws, const tsne_tuning_t *t)
+{
+    if (!ws || !t) return -1;
+    int cur_dev = 0;
+    if (hipGetDevice(&cur_dev) != hipSuccess) return -1;
+    if (cur_dev != ws->device && hipSetDevice(ws->device) != hipSuccess) return -1;
+    if (t->overlap == 1 && !ws->st2) {
+        // the second stream, its events and the split pass's attractive sums, made now
+        if (hipStreamCreateWithFlags(&ws->st2, hipStreamNonBlocking) != hipSuccess) {
+            ws->st2 = nullptr;
+            return -1;
+        }
+        if (hipEventCreateWithFlags(&ws->ev_y, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&ws->ev_a, hipEventDisableTiming) != hipSuccess ||
+            (!ws->d_A && dalloc(&ws->d_A, (size_t)ws->n * ws->dim, ws->st) != 0))
+            return -1;
+        if (ws->h2) h2_set_aux_stream(ws->h2, ws->st2);
+    }
+    if (t->overlap == 0 || t->overlap == 1) ws->overlap = t->overlap;
+    if (t->graph == 0 || t->graph == 1) tsne_set_graph_mode(ws, t->graph);
+    if (t->level_batch >= 1 && t->level_batch <= 8) ws->tune_level_batch = t->level_batch;
+    if (t->one_class_level >= 0) ws->tune_one_class = t->one_class_level;
+    if (ws->h2) h2_set_tuning(ws->h2, ws->tune_level_batch, ws->tune_one_class);
     return 0;
 }
 

@@ -108,6 +108,17 @@ class Quality(ctypes.Structure):
     ]
 
 
+class Tuning(ctypes.Structure):
+    """tsne_tuning_t: launch shapes of a job (none changes a result)."""
+    _fields_ = [
+        ('overlap', ctypes.c_int), ('level_batch', ctypes.c_int), ('graph', ctypes.c_int),
+        ('one_class_level', ctypes.c_int), ('ms', ctypes.c_double),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
 class JobInfo(ctypes.Structure):
     """tsne_job_info_t."""
     _fields_ = [
@@ -355,6 +366,35 @@ class TSNEBackend:
         self._lib.sparse_matrix_destroy(ptr)
         return csr_matrix((va, ci, rp), shape=(n, n)), st
 
+    def tuning_lookup(self, low_dim, n_point):
+        """The device tuning table's launch shapes for a job of this size: (dict, whether
+        the table holds an entry for its size bucket)."""
+        t = Tuning()
+        rc = self._lib.tsne_tuning_lookup(int(low_dim), int(n_point), ctypes.byref(t))
+        if rc < 0:
+            raise ValueError("tsne_tuning_lookup: low_dim must be 2 or 3 and n_point >= 1")
+        return t.as_dict(), rc == 1
+
+    def autotune(self, low_dim, n_point, n_iter=400):
+        """tsne_autotune: times a synthetic job of n_point points under each launch shape,
+        checks that every shape gives the same embedding bit for bit, and stores the
+        fastest in the device's persistent tuning table.  Returns the chosen shapes."""
+        t = Tuning()
+        if self._lib.tsne_autotune(int(low_dim), int(n_point), int(n_iter), ctypes.byref(t)) != 0:
+            raise RuntimeError("tsne_autotune failed")
+        return t.as_dict()
+
+    def tuning_path(self):
+        """Path of the device's tuning table file (None when the cache is off)."""
+        buf = ctypes.create_string_buffer(4096)
+        return buf.value.decode() if self._lib.tsne_tuning_path(buf, 4096) == 0 else None
+
+    def set_tuning(self, ws, overlap=-1, level_batch=0, graph=-1, one_class_level=-1):
+        """tsne_set_tuning on a workspace (out-of-range fields keep the current choice)."""
+        t = Tuning(int(overlap), int(level_batch), int(graph), int(one_class_level), 0.0)
+        if self._lib.tsne_set_tuning(ws, ctypes.byref(t)) != 0:
+            raise RuntimeError("tsne_set_tuning failed")
+
     def joint_P(self, knn_distances_csr, perplexity, num_threads=1):
         """Neighbour indexing on the host, then the joint P on the device."""
         data, indices, indptr = self.prepare_knn_csr(knn_distances_csr)
@@ -485,6 +525,14 @@ def configure_signatures(lib):
     lib.tsne_set_graph_mode.restype = ctypes.c_int
     lib.tsne_graph_launches.argtypes = [c_ws]
     lib.tsne_graph_launches.restype = ctypes.c_int
+    lib.tsne_set_tuning.argtypes = [c_ws, ctypes.POINTER(Tuning)]
+    lib.tsne_set_tuning.restype = ctypes.c_int
+    lib.tsne_tuning_lookup.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tuning)]
+    lib.tsne_tuning_lookup.restype = ctypes.c_int
+    lib.tsne_autotune.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.POINTER(Tuning)]
+    lib.tsne_autotune.restype = ctypes.c_int
+    lib.tsne_tuning_path.argtypes = [ctypes.c_char_p, ctypes.c_int]
+    lib.tsne_tuning_path.restype = ctypes.c_int
     lib.tsne_attr_probe.argtypes = [c_ws, ctypes.c_int, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
     lib.tsne_attr_probe.restype = ctypes.c_int
     lib.tsne_run_state.argtypes = [c_ws, ip, ip, dp, dp]

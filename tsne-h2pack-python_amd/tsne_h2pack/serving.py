@@ -124,3 +124,19 @@ def _info_dict(info: JobInfo) -> dict:
             "submit_s": info.submit_s, "start_s": info.start_s, "end_s": info.end_s,
             "latency_s": info.end_s - info.submit_s,
             "footprint_bytes": info.footprint_bytes}
+
+
+def autotune_sizes(sizes, dims=(2, 3), n_iter: int = 400) -> dict:
+    """Tune the launch shapes of every size bucket (round(log2 N)) that the job sizes
+    `sizes` fall in, for each embedding dimension in `dims`, with the library's autotuner
+    (tsne_autotune on a synthetic job of 2^bucket points; none of the tuned shapes changes a
+    result).  The table persists in the cache directory, and later jobs of those sizes --
+    served or not -- start with their bucket's shapes.  Returns {(dim, bucket): shapes}."""
+    be = get_backend()
+    buckets = sorted({int(round(np.log2(max(int(n), 2)))) for n in sizes})
+    out = {}
+    for d in dims:
+        for b in buckets:
+            n = max(2000, 1 << b)
+            out[(d, b)] = be.autotune(d, n, n_iter)
+    return out
