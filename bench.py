@@ -253,52 +253,50 @@ def gradient_error(be, ws, n, dim, n_rows=10000, seed=1):
                 Z=Z.value, Z64=Ze.value, rows=int(rows.size))
 
 
-def cpu_baseline(csr, n, dim, seconds_target=15.0):
-    """The oracle (FP64 C/OpenMP restatement of the reference's per-iteration path) on
-    the host cores, per phase, on a bounded sample of the benchmark job:
-      attractive  the full CSR-P pass over all N rows (timed whole);
-      update      the gains/momentum/position update and centring over all N rows
-                  (timed whole);
-      repulsion   the exact O(N^2) sum -- the reference's direct branch
-                  (src/tsne_h2pack.c:407-491), NOT its H2 approximation, which cannot be
-                  built here (its H2Pack submodule and BLAS/LAPACK are absent) -- on a row
-                  sample sized to about seconds_target, scaled to N rows.
-    value: iterations/s of that exact-repulsion path at N (it is the reference's
-    method='exact' path, not its H2 path)."""
+def cpu_baseline(be, csr, n, dim, tol, leaf, n_iter=1000, window=20):
+    """The oracle's FP64 C/OpenMP restatement of the reference's H2 path
+    (oracle/tsne_oracle_h2.c: Morton tree, admissible and near lists, proxy-point ID with
+    nested bases, upward / coupling / downward passes and the exact near field; pinned to
+    the exact sum within h2_tol by tests/test_oracle_h2.py) timed on the host cores over the
+    last `window` iterations [n_iter - window, n_iter) of the benchmark job.  The embedding
+    at iteration n_iter - window comes from the library (the same job run on the device up
+    to there, untimed); from it the oracle runs the window on the reference's schedule: a
+    fresh CPU H2 build and product at the window's first iteration and at every rebuild
+    iteration, the force reused between them, the attractive pass, the update and the
+    centring every iteration, the KL at the last.  Momentum and gains start at 0 and 1
+    (their values do not change the work).  The attractive pass and the update are also
+    timed alone over all N."""
     import oracle
     from scipy.sparse import csr_matrix
+    lo = n_iter - window
+    ws = make_workspace(be, csr, n, dim, "plain", n_iter, leaf, tol)
+    try:
+        assert be.lib.tsne_run_range(ws, 0, lo) == 0
+        Y = np.zeros((dim, n))
+        be.lib.tsne_get_embedding(ws, Y)
+    finally:
+        be.lib.tsne_destroy(ws)
     P = csr_matrix((csr[2], csr[1], csr[0]), shape=(n, n))
-    rng = np.random.default_rng(0)
-    Y = rng.normal(scale=30.0, size=(dim, n))
+    t0 = time.perf_counter()
+    _, kl, builds = oracle.run_h2(P, Y, n_iter=n_iter, it_begin=lo, h2_tol=tol, leaf=leaf)
+    dt = time.perf_counter() - t0
     t0 = time.perf_counter()
     g = oracle.attractive(P, Y, 1.0)
     t_attr = time.perf_counter() - t0
-    uY = np.zeros_like(Y)
-    gains = np.ones_like(Y)
     t0 = time.perf_counter()
-    oracle.update(Y, uY, gains, g, 0.8, max(n / 12.0, 200.0))
+    oracle.update(Y, np.zeros_like(Y), np.ones_like(Y), g, 0.8, max(n / 12.0, 200.0))
     t_upd = time.perf_counter() - t0
-    rows = np.sort(rng.choice(n, size=min(n, 64), replace=False)).astype(np.int32)
-    t0 = time.perf_counter()
-    oracle.repulsive_rows(Y, 1.0, rows)
-    t64 = time.perf_counter() - t0
-    m = int(min(n, max(64, 64 * seconds_target / max(t64, 1e-6))))
-    rows = np.sort(rng.choice(n, size=m, replace=False)).astype(np.int32)
-    t0 = time.perf_counter()
-    oracle.repulsive_rows(Y, 1.0, rows)
-    t_rows = time.perf_counter() - t0
-    t_rep = t_rows * n / rows.size
-    t_iter = t_attr + t_upd + t_rep
-    return {"value": 1.0 / t_iter, "unit": "iters/s", "cores": oracle.num_threads(),
+    t_h2 = max(dt - window * (t_attr + t_upd), 0.0) / max(builds, 1)
+    return {"value": window / dt, "unit": "iters/s", "cores": oracle.num_threads(),
             "kind": "port",
-            "sample": (f"oracle FP64 port of the reference's exact-repulsion path "
-                       f"(method='exact', src/tsne_h2pack.c:407-491; its H2 path is not "
-                       f"buildable here): attractive pass over all N {t_attr:.3f}s, update "
-                       f"over all N {t_upd:.3f}s, exact repulsion on {rows.size} sampled rows "
-                       f"{t_rows:.1f}s scaled to N={n}"),
-            "phases_s_per_iteration": {"attractive": t_attr, "update": t_upd,
-                                       "repulsion_exact": t_rep},
-            "attractive_plus_update_iters_per_s": 1.0 / (t_attr + t_upd),
+            "sample": (f"oracle FP64 CPU restatement of the reference's H2 path "
+                       f"(oracle/tsne_oracle_h2.c), iterations [{lo}, {n_iter}) of the "
+                       f"benchmark job from the library's embedding at iteration {lo}: "
+                       f"{builds} H2 builds + products, {window} attractive passes and "
+                       f"updates in {dt:.1f}s"),
+            "window": [lo, n_iter], "seconds": dt, "h2_builds": builds, "kl": kl,
+            "phases_s": {"attractive_per_iteration": t_attr, "update_per_iteration": t_upd,
+                         "h2_build_and_product_per_build": t_h2},
             "reference_published_h2_its_12_threads_m5max": 36.0}
 
 
@@ -788,7 +786,7 @@ def main():
     head = res.get("mixed") or res[next(iter(res))]
     nnz = int(csr[1].size)
     roof = attr_roofline(n, dim, nnz, head)
-    cpu = None if a.no_cpu_baseline else cpu_baseline(csr, n, dim)
+    cpu = None if a.no_cpu_baseline else cpu_baseline(be, csr, n, dim, a.tol, a.leaf)
     lo, hi = head["window"]
     out = {
         "metric": "iters/s at N=1M 2D, mixed vs plain FP32 GPU @<=1e-2 grad err; jobs/s + p99",

@@ -1,4 +1,4 @@
-"""ctypes wrapper of the CPU oracle (oracle/tsne_oracle.c).
+"""ctypes wrapper of the CPU oracle (oracle/tsne_oracle.c, oracle/tsne_oracle_h2.c).
 
 TEST INFRASTRUCTURE ONLY.  This module is the checker: tests/, the smoke() of
 __graft_entry__ and the cpu_baseline leg of bench.py import it.  The product
@@ -64,6 +64,20 @@ def lib():
         L.oracle_knn_exact.restype = None
         L.oracle_num_threads.argtypes = []
         L.oracle_num_threads.restype = ctypes.c_int
+        L.oracle_h2_build.argtypes = [ctypes.c_int, ctypes.c_int, _dp, ctypes.c_int,
+                                      ctypes.c_double]
+        L.oracle_h2_build.restype = ctypes.c_void_p
+        L.oracle_h2_repulsive.argtypes = [ctypes.c_void_p, _dp, _dp]
+        L.oracle_h2_repulsive.restype = ctypes.c_double
+        L.oracle_h2_free.argtypes = [ctypes.c_void_p]
+        L.oracle_h2_free.restype = None
+        ip_ = ctypes.POINTER(ctypes.c_int)
+        lp_ = ctypes.POINTER(ctypes.c_longlong)
+        L.oracle_h2_stats.argtypes = [ctypes.c_void_p, ip_, ip_, lp_, lp_, ip_,
+                                      ctypes.POINTER(ctypes.c_double)]
+        L.oracle_h2_stats.restype = None
+        L.oracle_run_h2.argtypes = L.oracle_run.argtypes + [ctypes.c_int, ctypes.c_double, ip_]
+        L.oracle_run_h2.restype = ctypes.c_double
         _lib = L
     return _lib
 
@@ -150,6 +164,51 @@ def run(P, Y0, n_iter=1000, lr=None, mom0=0.5, mom1=0.8, mom_switch=250, exag=12
                            float(mom0), float(mom1), int(mom_switch), float(exag),
                            int(exag_iter), int(freq), int(max(freq_early, freq)), 0)
     return Y, klv
+
+
+def repulsive_h2(Y, h2_tol=1e-2, leaf=100):
+    """The repulsive force from a fresh CPU H2 build and product (ID tolerance
+    0.5 h2_tol, as the reference's); returns (rep (dim, n), Z, stats dict)."""
+    Y = np.ascontiguousarray(Y, dtype=np.float64)
+    dim, n = Y.shape
+    L = lib()
+    h = L.oracle_h2_build(n, dim, Y, int(leaf), 0.5 * float(h2_tol))
+    if not h:
+        raise ValueError("oracle_h2_build: dim must be 2 or 3 and n >= 2")
+    try:
+        rep = np.zeros_like(Y)
+        Z = L.oracle_h2_repulsive(h, Y, rep)
+        nn, nl, rm = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        nf, nr = ctypes.c_longlong(), ctypes.c_longlong()
+        rmean = ctypes.c_double()
+        L.oracle_h2_stats(h, ctypes.byref(nn), ctypes.byref(nl), ctypes.byref(nf),
+                          ctypes.byref(nr), ctypes.byref(rm), ctypes.byref(rmean))
+        st = dict(n_nodes=nn.value, n_leaves=nl.value, n_far=nf.value, n_near=nr.value,
+                  rank_max=rm.value, rank_mean=rmean.value)
+    finally:
+        L.oracle_h2_free(h)
+    return rep, Z, st
+
+
+def run_h2(P, Y0, n_iter=1000, lr=None, mom0=0.5, mom1=0.8, mom_switch=250, exag=12.0,
+           exag_iter=250, freq=3, freq_early=10, it_begin=0, h2_tol=1e-2, leaf=100,
+           uY=None, gains=None):
+    """The reference's tsne_run loop with the CPU H2 product on rebuild iterations and the
+    force reused between them; returns (Y (dim, n), KL, builds).  it_begin > 0 continues a
+    run from the state (Y0, uY, gains) at that iteration."""
+    dim, n = Y0.shape
+    if lr is None:
+        lr = max(n / 12.0, 200.0)
+    rp, ci, va = _csr(P)
+    Y = np.array(Y0, dtype=np.float64, order="C")
+    uY = np.zeros_like(Y) if uY is None else np.array(uY, dtype=np.float64, order="C")
+    gains = np.ones_like(Y) if gains is None else np.array(gains, dtype=np.float64, order="C")
+    nb = ctypes.c_int()
+    klv = lib().oracle_run_h2(n, dim, rp, ci, va, Y, uY, gains, int(n_iter), float(lr),
+                              float(mom0), float(mom1), int(mom_switch), float(exag),
+                              int(exag_iter), int(freq), int(max(freq_early, freq)),
+                              int(it_begin), int(leaf), float(h2_tol), ctypes.byref(nb))
+    return Y, klv, nb.value
 
 
 def num_threads():

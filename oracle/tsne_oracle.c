@@ -8,7 +8,9 @@
  * 3.3.1) and cites the file:line it follows.  The H2 approximation of the
  * reference is replaced here by the exact O(N^2) sum it approximates (the
  * reference's own direct branch, src/tsne_h2pack.c:407-491), so that this file is
- * the FP64 exact gradient the GPU path is held to within h2_tol.
+ * the FP64 exact gradient the GPU path is held to within h2_tol.  The H2 product
+ * itself is restated on the CPU in tsne_oracle_h2.c (the CPU baseline of bench.py),
+ * pinned against the exact sum here.
  *
  * How each function is pinned (tests/test_oracle.py):
  *   oracle_perplexity_rows  row sums 1 (1e-10), row perplexity (1e-3), scale invariance
