@@ -354,15 +354,28 @@ def test_overlapped_attractive_pass_matches_fused(backend, problem):
     """On rebuild iterations the attractive sums run on a second stream next to the
     rebuild and the product, and a separate kernel applies the update with the fused
     kernel's order of centring sums: the result equals the fused single-stream pass bit
-    for bit."""
+    for bit, whether the split pass is queued beside the build's front or after it, and
+    whatever its grid (one workgroup per logical block, the default 7/8 of the CUs, or 16
+    workgroups walking the 376 logical blocks of this problem)."""
     import os
     _say("overlap vs fused")
     _, _, P = problem
     Y0 = init_normal(P.shape[0], 2, 23)
+    variants = {
+        "fused": {"TSNE_OVERLAP": "0"},
+        "beside-front": {"TSNE_OVERLAP": "1", "TSNE_ATTR_AFTER_FRONT": "0",
+                         "TSNE_ATTR_SPLIT_GRID": "0"},
+        "after-front": {"TSNE_OVERLAP": "1"},
+        "after-front-16": {"TSNE_OVERLAP": "1", "TSNE_ATTR_AFTER_FRONT": "1",
+                           "TSNE_ATTR_SPLIT_GRID": "16"},
+    }
+    keys = ("TSNE_OVERLAP", "TSNE_ATTR_AFTER_FRONT", "TSNE_ATTR_SPLIT_GRID")
     out = {}
-    for ov in ("0", "1"):
-        old = os.environ.get("TSNE_OVERLAP")
-        os.environ["TSNE_OVERLAP"] = ov
+    for name, env in variants.items():
+        old = {k: os.environ.get(k) for k in keys}
+        for k in keys:
+            os.environ.pop(k, None)
+        os.environ.update(env)
         try:
             p = backend.create_default_params()
             p.verbose = 0
@@ -372,20 +385,23 @@ def test_overlapped_attractive_pass_matches_fused(backend, problem):
             p.learning_rate = max(P.shape[0] / 12.0, 200.0)
             ws = backend.init_workspace(P, 2, p, init_Y=Y0.T.copy())
         finally:
-            if old is None:
-                del os.environ["TSNE_OVERLAP"]
-            else:
-                os.environ["TSNE_OVERLAP"] = old
+            for k, v in old.items():
+                if v is None:
+                    os.environ.pop(k, None)
+                else:
+                    os.environ[k] = v
         try:
             assert backend.lib.tsne_run(ws) == 0
             Y = np.zeros((2, P.shape[0]))
             backend.lib.tsne_get_embedding(ws, Y)
-            out[ov] = (Y, backend.lib.compute_kl_divergence(ws))
+            out[name] = (Y, backend.lib.compute_kl_divergence(ws))
         finally:
             backend.lib.tsne_destroy(ws)
-    rel = np.abs(out["0"][0] - out["1"][0]).max() / np.abs(out["0"][0]).max()
-    _say(f"  rel diff {rel:.2e}, KL {out['0'][1]:.8f} / {out['1'][1]:.8f}")
-    assert np.array_equal(out["0"][0], out["1"][0]) and out["0"][1] == out["1"][1]
+    ref = out["fused"]
+    for name, (Y, kl) in out.items():
+        rel = np.abs(ref[0] - Y).max() / np.abs(ref[0]).max()
+        _say(f"  {name}: rel diff {rel:.2e}, KL {ref[1]:.8f} / {kl:.8f}")
+        assert np.array_equal(ref[0], Y) and ref[1] == kl, name
 
 
 def test_oversized_deepest_leaf_3d(backend, problem):

@@ -45,9 +45,11 @@ int launch_attr_tile(int D, int n, const AttrTiles &tiles, int nt, const int *ro
                      hipStream_t st, hipEvent_t ev_begin = nullptr, hipEvent_t ev_end = nullptr);
 // The attractive pass split in two, for iterations whose repulsive force is rebuilt
 // meanwhile on another stream: the attractive sums into A (float [n][D]), then the
-// update from A and the repulsive sum F (nblocks as for launch_attr_update).
+// update from A and the repulsive sum F (nblocks as for launch_attr_update).  grid: the
+// attractive launch's workgroups (0: nblocks; fewer leave CUs to the rebuild's launches,
+// with the same sums).
 int launch_attr_only(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
-                     const void *Yg, float *A, int nblocks, hipStream_t st);
+                     const void *Yg, float *A, int nblocks, int grid, hipStream_t st);
 int launch_update(int D, int n, const float *A, const float *F, double *Y64, double *uY,
                   double *gains, void *Yg_next, const double *scal, double exag, double mom,
                   double lr, double *block_sums, int nblocks, hipStream_t st);

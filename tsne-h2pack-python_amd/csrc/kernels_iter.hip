@@ -99,7 +99,10 @@ __device__ __forceinline__ double row_epilogue(float ax, float ay, float az, int
 // ATTR_ONLY: the attractive sums only, written to A (float [n][D], unscaled) for
 // update_kernel; Y64, uY, gains, Yg_next and block_sums are left alone.  This form runs
 // on a second stream next to an H2 rebuild (the rebuild and the attractive force of an
-// iteration both read only the iteration's embedding).
+// iteration both read only the iteration's embedding).  Its launch may be smaller than
+// the nvblk logical blocks (a multiple of 8 workgroups): each workgroup then walks the
+// logical blocks b, b + G, ... (same XCD), leaving CUs to the rebuild's narrow launches;
+// every row's sums are the same whichever workgroup computes them.
 template <int D, int LANES, int BATCH, bool VEC, bool ATTR_ONLY = false>
 __global__ void __launch_bounds__(256)
 attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict__ col_idx,
@@ -108,21 +111,24 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
                    double *__restrict__ uY, double *__restrict__ gains,
                    typename Vec<D>::T *__restrict__ Yg_next, const double *__restrict__ scal,
                    double exag4, double mom, double lr, int rows_per_block,
-                   double *__restrict__ block_sums, float *__restrict__ A = nullptr)
+                   double *__restrict__ block_sums, float *__restrict__ A = nullptr,
+                   int nvblk = 0)
 {
     using V = typename Vec<D>::T;
     __shared__ double gsum[(256 / LANES)][3];
     const int sub = threadIdx.x & (LANES - 1);
     const int grp = threadIdx.x / LANES;
+    const double rep_scale = ATTR_ONLY ? 0.0 : rep_scale_of(scal);
+    const double m_d = (!ATTR_ONLY && sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
+    const int nv = ATTR_ONLY ? nvblk : (int)gridDim.x;
+    for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
     // XCD-aware placement: workgroups are dealt round-robin to the 8 XCDs, so logical
     // block L = (b % 8) * (G / 8) + b / 8 gives each XCD one contiguous eighth of the
     // rows; with the points renumbered along the tree, that XCD's gathers stay in its
-    // own L2.  (G, the launch size, is a multiple of 8.)
-    const int lblk = (blockIdx.x & 7) * (gridDim.x >> 3) + (blockIdx.x >> 3);
+    // own L2.  (G, the number of logical blocks, is a multiple of 8.)
+    const int lblk = (vb & 7) * (nv >> 3) + (vb >> 3);
     const int row0 = lblk * rows_per_block;
     const int row1 = min(n, row0 + rows_per_block);
-    const double rep_scale = ATTR_ONLY ? 0.0 : rep_scale_of(scal);
-    const double m_d = (!ATTR_ONLY && sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
     double my_sum = 0.0;  // lane d of the group: sum of new coordinate d over its rows
 
     for (int i = row0 + grp; i < row1; i += (256 / LANES)) {
@@ -219,14 +225,16 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
                                              rep_scale, mom, lr, Y64, uY, gains, Yg_next + i);
         }
     }
-    if constexpr (ATTR_ONLY) return;
-    // per-group sums (lane d holds coordinate d), then groups in order
-    if (sub < D) gsum[grp][sub] = my_sum;
-    __syncthreads();
-    if (threadIdx.x < D) {
-        double t = 0.0;
-        for (int w = 0; w < (256 / LANES); w++) t += gsum[w][threadIdx.x];
-        block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
+    if constexpr (!ATTR_ONLY) {
+        // per-group sums (lane d holds coordinate d), then groups in order
+        if (sub < D) gsum[grp][sub] = my_sum;
+        __syncthreads();
+        if (threadIdx.x < D) {
+            double t = 0.0;
+            for (int w = 0; w < (256 / LANES); w++) t += gsum[w][threadIdx.x];
+            block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
+        }
+    }
     }
 }
 
@@ -303,7 +311,7 @@ int launch_attr_probe(int D, int n, int mode, const int *row_ptr, const int *col
                       const float *pval, const void *Yg, float *A, int nblocks, hipStream_t st)
 {
     const int rpb = (n + nblocks - 1) / nblocks;
-    if (mode == 2) return launch_attr_only(D, n, row_ptr, col_idx, pval, Yg, A, nblocks, st);
+    if (mode == 2) return launch_attr_only(D, n, row_ptr, col_idx, pval, Yg, A, nblocks, 0, st);
     if (D == 2) {
         if (mode == 1)
             hipLaunchKernelGGL((attr_probe_kernel<2, true>), dim3(nblocks), dim3(256), 0, st, n,
@@ -1007,17 +1015,20 @@ int launch_attr_update(int D, int n, const int *row_ptr, const int *col_idx, con
 }
 
 int launch_attr_only(int D, int n, const int *row_ptr, const int *col_idx, const float *pval,
-                     const void *Yg, float *A, int nblocks, hipStream_t st)
+                     const void *Yg, float *A, int nblocks, int grid, hipStream_t st)
 {
     const int rpb = (n + nblocks - 1) / nblocks;
+    // grid: workgroups of the launch (0: one per logical block); a multiple of 8, so a
+    // workgroup's logical blocks all sit on its XCD
+    grid = (grid <= 0 || grid >= nblocks) ? nblocks : std::max(8, grid & ~7);
     if (D == 2)
-        hipLaunchKernelGGL((attr_update_kernel<2, 16, 3, true, true>), dim3(nblocks), dim3(256), 0, st,
+        hipLaunchKernelGGL((attr_update_kernel<2, 16, 3, true, true>), dim3(grid), dim3(256), 0, st,
                            n, row_ptr, col_idx, pval, (const float2 *)Yg, nullptr, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, rpb, nullptr, A);
+                           nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, rpb, nullptr, A, nblocks);
     else
-        hipLaunchKernelGGL((attr_update_kernel<3, 16, 3, true, true>), dim3(nblocks), dim3(256), 0, st,
+        hipLaunchKernelGGL((attr_update_kernel<3, 16, 3, true, true>), dim3(grid), dim3(256), 0, st,
                            n, row_ptr, col_idx, pval, (const float4 *)Yg, nullptr, nullptr, nullptr,
-                           nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, rpb, nullptr, A);
+                           nullptr, nullptr, nullptr, 0.0, 0.0, 0.0, rpb, nullptr, A, nblocks);
     return hipGetLastError() == hipSuccess ? 0 : -1;
 }
 
