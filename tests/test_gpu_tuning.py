@@ -3,7 +3,10 @@
 The tuned choices -- the second stream of the attractive pass on rebuild iterations, tree
 levels per host check, graph replay, and which workgroup shape runs each node's ID -- must
 leave every result unchanged, bit for bit; the autotuner checks that on each of its trials,
-and these tests check it on whole runs with renumbering."""
+and these tests check it on whole runs with renumbering.  So must the tree levels' grid
+hints from the previous build (TSNE_LEVEL_HINTS, off in one run and on in the other)."""
+import os
+
 import numpy as np
 import pytest
 
@@ -16,14 +19,23 @@ def _say(msg):
     print(msg, flush=True)
 
 
-def _run(backend, P, dim, Y0, shapes, n_iter=300):
+def _run(backend, P, dim, Y0, shapes, n_iter=300, hints=None):
     p = backend.create_default_params()
     p.verbose = 0
     p.n_iter = n_iter
     p.h2_tol = 1e-2 if dim == 2 else 3e-2
     p.h2_rebuild_freq = 3
     p.learning_rate = max(P.shape[0] / 12.0, 200.0)
-    ws = backend.init_workspace(P, dim, p, init_Y=Y0.T.copy())
+    old = os.environ.get("TSNE_LEVEL_HINTS")
+    if hints is not None:
+        os.environ["TSNE_LEVEL_HINTS"] = hints  # read when the workspace's H2 is made
+    try:
+        ws = backend.init_workspace(P, dim, p, init_Y=Y0.T.copy())
+    finally:
+        if old is None:
+            os.environ.pop("TSNE_LEVEL_HINTS", None)
+        else:
+            os.environ["TSNE_LEVEL_HINTS"] = old
     try:
         backend.set_tuning(ws, **shapes)
         assert backend.lib.tsne_run(ws) == 0
@@ -37,15 +49,16 @@ def _run(backend, P, dim, Y0, shapes, n_iter=300):
 @pytest.mark.parametrize("dim,n", [(2, 6000), (3, 4000)])
 def test_launch_shapes_do_not_change_results(backend, dim, n):
     """Two opposite launch-shape sets over 300 iterations (rebuilds, three renumberings,
-    graph replay in one of them): the same embedding and KL, bit for bit."""
+    graph replay in one of them, level grid hints in the other): the same embedding and KL,
+    bit for bit."""
     _say(f"launch shapes {dim}D n={n}")
     X, _ = gaussian_mixture(n, dim=12, n_clusters=6, seed=31)
     P = joint_P_oracle(knn_graph(X, 60), 20.0)
     Y0 = init_normal(n, dim, 32)
     a = dict(overlap=0, level_batch=1, graph=0, one_class_level=0)
     b = dict(overlap=1, level_batch=8, graph=1, one_class_level=1_000_000)
-    Ya, kla = _run(backend, P, dim, Y0, a)
-    Yb, klb = _run(backend, P, dim, Y0, b)
+    Ya, kla = _run(backend, P, dim, Y0, a, hints="0")
+    Yb, klb = _run(backend, P, dim, Y0, b, hints="1")
     _say(f"  KL {kla:.10f} / {klb:.10f}, max |dY| {np.abs(Ya - Yb).max():.3e}")
     assert np.isfinite(Ya).all()
     assert np.array_equal(Ya, Yb) and kla == klb

@@ -54,6 +54,8 @@ struct H2 {
     int *pin = nullptr;      // pinned host scratch
     int *pin_dev = nullptr;  // the same memory as the device sees it
     std::vector<int> lvl;  // level l nodes are [lvl[l], lvl[l+1])
+    std::vector<int> prev_cnt;  // node count per level of the last tree (level grid hints)
+    int level_hints = -1;       // TSNE_LEVEL_HINTS at creation: 0 off, 1 on, -1 by size
     int n_nodes = 0, n_leaves = 0, n_adm = 0, n_x = 0, n_w = 0, n_near = 0;
     int usable = 0;
     int pq = 0;  // proxy slots the ID processes: the table's largest count, rounded up
@@ -342,17 +344,37 @@ int build_once(H2 *h, const void *Yg)
     // past the last one are empty and cost only their launches.
     const int NCH = (D == 2) ? H2Cfg<2>::NCH : H2Cfg<3>::NCH;
     const int maxlev = (D == 2) ? H2Cfg<2>::B + 1 : H2Cfg<3>::B + 1;  // level B holds leaves only
+    // With the previous build's per-level node counts as hints (the points move little
+    // between rebuilds), the first batch holds that build's depth and the host reads its
+    // last record at once: the tree usually ends there, with one host wait for the whole
+    // tree and no empty level queued past it.  The hints also size each level's grids
+    // (the kernels' loops cover any count, so a level that outgrew its hint is still
+    // complete).  Used up to 2^18 points, where the host waits of a small tree are a large
+    // share of its build (serving mix of 2K-200K-point jobs: sequential 3.07 -> 3.20
+    // jobs/s, 12 slots 8.05 -> 8.27); at 500K 3D and 1M 2D they measured no faster (within
+    // 1%).  The results are the same bit for bit.  TSNE_LEVEL_HINTS=0 / 1 turns them off /
+    // on at every size.
     int *plev = h->pin + CNT_COUNT, *plev_dev = h->pin_dev + CNT_COUNT;
     for (int k = 0; k < maxlev * kPinRec; k++) plev[k] = -1;
     int l = 0, pend = 0;            // the level pass of level l appends to pend[pend]
     const int lb = h->tune_level_batch > 0 ? h->tune_level_batch : level_batch();
+    const bool use_hints = h->level_hints >= 0 ? h->level_hints == 1 : h->n <= (1 << 18);
+    static const std::vector<int> no_hint;
+    const std::vector<int> &hint = use_hints ? h->prev_cnt : no_hint;
+    const int first = hint.empty() ? lb : std::min<int>(maxlev, (int)hint.size());
     int known_l = 0, known_n = 1;   // level known_l has exactly known_n nodes
     hipEvent_t last_ev = nullptr;   // end of the last queued batch
+    std::vector<int> bend;          // end level of each queued batch
     for (int batch = 0; l < maxlev; batch++) {
-        for (int k = 0; k < lb && l < maxlev; k++, l++) {
+        const int bsz = (batch == 0) ? first : lb;
+        for (int k = 0; k < bsz && l < maxlev; k++, l++) {
             long long bound = known_n;
             for (int q = known_l; q < l && bound < d.node_cap; q++) bound *= NCH;
             bound = std::min<long long>(bound, d.node_cap);
+            if (l > known_l && !hint.empty()) {
+                const long long hb = (l < (int)hint.size()) ? hint[l] + hint[l] / 4 + 64 : 64;
+                bound = std::min(bound, hb);
+            }
             int nsplit = 1;
             if (launch_level_pass(D, d, l, (int)bound, pend ^ 1, pend, &nsplit, st) ||
                 launch_level_write(D, d, l, nsplit, pend ^ 1, plev_dev + kPinRec * l, st))
@@ -361,9 +383,13 @@ int build_once(H2 *h, const void *Yg)
         }
         if (hipEventRecord(h->ev_lvl[batch & 1], st) != hipSuccess) return -1;
         last_ev = h->ev_lvl[batch & 1];
-        if (batch == 0) continue;
-        if (hipEventSynchronize(h->ev_lvl[(batch - 1) & 1]) != hipSuccess) return -1;
-        const int lc = std::min(l, lb * batch) - 1;  // last level of batch - 1
+        bend.push_back(l);
+        if (batch == 0 && hint.empty()) continue;
+        // the batch whose records are read now: the first one at once when it holds the
+        // hinted depth, else the one before the batch just queued
+        const int wb = (batch == 0) ? 0 : batch - 1;
+        if (hipEventSynchronize(h->ev_lvl[wb & 1]) != hipSuccess) return -1;
+        const int lc = bend[wb] - 1;  // last level of batch wb
         const int *rec = plev + kPinRec * lc;
         if (rec[1] || rec[2]) {  // non-finite input / node arrays full: drain, then report
             if (hipStreamSynchronize(st) != hipSuccess) return -1;
@@ -399,6 +425,8 @@ int build_once(H2 *h, const void *Yg)
         h->pin[CNT_LEAF] = fin[8];
         h->lvl.assign({0, 1});
         for (int k = 1; k + 1 < (int)lv.size() && lv[k + 1] > lv[k]; k++) h->lvl.push_back(lv[k + 1]);
+        h->prev_cnt.resize(h->lvl.size() - 1);
+        for (size_t k = 0; k + 1 < h->lvl.size(); k++) h->prev_cnt[k] = h->lvl[k + 1] - h->lvl[k];
     }
     h->n_nodes = h->lvl.back();
     if (h->hflags[FLAG_NONFINITE]) return -2;
@@ -496,6 +524,7 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
     // add up over the levels, and the gradient error is what h2_tol bounds
     h->id_tol = 0.5 * tol;
     h->st = st;
+    if (const char *e = getenv("TSNE_LEVEL_HINTS")) h->level_hints = (e[0] == '1') ? 1 : 0;
     H2Dev &d = h->d;
     d.n = n;
     d.D = dim;
