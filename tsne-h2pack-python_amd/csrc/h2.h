@@ -48,7 +48,7 @@ int h2_product(H2 *h, const void *Yg, float *s, float *F, int mixed);
 // admissible pair falls back to the direct sum, as the reference does).
 int h2_usable(const H2 *h);
 
-int h2_stats(const H2 *h, tsne_h2_stats_t *st);
+int h2_stats(H2 *h, tsne_h2_stats_t *st);
 
 // Device array (n): the point at Morton position k of the last build, i.e. the tree order.
 const int *h2_order(const H2 *h);

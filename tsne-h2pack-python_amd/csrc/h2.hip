@@ -62,6 +62,10 @@ struct H2 {
     int hflags[FLAG_COUNT] = {0};
     int fallbacks = 0;  // builds that left the product to the direct sum, and why (first)
     int rank_capped = 0;  // builds in which some node's ID stopped at RMAX short of id_tol
+    // the ID's flags of the last build, copied to pinned memory behind the ID (ev_flags);
+    // absorbed at the next build or statistics call, so no host wait follows the ID
+    bool flags_pending = false;
+    hipEvent_t ev_flags = nullptr;
     // timing of the ID launches of each build (HIP events on the build stream)
     bool timing = false;
     hipEvent_t ev_id0 = nullptr, ev_id1 = nullptr;
@@ -254,7 +258,8 @@ int sort_pairs_list(H2 *h, uint64_t *&buf, int n, int *beg, int *end, hipStream_
 // scratch), kept on a process-wide free list per device: freeing pinned memory can
 // synchronise the whole device, which would stall every other job on it each time one
 // job ends.
-constexpr size_t kPinBytes = sizeof(int) * (CNT_COUNT + kPinRec * kMaxLevels);
+constexpr size_t kPinBytes = sizeof(int) * (CNT_COUNT + kPinRec * kMaxLevels + FLAG_COUNT);
+constexpr size_t kPinFlags = CNT_COUNT + kPinRec * kMaxLevels;  // the ID's flags (ints)
 struct PinBlock {
     int device;
     int *host, *dev;
@@ -295,6 +300,29 @@ void pin_give(int *host, int *dev)
     if (hipGetDevice(&device) != hipSuccess) return;  // kept out of the list (leaked)
     std::lock_guard<std::mutex> lk(g_pin_mu);
     g_pin_free.push_back({device, host, dev});
+}
+
+// The ID's flags of the last build, copied to pinned memory behind it: a rank that
+// reached its cap short of the ID tolerance is counted once per build and reported once.
+// A U-pool overflow cannot happen when the host skipped its wait (the tree's count of
+// children per part bounded every reservation); if it ever did, the next build fails
+// with a message.
+int absorb_id_flags(H2 *h)
+{
+    if (!h->flags_pending) return 0;
+    if (hipEventSynchronize(h->ev_flags) != hipSuccess) return -1;
+    h->flags_pending = false;
+    const int *f = h->pin + kPinFlags;
+    if (f[FLAG_POOL_OVF]) {
+        fprintf(stderr, "tsne_h2pack: an ID overflowed the U pool sized from the tree\n");
+        return -1;
+    }
+    if (f[FLAG_RANK_CAP] && h->rank_capped++ == 0)
+        fprintf(stderr, "tsne_h2pack: warning: a skeleton rank reached its cap (%d) before "
+                        "the ID tolerance %.3g; the product may exceed h2_tol = %.3g "
+                        "(counted in tsne_h2_stats.rank_capped_builds)\n",
+                rmax_of(h->D), h->id_tol, h->tol);
+    return 0;
 }
 
 // The caller's one-shot hook, queued once the build's front (root box, keys, sort, sorted
@@ -423,6 +451,7 @@ int build_once(H2 *h, const void *Yg)
         h->pin[CNT_W] = fin[6];
         h->pin[CNT_NEAR] = fin[7];
         h->pin[CNT_LEAF] = fin[8];
+        h->pin[CNT_NEED] = fin[10];  // children of the fullest U-pool part
         h->lvl.assign({0, 1});
         for (int k = 1; k + 1 < (int)lv.size() && lv[k + 1] > lv[k]; k++) h->lvl.push_back(lv[k + 1]);
         h->prev_cnt.resize(h->lvl.size() - 1);
@@ -444,6 +473,17 @@ int build_once(H2 *h, const void *Yg)
     h->n_w = h->pin[CNT_W];
     h->n_near = h->pin[CNT_NEAR];
     h->n_leaves = h->pin[CNT_LEAF];
+    // Whether the U pool holds every reservation the ID can make: an internal node
+    // reserves 2 m RMAX floats with m (its candidates) at most RMAX per child, in pool part
+    // (its index in the level) mod kPoolParts, and the tree counted the children per part
+    // (record word 10).  Then the ID cannot overflow the pool and the product follows the
+    // ID without the host reading the flags in between; otherwise (a pool sized below the
+    // worst case, as 3D trees can be) the host reads them after the ID and a build that
+    // overflowed is retried with a larger pool.
+    const bool pool_safe = [&] {
+        const int RMAX = rmax_of(D);
+        return 2LL * RMAX * RMAX * (long long)std::max(0, h->pin[CNT_NEED]) <= pool_part_cap(d, RMAX);
+    }();
     // ADM and X ranges were written per node by the list kernel; NEAR and W pairs are
     // appended by several levels and are grouped by target below.  Only the leaf product
     // reads those lists, so with a second stream the sorts run there, beside the ID.  The
@@ -478,12 +518,18 @@ int build_once(H2 *h, const void *Yg)
     if (h->st_aux && hipEventRecord(h->ev_sorted, h->st_aux) != hipSuccess) return -1;
     if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_SORTED], ss);
     if (h->st_aux && hipStreamWaitEvent(st, h->ev_sorted, 0) != hipSuccess) return -1;
-    if (hipMemcpyAsync(h->hflags, d.flags, sizeof(int) * FLAG_COUNT, hipMemcpyDeviceToHost, st) ||
-        hipStreamSynchronize(st) != hipSuccess)
+    if (hipMemcpyAsync(h->pin + kPinFlags, d.flags, sizeof(int) * FLAG_COUNT,
+                       hipMemcpyDeviceToHost, st) != hipSuccess ||
+        hipEventRecord(h->ev_flags, st) != hipSuccess)
         return -1;
-    if (h->hflags[FLAG_POOL_OVF]) {
-        h->pool_f *= 2.0;
-        return 1;
+    h->flags_pending = true;
+    if (!pool_safe) {
+        if (hipEventSynchronize(h->ev_flags) != hipSuccess) return -1;
+        if (h->pin[kPinFlags + FLAG_POOL_OVF]) {
+            h->flags_pending = false;
+            h->pool_f *= 2.0;
+            return 1;
+        }
     }
     if (h->tl_on) {
         // the end mark goes in after the host's last wait: the gap to the product's first
@@ -502,7 +548,9 @@ int build_once(H2 *h, const void *Yg)
     }
     if (tid) {
         float ms = 0.f;
-        if (hipEventElapsedTime(&ms, h->ev_id0, h->ev_id1) == hipSuccess) h->id_ms = ms;
+        if (hipEventSynchronize(h->ev_id1) == hipSuccess &&
+            hipEventElapsedTime(&ms, h->ev_id0, h->ev_id1) == hipSuccess)
+            h->id_ms = ms;
     }
     return 0;
 }
@@ -538,6 +586,7 @@ H2 *h2_create(int n, int dim, int leaf_size, double tol, hipStream_t st)
               dmalloc(&h->part, (size_t)h->nblk * 7, h->st) &&
               pin_take(&h->pin, &h->pin_dev) &&
               hipEventCreateWithFlags(&h->ev_lvl[0], hipEventDisableTiming) == hipSuccess &&
+              hipEventCreateWithFlags(&h->ev_flags, hipEventDisableTiming) == hipSuccess &&
               hipEventCreateWithFlags(&h->ev_lvl[1], hipEventDisableTiming) == hipSuccess;
     if (ok && getenv("TSNE_H2_TIMELINE") && getenv("TSNE_H2_TIMELINE")[0] == '1') {
         h->tl_on = true;
@@ -625,6 +674,10 @@ void h2_destroy(H2 *h)
         if (e) (void)hipEventDestroy(e);
     if (h->ev_id0) (void)hipEventDestroy(h->ev_id0);
     if (h->ev_id1) (void)hipEventDestroy(h->ev_id1);
+    if (h->ev_flags) {
+        (void)absorb_id_flags(h);  // a rank cap in the last build is still reported
+        (void)hipEventDestroy(h->ev_flags);
+    }
     free_sized(h);
     H2Dev &d = h->d;
     dfree(d.keys, h->st); dfree(h->keys_alt, h->st); dfree(d.perm, h->st); dfree(h->perm_alt, h->st); dfree(d.Ys, h->st);
@@ -654,16 +707,12 @@ static int build_all(H2 *h, const void *Yg, int *nonfinite)
 {
     *nonfinite = 0;
     h->usable = 0;
+    if (absorb_id_flags(h) != 0) return -1;  // the previous build's ID
     const char *why = "capacity retries exhausted";
     for (int attempt = 0; attempt < 6; attempt++) {
         const int rc = build_once(h, Yg);
         if (rc == 0) {
             h->usable = !h->hflags[FLAG_BIG_LEAF];
-            if (h->usable && h->hflags[FLAG_RANK_CAP] && h->rank_capped++ == 0)
-                fprintf(stderr, "tsne_h2pack: warning: a skeleton rank reached its cap (%d) before "
-                                "the ID tolerance %.3g; the product may exceed h2_tol = %.3g "
-                                "(counted in tsne_h2_stats.rank_capped_builds)\n",
-                        rmax_of(h->D), h->id_tol, h->tol);
             if (h->usable) return 0;
             why = "a leaf at the deepest level holds more points than a leaf can";
             break;
@@ -740,8 +789,9 @@ int h2_usable(const H2 *h) { return h->usable; }
 
 const int *h2_order(const H2 *h) { return h->d.perm; }
 
-int h2_stats(const H2 *h, tsne_h2_stats_t *st)
+int h2_stats(H2 *h, tsne_h2_stats_t *st)
 {
+    (void)absorb_id_flags(h);  // the last build's rank cap is counted
     memset(st, 0, sizeof(*st));
     st->levels = (int)h->lvl.size() - 1;
     st->n_nodes = h->n_nodes;

@@ -69,7 +69,10 @@ constexpr int kPoolParts = 8;
 enum H2Count { CNT_ADM = 0, CNT_X = kCntSlot, CNT_W = 2 * kCntSlot, CNT_NEAR = 3 * kCntSlot,
                CNT_PEND0 = 4 * kCntSlot, CNT_PEND1 = 5 * kCntSlot, CNT_LEAF = 6 * kCntSlot,
                CNT_POOL = 7 * kCntSlot,  // kPoolParts counters, one line each
-               CNT_COUNT = (7 + kPoolParts) * kCntSlot };
+               // per pool part, the children of the internal nodes whose ID reserves there
+               // (a node's reservation is at most 2 RMAX^2 floats per child)
+               CNT_NEED = (7 + kPoolParts) * kCntSlot,
+               CNT_COUNT = (7 + 2 * kPoolParts) * kCntSlot };
 
 struct RootInfo {
     double c[3];   // root box centre

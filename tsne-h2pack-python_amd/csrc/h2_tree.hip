@@ -369,6 +369,11 @@ __device__ __forceinline__ void publish_level(const H2Dev &h, int l, int le, int
     out[7] = h.cnt[CNT_NEAR];
     out[8] = h.cnt[CNT_LEAF];
     out[9] = ne;
+    // the most children over the U-pool parts, of the levels whose writes have finished
+    // (all of them once no level follows: the last level holds leaves only)
+    int need = 0;
+    for (int q = 0; q < kPoolParts; q++) need = max(need, h.cnt[CNT_NEED + q * kCntSlot]);
+    out[10] = need;
 }
 
 // The children of node i of level l (lb its first node), from child slot `base` on.
@@ -683,9 +688,14 @@ level_write_kernel(H2Dev h, int l, int nsplit, int pend_next, int *out)
     int c0, c1;
     split_chunk(cnt, nsplit, b, c0, c1);
     int run = le + pre;  // first child slot of the round
+    // children of this thread's nodes: the ID of node k of the level reserves its U in
+    // pool part k mod kPoolParts (one workgroup per node, blockIdx = k), the same part
+    // for every round since 256 is a multiple of kPoolParts
+    int kids = 0;
     for (int r0 = c0; r0 < c1; r0 += 256) {  // uniform trip count
         const int k = r0 + t;
         const int c = (k < c1) ? h.childcnt[k] : 0;
+        kids += c;
         int x = c;  // inclusive scan over the wavefront, then over the four wavefronts
         for (int off = 1; off < 64; off <<= 1) {
             const int y = __shfl_up(x, off, 64);
@@ -698,6 +708,18 @@ level_write_kernel(H2Dev h, int l, int nsplit, int pend_next, int *out)
         for (int q = 0; q < w; q++) wp += s_wsum[q];
         if (c > 0) split_write_node<D>(h, l, lb, lb + k, run + wp + x - c);
         run += s_wsum[0] + s_wsum[1] + s_wsum[2] + s_wsum[3];
+    }
+    // per-part sums: lanes with the same lane mod 8 share a part (lanes ^8, ^16, ^32),
+    // then the four wavefronts
+    static_assert(kPoolParts == 8, "parts by lane mod 8");
+    for (int o = 8; o < 64; o <<= 1) kids += __shfl_xor(kids, o, 64);
+    __shared__ int s_kids[4][8];
+    __syncthreads();
+    if (lane < 8) s_kids[w][lane] = kids;
+    __syncthreads();
+    if (t < 8) {
+        const int v = s_kids[0][t] + s_kids[1][t] + s_kids[2][t] + s_kids[3][t];
+        if (v) atomicAdd(&h.cnt[CNT_NEED + ((c0 + t) % kPoolParts) * kCntSlot], v);
     }
 }
 
