@@ -55,7 +55,7 @@ struct H2 {
     int *pin_dev = nullptr;  // the same memory as the device sees it
     std::vector<int> lvl;  // level l nodes are [lvl[l], lvl[l+1])
     std::vector<int> prev_cnt;  // node count per level of the last tree (level grid hints)
-    int level_hints = -1;       // TSNE_LEVEL_HINTS at creation: 0 off, 1 on, -1 by size
+    int level_hints = 1;        // TSNE_LEVEL_HINTS at creation (0: off)
     int n_nodes = 0, n_leaves = 0, n_adm = 0, n_x = 0, n_w = 0, n_near = 0;
     int usable = 0;
     int pq = 0;  // proxy slots the ID processes: the table's largest count, rounded up
@@ -372,21 +372,19 @@ int build_once(H2 *h, const void *Yg)
     // past the last one are empty and cost only their launches.
     const int NCH = (D == 2) ? H2Cfg<2>::NCH : H2Cfg<3>::NCH;
     const int maxlev = (D == 2) ? H2Cfg<2>::B + 1 : H2Cfg<3>::B + 1;  // level B holds leaves only
-    // With the previous build's per-level node counts as hints (the points move little
-    // between rebuilds), the first batch holds that build's depth and the host reads its
-    // last record at once: the tree usually ends there, with one host wait for the whole
-    // tree and no empty level queued past it.  The hints also size each level's grids
-    // (the kernels' loops cover any count, so a level that outgrew its hint is still
-    // complete).  Used up to 2^18 points, where the host waits of a small tree are a large
-    // share of its build (serving mix of 2K-200K-point jobs: sequential 3.07 -> 3.20
-    // jobs/s, 12 slots 8.05 -> 8.27); at 500K 3D and 1M 2D they measured no faster (within
-    // 1%).  The results are the same bit for bit.  TSNE_LEVEL_HINTS=0 / 1 turns them off /
-    // on at every size.
+    // With the previous build's tree as a hint (the points move little between rebuilds),
+    // the first batch holds that build's depth and the host reads its last record at
+    // once: the tree usually ends there, with one host wait for the whole tree and no
+    // empty level queued past it.  1M 2D whole schedule 753 -> 762 it/s; serving mix of
+    // 2K-200K-point jobs 8.05 -> 8.25 jobs/s (12 slots); 500K 3D 490 -> 492; the results
+    // are the same bit for bit.  TSNE_LEVEL_HINTS=0 turns it off.  Sizing each level's
+    // grids from the hint's per-level counts as well (TSNE_LEVEL_GRID_HINTS=1; the
+    // kernels' loops cover any count) measured slower at 1M (749 it/s) and is off.
     int *plev = h->pin + CNT_COUNT, *plev_dev = h->pin_dev + CNT_COUNT;
     for (int k = 0; k < maxlev * kPinRec; k++) plev[k] = -1;
     int l = 0, pend = 0;            // the level pass of level l appends to pend[pend]
     const int lb = h->tune_level_batch > 0 ? h->tune_level_batch : level_batch();
-    const bool use_hints = h->level_hints >= 0 ? h->level_hints == 1 : h->n <= (1 << 18);
+    const bool use_hints = h->level_hints != 0;
     static const std::vector<int> no_hint;
     const std::vector<int> &hint = use_hints ? h->prev_cnt : no_hint;
     const int first = hint.empty() ? lb : std::min<int>(maxlev, (int)hint.size());
@@ -399,7 +397,11 @@ int build_once(H2 *h, const void *Yg)
             long long bound = known_n;
             for (int q = known_l; q < l && bound < d.node_cap; q++) bound *= NCH;
             bound = std::min<long long>(bound, d.node_cap);
-            if (l > known_l && !hint.empty()) {
+            static const bool grid_hints = [] {
+                const char *e = getenv("TSNE_LEVEL_GRID_HINTS");
+                return e && e[0] == '1';
+            }();
+            if (l > known_l && !hint.empty() && grid_hints) {
                 const long long hb = (l < (int)hint.size()) ? hint[l] + hint[l] / 4 + 64 : 64;
                 bound = std::min(bound, hb);
             }
