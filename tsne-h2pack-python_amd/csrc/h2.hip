@@ -364,12 +364,13 @@ int build_once(H2 *h, const void *Yg)
     if (run_front_hook(h)) return -1;
     if (h->tl_on) (void)hipEventRecord(h->tl_ev[TL_FRONT], st);
     if (launch_init_root(d, st)) return -1;
-    // Levels are queued in batches of kLevelBatch; the host never waits for the level it
-    // has just queued.  After queuing batch k it waits for batch k - 1 (recorded in
-    // ev_lvl) and reads that batch's last published level from pinned memory (one
-    // record of kPinRec words per level, so later levels cannot overwrite it): whether
-    // the tree has ended, and the exact node count that bounds the next grids.  Levels
-    // past the last one are empty and cost only their launches.
+    // Levels are queued in batches of kLevelBatch (the first batch: the previous tree's
+    // depth, below).  After queuing batch k >= 1 the host waits for batch k - 1 (recorded
+    // in ev_lvl), never for the batch it has just queued, and reads that batch's last
+    // published level from pinned memory (one record of kPinRec words per level, so later
+    // levels cannot overwrite it): whether the tree has ended, and the exact node count
+    // that bounds the next grids.  Levels past the last one are empty and cost only their
+    // launches.
     const int NCH = (D == 2) ? H2Cfg<2>::NCH : H2Cfg<3>::NCH;
     const int maxlev = (D == 2) ? H2Cfg<2>::B + 1 : H2Cfg<3>::B + 1;  // level B holds leaves only
     // With the previous build's tree as a hint (the points move little between rebuilds),

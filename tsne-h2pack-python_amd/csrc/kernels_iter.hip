@@ -122,119 +122,119 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     const double m_d = (!ATTR_ONLY && sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
     const int nv = ATTR_ONLY ? nvblk : (int)gridDim.x;
     for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
-    // XCD-aware placement: workgroups are dealt round-robin to the 8 XCDs, so logical
-    // block L = (b % 8) * (G / 8) + b / 8 gives each XCD one contiguous eighth of the
-    // rows; with the points renumbered along the tree, that XCD's gathers stay in its
-    // own L2.  (G, the number of logical blocks, is a multiple of 8.)
-    const int lblk = (vb & 7) * (nv >> 3) + (vb >> 3);
-    const int row0 = lblk * rows_per_block;
-    const int row1 = min(n, row0 + rows_per_block);
-    double my_sum = 0.0;  // lane d of the group: sum of new coordinate d over its rows
+        // XCD-aware placement: workgroups are dealt round-robin to the 8 XCDs, so logical
+        // block L = (b % 8) * (G / 8) + b / 8 gives each XCD one contiguous eighth of the
+        // rows; with the points renumbered along the tree, that XCD's gathers stay in its
+        // own L2.  (G, the number of logical blocks, is a multiple of 8.)
+        const int lblk = (vb & 7) * (nv >> 3) + (vb >> 3);
+        const int row0 = lblk * rows_per_block;
+        const int row1 = min(n, row0 + rows_per_block);
+        double my_sum = 0.0;  // lane d of the group: sum of new coordinate d over its rows
 
-    for (int i = row0 + grp; i < row1; i += (256 / LANES)) {
-        const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
-        // update operands of this row, fetched now and used after the reduction
-        const size_t kd = (size_t)i * D + (sub < D ? sub : 0);
-        double f_d = 0.0, u_d = 0.0, g_d = 0.0, y_d = 0.0;
-        if (!ATTR_ONLY && sub < D) {
-            f_d = (double)F[kd];
-            u_d = uY[kd];
-            g_d = gains[kd];
-            y_d = Y64[kd];
-        }
-        const V yi = Yg[i];
-        float ax = 0.f, ay = 0.f, az = 0.f;
-        if constexpr (VEC) {
-            const int c1 = (e1 + 3) >> 2;  // chunks [e0/4, c1) cover the row
-            for (int cb = (e0 >> 2) + sub; cb < c1; cb += BATCH * LANES) {
-                int jj[BATCH][4];
-                float pp[BATCH][4];
-#pragma unroll
-                for (int t = 0; t < BATCH; t++) {
-                    const int c = cb + t * LANES;
-                    v4i cc = {i, i, i, i};
-                    v4f pv = {0.f, 0.f, 0.f, 0.f};
-                    if (c < c1) {
-                        cc = __builtin_nontemporal_load((const v4i *)col_idx + c);
-                        pv = __builtin_nontemporal_load((const v4f *)pval + c);
+        for (int i = row0 + grp; i < row1; i += (256 / LANES)) {
+            const int e0 = row_ptr[i], e1 = row_ptr[i + 1];
+            // update operands of this row, fetched now and used after the reduction
+            const size_t kd = (size_t)i * D + (sub < D ? sub : 0);
+            double f_d = 0.0, u_d = 0.0, g_d = 0.0, y_d = 0.0;
+            if (!ATTR_ONLY && sub < D) {
+                f_d = (double)F[kd];
+                u_d = uY[kd];
+                g_d = gains[kd];
+                y_d = Y64[kd];
+            }
+            const V yi = Yg[i];
+            float ax = 0.f, ay = 0.f, az = 0.f;
+            if constexpr (VEC) {
+                const int c1 = (e1 + 3) >> 2;  // chunks [e0/4, c1) cover the row
+                for (int cb = (e0 >> 2) + sub; cb < c1; cb += BATCH * LANES) {
+                    int jj[BATCH][4];
+                    float pp[BATCH][4];
+    #pragma unroll
+                    for (int t = 0; t < BATCH; t++) {
+                        const int c = cb + t * LANES;
+                        v4i cc = {i, i, i, i};
+                        v4f pv = {0.f, 0.f, 0.f, 0.f};
+                        if (c < c1) {
+                            cc = __builtin_nontemporal_load((const v4i *)col_idx + c);
+                            pv = __builtin_nontemporal_load((const v4f *)pval + c);
+                        }
+                        const int e = 4 * c;
+    #pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const bool ok = e + u >= e0 && e + u < e1;
+                            jj[t][u] = ok ? cc[u] : i;
+                            pp[t][u] = ok ? pv[u] : 0.f;
+                        }
                     }
-                    const int e = 4 * c;
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const bool ok = e + u >= e0 && e + u < e1;
-                        jj[t][u] = ok ? cc[u] : i;
-                        pp[t][u] = ok ? pv[u] : 0.f;
-                    }
+                    V yy[BATCH][4];
+    #pragma unroll
+                    for (int t = 0; t < BATCH; t++)
+    #pragma unroll
+                        for (int u = 0; u < 4; u++) yy[t][u] = Yg[jj[t][u]];
+    #pragma unroll
+                    for (int t = 0; t < BATCH; t++)
+    #pragma unroll
+                        for (int u = 0; u < 4; u++) {
+                            const float dx = yi.x - yy[t][u].x, dy = yi.y - yy[t][u].y;
+                            float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
+                            float dz = 0.f;
+                            if constexpr (D == 3) { dz = yi.z - yy[t][u].z; r2 = fmaf(dz, dz, r2); }
+                            const float q = pp[t][u] * fast_rcp(r2);
+                            ax = fmaf(q, dx, ax);
+                            ay = fmaf(q, dy, ay);
+                            if constexpr (D == 3) az = fmaf(q, dz, az);
+                        }
                 }
-                V yy[BATCH][4];
-#pragma unroll
-                for (int t = 0; t < BATCH; t++)
-#pragma unroll
-                    for (int u = 0; u < 4; u++) yy[t][u] = Yg[jj[t][u]];
-#pragma unroll
-                for (int t = 0; t < BATCH; t++)
-#pragma unroll
-                    for (int u = 0; u < 4; u++) {
-                        const float dx = yi.x - yy[t][u].x, dy = yi.y - yy[t][u].y;
-                        float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
-                        float dz = 0.f;
-                        if constexpr (D == 3) { dz = yi.z - yy[t][u].z; r2 = fmaf(dz, dz, r2); }
-                        const float q = pp[t][u] * fast_rcp(r2);
-                        ax = fmaf(q, dx, ax);
-                        ay = fmaf(q, dy, ay);
-                        if constexpr (D == 3) az = fmaf(q, dz, az);
-                    }
+            }
+            // a batch is BATCH nonzeros per lane: all its column loads are issued, then
+            // all its gathers, so a row costs two memory round trips instead of one per step
+            for (int base = e0 + sub; !VEC && base < e1; base += BATCH * LANES) {
+                int jj[BATCH];
+                float pp[BATCH];
+    #pragma unroll
+                for (int t = 0; t < BATCH; t++) {
+                    const int e = base + t * LANES;
+                    const bool ok = e < e1;
+                    jj[t] = ok ? col_idx[e] : i;
+                    pp[t] = ok ? pval[e] : 0.f;
+                }
+                V yy[BATCH];
+    #pragma unroll
+                for (int t = 0; t < BATCH; t++) yy[t] = Yg[jj[t]];
+    #pragma unroll
+                for (int t = 0; t < BATCH; t++) {
+                    const float dx = yi.x - yy[t].x, dy = yi.y - yy[t].y;
+                    float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
+                    float dz = 0.f;
+                    if constexpr (D == 3) { dz = yi.z - yy[t].z; r2 = fmaf(dz, dz, r2); }
+                    const float q = pp[t] * fast_rcp(r2);
+                    ax = fmaf(q, dx, ax);
+                    ay = fmaf(q, dy, ay);
+                    if constexpr (D == 3) az = fmaf(q, dz, az);
+                }
+            }
+            if constexpr (ATTR_ONLY) {
+                for (int off = LANES / 2; off > 0; off >>= 1) {
+                    ax += __shfl_xor(ax, off, 64);
+                    ay += __shfl_xor(ay, off, 64);
+                    if constexpr (D == 3) az += __shfl_xor(az, off, 64);
+                }
+                if (sub < D) A[kd] = (sub == 0) ? ax : (sub == 1) ? ay : az;
+            } else {
+                my_sum += row_epilogue<D, LANES>(ax, ay, az, sub, kd, f_d, u_d, g_d, y_d, m_d, exag4,
+                                                 rep_scale, mom, lr, Y64, uY, gains, Yg_next + i);
             }
         }
-        // a batch is BATCH nonzeros per lane: all its column loads are issued, then
-        // all its gathers, so a row costs two memory round trips instead of one per step
-        for (int base = e0 + sub; !VEC && base < e1; base += BATCH * LANES) {
-            int jj[BATCH];
-            float pp[BATCH];
-#pragma unroll
-            for (int t = 0; t < BATCH; t++) {
-                const int e = base + t * LANES;
-                const bool ok = e < e1;
-                jj[t] = ok ? col_idx[e] : i;
-                pp[t] = ok ? pval[e] : 0.f;
-            }
-            V yy[BATCH];
-#pragma unroll
-            for (int t = 0; t < BATCH; t++) yy[t] = Yg[jj[t]];
-#pragma unroll
-            for (int t = 0; t < BATCH; t++) {
-                const float dx = yi.x - yy[t].x, dy = yi.y - yy[t].y;
-                float r2 = fmaf(dx, dx, fmaf(dy, dy, 1.0f));
-                float dz = 0.f;
-                if constexpr (D == 3) { dz = yi.z - yy[t].z; r2 = fmaf(dz, dz, r2); }
-                const float q = pp[t] * fast_rcp(r2);
-                ax = fmaf(q, dx, ax);
-                ay = fmaf(q, dy, ay);
-                if constexpr (D == 3) az = fmaf(q, dz, az);
+        if constexpr (!ATTR_ONLY) {
+            // per-group sums (lane d holds coordinate d), then groups in order
+            if (sub < D) gsum[grp][sub] = my_sum;
+            __syncthreads();
+            if (threadIdx.x < D) {
+                double t = 0.0;
+                for (int w = 0; w < (256 / LANES); w++) t += gsum[w][threadIdx.x];
+                block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
             }
         }
-        if constexpr (ATTR_ONLY) {
-            for (int off = LANES / 2; off > 0; off >>= 1) {
-                ax += __shfl_xor(ax, off, 64);
-                ay += __shfl_xor(ay, off, 64);
-                if constexpr (D == 3) az += __shfl_xor(az, off, 64);
-            }
-            if (sub < D) A[kd] = (sub == 0) ? ax : (sub == 1) ? ay : az;
-        } else {
-            my_sum += row_epilogue<D, LANES>(ax, ay, az, sub, kd, f_d, u_d, g_d, y_d, m_d, exag4,
-                                             rep_scale, mom, lr, Y64, uY, gains, Yg_next + i);
-        }
-    }
-    if constexpr (!ATTR_ONLY) {
-        // per-group sums (lane d holds coordinate d), then groups in order
-        if (sub < D) gsum[grp][sub] = my_sum;
-        __syncthreads();
-        if (threadIdx.x < D) {
-            double t = 0.0;
-            for (int w = 0; w < (256 / LANES); w++) t += gsum[w][threadIdx.x];
-            block_sums[(size_t)lblk * 3 + threadIdx.x] = t;
-        }
-    }
     }
 }
 
