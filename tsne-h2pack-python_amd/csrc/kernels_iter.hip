@@ -120,6 +120,8 @@ attr_update_kernel(int n, const int *__restrict__ row_ptr, const int *__restrict
     const int grp = threadIdx.x / LANES;
     const double rep_scale = ATTR_ONLY ? 0.0 : rep_scale_of(scal);
     const double m_d = (!ATTR_ONLY && sub < D) ? scal[SLOT_MEAN + sub] : 0.0;
+    // the fused form runs exactly one logical block per workgroup (nv = the grid): its
+    // gsum reduction at the end of a block is not fenced against a next block's writes
     const int nv = ATTR_ONLY ? nvblk : (int)gridDim.x;
     for (int vb = blockIdx.x; vb < nv; vb += gridDim.x) {
         // XCD-aware placement: workgroups are dealt round-robin to the 8 XCDs, so logical
